@@ -1,0 +1,1032 @@
+/*
+ * etd_oracle.c -- FP64 CPU oracle for the ScLETD step (test infrastructure).
+ *
+ * Not the product: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, as the checker.  Function headers cite the
+ * reference lines restated (paths relative to the reference root) and what
+ * pins each one:
+ *
+ *   orc_eigbasis     closed forms; pinned by the reference's n=260 files
+ *                    (tests/golden/eig260.npz) and V*diag(D)*V^-1 = T.
+ *   orc_ghost_fill   pinned by the periodic-image invariant and the fixed-
+ *                    point KAT; the edge/corner order is restated from
+ *                    trans.c:154-251,1939-1996 (not independently pinned).
+ *   outside values   pinned by the uniform-state fixed point (T + outside
+ *                    term = periodic Laplacian) and translation equivariance.
+ *   orc_nonlinear    f' pinned pointwise (closed form); F2 pinned against an
+ *                    analytic Fourier mode away from the box edges.
+ *   orc_elastic      pinned by the k=0 KAT, B symmetry / evenness and a
+ *                    brute-force DFT convolution at N <= 6.
+ *   orc_fft3d        pinned by a brute-force DFT and round trip / Parseval.
+ *   orc_modeprod     pinned by a dense Kronecker product at n <= 8.
+ *   orc_step (ETD1)  pinned by dense expm(dt*L) at n = 6 with N == 0.
+ *   orc_step (ETD2)  pinned by order-2 convergence.
+ *   orc_diag         pinned by hand-computed 2^3 / 3^3 grids.
+ *   CH free energy   the reference's CH_f is broken (formula.c:40-56); the
+ *                    double well (c^2-1)^2/4 of field.c:575 / trans.c:993 is
+ *                    used, so CH parity is against this oracle only.
+ */
+#include "etd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX(n, k, j, i) ((((size_t)(k)) * (n) + (j)) * (n) + (i))
+
+struct orc_ctx {
+  orc_params p;
+  int V;                 /* nac + nch */
+  size_t n3;
+  double *eps0, *lam, *Vm, *Vim, *D;
+  double *DDX, *DDY, *DDZ;
+  double *field;         /* V * n3 */
+  double *N1, *N2;       /* V * n3: N(u_n), N(a) */
+  double *felas;         /* nac * n3 */
+  double *fbuf;          /* persistent elastic transfer buffer (elastic.c:228) */
+  double *phi1, *phi2;   /* (nac? 1:0 + nch? 1:0) tables, zxy layout */
+  double *t0, *t1, *t2, *t3;  /* n3 scratch */
+};
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Eigenbasis: vars.c:361-591 reads d/v/vi<id><n>.dat for the truncated
+ * second-difference matrix T of each boundary id (id = (k-1)+(l-1)*2 with
+ * k/l = 1 for a physical side, 2 for a neighbour side, vars.c:488-506).
+ * Angles are reduced as exact integers before sin/cos.
+ *   id 3 : Dirichlet truncation both sides, T = tridiag(1,-2,1):
+ *          lambda_m = -4 sin^2(m pi / (2(n+1))), v_i = sin(i m pi/(n+1)) (DST-I)
+ *   id 0 : reflecting both sides, rows [-2,2] ... [2,-2]:
+ *          lambda_m = -4 sin^2(m pi / (2(n-1))), v_i = cos(i m pi/(n-1)) (DCT-I)
+ *   id 1 : neighbour low side, reflecting high side (T[n-1][n-2] = 2):
+ *          lambda_m = -4 sin^2((2m+1) pi / (4n)), v_i = sin((i+1)(2m+1) pi/(2n))
+ *   id 2 : mirror image of id 1: v_i = cos(i (2m+1) pi / (2n))
+ * V columns are eigenvectors (A(r,c) at V[r + n*c]); V^-1 from Gauss-Jordan
+ * elimination with partial pivoting (orthonormal for id 3: V^-1 = V^T).
+ * Eigenvalue order: decreasing magnitude, as in the shipped d<id>260.dat. */
+static int invert(int n, const double *A, double *Ai) {
+  double *M = malloc(sizeof(double) * (size_t)n * 2 * n);
+  if (!M) return -1;
+  /* row-major augmented [A | I], A(r,c) = A[r + n c] */
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < 2 * n; c++)
+      M[(size_t)r * 2 * n + c] = c < n ? A[r + (size_t)n * c] : (c - n == r ? 1.0 : 0.0);
+  for (int col = 0; col < n; col++) {
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (fabs(M[(size_t)r * 2 * n + col]) > fabs(M[(size_t)piv * 2 * n + col])) piv = r;
+    if (M[(size_t)piv * 2 * n + col] == 0.0) { free(M); return -1; }
+    if (piv != col)
+      for (int c = 0; c < 2 * n; c++) {
+        double t = M[(size_t)col * 2 * n + c];
+        M[(size_t)col * 2 * n + c] = M[(size_t)piv * 2 * n + c];
+        M[(size_t)piv * 2 * n + c] = t;
+      }
+    double d = M[(size_t)col * 2 * n + col];
+    for (int c = 0; c < 2 * n; c++) M[(size_t)col * 2 * n + c] /= d;
+    for (int r = 0; r < n; r++) {
+      if (r == col) continue;
+      double f = M[(size_t)r * 2 * n + col];
+      if (f == 0.0) continue;
+      for (int c = 0; c < 2 * n; c++) M[(size_t)r * 2 * n + c] -= f * M[(size_t)col * 2 * n + c];
+    }
+  }
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) Ai[r + (size_t)n * c] = M[(size_t)r * 2 * n + n + c];
+  free(M);
+  return 0;
+}
+
+int orc_eigbasis(int n, int id, double *V, double *Vi, double *D) {
+  if (n < 3 || id < 0 || id > 3) return -1;
+  const double pi = 3.14159265358979323846;
+  for (int m = 0; m < n; m++) {
+    /* column c = m holds the mode of the m-th largest |lambda| */
+    double th, lam;
+    int mm;
+    switch (id) {
+      case 3:
+        mm = n - m;  /* modes 1..n, decreasing magnitude */
+        lam = -4.0 * pow(sin(mm * pi / (2.0 * (n + 1))), 2);
+        for (int i = 0; i < n; i++) V[i + (size_t)n * m] = sqrt(2.0 / (n + 1)) * sin((double)(((long)(i + 1) * mm) % (2L * (n + 1))) * pi / (n + 1));
+        break;
+      case 0:
+        mm = n - 1 - m;  /* modes 0..n-1 */
+        lam = -4.0 * pow(sin(mm * pi / (2.0 * (n - 1))), 2);
+        for (int i = 0; i < n; i++) V[i + (size_t)n * m] = cos((double)(((long)i * mm) % (2L * (n - 1))) * pi / (n - 1));
+        break;
+      case 1:
+        mm = n - 1 - m;
+        th = (2.0 * mm + 1.0) * pi / (2.0 * n);
+        lam = -4.0 * pow(sin(th / 2.0), 2);
+        for (int i = 0; i < n; i++) V[i + (size_t)n * m] = sin((double)(((long)(i + 1) * (2 * mm + 1)) % (4L * n)) * pi / (2.0 * n));
+        break;
+      default: /* 2 */
+        mm = n - 1 - m;
+        th = (2.0 * mm + 1.0) * pi / (2.0 * n);
+        lam = -4.0 * pow(sin(th / 2.0), 2);
+        for (int i = 0; i < n; i++) V[i + (size_t)n * m] = cos((double)(((long)i * (2 * mm + 1)) % (4L * n)) * pi / (2.0 * n));
+        break;
+    }
+    D[m] = lam;
+  }
+  if (id == 3) {
+    for (int r = 0; r < n; r++)
+      for (int c = 0; c < n; c++) Vi[r + (size_t)n * c] = V[c + (size_t)n * r];
+    return 0;
+  }
+  return invert(n, V, Vi);
+}
+
+/* ------------------------------------------------------------------ */
+/* Ghost fill of one field: the periodic self-exchange of transfer()
+ * (trans.c:1939-1996 with the packs 7-152 and unpack 154-251, mpi.c:71-82).
+ * Every ghost value is read from the pre-transfer state; z faces are written
+ * last (unpack front/back), then y faces, then x faces, so the cell (k,j,i)
+ * receives
+ *   k ghost          : u[wrap(k)][j][i]
+ *   else j ghost     : u[k][wrap(j)][i]
+ *   else i ghost     : u[k][j][wrap(i)]
+ * with wrap(e) = e+N (e<g), e-N (e>=n-g).  In place this is z faces first
+ * (they read y/x ghost cells of interior planes before those change), then
+ * y faces, then x faces. */
+static inline int wrap_e(int e, int n, int g) {
+  int N = n - 2 * g;
+  return e < g ? e + N : (e >= n - g ? e - N : e);
+}
+
+void orc_ghost_fill(int n, int g, double *u) {
+  for (int k = 0; k < n; k++) {
+    if (!(k < g || k >= n - g)) continue;
+    int ks = wrap_e(k, n, g);
+    memcpy(u + IDX(n, k, 0, 0), u + IDX(n, ks, 0, 0), sizeof(double) * (size_t)n * n);
+  }
+  for (int k = g; k < n - g; k++)
+    for (int j = 0; j < n; j++) {
+      if (!(j < g || j >= n - g)) continue;
+      int js = wrap_e(j, n, g);
+      memcpy(u + IDX(n, k, j, 0), u + IDX(n, k, js, 0), sizeof(double) * (size_t)n);
+    }
+  for (int k = g; k < n - g; k++)
+    for (int j = g; j < n - g; j++)
+      for (int i = 0; i < n; i++) {
+        if (!(i < g || i >= n - g)) continue;
+        u[IDX(n, k, j, i)] = u[IDX(n, k, j, wrap_e(i, n, g))];
+      }
+}
+
+void orc_transfer(orc_ctx *c) {
+  for (int v = 0; v < c->V; v++) orc_ghost_fill(c->p.n, c->p.g, c->field + c->n3 * v);
+}
+
+/* Periodic interior image P(z,y,x) for -2 <= z,y,x < n+2.  Every entry of the
+ * enlarged slabs fielde_* built by enlarge() (trans.c:253-925) from the
+ * exchanged slabs equals the pre-transfer value at the interior image of its
+ * position; the outside values fieldEu_* (ac_mu, trans.c:927-1432) and the
+ * CH chemical potential at the outside cells (ch_mu, trans.c:1433-1937) are
+ * read from these slabs.  Interior cells are untouched by the ghost fill, so
+ * P can be read from the post-fill field. */
+static inline double Pimg(const double *u, int n, int g, int z, int y, int x) {
+  return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+}
+
+/* ------------------------------------------------------------------ */
+/* Nonlinear term of an AC field: ac_calc_FU (calc.c:8-68) ->
+ * ac_calc_F1 (anisotropic.c:9-161) with SWTICH_AC_FIELD1 (ScLETD.h:108-120)
+ * and AC_f (formula.c:30-38) generalised to nac variants:
+ *   f1 = LE*KE*u_p - LE*(a2 u_p - a3 u_p^2 + a4 (sum_m u_m^2) u_p)
+ *   f1 += LE*epn2*u_out/h^2 on each of the 6 box faces (no kk factor)
+ *   N  = f1 - epn2*LE*F2 (ANISOTROPIC) - LE*ElasticScale*felas (ELASTIC=1)
+ * F2 (anisotropic.c:164-627): g = Lambda (gx, gy, gz) with central 2h
+ * differences where gy, gz use (u[-1] - u[+1]) (the reference's top = j-1,
+ * front = k-1 naming), then F2 = d/dx g_x + (g_y[-1]-g_y[+1])/2h + ...;
+ * neighbours outside the extended box read 0 (both sides have neighbours). */
+static void ac_f2(const orc_ctx *c, int pv, const double *u, double *gx_, double *gy_, double *gz_, double *f2) {
+  const int n = c->p.n;
+  const double hx = c->p.hx, hy = c->p.hy, hz = c->p.hz;
+  const double *L = c->lam + 9 * pv;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double a_left = i > 0 ? u[IDX(n, k, j, i - 1)] : 0.0;
+        double a_right = i < n - 1 ? u[IDX(n, k, j, i + 1)] : 0.0;
+        double a_top = j > 0 ? u[IDX(n, k, j - 1, i)] : 0.0;
+        double a_bottom = j < n - 1 ? u[IDX(n, k, j + 1, i)] : 0.0;
+        double a_front = k > 0 ? u[IDX(n, k - 1, j, i)] : 0.0;
+        double a_back = k < n - 1 ? u[IDX(n, k + 1, j, i)] : 0.0;
+        double gradx = (a_right - a_left) / (2.0 * hx);
+        double grady = (a_top - a_bottom) / (2.0 * hy);
+        double gradz = (a_front - a_back) / (2.0 * hz);
+        size_t l = IDX(n, k, j, i);
+        gx_[l] = L[0] * gradx + L[1] * grady + L[2] * gradz;
+        gy_[l] = L[3] * gradx + L[4] * grady + L[5] * gradz;
+        gz_[l] = L[6] * gradx + L[7] * grady + L[8] * gradz;
+      }
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double a_left = i > 0 ? gx_[IDX(n, k, j, i - 1)] : 0.0;
+        double a_right = i < n - 1 ? gx_[IDX(n, k, j, i + 1)] : 0.0;
+        double a_top = j > 0 ? gy_[IDX(n, k, j - 1, i)] : 0.0;
+        double a_bottom = j < n - 1 ? gy_[IDX(n, k, j + 1, i)] : 0.0;
+        double a_front = k > 0 ? gz_[IDX(n, k - 1, j, i)] : 0.0;
+        double a_back = k < n - 1 ? gz_[IDX(n, k + 1, j, i)] : 0.0;
+        double tmp = 0.0;
+        tmp += (a_right - a_left) / 2.0 / hx;
+        tmp += (a_top - a_bottom) / 2.0 / hy;
+        tmp += (a_front - a_back) / 2.0 / hz;
+        f2[IDX(n, k, j, i)] = tmp;
+      }
+}
+
+static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g, nac = p->nac;
+  const double LE = p->LE, KE = p->KE, epn2 = p->epn2;
+  const double hx = p->hx, hy = p->hy, hz = p->hz;
+  const double *u = c->field + c->n3 * pv;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        size_t l = IDX(n, k, j, i);
+        double s = 0.0;
+        for (int m = 0; m < nac; m++) {
+          double um = c->field[c->n3 * m + l];
+          s = s + um * um;
+        }
+        double up = u[l];
+        double fp = p->a2 * up - p->a3 * up * up + p->a4 * s * up;
+        double f = LE * KE * up;
+        f -= LE * fp;
+        out[l] = f;
+      }
+  /* boundary coupling, order left,right,top,bottom,front,back */
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] += LE * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, n - 1)] += LE * epn2 * Pimg(u, n, g, k, j, n) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) out[IDX(n, k, 0, i)] += LE * epn2 * Pimg(u, n, g, k, -1, i) / hy / hy;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) out[IDX(n, k, n - 1, i)] += LE * epn2 * Pimg(u, n, g, k, n, i) / hy / hy;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] += LE * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] += LE * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+  if (p->aniso) {
+    ac_f2(c, pv, u, c->t0, c->t1, c->t2, c->t3);
+  }
+  const double *felas = p->elastic == 1 ? c->felas + c->n3 * pv : NULL;
+#pragma omp parallel for schedule(static)
+  for (size_t l = 0; l < c->n3; l++) {
+    double f = out[l];
+    if (p->aniso) f -= epn2 * LE * c->t3[l];
+    if (felas) f -= LE * p->elastic_scale * felas[l];
+    out[l] = f;
+  }
+}
+
+/* CH nonlinear term: ch_calc_FU_ConstantMobility (calc.c:70-473) with the
+ * chemical potential at the outside cells from ch_mu (trans.c:1433-1937):
+ *   t   = -LCI*KCI*c + LCI*CH_f(c)          (SWITCH_CH_FIELD1, ScLETD.h:125)
+ *   t  -= LCI*epn2*c_out/h^2 on each face
+ *   out = Lap_T t  (out-of-box neighbours 0)
+ *   out -= mu_out/h^2 on each face, mu_out = LCI*(epn2*Lap c - CH_f(c)) at
+ *          the outside cell, evaluated on the periodic image.
+ * CH_f(c) = c (c^2 - 1), the derivative of (c^2-1)^2/4 (field.c:575). */
+static inline double ch_f(double cc) { return cc * (cc * cc - 1.0); }
+
+static double ch_mu_out(const orc_ctx *c, const double *u, int z, int y, int x) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g;
+  double v0 = Pimg(u, n, g, z, y, x);
+  double mu = (Pimg(u, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(u, n, g, z, y, x + 1)) / p->hx / p->hx;
+  mu += (Pimg(u, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(u, n, g, z, y + 1, x)) / p->hy / p->hy;
+  mu += (Pimg(u, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(u, n, g, z + 1, y, x)) / p->hz / p->hz;
+  mu = p->LCI * p->epn2 * mu;
+  mu -= p->LCI * ch_f(v0);
+  return mu;
+}
+
+static void ch_nonlinear(orc_ctx *c, int v, double *out) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g;
+  const double LCI = p->LCI, KCI = p->KCI, epn2 = p->epn2;
+  const double hx = p->hx, hy = p->hy, hz = p->hz;
+  const double *u = c->field + c->n3 * v;
+  double *t = c->t0;
+#pragma omp parallel for schedule(static)
+  for (size_t l = 0; l < c->n3; l++) {
+    double cc = u[l];
+    double f = -LCI * KCI * cc;
+    f += LCI * ch_f(cc);
+    t[l] = f;
+  }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) t[IDX(n, k, j, 0)] -= LCI * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) t[IDX(n, k, j, n - 1)] -= LCI * epn2 * Pimg(u, n, g, k, j, n) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) t[IDX(n, k, 0, i)] -= LCI * epn2 * Pimg(u, n, g, k, -1, i) / hy / hy;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) t[IDX(n, k, n - 1, i)] -= LCI * epn2 * Pimg(u, n, g, k, n, i) / hy / hy;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) t[IDX(n, 0, j, i)] -= LCI * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) t[IDX(n, n - 1, j, i)] -= LCI * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double tc = t[IDX(n, k, j, i)];
+        double f_left = i > 0 ? t[IDX(n, k, j, i - 1)] : 0.0;
+        double f_right = i < n - 1 ? t[IDX(n, k, j, i + 1)] : 0.0;
+        double f_top = j > 0 ? t[IDX(n, k, j - 1, i)] : 0.0;
+        double f_bottom = j < n - 1 ? t[IDX(n, k, j + 1, i)] : 0.0;
+        double f_front = k > 0 ? t[IDX(n, k - 1, j, i)] : 0.0;
+        double f_back = k < n - 1 ? t[IDX(n, k + 1, j, i)] : 0.0;
+        double r = (f_left + f_right - 2.0 * tc) / hx / hx;
+        r += (f_top + f_bottom - 2.0 * tc) / hy / hy;
+        r += (f_front + f_back - 2.0 * tc) / hz / hz;
+        out[IDX(n, k, j, i)] = r;
+      }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] -= ch_mu_out(c, u, k, j, -1) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, n - 1)] -= ch_mu_out(c, u, k, j, n) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) out[IDX(n, k, 0, i)] -= ch_mu_out(c, u, k, -1, i) / hy / hy;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) out[IDX(n, k, n - 1, i)] -= ch_mu_out(c, u, k, n, i) / hy / hy;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] -= ch_mu_out(c, u, -1, j, i) / hz / hz;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] -= ch_mu_out(c, u, n, j, i) / hz / hz;
+}
+
+void orc_nonlinear(orc_ctx *c, int v, double *out) {
+  if (v < c->p.nac) ac_nonlinear(c, v, out);
+  else ch_nonlinear(c, v, out);
+}
+
+/* ------------------------------------------------------------------ */
+/* Own mixed-radix complex FFT (the reference links fftMPI, fft3d.c:63-131,
+ * which is not vendored).  Recursive decimation in time over the prime
+ * factors of N with exact cos/sin twiddles.  Forward: exp(-2 pi i jk/N),
+ * unscaled; backward: exp(+...), scaled by 1/N^3 over the 3D transform
+ * (fftMPI default scaled = 1; fixed by the k = 0 KAT). */
+typedef struct { double re, im; } cplx;
+
+/* work: N scratch entries shared down the recursion (a parent only uses it
+ * after all its children have returned) */
+static void fft1d_rec(int N, const cplx *in, int stride, cplx *out, int sign, const int *fac, const cplx *tw, int twN, cplx *res) {
+  if (N == 1) { out[0] = in[0]; return; }
+  int r = fac[0], m = N / r;
+  /* sub-transforms of the r decimated sequences */
+  for (int q = 0; q < r; q++) fft1d_rec(m, in + (size_t)q * stride, stride * r, out + (size_t)q * m, sign, fac + 1, tw, twN, res);
+  for (int k = 0; k < m; k++)
+    for (int s = 0; s < r; s++) {
+      int kk = k + s * m; /* output index */
+      double sr = 0.0, si = 0.0;
+      for (int q = 0; q < r; q++) {
+        /* twiddle exp(sign*2 pi i q kk / N) */
+        size_t e = ((size_t)q * kk) % N;
+        size_t ti = e * (size_t)(twN / N);
+        double wr = tw[ti].re, wi = sign * tw[ti].im;
+        cplx x = out[(size_t)q * m + k];
+        sr += x.re * wr - x.im * wi;
+        si += x.re * wi + x.im * wr;
+      }
+      res[kk].re = sr;
+      res[kk].im = si;
+    }
+  memcpy(out, res, sizeof(cplx) * (size_t)N);
+}
+
+static int factorize(int N, int *fac) {
+  int nf = 0;
+  int x = N;
+  for (int pr = 2; pr * pr <= x; pr++)
+    while (x % pr == 0) { fac[nf++] = pr; x /= pr; }
+  if (x > 1) fac[nf++] = x;
+  return nf;
+}
+
+int orc_fft3d(int N, double *data, int dir) {
+  if (N < 1) return -1;
+  int fac[64];
+  factorize(N, fac);
+  cplx *tw = malloc(sizeof(cplx) * (size_t)N);
+  const double pi = 3.14159265358979323846;
+  for (int e = 0; e < N; e++) { /* exp(-2 pi i e / N) */
+    tw[e].re = cos(2.0 * pi * e / N);
+    tw[e].im = -sin(2.0 * pi * e / N);
+  }
+  int sign = dir > 0 ? 1 : -1;
+  cplx *d = (cplx *)data;
+  size_t NN = (size_t)N * N;
+  /* lines along x, y, z */
+  for (int axis = 0; axis < 3; axis++) {
+#pragma omp parallel
+    {
+      cplx *line = malloc(sizeof(cplx) * (size_t)N), *res = malloc(sizeof(cplx) * (size_t)N);
+      cplx *work = malloc(sizeof(cplx) * (size_t)N);
+#pragma omp for schedule(static)
+      for (size_t l = 0; l < NN; l++) {
+        size_t base, st;
+        if (axis == 0) { base = l * N; st = 1; }
+        else if (axis == 1) { base = (l / N) * NN + (l % N); st = N; }
+        else { base = l; st = NN; }
+        for (int t = 0; t < N; t++) line[t] = d[base + t * st];
+        fft1d_rec(N, line, 1, res, sign, fac, tw, N, work);
+        for (int t = 0; t < N; t++) d[base + t * st] = res[t];
+      }
+      free(line);
+      free(res);
+      free(work);
+    }
+  }
+  if (dir < 0) {
+    double s = 1.0 / ((double)N * N * N);
+    for (size_t l = 0; l < NN * N; l++) { d[l].re *= s; d[l].im *= s; }
+  }
+  free(tw);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Khachaturyan B_pq(n) (elastic.c:460-605, inverse_3x3 442-457, C_ijkl and
+ * sigma0 = C:eps0 from elastic_input 349-438):
+ *   B = C_ijkl e_p,ij e_q,kl - n_i s_p,ij Omega_jk s_q,kl n_l,
+ *   Omega = (C_ijkl n_j n_l)^-1, n from signed wavenumbers (g < N/2 ? g : g-N),
+ *   k = 0 uses n = (0,1,0). */
+static int voigt(int i, int j) {
+  if (i == j) return i;
+  if ((i == 1 && j == 2) || (i == 2 && j == 1)) return 3;
+  if ((i == 0 && j == 2) || (i == 2 && j == 0)) return 4;
+  return 5;
+}
+
+static void c4_from_voigt(const double *C2, double *C4) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) C4[((i * 3 + j) * 3 + k) * 3 + l] = C2[voigt(i, j) * 6 + voigt(k, l)];
+}
+
+static void inverse3(double *om, const double *oi) {
+  double nr = 0.0;
+  nr += oi[0] * (oi[4] * oi[8] - oi[5] * oi[7]);
+  nr -= oi[1] * (oi[3] * oi[8] - oi[5] * oi[6]);
+  nr += oi[2] * (oi[3] * oi[7] - oi[4] * oi[6]);
+  om[0] = (+1) * (oi[4] * oi[8] - oi[5] * oi[7]) / nr;
+  om[1] = (-1) * (oi[3] * oi[8] - oi[6] * oi[5]) / nr;
+  om[2] = (+1) * (oi[3] * oi[7] - oi[6] * oi[4]) / nr;
+  om[3] = (-1) * (oi[1] * oi[8] - oi[7] * oi[2]) / nr;
+  om[4] = (+1) * (oi[0] * oi[8] - oi[2] * oi[6]) / nr;
+  om[5] = (-1) * (oi[0] * oi[7] - oi[6] * oi[1]) / nr;
+  om[6] = (+1) * (oi[1] * oi[5] - oi[4] * oi[2]) / nr;
+  om[7] = (-1) * (oi[0] * oi[5] - oi[3] * oi[2]) / nr;
+  om[8] = (+1) * (oi[0] * oi[4] - oi[1] * oi[3]) / nr;
+}
+
+double orc_bpq(const orc_params *p, int N, int gx, int gy, int gz, int pp, int qq) {
+  double C4[81];
+  c4_from_voigt(p->C, C4);
+  const double *ep = p->eps0 + 9 * pp, *eq = p->eps0 + 9 * qq;
+  double sp[9], sq[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      sp[i * 3 + j] = 0;
+      sq[i * 3 + j] = 0;
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) {
+          sp[i * 3 + j] += C4[((i * 3 + j) * 3 + k) * 3 + l] * ep[k * 3 + l];
+          sq[i * 3 + j] += C4[((i * 3 + j) * 3 + k) * 3 + l] * eq[k * 3 + l];
+        }
+    }
+  int cnt = N / 2;
+  double n1 = gx < cnt ? 1.0 * gx : -1.0 * (N - gx);
+  double n2 = gy < cnt ? 1.0 * gy : -1.0 * (N - gy);
+  double n3 = gz < cnt ? 1.0 * gz : -1.0 * (N - gz);
+  double nv[3];
+  if (gx == 0 && gy == 0 && gz == 0) {
+    nv[0] = 0; nv[1] = 1; nv[2] = 0;
+  } else {
+    double r = pow((pow(n1, 2) + pow(n2, 2) + pow(n3, 2)), 0.5);
+    nv[0] = n1 / r; nv[1] = n2 / r; nv[2] = n3 / r;
+  }
+  double oi[9], om[9];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) {
+      oi[i * 3 + k] = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int l = 0; l < 3; ++l) oi[i * 3 + k] += C4[((i * 3 + j) * 3 + k) * 3 + l] * nv[j] * nv[l];
+    }
+  inverse3(om, oi);
+  double t0 = 0, t1 = 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) t0 += C4[((i * 3 + j) * 3 + k) * 3 + l] * ep[i * 3 + j] * eq[k * 3 + l];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) t1 += nv[i] * sp[i * 3 + j] * om[j * 3 + k] * sq[k * 3 + l] * nv[l];
+  return t0 - t1;
+}
+
+/* elastic_calculate (elastic.c:689-762) -> elastic_calculate_ElasDri
+ * (617-687): felas_p = Re IFFT[sum_q B_pq FFT(eta_q)] on the interior N^3,
+ * written into the interior of the persistent buffer f, then f takes the
+ * same self-exchange as the fields (its ghost cells start from the previous
+ * variant's values, as the single buffer f does), then felas_p = f.
+ * The reference runs nac^2 forward/backward pairs; by linearity nac forward
+ * transforms and one backward per p give the same values to roundoff. */
+/* B_pq for all q at one wavevector, factored as
+ *   tmp0_pq - (sigma_p^T n) . Omega . (sigma_q n)
+ * (the same sums as orc_bpq regrouped; equal to roundoff). */
+typedef struct {
+  double C4[81];
+  double sig[9 * 64];
+  double t0[64 * 64];
+} bfast;
+
+static void bfast_init(const orc_params *p, bfast *b) {
+  c4_from_voigt(p->C, b->C4);
+  for (int q = 0; q < p->nac; q++) {
+    const double *e = p->eps0 + 9 * q;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++) s += b->C4[((i * 3 + j) * 3 + k) * 3 + l] * e[k * 3 + l];
+        b->sig[9 * q + i * 3 + j] = s;
+      }
+  }
+  for (int pp = 0; pp < p->nac; pp++)
+    for (int qq = 0; qq < p->nac; qq++) {
+      double t0 = 0.0;
+      const double *ep = p->eps0 + 9 * pp, *eq = p->eps0 + 9 * qq;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) t0 += b->C4[((i * 3 + j) * 3 + k) * 3 + l] * ep[i * 3 + j] * eq[k * 3 + l];
+      b->t0[pp * 64 + qq] = t0;
+    }
+}
+
+static void bfast_row(const orc_params *p, const bfast *b, int N, int gx, int gy, int gz, int pp, double *row) {
+  int cnt = N / 2;
+  double n1 = gx < cnt ? 1.0 * gx : -1.0 * (N - gx);
+  double n2 = gy < cnt ? 1.0 * gy : -1.0 * (N - gy);
+  double n3 = gz < cnt ? 1.0 * gz : -1.0 * (N - gz);
+  double nv[3];
+  if (gx == 0 && gy == 0 && gz == 0) {
+    nv[0] = 0; nv[1] = 1; nv[2] = 0;
+  } else {
+    double r = pow((pow(n1, 2) + pow(n2, 2) + pow(n3, 2)), 0.5);
+    nv[0] = n1 / r; nv[1] = n2 / r; nv[2] = n3 / r;
+  }
+  double oi[9], om[9];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) {
+      oi[i * 3 + k] = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int l = 0; l < 3; ++l) oi[i * 3 + k] += b->C4[((i * 3 + j) * 3 + k) * 3 + l] * nv[j] * nv[l];
+    }
+  inverse3(om, oi);
+  const double *sp = b->sig + 9 * pp;
+  double a[3], w[3];
+  for (int j = 0; j < 3; j++) a[j] = nv[0] * sp[0 * 3 + j] + nv[1] * sp[1 * 3 + j] + nv[2] * sp[2 * 3 + j];
+  for (int k = 0; k < 3; k++) w[k] = a[0] * om[0 * 3 + k] + a[1] * om[1 * 3 + k] + a[2] * om[2 * 3 + k];
+  for (int qq = 0; qq < p->nac; qq++) {
+    const double *sq = b->sig + 9 * qq;
+    double t1 = 0.0;
+    for (int k = 0; k < 3; k++) t1 += w[k] * (sq[k * 3 + 0] * nv[0] + sq[k * 3 + 1] * nv[1] + sq[k * 3 + 2] * nv[2]);
+    row[qq] = b->t0[pp * 64 + qq] - t1;
+  }
+}
+
+void orc_elastic(orc_ctx *c) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g, N = n - 2 * g, nac = p->nac;
+  if (p->elastic != 1 || nac == 0 || nac > 64) return;
+  size_t N3 = (size_t)N * N * N;
+  double *spec = malloc(sizeof(double) * 2 * N3 * nac);
+  double *acc = malloc(sizeof(double) * 2 * N3);
+  bfast *bf = malloc(sizeof(bfast));
+  bfast_init(p, bf);
+  for (int q = 0; q < nac; q++) {
+    double *s = spec + 2 * N3 * q;
+    const double *u = c->field + c->n3 * q;
+    for (int z = 0; z < N; z++)
+      for (int y = 0; y < N; y++)
+        for (int x = 0; x < N; x++) {
+          size_t m = ((size_t)z * N + y) * N + x;
+          s[2 * m] = u[IDX(n, z + g, y + g, x + g)];
+          s[2 * m + 1] = 0.0;
+        }
+    orc_fft3d(N, s, +1);
+  }
+  for (int pp = 0; pp < nac; pp++) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; z++) {
+      double row[64];
+      for (int y = 0; y < N; y++)
+        for (int x = 0; x < N; x++) {
+          size_t m = ((size_t)z * N + y) * N + x;
+          bfast_row(p, bf, N, x, y, z, pp, row);
+          double re = 0.0, im = 0.0;
+          for (int qq = 0; qq < nac; qq++) {
+            re += spec[2 * N3 * qq + 2 * m] * row[qq];
+            im += spec[2 * N3 * qq + 2 * m + 1] * row[qq];
+          }
+          acc[2 * m] = re;
+          acc[2 * m + 1] = im;
+        }
+    }
+    orc_fft3d(N, acc, -1);
+    double *f = c->fbuf;
+    for (int z = 0; z < N; z++)
+      for (int y = 0; y < N; y++)
+        for (int x = 0; x < N; x++) f[IDX(n, z + g, y + g, x + g)] = acc[2 * (((size_t)z * N + y) * N + x)];
+    orc_ghost_fill(n, g, f);
+    memcpy(c->felas + c->n3 * pp, f, sizeof(double) * c->n3);
+  }
+  free(spec);
+  free(acc);
+  free(bf);
+}
+
+/* ------------------------------------------------------------------ */
+/* Mode product: PUX/PUY/PUZ (calc.c:631-684) = cblas_dgemm(ColMajor, NN,
+ * m=n, n=n^2, k=n) followed by the cyclic rotation of xyz_yzx/yzx_zxy/
+ * zxy_xyz (calc.c:585-629); flat (a,b,c) -> (b,c,a).  rotate = 0 keeps the
+ * GEMM layout (PUZ stage 0/1). */
+void orc_modeprod(int n, const double *A, const double *in, double *out, int rotate) {
+  const size_t n2 = (size_t)n * n;
+  double *At = malloc(sizeof(double) * n2); /* At[m][a] = A(m,a) */
+  for (int m = 0; m < n; m++)
+    for (int a = 0; a < n; a++) At[(size_t)m * n + a] = A[m + (size_t)n * a];
+  const int CB = 64;
+  size_t nblk = (n2 + CB - 1) / CB;
+#pragma omp parallel
+  {
+    double *Bt = calloc((size_t)n * CB, sizeof(double));
+    double *Cb = malloc(sizeof(double) * 4 * CB);
+#pragma omp for schedule(static)
+    for (size_t b = 0; b < nblk; b++) {
+      size_t c0 = b * CB;
+      int cw = (int)((c0 + CB <= n2) ? CB : n2 - c0);
+      for (int cc = 0; cc < cw; cc++)
+        for (int a = 0; a < n; a++) Bt[(size_t)a * CB + cc] = in[a + (size_t)n * (c0 + cc)];
+      for (int m0 = 0; m0 < n; m0 += 4) {
+        int mw = n - m0 < 4 ? n - m0 : 4;
+        for (int t = 0; t < 4 * CB; t++) Cb[t] = 0.0;
+        for (int a = 0; a < n; a++) {
+          const double *br = Bt + (size_t)a * CB;
+          for (int r = 0; r < mw; r++) {
+            double av = At[(size_t)(m0 + r) * n + a];
+            double *cr = Cb + r * CB;
+            for (int cc = 0; cc < CB; cc++) cr[cc] += av * br[cc];
+          }
+        }
+        for (int r = 0; r < mw; r++) {
+          int m = m0 + r;
+          if (rotate)
+            for (int cc = 0; cc < cw; cc++) out[(c0 + cc) + n2 * m] = Cb[r * CB + cc];
+          else
+            for (int cc = 0; cc < cw; cc++) out[m + (size_t)n * (c0 + cc)] = Cb[r * CB + cc];
+        }
+      }
+    }
+    free(Bt);
+    free(Cb);
+  }
+  free(At);
+}
+
+/* flat (a,b,c) -> (b,c,a): zxy_xyz (calc.c:615-629) */
+static void rotate3(int n, const double *f, double *ft) {
+  const size_t n2 = (size_t)n * n;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < n; c++)
+    for (int b = 0; b < n; b++)
+      for (int a = 0; a < n; a++) ft[b + (size_t)n * c + n2 * a] = f[a + (size_t)n * b + n2 * c];
+}
+
+/* ------------------------------------------------------------------ */
+/* phi tables: ac_init_phi / ch_init_phi (vars.c:302-359), zxy layout
+ * l = j n^2 + i n + k:
+ *   AC: H = -LE (epn2 (kkz DDZ_k + kky DDY_j + kkx DDX_i) - KE)
+ *   CH: H = LCI (epn2 lam^2 - KCI lam), lam = DDZ_k + DDY_j + DDX_i
+ *   |H| > 1e-8: phi1 = (1 - exp(-dt H))/H, phi2 = (1 - phi1/dt)/H
+ *   else phi1 = dt, phi2 = dt/2 */
+static double h_ac(const orc_ctx *c, int i, int j, int k) {
+  const orc_params *p = &c->p;
+  double tmp = p->kk[2] * c->DDZ[k] + p->kk[1] * c->DDY[j] + p->kk[0] * c->DDX[i];
+  return -p->LE * (tmp * p->epn2 - p->KE);
+}
+static double h_ch(const orc_ctx *c, int i, int j, int k) {
+  const orc_params *p = &c->p;
+  double tmp = c->DDZ[k] + c->DDY[j] + c->DDX[i];
+  return p->LCI * (tmp * tmp * p->epn2 - p->KCI * tmp);
+}
+
+static void init_phi(orc_ctx *c) {
+  const int n = c->p.n;
+  const double dt = c->p.dt;
+  for (int t = 0; t < 2; t++) {
+    double *phi = c->phi1 + c->n3 * t, *phi2 = c->phi2 + c->n3 * t;
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) {
+          size_t l = (size_t)j * n * n + (size_t)i * n + k;
+          double H = t == 0 ? h_ac(c, i, j, k) : h_ch(c, i, j, k);
+          if (fabs(H) > 1.0e-8) {
+            double e = exp(-dt * H);
+            phi[l] = (1.0 - e) / H;
+            phi2[l] = (1.0 - phi[l] / dt) / H;
+          } else {
+            phi[l] = dt;
+            phi2[l] = dt / 2;
+          }
+        }
+  }
+}
+
+/* ac_updateU_new / ch_updateU_new (calc.c:489-535):
+ *   uhat <- (1 - phi1 H') uhat + phi1 nhat, H' = 0 where |H| < 1e-8 */
+void orc_phi_update(orc_ctx *c, int v, double *uhat, const double *nhat) {
+  const int n = c->p.n;
+  const int isch = v >= c->p.nac;
+  const double *phi = c->phi1 + c->n3 * isch;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < n; k++) {
+        size_t l = (size_t)j * n * n + (size_t)i * n + k;
+        double H = isch ? h_ch(c, i, j, k) : h_ac(c, i, j, k);
+        if (fabs(H) < 1.0e-8) H = 0.0;
+        double tmp = 1.0 - phi[l] * H;
+        uhat[l] = tmp * uhat[l] + phi[l] * nhat[l];
+      }
+}
+
+/* ------------------------------------------------------------------ */
+orc_ctx *orc_create(const orc_params *p) {
+  if (!p || p->n < 2 * p->g + 2 || p->g < 1 || p->nac < 0 || p->nch < 0 || p->nac + p->nch < 1) return NULL;
+  orc_ctx *c = calloc(1, sizeof(orc_ctx));
+  c->p = *p;
+  c->V = p->nac + p->nch;
+  const int n = p->n;
+  c->n3 = (size_t)n * n * n;
+  size_t nn = (size_t)n * n;
+  c->Vm = malloc(sizeof(double) * nn);
+  c->Vim = malloc(sizeof(double) * nn);
+  c->D = malloc(sizeof(double) * n);
+  memcpy(c->Vm, p->V, sizeof(double) * nn);
+  memcpy(c->Vim, p->Vi, sizeof(double) * nn);
+  memcpy(c->D, p->D, sizeof(double) * n);
+  c->eps0 = calloc((size_t)9 * (p->nac > 0 ? p->nac : 1), sizeof(double));
+  c->lam = calloc((size_t)9 * (p->nac > 0 ? p->nac : 1), sizeof(double));
+  if (p->eps0) memcpy(c->eps0, p->eps0, sizeof(double) * 9 * p->nac);
+  if (p->lam) memcpy(c->lam, p->lam, sizeof(double) * 9 * p->nac);
+  c->p.eps0 = c->eps0;
+  c->p.lam = c->lam;
+  c->p.V = c->Vm;
+  c->p.Vi = c->Vim;
+  c->p.D = c->D;
+  c->DDX = malloc(sizeof(double) * n);
+  c->DDY = malloc(sizeof(double) * n);
+  c->DDZ = malloc(sizeof(double) * n);
+  for (int l = 0; l < n; l++) {
+    c->DDX[l] = c->D[l] / p->hx / p->hx;
+    c->DDY[l] = c->D[l] / p->hy / p->hy;
+    c->DDZ[l] = c->D[l] / p->hz / p->hz;
+  }
+  c->field = calloc(c->n3 * c->V, sizeof(double));
+  c->N1 = calloc(c->n3 * c->V, sizeof(double));
+  c->N2 = calloc(c->n3 * c->V, sizeof(double));
+  c->felas = calloc(c->n3 * (p->nac > 0 ? p->nac : 1), sizeof(double));
+  c->fbuf = calloc(c->n3, sizeof(double));
+  c->phi1 = calloc(c->n3 * 2, sizeof(double));
+  c->phi2 = calloc(c->n3 * 2, sizeof(double));
+  c->t0 = calloc(c->n3, sizeof(double));
+  c->t1 = calloc(c->n3, sizeof(double));
+  c->t2 = calloc(c->n3, sizeof(double));
+  c->t3 = calloc(c->n3, sizeof(double));
+  init_phi(c);
+  return c;
+}
+
+void orc_destroy(orc_ctx *c) {
+  if (!c) return;
+  free(c->Vm); free(c->Vim); free(c->D); free(c->eps0); free(c->lam);
+  free(c->DDX); free(c->DDY); free(c->DDZ);
+  free(c->field); free(c->N1); free(c->N2); free(c->felas); free(c->fbuf);
+  free(c->phi1); free(c->phi2); free(c->t0); free(c->t1); free(c->t2); free(c->t3);
+  free(c);
+}
+
+double *orc_field(orc_ctx *c, int v) { return c->field + c->n3 * v; }
+double *orc_felas(orc_ctx *c, int p) { return c->felas + c->n3 * p; }
+
+/* S^-1 along x, y, z: PUX, PUY, PUZ(stage 0/1) -> zxy spectral layout */
+static void fwd3(orc_ctx *c, const double *in, double *out, double *w1, double *w2) {
+  const int n = c->p.n;
+  orc_modeprod(n, c->Vim, in, w1, 1);
+  orc_modeprod(n, c->Vim, w1, w2, 1);
+  orc_modeprod(n, c->Vim, w2, out, 0);
+}
+/* zxy_xyz then S along x, y, z (stage 2), result in xyz */
+static void bwd3(orc_ctx *c, const double *spec_zxy, double *out, double *w1, double *w2) {
+  const int n = c->p.n;
+  rotate3(n, spec_zxy, w1);
+  orc_modeprod(n, c->Vm, w1, w2, 1);
+  orc_modeprod(n, c->Vm, w2, w1, 1);
+  orc_modeprod(n, c->Vm, w1, out, 1);
+}
+
+/* One time step: ScLETD.c:86-189 (check_soln_new is separate: orc_diag). */
+void orc_step(orc_ctx *c) {
+  const orc_params *p = &c->p;
+  const size_t n3 = c->n3;
+  double *uh = malloc(sizeof(double) * n3), *nh = malloc(sizeof(double) * n3);
+  double *w1 = malloc(sizeof(double) * n3), *w2 = malloc(sizeof(double) * n3);
+  orc_transfer(c);
+  if (p->elastic == 1) orc_elastic(c);
+  for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N1 + n3 * v);
+  for (int v = 0; v < c->V; v++) {
+    double *u = c->field + n3 * v;
+    fwd3(c, u, uh, w1, w2);
+    fwd3(c, c->N1 + n3 * v, nh, w1, w2);
+    orc_phi_update(c, v, uh, nh);
+    bwd3(c, uh, u, w1, w2);
+  }
+  if (p->etd2) {
+    orc_transfer(c);
+    if (p->elastic == 1) orc_elastic(c);
+    for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N2 + n3 * v);
+    for (int v = 0; v < c->V; v++) {
+      double *dn = c->N2 + n3 * v;
+      const double *n1 = c->N1 + n3 * v;
+      for (size_t l = 0; l < n3; l++) dn[l] = dn[l] - n1[l];       /* prepare_U1_new */
+      fwd3(c, dn, nh, w1, w2);
+      const double *ph2 = c->phi2 + n3 * (v >= p->nac);
+      for (size_t l = 0; l < n3; l++) nh[l] = ph2[l] * nh[l];      /* prepare_U2_new */
+      bwd3(c, nh, uh, w1, w2);
+      double *u = c->field + n3 * v;
+      for (size_t l = 0; l < n3; l++) u[l] += uh[l];               /* correct_U_new */
+    }
+  }
+  free(uh); free(nh); free(w1); free(w2);
+}
+
+/* ------------------------------------------------------------------ */
+/* check_soln_new (field.c:407-678).  Per AC field: max, min, vol over the
+ * interior [g, n-g)^3; eng over cells [g, n-g-1)^3 with the 8-corner mean
+ * and 4-edge-averaged gradient, summed over all AC fields m:
+ *   epn2/2 grad.(Lambda^T grad) + a2/2 sum u^2 - a3/3 sum u^3 + a4/4 (sum u^2)^2
+ * (the reference hard-codes 4 variants and a2,a3,a4 = 2376/2372, ...).
+ * The same eng is reported for every AC field.  CH: epn2/2 |grad c|^2 +
+ * (c^2-1)^2/4.  Couple (nac>0 and nch>0): continues the last CH accumulator
+ * (not reset, field.c:597-644) with gradient terms of AC fields 0..nch-1 and
+ * CH fields, then scales by h^3 again.  Partial sums per plane make the
+ * result independent of the thread count. */
+static void cell_grad(const double *u, int n, int k, int j, int i, double hx, double hy, double hz, double *m, double *ux, double *uy, double *uz) {
+  double f[8];
+  f[0] = u[IDX(n, k, j, i)];
+  f[1] = u[IDX(n, k, j, i + 1)];
+  f[2] = u[IDX(n, k, j + 1, i)];
+  f[3] = u[IDX(n, k, j + 1, i + 1)];
+  f[4] = u[IDX(n, k + 1, j, i)];
+  f[5] = u[IDX(n, k + 1, j, i + 1)];
+  f[6] = u[IDX(n, k + 1, j + 1, i)];
+  f[7] = u[IDX(n, k + 1, j + 1, i + 1)];
+  *m = (f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7]) / 8;
+  *ux = ((f[1] - f[0]) + (f[3] - f[2]) + (f[5] - f[4]) + (f[7] - f[6])) / 4 / hx;
+  *uy = ((f[2] - f[0]) + (f[3] - f[1]) + (f[6] - f[4]) + (f[7] - f[5])) / 4 / hy;
+  *uz = ((f[4] - f[0]) + (f[5] - f[1]) + (f[6] - f[2]) + (f[7] - f[3])) / 4 / hz;
+}
+
+void orc_diag(orc_ctx *c, double *stats) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g, nac = p->nac, nch = p->nch;
+  const double hx = p->hx, hy = p->hy, hz = p->hz, epn2 = p->epn2;
+  const int K = n - 2 * g;
+  double *plane = calloc((size_t)n, sizeof(double));
+  for (int v = 0; v < c->V; v++) {
+    const double *u = c->field + c->n3 * v;
+    double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
+    for (int k = g; k < n - g; k++)
+      for (int j = g; j < n - g; j++)
+        for (int i = g; i < n - g; i++) {
+          double t = u[IDX(n, k, j, i)];
+          if (t > mx) mx = t;
+          if (t < mn) mn = t;
+        }
+#pragma omp parallel for schedule(static)
+    for (int k = g; k < n - g; k++) {
+      double s = 0.0;
+      for (int j = g; j < n - g; j++)
+        for (int i = g; i < n - g; i++) s += u[IDX(n, k, j, i)] + 1.0;
+      plane[k] = s;
+    }
+    for (int k = g; k < n - g; k++) vol += plane[k];
+    vol *= hx * hy * hz;
+    stats[4 * v + 0] = mx;
+    stats[4 * v + 1] = mn;
+    stats[4 * v + 2] = vol;
+  }
+  (void)K;
+  /* AC energy (common to all AC fields) */
+  if (nac > 0) {
+#pragma omp parallel for schedule(static)
+    for (int k = g; k < n - g - 1; k++) {
+      double e = 0.0;
+      double *um = malloc(sizeof(double) * nac);
+      for (int j = g; j < n - g - 1; j++)
+        for (int i = g; i < n - g - 1; i++) {
+          for (int m = 0; m < nac; m++) {
+            double ux, uy, uz;
+            cell_grad(c->field + c->n3 * m, n, k, j, i, hx, hy, hz, &um[m], &ux, &uy, &uz);
+            const double *L = c->lam + 9 * m;
+            double ffx = ux * L[0] + uy * L[3] + uz * L[6];
+            double ffy = ux * L[1] + uy * L[4] + uz * L[7];
+            double ffz = ux * L[2] + uy * L[5] + uz * L[8];
+            e += epn2 * (ffx * ux + ffy * uy + ffz * uz) / 2;
+          }
+          double s2 = 0.0, s3 = 0.0;
+          for (int m = 0; m < nac; m++) { s2 += um[m] * um[m]; s3 += um[m] * um[m] * um[m]; }
+          e += (p->a2 / 2.0) * s2 - (p->a3 / 3.0) * s3 + (p->a4 / 4.0) * s2 * s2;
+        }
+      free(um);
+      plane[k] = e;
+    }
+    double eng = 0.0;
+    for (int k = g; k < n - g - 1; k++) eng += plane[k];
+    eng = eng * hx * hy * hz;
+    for (int v = 0; v < nac; v++) stats[4 * v + 3] = eng;
+  }
+  double engtmp = 0.0;
+  for (int q = 0; q < nch; q++) {
+    const double *u = c->field + c->n3 * (nac + q);
+#pragma omp parallel for schedule(static)
+    for (int k = g; k < n - g - 1; k++) {
+      double e = 0.0;
+      for (int j = g; j < n - g - 1; j++)
+        for (int i = g; i < n - g - 1; i++) {
+          double cm, ux, uy, uz;
+          cell_grad(u, n, k, j, i, hx, hy, hz, &cm, &ux, &uy, &uz);
+          e += epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+          e += ((cm * cm - 1) * (cm * cm - 1) / 4);
+        }
+      plane[k] = e;
+    }
+    engtmp = 0.0;
+    for (int k = g; k < n - g - 1; k++) engtmp += plane[k];
+    engtmp = engtmp * hx * hy * hz;
+    stats[4 * (nac + q) + 3] = engtmp;
+  }
+  double couple = 0.0;
+  if (nac > 0 && nch > 0) {
+#pragma omp parallel for schedule(static)
+    for (int k = g; k < n - g - 1; k++) {
+      double e = 0.0;
+      for (int j = g; j < n - g - 1; j++)
+        for (int i = g; i < n - g - 1; i++) {
+          for (int q = 0; q < nch && q < nac; q++) {
+            double cm, ux, uy, uz;
+            cell_grad(c->field + c->n3 * q, n, k, j, i, hx, hy, hz, &cm, &ux, &uy, &uz);
+            e += epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+          }
+          for (int q = 0; q < nch; q++) {
+            double cm, ux, uy, uz;
+            cell_grad(c->field + c->n3 * (nac + q), n, k, j, i, hx, hy, hz, &cm, &ux, &uy, &uz);
+            e += epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+          }
+        }
+      plane[k] = e;
+    }
+    couple = engtmp;
+    for (int k = g; k < n - g - 1; k++) couple += plane[k];
+    couple = couple * hx * hy * hz;
+  }
+  stats[4 * c->V] = couple;
+  free(plane);
+}
