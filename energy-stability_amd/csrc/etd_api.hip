@@ -1,0 +1,530 @@
+// etd_api.hip -- the C ABI of libetd: context lifetime, host<->device copies,
+// the step orchestration and the per-kernel / timing entry points.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "etd_internal.h"
+
+#define HCHK(c, call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      if (c) snprintf((c)->err, sizeof((c)->err), "%s: %s", #call, hipGetErrorString(e_)); \
+      return ETD_ERR_DEVICE;                                                           \
+    }                                                                                  \
+  } while (0)
+
+static int seterr(etd_ctx *c, int code, const char *fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+extern "C" const char *etd_error_string(int code) {
+  switch (code) {
+    case ETD_OK: return "ok";
+    case ETD_ERR_ARG: return "invalid argument";
+    case ETD_ERR_IO: return "input file missing or malformed";
+    case ETD_ERR_UNSUPPORTED: return "unsupported configuration";
+    case ETD_ERR_DEVICE: return "HIP device error";
+    case ETD_ERR_NOMEM: return "out of memory";
+    default: return "unknown error";
+  }
+}
+
+extern "C" const char *etd_last_error(const etd_ctx *c) { return c ? c->err : "no context"; }
+
+// ------------------------------------------------------------------ profiling
+void etd_prof_begin(etd_ctx *c, int cls) {
+  if (!c->prof || c->nev + 2 > c->cap_ev) return;
+  (void)hipEventRecord(c->ev[c->nev], c->stream);
+  c->ev_class[c->nev] = cls;
+  c->nev++;
+}
+void etd_prof_end(etd_ctx *c, int cls) {
+  if (!c->prof || c->nev + 1 > c->cap_ev) return;
+  (void)hipEventRecord(c->ev[c->nev], c->stream);
+  c->ev_class[c->nev] = -1 - cls;
+  c->nev++;
+}
+
+// ------------------------------------------------------------------ helpers
+static int fft_plan_radices(int N, int *rad) {
+  int np = 0, x = N;
+  while (x % 4 == 0) { rad[np++] = 4; x /= 4; }
+  while (x % 2 == 0) { rad[np++] = 2; x /= 2; }
+  while (x % 3 == 0) { rad[np++] = 3; x /= 3; }
+  while (x % 5 == 0) { rad[np++] = 5; x /= 5; }
+  if (x != 1 || np > 16) return -1;
+  return np;
+}
+
+static void c4_from_voigt(const double *C2, double *C4) {
+  auto vg = [](int i, int j) {
+    if (i == j) return i;
+    if ((i == 1 && j == 2) || (i == 2 && j == 1)) return 3;
+    if ((i == 0 && j == 2) || (i == 2 && j == 0)) return 4;
+    return 5;
+  };
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) C4[((i * 3 + j) * 3 + k) * 3 + l] = C2[vg(i, j) * 6 + vg(k, l)];
+}
+
+static int validate(etd_ctx *c, const etd_params *p) {
+  if (p->nx != p->ny || p->nx != p->nz)
+    return seterr(c, ETD_ERR_UNSUPPORTED, "nx, ny, nz must be equal (the axis rotation of calc.c:585-629 needs a cube)");
+  if (p->px != 1 || p->py != 1 || p->pz != 1)
+    return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz = %d,%d,%d: one subdomain per GPU (use 1,1,1)", p->px, p->py, p->pz);
+  if (!p->periodic) return seterr(c, ETD_ERR_UNSUPPORTED, "periodic = 0 is not supported");
+  if (p->variable_mobility) return seterr(c, ETD_ERR_UNSUPPORTED, "VariableMobility = 1 is not supported");
+  if (p->elastic != 0 && p->elastic != 1) return seterr(c, ETD_ERR_UNSUPPORTED, "ELASTIC = %d is not supported", p->elastic);
+  if (p->nac < 0 || p->nch < 0 || p->nac + p->nch < 1) return seterr(c, ETD_ERR_ARG, "nac = 0 and nch = 0");
+  if (p->nac + p->nch > ETD_MAX_VAR) return seterr(c, ETD_ERR_UNSUPPORTED, "too many fields");
+  if (p->nghost < 1 || p->nx < 2 * p->nghost + 2) return seterr(c, ETD_ERR_ARG, "nghost / nx");
+  if (p->nx > 16 * 17) return seterr(c, ETD_ERR_UNSUPPORTED, "n > 272");
+  if (p->elastic == 1 && p->nac > 16) return seterr(c, ETD_ERR_UNSUPPORTED, "elasticity supports nac <= 16");
+  if (p->precision != ETD_PREC_FP64) return seterr(c, ETD_ERR_UNSUPPORTED, "only the FP64 path is built");
+  return ETD_OK;
+}
+
+extern "C" int etd_create(const etd_params *p, const double *Vh, const double *Vih, const double *Dh, etd_ctx **out) {
+  if (!p || !out) return ETD_ERR_ARG;
+  *out = nullptr;
+  etd_ctx *c = (etd_ctx *)calloc(1, sizeof(etd_ctx));
+  if (!c) return ETD_ERR_NOMEM;
+  int rc = validate(c, p);
+  if (rc) {
+    fprintf(stderr, "etd_create: %s\n", c->err);
+    free(c);
+    return rc;
+  }
+  c->p = *p;
+  c->n = p->nx;
+  c->g = p->nghost;
+  c->N = c->n - 2 * c->g;
+  c->nac = p->nac;
+  c->nch = p->nch;
+  c->V = p->nac + p->nch;
+  const int n = c->n, V = c->V;
+  c->n3 = (size_t)n * n * n;
+  c->N3 = (size_t)c->N * c->N * c->N;
+  *out = c;  // from here on etd_close releases partial state
+  c->hV = (double *)malloc(sizeof(double) * n * n);
+  c->hVi = (double *)malloc(sizeof(double) * n * n);
+  c->hD = (double *)malloc(sizeof(double) * n);
+  if (Vh && Vih && Dh) {
+    memcpy(c->hV, Vh, sizeof(double) * n * n);
+    memcpy(c->hVi, Vih, sizeof(double) * n * n);
+    memcpy(c->hD, Dh, sizeof(double) * n);
+  } else if ((rc = etd_eigbasis(n, 3, c->hV, c->hVi, c->hD)) != ETD_OK) {
+    return seterr(c, rc, "eigenbasis");
+  }
+  int dev = 0;
+  const char *ev = getenv("ETD_DEVICE");
+  if (ev) dev = atoi(ev);
+  HCHK(c, hipSetDevice(dev));
+  HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t n3 = c->n3;
+  HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
+  c->N1 = c->u + n3 * V;
+  HCHK(c, hipMemsetAsync(c->u, 0, sizeof(double) * n3 * 2 * V, c->stream));
+  if (p->etd2) HCHK(c, hipMalloc(&c->N2, sizeof(double) * n3 * V));
+  HCHK(c, hipMalloc(&c->w1, sizeof(double) * n3 * 2 * V));
+  HCHK(c, hipMalloc(&c->w2, sizeof(double) * n3 * 2 * V));
+  HCHK(c, hipMalloc(&c->spec, sizeof(double) * n3 * 2 * V));
+  HCHK(c, hipMalloc(&c->gtmp, sizeof(double) * n3 * 3));
+  // eigenbasis as MFMA A fragments
+  const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
+  const size_t a4n = arrange_A_chunk(MT) * nkc;
+  double *h4 = (double *)malloc(sizeof(double) * a4n);
+  HCHK(c, hipMalloc(&c->V4, sizeof(double) * a4n));
+  HCHK(c, hipMalloc(&c->Vi4, sizeof(double) * a4n));
+  arrange_A(n, c->hVi, h4, MT, nkc);
+  HCHK(c, hipMemcpy(c->Vi4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
+  arrange_A(n, c->hV, h4, MT, nkc);
+  HCHK(c, hipMemcpy(c->V4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
+  free(h4);
+  // D / h^2 per axis (read_matrices, vars.c:512,544,576)
+  double *dd = (double *)malloc(sizeof(double) * 3 * n);
+  for (int l = 0; l < n; l++) {
+    dd[l] = c->hD[l] / p->hx / p->hx;
+    dd[n + l] = c->hD[l] / p->hy / p->hy;
+    dd[2 * n + l] = c->hD[l] / p->hz / p->hz;
+  }
+  HCHK(c, hipMalloc(&c->DDX, sizeof(double) * 3 * n));
+  c->DDY = c->DDX + n;
+  c->DDZ = c->DDX + 2 * n;
+  HCHK(c, hipMemcpy(c->DDX, dd, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  free(dd);
+  HCHK(c, hipMalloc(&c->lam, sizeof(double) * 9 * (V > 0 ? V : 1)));
+  HCHK(c, hipMemcpy(c->lam, p->lam, sizeof(double) * 9 * (p->nac > 0 ? p->nac : 1), hipMemcpyHostToDevice));
+  // elasticity
+  if (p->elastic == 1 && p->nac > 0) {
+    c->fft_npass = fft_plan_radices(c->N, c->fft_radix);
+    if (c->fft_npass < 0) return seterr(c, ETD_ERR_UNSUPPORTED, "N = %d has prime factors other than 2, 3, 5", c->N);
+    c->npair = (p->nac + 1) / 2;
+    c->ring = p->nac + 2;
+    HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->N3 * c->npair));
+    HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->N3 * c->npair));
+    HCHK(c, hipMalloc(&c->E, sizeof(double) * c->N3 * c->ring));
+    HCHK(c, hipMemsetAsync(c->E, 0, sizeof(double) * c->N3 * c->ring, c->stream));
+    double *tw = (double *)malloc(sizeof(double) * 2 * c->N);
+    const double pi = 3.14159265358979323846;
+    for (int e = 0; e < c->N; e++) {
+      tw[2 * e] = cos(2.0 * pi * e / c->N);
+      tw[2 * e + 1] = -sin(2.0 * pi * e / c->N);
+    }
+    HCHK(c, hipMalloc(&c->tw, sizeof(double) * 2 * c->N));
+    HCHK(c, hipMemcpy(c->tw, tw, sizeof(double) * 2 * c->N, hipMemcpyHostToDevice));
+    free(tw);
+    etd_dev_consts *k = (etd_dev_consts *)calloc(1, sizeof(etd_dev_consts));
+    c4_from_voigt(p->C, k->C4);
+    for (int q = 0; q < p->nac; q++) {
+      const double *e = p->eps0 + 9 * q;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double s = 0.0;
+          for (int kk = 0; kk < 3; kk++)
+            for (int l = 0; l < 3; l++) s += k->C4[((i * 3 + j) * 3 + kk) * 3 + l] * e[kk * 3 + l];
+          k->sig[9 * q + i * 3 + j] = s;
+        }
+    }
+    for (int a = 0; a < p->nac; a++)
+      for (int b = 0; b < p->nac; b++) {
+        double t0 = 0.0;
+        const double *ea = p->eps0 + 9 * a, *eb = p->eps0 + 9 * b;
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            for (int kk = 0; kk < 3; kk++)
+              for (int l = 0; l < 3; l++) t0 += k->C4[((i * 3 + j) * 3 + kk) * 3 + l] * ea[i * 3 + j] * eb[kk * 3 + l];
+        k->t0[a * ETD_MAX_VAR + b] = t0;
+      }
+    HCHK(c, hipMalloc(&c->dconst, sizeof(etd_dev_consts)));
+    HCHK(c, hipMemcpy(c->dconst, k, sizeof(etd_dev_consts), hipMemcpyHostToDevice));
+    free(k);
+  }
+  // energy partials
+  const int ms = n - 2 * c->g, me = n - 2 * c->g - 1;
+  c->npart = 3 * ms * V + me * (2 + c->nch);
+  HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
+  HCHK(c, hipHostMalloc(&c->hpart, sizeof(double) * c->npart, hipHostMallocDefault));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" void etd_close(etd_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->u);
+  (void)hipFree(c->N2);
+  (void)hipFree(c->w1);
+  (void)hipFree(c->w2);
+  (void)hipFree(c->spec);
+  (void)hipFree(c->gtmp);
+  (void)hipFree(c->V4);
+  (void)hipFree(c->Vi4);
+  (void)hipFree(c->DDX);
+  (void)hipFree(c->lam);
+  (void)hipFree(c->Z);
+  (void)hipFree(c->Y);
+  (void)hipFree(c->E);
+  (void)hipFree(c->tw);
+  (void)hipFree(c->dconst);
+  (void)hipFree(c->part);
+  if (c->hpart) (void)hipHostFree(c->hpart);
+  for (int i = 0; i < c->cap_ev; i++) (void)hipEventDestroy(c->ev[i]);
+  free(c->ev);
+  free(c->ev_class);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  free(c->hV);
+  free(c->hVi);
+  free(c->hD);
+  free(c);
+}
+
+extern "C" int etd_get_params(const etd_ctx *c, etd_params *p) {
+  if (!c || !p) return ETD_ERR_ARG;
+  *p = c->p;
+  return ETD_OK;
+}
+
+extern "C" int etd_open(const char *dir, const char *suffix, etd_ctx **out) {
+  if (!dir || !suffix || !out) return ETD_ERR_ARG;
+  etd_params p;
+  etd_default_params(&p);
+  char fn[2048];
+  snprintf(fn, sizeof fn, "%s/input%s.ini", dir, suffix);
+  int rc = etd_parse_ini(fn, &p);
+  if (rc) {
+    fprintf(stderr, "etd_open: cannot parse %s\n", fn);
+    return rc;
+  }
+  if (p.elastic != 0) {
+    snprintf(fn, sizeof fn, "%s/elastic_input.txt", dir);
+    if ((rc = etd_parse_elastic(fn, &p))) {
+      fprintf(stderr, "open elastic_input.txt Fail!");
+      return rc;
+    }
+  }
+  if (p.anisotropic == 1) {
+    snprintf(fn, sizeof fn, "%s/anisotropic_input.txt", dir);
+    if ((rc = etd_parse_anisotropic(fn, &p))) {
+      fprintf(stderr, "open anisotropic_input.txt Fail!");
+      return rc;
+    }
+  }
+  snprintf(fn, sizeof fn, "%s/landau_input.txt", dir);
+  FILE *f = fopen(fn, "r");
+  if (f) {
+    fclose(f);
+    if ((rc = etd_parse_landau(fn, &p))) return rc;
+  }
+  const int n = p.nx;
+  double *V = (double *)malloc(sizeof(double) * n * n), *Vi = (double *)malloc(sizeof(double) * n * n),
+         *D = (double *)malloc(sizeof(double) * n);
+  int have = etd_read_matrices(dir, n, 3, V, Vi, D) == ETD_OK;
+  rc = etd_create(&p, have ? V : nullptr, have ? Vi : nullptr, have ? D : nullptr, out);
+  free(V);
+  free(Vi);
+  free(D);
+  return rc;
+}
+
+// ------------------------------------------------------------------ fields
+extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
+  if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "set_field: bad variable %d", var);
+  HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_get_field(etd_ctx *c, int var, double *host) {
+  if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "get_field: bad variable %d", var);
+  HCHK(c, hipMemcpyAsync(host, c->u + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, double mean, double amp) {
+  if (!c || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "init_noise: bad variable %d", var);
+  double *h = (double *)malloc(sizeof(double) * c->n3);
+  if (!h) return ETD_ERR_NOMEM;
+  etd_host_noise(c->n, c->g, seed, mean, amp, h);
+  int rc = etd_set_field(c, var, h);
+  free(h);
+  return rc;
+}
+
+// ------------------------------------------------------------------ step
+static void fwd3(etd_ctx *c, const double *in, double *out, int nf) {
+  const size_t n3 = c->n3;
+  launch_modeprod(c, c->Vi4, in, n3, c->w1, n3, nf, 1, 0);
+  launch_modeprod(c, c->Vi4, c->w1, n3, c->w2, n3, nf, 1, 0);
+  launch_modeprod(c, c->Vi4, c->w2, n3, out, n3, nf, 1, 0);
+}
+static void bwd3(etd_ctx *c, const double *in, double *out, int nf, int accumulate) {
+  const size_t n3 = c->n3;
+  launch_modeprod(c, c->V4, in, n3, c->w1, n3, nf, 1, 0);
+  launch_modeprod(c, c->V4, c->w1, n3, c->w2, n3, nf, 1, 0);
+  launch_modeprod(c, c->V4, c->w2, n3, out, n3, nf, 1, accumulate);
+}
+
+// ScLETD.c:87-189 for every field at once: u and N(u) share the forward
+// transforms (one launch over 2V fields), the phi update writes u-hat in
+// place, three inverse products return u in xyz.
+static int step_once(etd_ctx *c) {
+  const int V = c->V;
+  const size_t n3 = c->n3;
+  launch_ghost_fill(c, c->u, V, n3);
+  if (c->p.elastic == 1 && c->nac > 0) launch_elastic(c);
+  launch_nonlinear(c, c->N1, nullptr);
+  fwd3(c, c->u, c->spec, 2 * V);
+  launch_phiscale(c, c->spec, c->spec + n3 * V, 0, V, 0);
+  bwd3(c, c->spec, c->u, V, 0);
+  if (c->p.etd2) {
+    launch_ghost_fill(c, c->u, V, n3);
+    if (c->p.elastic == 1 && c->nac > 0) launch_elastic(c);
+    launch_nonlinear(c, c->N2, c->N1);  // N(a) - N(u_n)
+    fwd3(c, c->N2, c->spec, V);
+    launch_phiscale(c, c->spec, nullptr, 0, V, 1);
+    bwd3(c, c->spec, c->u, V, 1);  // u = a + S phi2 S^-1 dN
+  }
+  HCHK(c, hipGetLastError());
+  return ETD_OK;
+}
+
+extern "C" int etd_step(etd_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return ETD_ERR_ARG;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = step_once(c);
+    if (rc) return rc;
+  }
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_diag(etd_ctx *c, double *stats) {
+  if (!c || !stats) return ETD_ERR_ARG;
+  int rc = launch_diag(c, stats);
+  if (rc) return seterr(c, rc, "diag: %s", hipGetErrorString(hipGetLastError()));
+  return ETD_OK;
+}
+
+extern "C" int etd_write_csv(etd_ctx *c, int iter, const double *stats) {
+  if (!c || !stats) return ETD_ERR_ARG;
+  int rc = etd_csv_rows(c->p.work_dir, c->nac, c->nch, iter, stats);
+  if (rc) return seterr(c, rc, "cannot write CSV under %s", c->p.work_dir);
+  return ETD_OK;
+}
+
+extern "C" int etd_write_field(etd_ctx *c, int var, int ioutput) {
+  if (!c || var < 0 || var >= c->V) return ETD_ERR_ARG;
+  double *h = (double *)malloc(sizeof(double) * c->n3);
+  int rc = etd_get_field(c, var, h);
+  if (!rc) rc = etd_field_file(c->p.data_dir, var, ioutput, c->n, h);
+  free(h);
+  return rc;
+}
+
+// ------------------------------------------------------------------ per-kernel entry points
+extern "C" int etd_k_transfer(etd_ctx *c) {
+  if (!c) return ETD_ERR_ARG;
+  launch_ghost_fill(c, c->u, c->V, c->n3);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_k_elastic(etd_ctx *c, double *host_felas) {
+  if (!c) return ETD_ERR_ARG;
+  if (c->p.elastic != 1 || c->nac == 0) return seterr(c, ETD_ERR_ARG, "elasticity is off");
+  launch_elastic(c);
+  if (host_felas) {
+    double *d;
+    HCHK(c, hipMalloc(&d, sizeof(double) * c->n3 * c->nac));
+    launch_felas_materialize(c, d);
+    HCHK(c, hipMemcpyAsync(host_felas, d, sizeof(double) * c->n3 * c->nac, hipMemcpyDeviceToHost, c->stream));
+    HCHK(c, hipStreamSynchronize(c->stream));
+    HCHK(c, hipFree(d));
+  }
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
+  if (!c || !host_out || var < 0 || var >= c->V) return ETD_ERR_ARG;
+  launch_nonlinear(c, c->N1, nullptr);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipMemcpyAsync(host_out, c->N1 + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *in, double *out, int rotate) {
+  if (!c || !A || !in || !out) return ETD_ERR_ARG;
+  if (n != c->n) return seterr(c, ETD_ERR_ARG, "modeprod: n must equal the context edge %d", c->n);
+  const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
+  const size_t a4n = arrange_A_chunk(MT) * nkc;
+  double *h4 = (double *)malloc(sizeof(double) * a4n);
+  arrange_A(n, A, h4, MT, nkc);
+  double *dA, *din, *dout;
+  HCHK(c, hipMalloc(&dA, sizeof(double) * a4n));
+  HCHK(c, hipMalloc(&din, sizeof(double) * c->n3));
+  HCHK(c, hipMalloc(&dout, sizeof(double) * c->n3));
+  HCHK(c, hipMemcpy(dA, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
+  HCHK(c, hipMemcpy(din, in, sizeof(double) * c->n3, hipMemcpyHostToDevice));
+  free(h4);
+  launch_modeprod(c, dA, din, c->n3, dout, c->n3, 1, rotate, 0);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  HCHK(c, hipMemcpy(out, dout, sizeof(double) * c->n3, hipMemcpyDeviceToHost));
+  (void)hipFree(dA);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return ETD_OK;
+}
+
+extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat) {
+  if (!c || !uhat || !nhat || var < 0 || var >= c->V) return ETD_ERR_ARG;
+  double *d;
+  HCHK(c, hipMalloc(&d, sizeof(double) * 2 * c->n3));
+  HCHK(c, hipMemcpy(d, uhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
+  HCHK(c, hipMemcpy(d + c->n3, nhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
+  launch_phiscale(c, d, d + c->n3, var, 1, 0);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  HCHK(c, hipMemcpy(uhat, d, sizeof(double) * c->n3, hipMemcpyDeviceToHost));
+  (void)hipFree(d);
+  return ETD_OK;
+}
+
+// ------------------------------------------------------------------ timing
+extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches) {
+  if (!c || nsteps < 1 || !ms) return ETD_ERR_ARG;
+  // enough event slots for every bracketed launch group of nsteps steps
+  const int per_step = 2 * (2 * 12 + 2 * 4 + 8);
+  const int need = per_step * nsteps + 16;
+  if (c->cap_ev < need) {
+    for (int i = 0; i < c->cap_ev; i++) (void)hipEventDestroy(c->ev[i]);
+    free(c->ev);
+    free(c->ev_class);
+    c->ev = (hipEvent_t *)calloc(need, sizeof(hipEvent_t));
+    c->ev_class = (int *)calloc(need, sizeof(int));
+    c->cap_ev = 0;
+    for (int i = 0; i < need; i++) {
+      HCHK(c, hipEventCreate(&c->ev[i]));
+      c->cap_ev++;
+    }
+  }
+  hipEvent_t t0, t1;
+  HCHK(c, hipEventCreate(&t0));
+  HCHK(c, hipEventCreate(&t1));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  c->prof = 1;
+  c->nev = 0;
+  double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
+  HCHK(c, hipEventRecord(t0, c->stream));
+  int rc = ETD_OK;
+  for (int s = 0; s < nsteps && rc == ETD_OK; s++) {
+    if (with_diag) rc = launch_diag(c, st);
+    if (rc == ETD_OK) rc = step_once(c);
+  }
+  HCHK(c, hipEventRecord(t1, c->stream));
+  HCHK(c, hipEventSynchronize(t1));
+  c->prof = 0;
+  free(st);
+  if (rc) return rc;
+  float tot;
+  HCHK(c, hipEventElapsedTime(&tot, t0, t1));
+  for (int k = 0; k < 7; k++) ms[k] = 0.0;
+  ms[0] = tot;
+  int cnt[PC_COUNT] = {0};
+  for (int i = 0; i < c->nev; i++) {
+    if (c->ev_class[i] < 0) continue;
+    const int cls = c->ev_class[i];
+    int j = i + 1;
+    while (j < c->nev && c->ev_class[j] != -1 - cls) j++;
+    if (j >= c->nev) break;
+    float e;
+    HCHK(c, hipEventElapsedTime(&e, c->ev[i], c->ev[j]));
+    ms[1 + cls] += e;
+    cnt[cls]++;
+  }
+  if (launches) {
+    for (int k = 0; k < PC_COUNT; k++) launches[k] = cnt[k];
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return ETD_OK;
+}

@@ -1,0 +1,90 @@
+/*
+ * etd_internal.h -- device state and kernel launchers of libetd (gfx950).
+ *
+ * HBM layout (plain FP64 path), n = extended edge, n3 = n^3, V = nac + nch:
+ *   u     V x n3   fields, xyz (x fastest), ghosts included
+ *   N1    V x n3   N(u_n); N2 V x n3: N(a) - N(u_n) for the ETD2 corrector
+ *   w1,w2 2V x n3  mode-product intermediates (u and N batched together)
+ *   spec  2V x n3  spectral u-hat / N-hat in the rotated x'y'z' layout
+ *   Vi4,V4         V^-1 and V rearranged as MFMA A-fragments (see modeprod)
+ *   Z,Y   P x N^3  complex pairs (eta_2r + i eta_2r+1) for the elastic FFTs
+ *   E     R x N^3  ring of elastic driving forces (interior), R = nac + 2
+ */
+#ifndef ETD_INTERNAL_H
+#define ETD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include "etd.h"
+
+#define ETD_MAXN 512
+
+struct etd_dev_consts {
+  /* elasticity, shared by every k-point */
+  double C4[81];
+  double sig[ETD_MAX_VAR * 9];
+  double t0[ETD_MAX_VAR * ETD_MAX_VAR];
+};
+
+struct etd_ctx {
+  etd_params p;
+  int n, g, N, V, nac, nch;
+  size_t n3, N3;
+  hipStream_t stream;
+  char err[512];
+  /* host copies */
+  double *hV, *hVi, *hD;
+  /* device */
+  double *u, *N1, *N2, *w1, *w2, *spec;
+  double *V4, *Vi4;        /* MFMA-arranged A operands */
+  double *DDX, *DDY, *DDZ; /* D / h^2 per axis */
+  double *lam;             /* nac x 9 */
+  double *gtmp;            /* 3 x n3 gradient scratch for F2, or CH t */
+  /* elastic */
+  double *Z, *Y;           /* npair x N^3 complex */
+  double *E;               /* ring x N^3 */
+  double *tw;              /* N complex twiddles exp(-2 pi i e / N) */
+  etd_dev_consts *dconst;
+  int npair, ring;
+  long long elastic_evals; /* completed elastic evaluations */
+  int fft_radix[16], fft_npass;
+  /* energy partials */
+  double *part;            /* device partial sums */
+  double *hpart;           /* pinned host copy */
+  int npart;
+  /* profiling */
+  int prof;                /* record per-class event pairs */
+  hipEvent_t *ev;
+  int *ev_class;
+  int nev, cap_ev;
+};
+
+enum { PC_MODEPROD = 0, PC_ELASTIC, PC_NONLIN, PC_PHI, PC_ENERGY, PC_HALO, PC_COUNT };
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* host-side helpers in etd_io.c */
+void etd_host_noise(int n, int g, unsigned long long seed, double mean, double amp, double *u);
+int etd_csv_rows(const char *work_dir, int nac, int nch, int iter, const double *stats);
+int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u);
+#ifdef __cplusplus
+}
+#endif
+
+/* launchers (etd_kernels.hip) */
+void etd_prof_begin(etd_ctx *c, int cls);
+void etd_prof_end(etd_ctx *c, int cls);
+
+void launch_ghost_fill(etd_ctx *c, double *base, int nfields, size_t fstride);
+void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
+                     int nfields, int rotate, int accumulate);
+void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var, int nvars, int order2);
+void launch_nonlinear(etd_ctx *c, double *out, const double *sub);
+void launch_elastic(etd_ctx *c);
+void launch_felas_materialize(etd_ctx *c, double *felas);
+int launch_diag(etd_ctx *c, double *stats);
+size_t arrange_A_chunk(int MT);
+void arrange_A(int n, const double *Acm, double *A4host, int MT, int nkc);
+
+#endif

@@ -1,0 +1,1014 @@
+// etd_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the ScLETD step.
+//
+// Every kernel takes n^3 fields in the reference's xyz layout (x fastest)
+// unless stated; spectral data is in the rotated x'y'z' layout produced by
+// three rotating mode products.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <utility>
+
+#include "etd_internal.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int wrap_e(int e, int n, int g) {
+  const int N = n - 2 * g;
+  return e < g ? e + N : (e >= n - g ? e - N : e);
+}
+__device__ __forceinline__ size_t IDX(int n, int k, int j, int i) { return ((size_t)k * n + j) * n + i; }
+
+// ------------------------------------------------------------------ halo
+// Periodic self-exchange of transfer() (trans.c:1939-1996): every ghost value
+// comes from the pre-transfer state, z faces override y faces override x
+// faces.  In place: z phase, then y phase (reads x-ghost cells of interior
+// planes, not yet rewritten), then x phase (reads interior cells only).
+__global__ void k_ghost_z(double *base, size_t fs, int n, int g) {
+  double *u = base + fs * blockIdx.y;
+  const size_t n2 = (size_t)n * n;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)2 * g * n2) return;
+  int pl = (int)(t / n2);
+  size_t r = t % n2;
+  int k = pl < g ? pl : n - 2 * g + pl;
+  u[(size_t)k * n2 + r] = u[(size_t)wrap_e(k, n, g) * n2 + r];
+}
+__global__ void k_ghost_y(double *base, size_t fs, int n, int g) {
+  double *u = base + fs * blockIdx.y;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = 2 * g;
+  if (t >= (size_t)(n - 2 * g) * rows * n) return;
+  int i = (int)(t % n);
+  int rr = (int)((t / n) % rows);
+  int k = g + (int)(t / ((size_t)n * rows));
+  int j = rr < g ? rr : n - 2 * g + rr;
+  u[IDX(n, k, j, i)] = u[IDX(n, k, wrap_e(j, n, g), i)];
+}
+__global__ void k_ghost_x(double *base, size_t fs, int n, int g) {
+  double *u = base + fs * blockIdx.y;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cols = 2 * g, m = n - 2 * g;
+  if (t >= (size_t)m * m * cols) return;
+  int cc = (int)(t % cols);
+  int j = g + (int)((t / cols) % m);
+  int k = g + (int)(t / ((size_t)cols * m));
+  int i = cc < g ? cc : n - 2 * g + cc;
+  u[IDX(n, k, j, i)] = u[IDX(n, k, j, wrap_e(i, n, g))];
+}
+
+void launch_ghost_fill(etd_ctx *c, double *base, int nfields, size_t fstride) {
+  const int n = c->n, g = c->g;
+  const size_t n2 = (size_t)n * n;
+  size_t tz = 2 * g * n2, ty = (size_t)(n - 2 * g) * 2 * g * n, tx = (size_t)(n - 2 * g) * (n - 2 * g) * 2 * g;
+  etd_prof_begin(c, PC_HALO);
+  k_ghost_z<<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  k_ghost_y<<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  k_ghost_x<<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  etd_prof_end(c, PC_HALO);
+}
+
+// ------------------------------------------------------------------ mode product
+// out = A x_1 in for each of nfields fields: the contraction over the fastest
+// axis a of in[a + n*b + n^2*c], C(m, col) = sum_a A(m,a) in[a + n*col],
+// col = b + n*c, written rotated (out[col + n^2*m], the reference's
+// cblas_dgemm + xyz_yzx, calc.c:631-684) or in GEMM layout (out[m + n*col]).
+//
+// FP64 MFMA 16x16x4: a wave owns 16*NB columns and all MT = ceil(n/16) row
+// tiles.  K runs in chunks of 16; inside a chunk lane group q = lane/16
+// supplies k = 16 kc + 4 q + s at MFMA step s, so each lane's four B values
+// are contiguous in memory (one 32-byte read) and its A values are a
+// pre-arranged double4.  A chunks are staged through LDS (double-buffered)
+// and shared by the workgroup's 4 waves; B is read straight from HBM one
+// chunk ahead.
+template <int MT, int NB, bool VEC>
+__global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__ A4, const double *__restrict__ in,
+                                                      size_t in_fs, double *__restrict__ out, size_t out_fs, int n,
+                                                      int rotate, int accumulate) {
+  // A chunk: MT*64 double4 = MT*128 double2, padded to APT*256 (zeros) so
+  // every thread moves exactly APT double2 per chunk
+  constexpr int A2 = MT * 128;
+  constexpr int APT = (A2 + 255) / 256;
+  __shared__ double2 sA2[2][APT * 256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  const double *src = in + in_fs * blockIdx.y;
+  double *dst = out + out_fs * blockIdx.y;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 15) / 16;
+  const long colbase = (long)blockIdx.x * (64 * NB) + wave * (16 * NB);
+
+  v4d acc[MT][NB];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB; u++) acc[t][u] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  // B fragment of column tile u for K chunk kc (zero outside the array)
+  auto loadB = [&](int kc, int u) -> v4d {
+    const long col = colbase + 16 * u + r16;
+    const int k0 = 16 * kc + 4 * q;
+    v4d v = (v4d){0.0, 0.0, 0.0, 0.0};
+    if (col < n2) {
+      const double *p = src + k0 + (long)n * col;
+      if (VEC && k0 + 3 < n) {
+        const double2 lo = *reinterpret_cast<const double2 *>(p);
+        const double2 hi = *reinterpret_cast<const double2 *>(p + 2);
+        v = (v4d){lo.x, lo.y, hi.x, hi.y};
+      } else {
+        if (k0 + 0 < n) v.x = p[0];
+        if (k0 + 1 < n) v.y = p[1];
+        if (k0 + 2 < n) v.z = p[2];
+        if (k0 + 3 < n) v.w = p[3];
+      }
+    }
+    return v;
+  };
+  // register staging of the next A chunk (fold expressions keep every index a
+  // compile-time constant so the array lives in VGPRs)
+  double2 aprf[APT];
+  constexpr auto AIDX = std::make_index_sequence<APT>{};
+  auto fetch_a = [&](int kc, auto seq) {
+    const double2 *g2 = reinterpret_cast<const double2 *>(A4) + (size_t)kc * (APT * 256);
+    [&]<size_t... I>(std::index_sequence<I...>) { ((aprf[I] = g2[threadIdx.x + 256 * I]), ...); }(seq);
+  };
+  auto store_a = [&](int b, auto seq) {
+    [&]<size_t... I>(std::index_sequence<I...>) { ((sA2[b][threadIdx.x + 256 * I] = aprf[I]), ...); }(seq);
+  };
+
+  v4d bcur[NB], bnext[NB];
+  fetch_a(0, AIDX);
+#pragma unroll
+  for (int u = 0; u < NB; u++) bcur[u] = loadB(0, u);
+  store_a(0, AIDX);
+  __syncthreads();
+  for (int kc = 0; kc < nkc; kc++) {
+    const int buf = kc & 1;
+    const bool more = kc + 1 < nkc;
+    if (more) {
+      fetch_a(kc + 1, AIDX);
+#pragma unroll
+      for (int u = 0; u < NB; u++) bnext[u] = loadB(kc + 1, u);
+    }
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      const double2 alo = sA2[buf][2 * (t * 64 + lane)], ahi = sA2[buf][2 * (t * 64 + lane) + 1];
+      const v4d a = (v4d){alo.x, alo.y, ahi.x, ahi.y};
+#pragma unroll
+      for (int u = 0; u < NB; u++) {
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bcur[u].x, acc[t][u], 0, 0, 0);
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bcur[u].y, acc[t][u], 0, 0, 0);
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.z, bcur[u].z, acc[t][u], 0, 0, 0);
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.w, bcur[u].w, acc[t][u], 0, 0, 0);
+      }
+    }
+    if (more) {
+      store_a(buf ^ 1, AIDX);
+#pragma unroll
+      for (int u = 0; u < NB; u++) bcur[u] = bnext[u];
+    }
+    __syncthreads();
+  }
+  // D(16x16): lane holds rows 4q..4q+3 of column r16
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const long col = colbase + 16 * u + r16;
+      if (col >= n2) continue;
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int m = 16 * t + 4 * q + rr;
+        if (m >= n) continue;
+        const double val = acc[t][u][rr];
+        const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
+        if (accumulate) dst[o] += val;
+        else dst[o] = val;
+      }
+    }
+}
+
+// host: A(m,k) column-major -> A4[kc*CH + (t*64 + lane)*4 + s]
+//       = A(16t + lane%16, 16kc + 4(lane/16) + s), zero outside n x n;
+// CH = per-chunk stride (MT*256 doubles rounded up to 512)
+size_t arrange_A_chunk(int MT) { return (size_t)((MT * 128 + 255) / 256) * 256 * 2; }
+void arrange_A(int n, const double *Acm, double *A4, int MT, int nkc) {
+  const size_t CH = arrange_A_chunk(MT);
+  memset(A4, 0, sizeof(double) * CH * nkc);
+  for (int kc = 0; kc < nkc; kc++)
+    for (int t = 0; t < MT; t++)
+      for (int lane = 0; lane < 64; lane++)
+        for (int s = 0; s < 4; s++) {
+          int m = 16 * t + (lane & 15), k = 16 * kc + 4 * (lane >> 4) + s;
+          A4[(size_t)kc * CH + ((size_t)t * 64 + lane) * 4 + s] = (m < n && k < n) ? Acm[m + (size_t)n * k] : 0.0;
+        }
+}
+
+template <int MT>
+static void modeprod_mt(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
+                        int nfields, int rotate, int accumulate) {
+  constexpr int NB = 2;
+  const int n = c->n;
+  const long n2 = (long)n * n;
+  dim3 grid((unsigned)((n2 + 64 * NB - 1) / (64 * NB)), nfields);
+  if (n % 2 == 0)
+    k_modeprod_f64<MT, NB, true><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+  else
+    k_modeprod_f64<MT, NB, false><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+}
+
+void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
+                     int nfields, int rotate, int accumulate) {
+  const int MT = (c->n + 15) / 16;
+  etd_prof_begin(c, PC_MODEPROD);
+  switch (MT) {
+#define MTC(X) case X: modeprod_mt<X>(c, A4, in, in_fs, out, out_fs, nfields, rotate, accumulate); break;
+    MTC(1) MTC(2) MTC(3) MTC(4) MTC(5) MTC(6) MTC(7) MTC(8) MTC(9) MTC(10) MTC(11) MTC(12) MTC(13) MTC(14)
+    MTC(15) MTC(16) MTC(17)
+#undef MTC
+    default: break;
+  }
+  etd_prof_end(c, PC_MODEPROD);
+}
+
+// ------------------------------------------------------------------ phi scaling
+// ac_updateU_new / ch_updateU_new (calc.c:489-535) and prepare_U2_new
+// (553-567) on the rotated spectral layout l = i' + n j' + n^2 k'.  phi1 and
+// phi2 (vars.c:302-359) are recomputed in registers from D instead of read
+// from n^3 tables.
+struct phi_args {
+  int n, nac;
+  double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
+};
+
+__global__ void k_phiscale(double *uhat, const double *nhat, size_t fs, int first_var, const double *DDX,
+                           const double *DDY, const double *DDZ, phi_args a, int order2) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int var = first_var + blockIdx.y;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  double H;
+  if (var >= a.nac) {
+    const double tmp = DDZ[k] + DDY[j] + DDX[i];
+    H = a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp);
+  } else {
+    const double tmp = a.kkz * DDZ[k] + a.kky * DDY[j] + a.kkx * DDX[i];
+    H = -a.LE * (tmp * a.epn2 - a.KE);
+  }
+  double phi1, phi2;
+  if (fabs(H) > 1.0e-8) {
+    const double e = exp(-a.dt * H);
+    phi1 = (1.0 - e) / H;
+    phi2 = (1.0 - phi1 / a.dt) / H;
+  } else {
+    phi1 = a.dt;
+    phi2 = a.dt / 2;
+  }
+  double *uh = uhat + fs * blockIdx.y;
+  if (order2) {
+    uh[l] = phi2 * uh[l];
+  } else {
+    const double *nh = nhat + fs * blockIdx.y;
+    const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
+    const double tmp = 1.0 - phi1 * Hp;
+    uh[l] = tmp * uh[l] + phi1 * nh[l];
+  }
+}
+
+void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var, int nvars, int order2) {
+  phi_args a;
+  a.n = c->n;
+  a.nac = c->nac;
+  a.dt = c->p.dt;
+  a.epn2 = c->p.epn2;
+  a.LE = c->p.LE;
+  a.KE = c->p.KE;
+  a.LCI = c->p.LCI;
+  a.KCI = c->p.KCI;
+  a.kkx = c->p.kkx;
+  a.kky = c->p.kky;
+  a.kkz = c->p.kkz;
+  etd_prof_begin(c, PC_PHI);
+  k_phiscale<<<dim3((unsigned)((c->n3 + 255) / 256), nvars), 256, 0, c->stream>>>(uhat, nhat, c->n3, first_var, c->DDX,
+                                                                                  c->DDY, c->DDZ, a, order2);
+  etd_prof_end(c, PC_PHI);
+}
+
+// ------------------------------------------------------------------ nonlinear
+struct nl_args {
+  int n, g, N, nac, nch, aniso, elastic, ring;
+  long long Lbase;  // linear index of AC field 0 of the latest elastic evaluation
+  double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
+};
+
+__device__ __forceinline__ double Pimg(const double *u, int n, int g, int z, int y, int x) {
+  return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+}
+
+// Elastic driving force of AC field p at extended cell (k,j,i).  The
+// reference writes IFFT results into the interior of ONE buffer f and
+// refreshes its ghosts with the same self-exchange before copying it to
+// felas_p (elastic.c:664-687, 714-723), so ghost cells whose source is itself
+// a ghost of f carry values of earlier variants: with d = number of ghost
+// coordinates, the value is the interior of variant p - max(0, d-1) at the
+// fully wrapped position (variants before 0 come from the previous call).
+__device__ __forceinline__ double felas_at(const double *E, const nl_args &a, int p, int k, int j, int i) {
+  const int n = a.n, g = a.g, N = a.N;
+  const int gk = (k < g || k >= n - g), gj = (j < g || j >= n - g), gi = (i < g || i >= n - g);
+  int lag = gk + gj + gi - 1;
+  lag = lag < 0 ? 0 : lag;
+  long long L = a.Lbase + p - lag;
+  int slot = (int)(((L % a.ring) + a.ring) % a.ring);
+  const int wk = wrap_e(k, n, g) - g, wj = wrap_e(j, n, g) - g, wi = wrap_e(i, n, g) - g;
+  return E[(size_t)slot * N * N * N + ((size_t)wk * N + wj) * N + wi];
+}
+
+// ac_calc_FU (calc.c:8-68) -> ac_calc_F1 (anisotropic.c:9-161): all AC fields
+// of one point per thread (sum u^2 couples them).
+__global__ void k_nonlin_ac(const double *__restrict__ u, double *__restrict__ out, const double *__restrict__ sub,
+                            const double *__restrict__ E, nl_args a, int with_felas) {
+  const int n = a.n, g = a.g;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  double s = 0.0;
+  for (int m = 0; m < a.nac; m++) {
+    const double um = u[n3 * m + l];
+    s = s + um * um;
+  }
+  for (int p = 0; p < a.nac; p++) {
+    const double *up_f = u + n3 * p;
+    const double up = up_f[l];
+    const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
+    double f = a.LE * a.KE * up;
+    f -= a.LE * fp;
+    const double ce = a.LE * a.epn2;
+    if (i == 0) f += ce * Pimg(up_f, n, g, k, j, -1) / a.hx / a.hx;
+    if (i == n - 1) f += ce * Pimg(up_f, n, g, k, j, n) / a.hx / a.hx;
+    if (j == 0) f += ce * Pimg(up_f, n, g, k, -1, i) / a.hy / a.hy;
+    if (j == n - 1) f += ce * Pimg(up_f, n, g, k, n, i) / a.hy / a.hy;
+    if (k == 0) f += ce * Pimg(up_f, n, g, -1, j, i) / a.hz / a.hz;
+    if (k == n - 1) f += ce * Pimg(up_f, n, g, n, j, i) / a.hz / a.hz;
+    if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+    if (sub) f -= sub[n3 * p + l];
+    out[n3 * p + l] = f;
+  }
+}
+
+// ac_calc_F2 (anisotropic.c:164-627), pass 1: G = Lambda (gx, gy, gz) with
+// gy, gz taken as (u[-1] - u[+1]) / 2h; out-of-box neighbours read 0.
+__global__ void k_f2_grad(const double *__restrict__ u, double *__restrict__ G, const double *__restrict__ lam,
+                          nl_args a) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  const double a_left = i > 0 ? u[l - 1] : 0.0;
+  const double a_right = i < n - 1 ? u[l + 1] : 0.0;
+  const double a_top = j > 0 ? u[l - n] : 0.0;
+  const double a_bottom = j < n - 1 ? u[l + n] : 0.0;
+  const double a_front = k > 0 ? u[l - (size_t)n * n] : 0.0;
+  const double a_back = k < n - 1 ? u[l + (size_t)n * n] : 0.0;
+  const double gx = (a_right - a_left) / (2.0 * a.hx);
+  const double gy = (a_top - a_bottom) / (2.0 * a.hy);
+  const double gz = (a_front - a_back) / (2.0 * a.hz);
+  G[l] = lam[0] * gx + lam[1] * gy + lam[2] * gz;
+  G[n3 + l] = lam[3] * gx + lam[4] * gy + lam[5] * gz;
+  G[2 * n3 + l] = lam[6] * gx + lam[7] * gy + lam[8] * gz;
+}
+
+// pass 2: out -= epn2 LE div(G) (then the elastic term, keeping the
+// reference's order f1 - epn2 LE F2 - LE ES felas)
+__global__ void k_f2_div(const double *__restrict__ G, double *__restrict__ out, const double *__restrict__ E,
+                         nl_args a, int p, int with_felas) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  const double *gx = G, *gy = G + n3, *gz = G + 2 * n3;
+  const double a_left = i > 0 ? gx[l - 1] : 0.0;
+  const double a_right = i < n - 1 ? gx[l + 1] : 0.0;
+  const double a_top = j > 0 ? gy[l - n] : 0.0;
+  const double a_bottom = j < n - 1 ? gy[l + n] : 0.0;
+  const double a_front = k > 0 ? gz[l - (size_t)n * n] : 0.0;
+  const double a_back = k < n - 1 ? gz[l + (size_t)n * n] : 0.0;
+  double tmp = 0.0;
+  tmp += (a_right - a_left) / 2.0 / a.hx;
+  tmp += (a_top - a_bottom) / 2.0 / a.hy;
+  tmp += (a_front - a_back) / 2.0 / a.hz;
+  double f = out[l];
+  f -= a.epn2 * a.LE * tmp;
+  if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+  out[l] = f;
+}
+
+__device__ __forceinline__ double ch_f(double cc) { return cc * (cc * cc - 1.0); }
+
+// ch_calc_FU_ConstantMobility (calc.c:70-473), pass 1:
+// t = -LCI KCI c + LCI CH_f(c), minus LCI epn2 c_out / h^2 on the faces.
+__global__ void k_ch_t(const double *__restrict__ cfield, double *__restrict__ t, nl_args a) {
+  const int n = a.n, g = a.g;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  const double cc = cfield[l];
+  double f = -a.LCI * a.KCI * cc;
+  f += a.LCI * ch_f(cc);
+  const double ce = a.LCI * a.epn2;
+  if (i == 0) f -= ce * Pimg(cfield, n, g, k, j, -1) / a.hx / a.hx;
+  if (i == n - 1) f -= ce * Pimg(cfield, n, g, k, j, n) / a.hx / a.hx;
+  if (j == 0) f -= ce * Pimg(cfield, n, g, k, -1, i) / a.hy / a.hy;
+  if (j == n - 1) f -= ce * Pimg(cfield, n, g, k, n, i) / a.hy / a.hy;
+  if (k == 0) f -= ce * Pimg(cfield, n, g, -1, j, i) / a.hz / a.hz;
+  if (k == n - 1) f -= ce * Pimg(cfield, n, g, n, j, i) / a.hz / a.hz;
+  t[l] = f;
+}
+
+// chemical potential LCI (epn2 Lap c - CH_f(c)) at an outside cell, from the
+// periodic image (ch_mu, trans.c:1433-1937)
+__device__ double ch_mu_out(const double *c, const nl_args &a, int z, int y, int x) {
+  const int n = a.n, g = a.g;
+  const double v0 = Pimg(c, n, g, z, y, x);
+  double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) / a.hx / a.hx;
+  mu += (Pimg(c, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(c, n, g, z, y + 1, x)) / a.hy / a.hy;
+  mu += (Pimg(c, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(c, n, g, z + 1, y, x)) / a.hz / a.hz;
+  mu = a.LCI * a.epn2 * mu;
+  mu -= a.LCI * ch_f(v0);
+  return mu;
+}
+
+// pass 2: out = Lap_T t (out-of-box neighbours 0) - mu_out / h^2 on faces
+__global__ void k_ch_lap(const double *__restrict__ cfield, const double *__restrict__ t, double *__restrict__ out,
+                         const double *__restrict__ sub, nl_args a) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  const double tc = t[l];
+  const double f_left = i > 0 ? t[l - 1] : 0.0;
+  const double f_right = i < n - 1 ? t[l + 1] : 0.0;
+  const double f_top = j > 0 ? t[l - n] : 0.0;
+  const double f_bottom = j < n - 1 ? t[l + n] : 0.0;
+  const double f_front = k > 0 ? t[l - (size_t)n * n] : 0.0;
+  const double f_back = k < n - 1 ? t[l + (size_t)n * n] : 0.0;
+  double r = (f_left + f_right - 2.0 * tc) / a.hx / a.hx;
+  r += (f_top + f_bottom - 2.0 * tc) / a.hy / a.hy;
+  r += (f_front + f_back - 2.0 * tc) / a.hz / a.hz;
+  if (i == 0) r -= ch_mu_out(cfield, a, k, j, -1) / a.hx / a.hx;
+  if (i == n - 1) r -= ch_mu_out(cfield, a, k, j, n) / a.hx / a.hx;
+  if (j == 0) r -= ch_mu_out(cfield, a, k, -1, i) / a.hy / a.hy;
+  if (j == n - 1) r -= ch_mu_out(cfield, a, k, n, i) / a.hy / a.hy;
+  if (k == 0) r -= ch_mu_out(cfield, a, -1, j, i) / a.hz / a.hz;
+  if (k == n - 1) r -= ch_mu_out(cfield, a, n, j, i) / a.hz / a.hz;
+  if (sub) r -= sub[l];
+  out[l] = r;
+}
+
+__global__ void k_sub(double *out, const double *sub, size_t cnt) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < cnt) out[l] = out[l] - sub[l];
+}
+void launch_sub(etd_ctx *c, double *out, const double *sub, size_t cnt) {
+  k_sub<<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(out, sub, cnt);
+}
+
+static nl_args make_nl(etd_ctx *c) {
+  nl_args a;
+  a.n = c->n;
+  a.g = c->g;
+  a.N = c->N;
+  a.nac = c->nac;
+  a.nch = c->nch;
+  a.aniso = c->p.anisotropic;
+  a.elastic = c->p.elastic;
+  a.ring = c->ring > 0 ? c->ring : 1;
+  a.Lbase = (c->elastic_evals - 1) * (long long)c->nac;
+  a.LE = c->p.LE;
+  a.KE = c->p.KE;
+  a.a2 = c->p.a2;
+  a.a3 = c->p.a3;
+  a.a4 = c->p.a4;
+  a.epn2 = c->p.epn2;
+  a.hx = c->p.hx;
+  a.hy = c->p.hy;
+  a.hz = c->p.hz;
+  a.LCI = c->p.LCI;
+  a.KCI = c->p.KCI;
+  a.es = c->p.elastic_scale;
+  return a;
+}
+
+// N for every variable into out (V x n3); when sub != NULL the result is
+// N - sub (the ETD2 difference prepare_U1_new, calc.c:537-551).
+void launch_nonlinear(etd_ctx *c, double *out, const double *sub) {
+  nl_args a = make_nl(c);
+  const size_t n3 = c->n3;
+  const unsigned nb = (unsigned)((n3 + 255) / 256);
+  const int felas = c->p.elastic == 1 && c->nac > 0;
+  etd_prof_begin(c, PC_NONLIN);
+  if (c->nac > 0) {
+    const int aniso = c->p.anisotropic;
+    // with F2 the felas and ETD2 subtraction move to the divergence pass
+    k_nonlin_ac<<<nb, 256, 0, c->stream>>>(c->u, out, aniso ? nullptr : sub, c->E, a, felas && !aniso);
+    if (aniso)
+      for (int p = 0; p < c->nac; p++) {
+        k_f2_grad<<<nb, 256, 0, c->stream>>>(c->u + n3 * p, c->gtmp, c->lam + 9 * p, a);
+        k_f2_div<<<nb, 256, 0, c->stream>>>(c->gtmp, out + n3 * p, c->E, a, p, felas);
+        if (sub) launch_sub(c, out + n3 * p, sub + n3 * p, n3);  // after the F2 / felas terms
+      }
+  }
+  for (int q = 0; q < c->nch; q++) {
+    const int v = c->nac + q;
+    k_ch_t<<<nb, 256, 0, c->stream>>>(c->u + n3 * v, c->gtmp, a);
+    k_ch_lap<<<nb, 256, 0, c->stream>>>(c->u + n3 * v, c->gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr, a);
+  }
+  etd_prof_end(c, PC_NONLIN);
+}
+
+
+// ------------------------------------------------------------------ elastic
+// Own FFT: each workgroup transforms LINES lines of length N held in LDS with
+// mixed-radix (2,3,4,5) Stockham passes:
+//   pass (radix R, p = product of earlier radices), i in [0, N/R):
+//   k = i mod p, a_t = x[i + t N/R] w^(t k N/(pR)), y = DFT_R(a),
+//   x'[(i-k) R + k + t p] = y_t
+// Forward uses w = exp(-2 pi i/N); the inverse conjugates (scaling 1/N^3 is
+// applied once when results leave the transform).
+#define FFT_LINES 8
+struct fft_plan {
+  int N, npass;
+  int radix[16];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by -i (sgn = +1, forward) or +i (sgn = -1, inverse)
+__device__ __forceinline__ double2 cmi(double2 a, int sgn) {
+  return sgn > 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+template <int SGN>
+__device__ void dft_r(int R, double2 *a) {
+  if (R == 2) {
+    double2 y0 = cadd(a[0], a[1]), y1 = csub(a[0], a[1]);
+    a[0] = y0;
+    a[1] = y1;
+  } else if (R == 4) {
+    double2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+    double2 s13 = cadd(a[1], a[3]), d13 = cmi(csub(a[1], a[3]), SGN);
+    a[0] = cadd(s02, s13);
+    a[1] = cadd(d02, d13);
+    a[2] = csub(s02, s13);
+    a[3] = csub(d02, d13);
+  } else if (R == 3) {
+    const double h = 0.86602540378443864676;  // sin(2 pi / 3)
+    double2 s = cadd(a[1], a[2]), d = csub(a[1], a[2]);
+    double2 y0 = cadd(a[0], s);
+    double2 t = make_double2(a[0].x - 0.5 * s.x, a[0].y - 0.5 * s.y);
+    double2 id = cmi(make_double2(h * d.x, h * d.y), SGN);  // -i h d (forward)
+    a[0] = y0;
+    a[1] = cadd(t, id);
+    a[2] = csub(t, id);
+  } else {  // 5
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    double2 b1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
+    double2 b2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
+    double2 y0 = cadd(a[0], cadd(b1, b2));
+    double2 t1 = make_double2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+    double2 t2 = make_double2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+    // forward: y1 = t1 - i (s1 d1 + s2 d2), y4 = t1 + i(...); y2 = t2 - i (s2 d1 - s1 d2), y3 = t2 + i(...)
+    double2 e1 = cmi(make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), SGN);
+    double2 e2 = cmi(make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), SGN);
+    a[0] = y0;
+    a[1] = cadd(t1, e1);
+    a[4] = csub(t1, e1);
+    a[2] = cadd(t2, e2);
+    a[3] = csub(t2, e2);
+  }
+}
+
+template <int SGN>
+__global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int axis, fft_plan pl,
+                                                  const double2 *__restrict__ tw) {
+  extern __shared__ double2 sbuf[];
+  const int N = pl.N;
+  double2 *buf0 = sbuf, *buf1 = sbuf + FFT_LINES * N;
+  double2 *d = data + fs * blockIdx.y;
+  const size_t N2 = (size_t)N * N;
+  const size_t line0 = (size_t)blockIdx.x * FFT_LINES;  // first line of the group
+  const int tid = threadIdx.x;
+  const int tot = FFT_LINES * N;
+  const size_t nlines = N2;
+  // gather
+  for (int e = tid; e < tot; e += blockDim.x) {
+    int ln, t;
+    size_t addr;
+    if (axis == 0) {
+      t = e % N;
+      ln = e / N;
+      addr = (line0 + ln) * N + t;
+    } else {
+      ln = e % FFT_LINES;
+      t = e / FFT_LINES;
+      size_t L = line0 + ln;  // L = x + N * (other)
+      size_t x = L % N, o = L / N;
+      addr = axis == 1 ? x + (size_t)N * t + N2 * o : x + (size_t)N * o + N2 * t;
+    }
+    buf0[ln * N + t] = line0 + ln < nlines ? d[addr] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2 *src = buf0, *dst = buf1;
+  int p = 1;
+  for (int ps = 0; ps < pl.npass; ps++) {
+    const int R = pl.radix[ps];
+    const int M = N / R;
+    const int stw = N / (p * R);
+    for (int b = tid; b < FFT_LINES * M; b += blockDim.x) {
+      const int ln = b / M, i = b % M;
+      const int k = i % p;
+      double2 a[5];
+      const double2 *s = src + ln * N;
+      for (int t = 0; t < R; t++) {
+        double2 x = s[i + t * M];
+        if (t > 0 && k > 0) {
+          double2 w = tw[(t * k * stw) % N];
+          if (SGN < 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        a[t] = x;
+      }
+      dft_r<SGN>(R, a);
+      double2 *o = dst + ln * N;
+      const int base = (i - k) * R + k;
+      for (int t = 0; t < R; t++) o[base + t * p] = a[t];
+    }
+    __syncthreads();
+    double2 *tmp = src;
+    src = dst;
+    dst = tmp;
+    p *= R;
+  }
+  for (int e = tid; e < tot; e += blockDim.x) {
+    int ln, t;
+    size_t addr;
+    if (axis == 0) {
+      t = e % N;
+      ln = e / N;
+      addr = (line0 + ln) * N + t;
+    } else {
+      ln = e % FFT_LINES;
+      t = e / FFT_LINES;
+      size_t L = line0 + ln;
+      size_t x = L % N, o = L / N;
+      addr = axis == 1 ? x + (size_t)N * t + N2 * o : x + (size_t)N * o + N2 * t;
+    }
+    if (line0 + ln < nlines) d[addr] = src[ln * N + t];
+  }
+}
+
+static void fft3d(etd_ctx *c, double *data, int npair, int sgn) {
+  fft_plan pl;
+  pl.N = c->N;
+  pl.npass = c->fft_npass;
+  for (int i = 0; i < 16; i++) pl.radix[i] = c->fft_radix[i];
+  const size_t N2 = (size_t)c->N * c->N;
+  dim3 grid((unsigned)((N2 + FFT_LINES - 1) / FFT_LINES), npair);
+  size_t lds = 2 * FFT_LINES * (size_t)c->N * sizeof(double2);
+  for (int ax = 0; ax < 3; ax++) {
+    if (sgn > 0)
+      k_fft_axis<1><<<grid, 256, lds, c->stream>>>((double2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+    else
+      k_fft_axis<-1><<<grid, 256, lds, c->stream>>>((double2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+  }
+}
+
+// pack interior eta_2r + i eta_2r+1
+__global__ void k_el_pack(const double *__restrict__ u, double2 *__restrict__ Z, int n, int g, int N, int nac) {
+  const size_t N3 = (size_t)N * N * N;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N3) return;
+  const int r = blockIdx.y;
+  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  const size_t n3 = (size_t)n * n * n;
+  const size_t src = IDX(n, z + g, y + g, x + g);
+  const int pa = 2 * r, pb = 2 * r + 1;
+  double re = u[n3 * pa + src];
+  double im = pb < nac ? u[n3 * pb + src] : 0.0;
+  Z[N3 * r + m] = make_double2(re, im);
+}
+
+// k-space contraction acc_p = sum_q B_pq(n) F_q with B computed on the fly
+// (elastic_calculate_BN, elastic.c:460-605: n from signed wavenumbers, k = 0
+// uses n = (0,1,0); Omega = (C n n)^-1 by the adjugate as inverse_3x3).
+#define EL_MAXV 16
+__global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+                                                     int nac, int npair, const etd_dev_consts *__restrict__ K) {
+  const size_t N3 = (size_t)N * N * N;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N3) return;
+  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  const size_t mm = ((size_t)((N - z) % N) * N + (N - y) % N) * N + (N - x) % N;
+  double2 F[EL_MAXV];
+  for (int r = 0; r < npair; r++) {
+    const double2 a = Z[N3 * r + m], b = Z[N3 * r + mm];
+    // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
+    F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    if (2 * r + 1 < nac) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+  }
+  const int cnt = N / 2;
+  const double n1 = x < cnt ? 1.0 * x : -1.0 * (N - x);
+  const double n2 = y < cnt ? 1.0 * y : -1.0 * (N - y);
+  const double n3_ = z < cnt ? 1.0 * z : -1.0 * (N - z);
+  double nv[3];
+  if (x == 0 && y == 0 && z == 0) {
+    nv[0] = 0.0;
+    nv[1] = 1.0;
+    nv[2] = 0.0;
+  } else {
+    const double rr = sqrt(n1 * n1 + n2 * n2 + n3_ * n3_);
+    nv[0] = n1 / rr;
+    nv[1] = n2 / rr;
+    nv[2] = n3_ / rr;
+  }
+  double oi[9], om[9];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int l = 0; l < 3; ++l) s += K->C4[((i * 3 + j) * 3 + k) * 3 + l] * nv[j] * nv[l];
+      oi[i * 3 + k] = s;
+    }
+  {
+    double nr = 0.0;
+    nr += oi[0] * (oi[4] * oi[8] - oi[5] * oi[7]);
+    nr -= oi[1] * (oi[3] * oi[8] - oi[5] * oi[6]);
+    nr += oi[2] * (oi[3] * oi[7] - oi[4] * oi[6]);
+    om[0] = (oi[4] * oi[8] - oi[5] * oi[7]) / nr;
+    om[1] = -(oi[3] * oi[8] - oi[6] * oi[5]) / nr;
+    om[2] = (oi[3] * oi[7] - oi[6] * oi[4]) / nr;
+    om[3] = -(oi[1] * oi[8] - oi[7] * oi[2]) / nr;
+    om[4] = (oi[0] * oi[8] - oi[2] * oi[6]) / nr;
+    om[5] = -(oi[0] * oi[7] - oi[6] * oi[1]) / nr;
+    om[6] = (oi[1] * oi[5] - oi[4] * oi[2]) / nr;
+    om[7] = -(oi[0] * oi[5] - oi[3] * oi[2]) / nr;
+    om[8] = (oi[0] * oi[4] - oi[1] * oi[3]) / nr;
+  }
+  double bq[EL_MAXV][3];
+  for (int qq = 0; qq < nac; qq++) {
+    const double *s = K->sig + 9 * qq;
+    for (int k = 0; k < 3; k++) bq[qq][k] = s[k * 3 + 0] * nv[0] + s[k * 3 + 1] * nv[1] + s[k * 3 + 2] * nv[2];
+  }
+  double2 acc_prev = make_double2(0.0, 0.0);
+  for (int pp = 0; pp < nac; pp++) {
+    const double *sp = K->sig + 9 * pp;
+    double av[3], w[3];
+    for (int j = 0; j < 3; j++) av[j] = nv[0] * sp[0 * 3 + j] + nv[1] * sp[1 * 3 + j] + nv[2] * sp[2 * 3 + j];
+    for (int k = 0; k < 3; k++) w[k] = av[0] * om[0 * 3 + k] + av[1] * om[1 * 3 + k] + av[2] * om[2 * 3 + k];
+    double re = 0.0, im = 0.0;
+    for (int qq = 0; qq < nac; qq++) {
+      const double t1 = w[0] * bq[qq][0] + w[1] * bq[qq][1] + w[2] * bq[qq][2];
+      const double B = K->t0[pp * ETD_MAX_VAR + qq] - t1;
+      re += F[qq].x * B;
+      im += F[qq].y * B;
+    }
+    if ((pp & 1) == 0) {
+      acc_prev = make_double2(re, im);
+      if (pp + 1 == nac) Y[N3 * (pp / 2) + m] = acc_prev;
+    } else {
+      // Y_r = acc_2r + i acc_2r+1
+      Y[N3 * (pp / 2) + m] = make_double2(acc_prev.x - im, acc_prev.y + re);
+    }
+  }
+}
+
+__global__ void k_el_unpack(const double2 *__restrict__ Y, double *__restrict__ E, int N, int nac, int ring,
+                            long long Lbase) {
+  const size_t N3 = (size_t)N * N * N;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N3) return;
+  const int r = blockIdx.y;
+  const double inv = 1.0 / ((double)N * N * N);
+  const double2 v = Y[N3 * r + m];
+  int pa = 2 * r, pb = 2 * r + 1;
+  E[(size_t)((Lbase + pa) % ring) * N3 + m] = v.x * inv;
+  if (pb < nac) E[(size_t)((Lbase + pb) % ring) * N3 + m] = v.y * inv;
+}
+
+void launch_elastic(etd_ctx *c) {
+  const int N = c->N;
+  const size_t N3 = c->N3;
+  const unsigned nb = (unsigned)((N3 + 255) / 256);
+  etd_prof_begin(c, PC_ELASTIC);
+  k_el_pack<<<dim3(nb, c->npair), 256, 0, c->stream>>>(c->u, (double2 *)c->Z, c->n, c->g, N, c->nac);
+  fft3d(c, c->Z, c->npair, +1);
+  k_el_contract<<<nb, 256, 0, c->stream>>>((const double2 *)c->Z, (double2 *)c->Y, N, c->nac, c->npair, c->dconst);
+  fft3d(c, c->Y, c->npair, -1);
+  const long long Lbase = c->elastic_evals * (long long)c->nac;
+  k_el_unpack<<<dim3(nb, c->npair), 256, 0, c->stream>>>((const double2 *)c->Y, c->E, N, c->nac, c->ring, Lbase);
+  c->elastic_evals++;
+  etd_prof_end(c, PC_ELASTIC);
+}
+
+__global__ void k_felas_mat(double *felas, const double *E, nl_args a) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int p = blockIdx.y;
+  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  felas[n3 * p + l] = felas_at(E, a, p, k, j, i);
+}
+
+void launch_felas_materialize(etd_ctx *c, double *felas) {
+  nl_args a = make_nl(c);
+  k_felas_mat<<<dim3((unsigned)((c->n3 + 255) / 256), c->nac), 256, 0, c->stream>>>(felas, c->E, a);
+}
+
+// ------------------------------------------------------------------ energy
+// check_soln_new (field.c:407-678): per-plane partial sums in FP64 (wave
+// shuffles, then LDS across the 4 waves), summed in plane order on the host.
+struct diag_args {
+  int n, g, nac, nch;
+  double hx, hy, hz, epn2, a2, a3, a4;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// grid (planes = n - 2g, V): max, min, sum(u + 1) over the plane interior
+__global__ __launch_bounds__(256) void k_diag_stats(const double *__restrict__ u, double *__restrict__ part,
+                                                    diag_args a) {
+  const int n = a.n, g = a.g, m = n - 2 * g;
+  const int k = g + blockIdx.x, v = blockIdx.y;
+  const double *f = u + (size_t)n * n * n * v;
+  double mx = -1.0e30, mn = 1.0e30, s = 0.0;
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+    const int j = g + t / m, i = g + t % m;
+    const double x = f[IDX(n, k, j, i)];
+    mx = fmax(mx, x);
+    mn = fmin(mn, x);
+    s += x + 1.0;
+  }
+  __shared__ double sh[3][4];
+  mx = wave_max(mx);
+  mn = wave_min(mn);
+  s = wave_sum(s);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = mx;
+    sh[1][w] = mn;
+    sh[2][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)v * m + blockIdx.x) * 3;
+    o[0] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    o[1] = fmin(fmin(sh[1][0], sh[1][1]), fmin(sh[1][2], sh[1][3]));
+    o[2] = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
+  }
+}
+
+__device__ __forceinline__ void cell_grad(const double *u, int n, int k, int j, int i, double hx, double hy, double hz,
+                                          double &mean, double &ux, double &uy, double &uz) {
+  double f0 = u[IDX(n, k, j, i)], f1 = u[IDX(n, k, j, i + 1)], f2 = u[IDX(n, k, j + 1, i)],
+         f3 = u[IDX(n, k, j + 1, i + 1)];
+  double f4 = u[IDX(n, k + 1, j, i)], f5 = u[IDX(n, k + 1, j, i + 1)], f6 = u[IDX(n, k + 1, j + 1, i)],
+         f7 = u[IDX(n, k + 1, j + 1, i + 1)];
+  mean = (f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7) / 8;
+  ux = ((f1 - f0) + (f3 - f2) + (f5 - f4) + (f7 - f6)) / 4 / hx;
+  uy = ((f2 - f0) + (f3 - f1) + (f6 - f4) + (f7 - f5)) / 4 / hy;
+  uz = ((f4 - f0) + (f5 - f1) + (f6 - f2) + (f7 - f3)) / 4 / hz;
+}
+
+// grid (planes = n - 2g - 1, 1 + nch + couple): y = 0 AC energy, 1..nch CH
+// energy of field nac + y - 1, nch + 1 couple gradient terms
+__global__ __launch_bounds__(256) void k_diag_energy(const double *__restrict__ u, const double *__restrict__ lam,
+                                                     double *__restrict__ part, diag_args a) {
+  const int n = a.n, g = a.g, m = n - 2 * g - 1;
+  const int k = g + blockIdx.x, kind = blockIdx.y;
+  const size_t n3 = (size_t)n * n * n;
+  double e = 0.0;
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+    const int j = g + t / m, i = g + t % m;
+    if (kind == 0) {
+      double s2 = 0.0, s3 = 0.0;
+      for (int mm = 0; mm < a.nac; mm++) {
+        double um, ux, uy, uz;
+        cell_grad(u + n3 * mm, n, k, j, i, a.hx, a.hy, a.hz, um, ux, uy, uz);
+        const double *L = lam + 9 * mm;
+        const double ffx = ux * L[0] + uy * L[3] + uz * L[6];
+        const double ffy = ux * L[1] + uy * L[4] + uz * L[7];
+        const double ffz = ux * L[2] + uy * L[5] + uz * L[8];
+        e += a.epn2 * (ffx * ux + ffy * uy + ffz * uz) / 2;
+        s2 += um * um;
+        s3 += um * um * um;
+      }
+      e += (a.a2 / 2.0) * s2 - (a.a3 / 3.0) * s3 + (a.a4 / 4.0) * s2 * s2;
+    } else if (kind <= a.nch) {
+      double cm, ux, uy, uz;
+      cell_grad(u + n3 * (a.nac + kind - 1), n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
+      e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+      e += ((cm * cm - 1) * (cm * cm - 1) / 4);
+    } else {
+      for (int q = 0; q < a.nch && q < a.nac; q++) {
+        double cm, ux, uy, uz;
+        cell_grad(u + n3 * q, n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
+        e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+      }
+      for (int q = 0; q < a.nch; q++) {
+        double cm, ux, uy, uz;
+        cell_grad(u + n3 * (a.nac + q), n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
+        e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+      }
+    }
+  }
+  __shared__ double sh[4];
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(size_t)kind * m + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+int launch_diag(etd_ctx *c, double *stats) {
+  diag_args a;
+  a.n = c->n;
+  a.g = c->g;
+  a.nac = c->nac;
+  a.nch = c->nch;
+  a.hx = c->p.hx;
+  a.hy = c->p.hy;
+  a.hz = c->p.hz;
+  a.epn2 = c->p.epn2;
+  a.a2 = c->p.a2;
+  a.a3 = c->p.a3;
+  a.a4 = c->p.a4;
+  const int n = c->n, g = c->g, V = c->V;
+  const int ms = n - 2 * g, me = n - 2 * g - 1;
+  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
+  double *pst = c->part, *pen = c->part + (size_t)3 * ms * V;
+  etd_prof_begin(c, PC_ENERGY);
+  k_diag_stats<<<dim3(ms, V), 256, 0, c->stream>>>(c->u, pst, a);
+  k_diag_energy<<<dim3(me, kinds), 256, 0, c->stream>>>(c->u, c->lam, pen, a);
+  etd_prof_end(c, PC_ENERGY);
+  const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds;
+  if (hipMemcpyAsync(c->hpart, c->part, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    return ETD_ERR_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return ETD_ERR_DEVICE;
+  const double h3 = c->p.hx * c->p.hy * c->p.hz;
+  const double *hs = c->hpart, *he = c->hpart + (size_t)3 * ms * V;
+  for (int v = 0; v < V; v++) {
+    double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
+    for (int k = 0; k < ms; k++) {
+      const double *o = hs + ((size_t)v * ms + k) * 3;
+      if (o[0] > mx) mx = o[0];
+      if (o[1] < mn) mn = o[1];
+      vol += o[2];
+    }
+    stats[4 * v + 0] = mx;
+    stats[4 * v + 1] = mn;
+    stats[4 * v + 2] = vol * h3;
+    stats[4 * v + 3] = 0.0;
+  }
+  if (c->nac > 0) {
+    double e = 0.0;
+    for (int k = 0; k < me; k++) e += he[k];
+    e = e * h3;
+    for (int v = 0; v < c->nac; v++) stats[4 * v + 3] = e;
+  }
+  double engtmp = 0.0;
+  for (int q = 0; q < c->nch; q++) {
+    engtmp = 0.0;
+    for (int k = 0; k < me; k++) engtmp += he[(size_t)(1 + q) * me + k];
+    engtmp = engtmp * h3;
+    stats[4 * (c->nac + q) + 3] = engtmp;
+  }
+  double couple = 0.0;
+  if (c->nac > 0 && c->nch > 0) {
+    couple = engtmp;
+    for (int k = 0; k < me; k++) couple += he[(size_t)(1 + c->nch) * me + k];
+    couple = couple * h3;
+  }
+  stats[4 * V] = couple;
+  return ETD_OK;
+}

@@ -1,0 +1,308 @@
+"""Python host binding of libetd (ctypes over the C ABI in include/etd.h).
+
+Mirrors the reference's step boundary: an Etd context owns the device state
+of one periodic subdomain; step() is the body of the time loop of
+source_beta_to_omega/src/ScLETD.c:84-191 and diag() is check_soln_new
+(source_beta_to_omega/src/field.c:407-678).  Errors raise EtdError with the
+library's message (the reference prints and exits).
+
+The binding never falls back to Python or CPU arithmetic: if libetd.so is
+missing the import-time load raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libetd.so")
+MAX_VAR = 64
+
+_dp = C.POINTER(C.c_double)
+
+
+class EtdError(RuntimeError):
+    pass
+
+
+class EtdParams(C.Structure):
+    """Mirror of etd_params (include/etd.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("px", C.c_int), ("py", C.c_int), ("pz", C.c_int),
+        ("nghost", C.c_int),
+        ("work_dir", C.c_char * 1024), ("data_dir", C.c_char * 1024),
+        ("dt", C.c_double), ("t_total", C.c_double), ("epn2", C.c_double),
+        ("periodic", C.c_int), ("nac", C.c_int), ("nch", C.c_int),
+        ("restart", C.c_int), ("restart_iter", C.c_int), ("restart_dir", C.c_char * 1024),
+        ("xmin", C.c_double), ("ymin", C.c_double), ("zmin", C.c_double),
+        ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("etd2", C.c_int), ("elastic", C.c_int), ("anisotropic", C.c_int),
+        ("kkx", C.c_double), ("kky", C.c_double), ("kkz", C.c_double),
+        ("variable_mobility", C.c_int),
+        ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+        ("LE", C.c_double), ("KE", C.c_double),
+        ("a2", C.c_double), ("a3", C.c_double), ("a4", C.c_double),
+        ("LCI", C.c_double), ("KCI", C.c_double),
+        ("elastic_scale", C.c_double),
+        ("C", C.c_double * 36),
+        ("eps0", C.c_double * (MAX_VAR * 9)),
+        ("lam", C.c_double * (MAX_VAR * 9)),
+        ("precision", C.c_int),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise EtdError(f"{LIB_PATH} is not built (run `make -C energy-stability_amd`)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(EtdParams)
+        ctxp = C.c_void_p
+        sig = {
+            "etd_default_params": (None, [P]),
+            "etd_parse_ini": (C.c_int, [C.c_char_p, P]),
+            "etd_parse_ini_alpha": (C.c_int, [C.c_char_p, P]),
+            "etd_parse_elastic": (C.c_int, [C.c_char_p, P]),
+            "etd_parse_anisotropic": (C.c_int, [C.c_char_p, P]),
+            "etd_parse_landau": (C.c_int, [C.c_char_p, P]),
+            "etd_read_matrices": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp]),
+            "etd_eigbasis": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp]),
+            "etd_create": (C.c_int, [P, _dp, _dp, _dp, C.POINTER(ctxp)]),
+            "etd_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(ctxp)]),
+            "etd_close": (None, [ctxp]),
+            "etd_last_error": (C.c_char_p, [ctxp]),
+            "etd_error_string": (C.c_char_p, [C.c_int]),
+            "etd_get_params": (C.c_int, [ctxp, P]),
+            "etd_set_field": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_get_field": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_init_noise": (C.c_int, [ctxp, C.c_int, C.c_ulonglong, C.c_double, C.c_double]),
+            "etd_step": (C.c_int, [ctxp, C.c_int]),
+            "etd_diag": (C.c_int, [ctxp, _dp]),
+            "etd_write_csv": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_write_field": (C.c_int, [ctxp, C.c_int, C.c_int]),
+            "etd_k_transfer": (C.c_int, [ctxp]),
+            "etd_k_elastic": (C.c_int, [ctxp, _dp]),
+            "etd_k_nonlinear": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_k_modeprod": (C.c_int, [ctxp, C.c_int, _dp, _dp, _dp, C.c_int]),
+            "etd_k_phiscale": (C.c_int, [ctxp, C.c_int, _dp, _dp]),
+            "etd_time_steps": (C.c_int, [ctxp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = lib().etd_error_string(rc).decode()
+        if ctx:
+            detail = lib().etd_last_error(ctx)
+            if detail:
+                msg += ": " + detail.decode()
+        raise EtdError(f"{what}: {msg}")
+
+
+def default_params():
+    p = EtdParams()
+    lib().etd_default_params(C.byref(p))
+    return p
+
+
+def parse_inputs(directory, suffix):
+    """input<suffix>.ini + elastic_input.txt + anisotropic_input.txt (+ landau_input.txt)."""
+    p = default_params()
+    d = os.fspath(directory)
+    _check(lib().etd_parse_ini(os.path.join(d, f"input{suffix}.ini").encode(), C.byref(p)), what="parse_ini")
+    if p.elastic:
+        _check(lib().etd_parse_elastic(os.path.join(d, "elastic_input.txt").encode(), C.byref(p)), what="elastic_input")
+    if p.anisotropic == 1:
+        _check(lib().etd_parse_anisotropic(os.path.join(d, "anisotropic_input.txt").encode(), C.byref(p)),
+               what="anisotropic_input")
+    lp = os.path.join(d, "landau_input.txt")
+    if os.path.exists(lp):
+        _check(lib().etd_parse_landau(lp.encode(), C.byref(p)), what="landau_input")
+    return p
+
+
+def eigbasis(n, bcid=3):
+    V = np.zeros(n * n)
+    Vi = np.zeros(n * n)
+    D = np.zeros(n)
+    _check(lib().etd_eigbasis(n, bcid, _ptr(V), _ptr(Vi), _ptr(D)), what="eigbasis")
+    return V.reshape(n, n).T.copy(), Vi.reshape(n, n).T.copy(), D
+
+
+def params_from_dict(prm):
+    """Build etd_params from the test/bench dictionary form (same keys as the oracle)."""
+    p = default_params()
+    n = prm["n"]
+    p.nx = p.ny = p.nz = n
+    p.nghost = prm.get("g", 2)
+    p.dt = prm["dt"]
+    p.t_total = prm.get("t_total", 1.0)
+    p.epn2 = prm["epn2"]
+    p.nac = prm["nac"]
+    p.nch = prm["nch"]
+    p.etd2 = prm.get("etd2", 0)
+    p.elastic = prm.get("elastic", 0)
+    p.anisotropic = prm.get("aniso", 0)
+    kk = prm.get("kk", (1.0, 1.0, 1.0))
+    p.kkx, p.kky, p.kkz = kk
+    h = prm.get("h", (1.0, 1.0, 1.0))
+    N = n - 2 * p.nghost
+    p.xmin = p.ymin = p.zmin = 0.0
+    p.xmax, p.ymax, p.zmax = h[0] * N, h[1] * N, h[2] * N
+    p.hx, p.hy, p.hz = h
+    p.LE = prm.get("LE", 1.0)
+    p.KE = prm.get("KE", 42.0)
+    if "landau" in prm:
+        p.a2, p.a3, p.a4 = prm["landau"]
+    p.LCI = prm.get("LCI", 1.0)
+    p.KCI = prm.get("KCI", 2.0)
+    p.elastic_scale = prm.get("elastic_scale", 1.0)
+    Cv = np.asarray(prm.get("C", np.zeros(36)), dtype=np.float64).reshape(-1)
+    for i in range(36):
+        p.C[i] = Cv[i]
+    nac = p.nac
+    if nac:
+        eps = np.asarray(prm.get("eps0", np.zeros((nac, 3, 3))), dtype=np.float64).reshape(-1)[: 9 * nac]
+        for i, x in enumerate(eps):
+            p.eps0[i] = x
+        lam = prm.get("lam")
+        if lam is None:
+            lam = np.array([np.diag(kk)] * nac)
+        lam = np.asarray(lam, dtype=np.float64).reshape(-1)[: 9 * nac]
+        for i, x in enumerate(lam):
+            p.lam[i] = x
+    p.precision = prm.get("precision", 0)
+    return p
+
+
+class Etd:
+    """One periodic subdomain on the GPU."""
+
+    def __init__(self, params, V=None, Vi=None, D=None):
+        self.ctx = C.c_void_p()
+        self.p = params
+        self.n = params.nx
+        self.V = params.nac + params.nch
+        if V is not None:
+            Vc = np.ascontiguousarray(np.asarray(V).T).reshape(-1)
+            Vic = np.ascontiguousarray(np.asarray(Vi).T).reshape(-1)
+            Dc = np.ascontiguousarray(np.asarray(D, dtype=np.float64))
+            rc = lib().etd_create(C.byref(params), _ptr(Vc), _ptr(Vic), _ptr(Dc), C.byref(self.ctx))
+        else:
+            rc = lib().etd_create(C.byref(params), None, None, None, C.byref(self.ctx))
+        if rc != 0:
+            ctx = self.ctx if self.ctx else None
+            err = lib().etd_last_error(ctx).decode() if ctx else ""
+            if ctx:
+                lib().etd_close(ctx)
+            self.ctx = None
+            raise EtdError(f"etd_create: {lib().etd_error_string(rc).decode()} {err}")
+
+    @classmethod
+    def from_dict(cls, prm):
+        return cls(params_from_dict(prm))
+
+    @classmethod
+    def open(cls, directory, suffix):
+        obj = cls.__new__(cls)
+        obj.ctx = C.c_void_p()
+        rc = lib().etd_open(os.fspath(directory).encode(), str(suffix).encode(), C.byref(obj.ctx))
+        if rc != 0:
+            raise EtdError(f"etd_open: {lib().etd_error_string(rc).decode()}")
+        obj.p = EtdParams()
+        lib().etd_get_params(obj.ctx, C.byref(obj.p))
+        obj.n = obj.p.nx
+        obj.V = obj.p.nac + obj.p.nch
+        return obj
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().etd_close(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # fields
+    def set_field(self, var, data):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
+        if a.size != self.n ** 3:
+            raise EtdError("set_field: wrong size")
+        _check(lib().etd_set_field(self.ctx, var, _ptr(a)), self.ctx, "set_field")
+
+    def get_field(self, var):
+        a = np.empty(self.n ** 3)
+        _check(lib().etd_get_field(self.ctx, var, _ptr(a)), self.ctx, "get_field")
+        return a.reshape(self.n, self.n, self.n)
+
+    def init_noise(self, var, seed, mean, amp):
+        _check(lib().etd_init_noise(self.ctx, var, seed, mean, amp), self.ctx, "init_noise")
+
+    # the step and its parts
+    def step(self, nsteps=1):
+        _check(lib().etd_step(self.ctx, nsteps), self.ctx, "step")
+
+    def diag(self):
+        st = np.zeros(4 * self.V + 1)
+        _check(lib().etd_diag(self.ctx, _ptr(st)), self.ctx, "diag")
+        return st[:-1].reshape(self.V, 4), st[-1]
+
+    def transfer(self):
+        _check(lib().etd_k_transfer(self.ctx), self.ctx, "transfer")
+
+    def elastic(self, want=True):
+        if want:
+            out = np.empty(self.p.nac * self.n ** 3)
+            _check(lib().etd_k_elastic(self.ctx, _ptr(out)), self.ctx, "elastic")
+            return out.reshape(self.p.nac, self.n, self.n, self.n)
+        _check(lib().etd_k_elastic(self.ctx, None), self.ctx, "elastic")
+        return None
+
+    def nonlinear(self, var):
+        out = np.empty(self.n ** 3)
+        _check(lib().etd_k_nonlinear(self.ctx, var, _ptr(out)), self.ctx, "nonlinear")
+        return out.reshape(self.n, self.n, self.n)
+
+    def modeprod(self, A, u, rotate=True):
+        n = self.n
+        Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T).reshape(-1)
+        inp = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        out = np.empty(n ** 3)
+        _check(lib().etd_k_modeprod(self.ctx, n, _ptr(Acm), _ptr(inp), _ptr(out), 1 if rotate else 0),
+               self.ctx, "modeprod")
+        return out
+
+    def phiscale(self, var, uhat, nhat):
+        u = np.ascontiguousarray(np.asarray(uhat, dtype=np.float64).reshape(-1)).copy()
+        nh = np.ascontiguousarray(np.asarray(nhat, dtype=np.float64).reshape(-1))
+        _check(lib().etd_k_phiscale(self.ctx, var, _ptr(u), _ptr(nh)), self.ctx, "phiscale")
+        return u
+
+    def time_steps(self, nsteps, with_diag=True):
+        ms = np.zeros(7)
+        launches = (C.c_int * 6)()
+        _check(lib().etd_time_steps(self.ctx, nsteps, 1 if with_diag else 0, _ptr(ms), launches), self.ctx,
+               "time_steps")
+        return ms, list(launches)

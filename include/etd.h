@@ -1,0 +1,180 @@
+/*
+ * etd.h -- C ABI of libetd, the MI355X (gfx950) implementation of the ScLETD
+ * exponential-time-differencing step of moxiaoan/Energy-stability.
+ *
+ * The reference has no plugin or FFI: its time-step loop
+ * (source_beta_to_omega/src/ScLETD.c:84-191) calls a fixed sequence of void
+ * functions on process-global state.  Each entry point below replaces one of
+ * those functions (cited per declaration) or the init chain around them, with
+ * explicit state (etd_ctx), plain pointers and sizes, int return codes
+ * (0 = OK, negative = error) and etd_last_error() instead of printf+exit.
+ *
+ * Data layout at the boundary: one field = n^3 doubles, index (k*n + j)*n + i
+ * with i (x) fastest, ghost cells included, exactly the reference's
+ * ac[v].fieldE / ch[v].fieldCI arrays (source_beta_to_omega/src/vars.c:175).
+ * Variables are numbered AC fields first (0..nac-1), then CH (nac..nac+nch-1).
+ * Matrices use the reference's in-memory layout: A(r,c) at A[r + n*c].
+ *
+ * Threading: an etd_ctx owns its device buffers and one HIP stream; it is not
+ * thread-safe.  Several contexts may share a device.
+ */
+#ifndef ETD_H
+#define ETD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ETD_MAX_VAR 64
+
+enum {
+  ETD_OK = 0,
+  ETD_ERR_ARG = -1,      /* invalid argument */
+  ETD_ERR_IO = -2,       /* missing / malformed input file */
+  ETD_ERR_UNSUPPORTED = -3,
+  ETD_ERR_DEVICE = -4,   /* HIP runtime failure */
+  ETD_ERR_NOMEM = -5
+};
+
+/* precision of the step */
+enum { ETD_PREC_FP64 = 0, ETD_PREC_MIXED = 1 };
+
+/* Everything the reference reads from input<s>.ini (ScLETD.c:18-34),
+ * elastic_input.txt (elastic.c:375-418), anisotropic_input.txt
+ * (anisotropic.c:645-673) and the constants hard-coded in formula.c:9-29. */
+typedef struct etd_params {
+  int nx, ny, nz;          /* extended edge (must be equal) */
+  int px, py, pz;          /* process grid (must be 1,1,1: one subdomain) */
+  int nghost;
+  char work_dir[1024];
+  char data_dir[1024];
+  double dt, t_total;
+  double epn2;
+  int periodic;            /* must be 1 */
+  int nac, nch;
+  int restart, restart_iter;
+  char restart_dir[1024];
+  double xmin, ymin, zmin, xmax, ymax, zmax;
+  int etd2;
+  int elastic;             /* 0 none, 1 AC (2 CH / 3 both: unsupported) */
+  int anisotropic;
+  double kkx, kky, kkz;
+  int variable_mobility;   /* must be 0 */
+  /* derived / hard-coded in the reference */
+  double hx, hy, hz;       /* (max-min)/gnx, vars.c:153-155 */
+  double LE, KE;           /* formula.c:9-18: 1, 42 */
+  double a2, a3, a4;       /* formula.c:126-128 */
+  double LCI, KCI;         /* CH mobility / stabiliser (unset in the reference) */
+  double elastic_scale;
+  double C[36];            /* Voigt 6x6, row-major */
+  double eps0[ETD_MAX_VAR * 9];
+  double lam[ETD_MAX_VAR * 9];  /* Lambda = diag(kk) - lambda_in */
+  int precision;           /* ETD_PREC_FP64 or ETD_PREC_MIXED */
+} etd_params;
+
+typedef struct etd_ctx etd_ctx;
+
+/* Fills *p with the reference's defaults (ω constants of formula.c:9-29). */
+void etd_default_params(etd_params *p);
+
+/* Replaces the positional fscanf block of main (ScLETD.c:16-35), including
+ * CRLF tolerance.  Derives hx,hy,hz as init_para does (vars.c:58-157). */
+int etd_parse_ini(const char *path, etd_params *p);
+
+/* The beta->alpha ini order (source_beta_to_alpha/src/ScLETD.cpp:15-31;
+ * epn2 = 0.048 and ETD1 as that driver sets them). */
+int etd_parse_ini_alpha(const char *path, etd_params *p);
+
+/* Replaces elastic_input (elastic.c:349-438): ElasticScale, 36 Voigt
+ * entries, then nac 3x3 eigenstrains. */
+int etd_parse_elastic(const char *path, etd_params *p);
+
+/* Replaces anisotropic_input (anisotropic.c:630-674): nac 3x3 lambda, then
+ * Lambda = diag(kk) - lambda. */
+int etd_parse_anisotropic(const char *path, etd_params *p);
+
+/* Optional extension file landau_input.txt: "LE KE a2 a3 a4 [LCI KCI]"
+ * (the reference hard-codes these in formula.c:9-29). */
+int etd_parse_landau(const char *path, etd_params *p);
+
+/* Replaces read_matrices' file reads (vars.c:370-476): d<id><n>.dat (n
+ * values), v<id><n>.dat and vi<id><n>.dat (n*n values, row-major file,
+ * stored column-major).  Returns ETD_ERR_IO when a file is missing. */
+int etd_read_matrices(const char *dir, int n, int id, double *V, double *Vi, double *D);
+
+/* Closed-form eigenbasis of the truncated second-difference matrix for
+ * boundary id 0..3 (what the .dat files hold). */
+int etd_eigbasis(int n, int id, double *V, double *Vi, double *D);
+
+/* Creates a context: allocates device state, uploads the eigenbasis (V, Vi,
+ * D may be NULL: closed form for id 3), builds the phi tables
+ * (read_matrices -> ac_init_phi / ch_init_phi, vars.c:302-359,583-590).
+ * Replaces alloc_vars + init_vars + init_para + read_matrices +
+ * elastic_init (vars.c:166-300, elastic.c:607-613). */
+int etd_create(const etd_params *p, const double *V, const double *Vi, const double *D, etd_ctx **out);
+
+/* Parses <dir>/input<suffix>.ini, elastic_input.txt, anisotropic_input.txt
+ * (and landau_input.txt when present), reads d/v/vi<3><n>.dat from dir when
+ * present, then etd_create. */
+int etd_open(const char *dir, const char *suffix, etd_ctx **out);
+
+void etd_close(etd_ctx *c);
+const char *etd_last_error(const etd_ctx *c);
+const char *etd_error_string(int code);
+int etd_get_params(const etd_ctx *c, etd_params *p);
+
+/* Host <-> device copies of one field (n^3 doubles, ghosts included). */
+int etd_set_field(etd_ctx *c, int var, const double *host);
+int etd_get_field(etd_ctx *c, int var, double *host);
+
+/* Seeded initial state: interior U[mean-amp, mean+amp) from splitmix64
+ * ((x >> 11) * 2^-53), ghosts as periodic images.  Replaces init_field
+ * (field.c:742-777) for the synthetic workloads. */
+int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, double mean, double amp);
+
+/* One or more full time steps: transfer, elastic_calculate, calc_mu,
+ * ac_calc_FU / ch_calc_FU, mode products, phi update, and the ETD2
+ * corrector (ScLETD.c:87-189).  Does not run the diagnostics. */
+int etd_step(etd_ctx *c, int nsteps);
+
+/* check_soln_new (field.c:407-678) without file output:
+ * stats[4*v + {0,1,2,3}] = max, min, vol, eng for each variable v, then
+ * stats[4*(nac+nch)] = couple energy (0 unless nac > 0 and nch > 0). */
+int etd_diag(etd_ctx *c, double *stats);
+
+/* Appends the CSV rows of check_soln_new (phi_ac_<v>.csv, phi_ch_<v>.csv,
+ * phi_couple.csv, format "%d,%+1.15lf,...") under work_dir; iter == 0 also
+ * writes the headers. */
+int etd_write_csv(etd_ctx *c, int iter, const double *stats);
+
+/* Writes <data_dir>eta<v>_<ioutput:06d>_000000.dat, text %+1.15lf, full n^3
+ * (outputfield, field.c:17-56). */
+int etd_write_field(etd_ctx *c, int var, int ioutput);
+
+/* ---- per-kernel entry points (unit parity; host buffers in/out) ---- */
+
+/* transfer() (trans.c:1939-1996): periodic ghost fill of every field. */
+int etd_k_transfer(etd_ctx *c);
+/* elastic_calculate (elastic.c:689-762); felas_p copied to host_felas
+ * (nac * n^3 doubles) when non-NULL. */
+int etd_k_elastic(etd_ctx *c, double *host_felas);
+/* ac_calc_FU / ch_calc_FU (calc.c:8-68,475-487) for variable var on the
+ * current device state (after etd_k_transfer / etd_k_elastic). */
+int etd_k_nonlinear(etd_ctx *c, int var, double *host_out);
+/* PUX/PUY/PUZ (calc.c:631-684): out = A x_1 in, optionally rotated
+ * (flat (a,b,c) -> (b,c,a)); A column-major n x n; n^3 in/out. */
+int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *in, double *out, int rotate);
+/* ac_updateU_new / ch_updateU_new (calc.c:489-535) on spectral data in
+ * the rotated (x'y'z', x' fastest) layout. */
+int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat);
+
+/* ---- timing: runs nsteps on the context stream between HIP events ---- */
+/* ms[0] = total, ms[1] = mode-product kernels, ms[2] = elastic (FFT +
+ * contraction), ms[3] = nonlinear, ms[4] = phi scaling, ms[5] = energy,
+ * ms[6] = halo; launches[] counts the dominant kernel's launches. */
+int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
