@@ -422,6 +422,21 @@ extern "C" int etd_k_elastic(etd_ctx *c, double *host_felas) {
   return ETD_OK;
 }
 
+extern "C" int etd_k_fft3d(etd_ctx *c, double *data, int dir) {
+  if (!c || !data || (dir != 1 && dir != -1)) return ETD_ERR_ARG;
+  if (!c->tw) return seterr(c, ETD_ERR_ARG, "fft3d: the context has no elastic FFT plan");
+  double *d;
+  const size_t bytes = sizeof(double) * 2 * c->N3;
+  HCHK(c, hipMalloc(&d, bytes));
+  HCHK(c, hipMemcpy(d, data, bytes, hipMemcpyHostToDevice));
+  fft3d(c, d, 1, dir);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  HCHK(c, hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost));
+  (void)hipFree(d);
+  return ETD_OK;
+}
+
 extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
   if (!c || !host_out || var < 0 || var >= c->V) return ETD_ERR_ARG;
   launch_nonlinear(c, c->N1, nullptr);

@@ -169,7 +169,8 @@ __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__
     }
     __syncthreads();
   }
-  // D(16x16): lane holds rows 4q..4q+3 of column r16
+  // D(16x16) of v_mfma_f64_16x16x4f64: register rr of lane (q, r16) holds
+  // row q + 4 rr, column r16
 #pragma unroll
   for (int t = 0; t < MT; t++)
 #pragma unroll
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__
       if (col >= n2) continue;
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int m = 16 * t + 4 * q + rr;
+        const int m = 16 * t + q + 4 * rr;
         if (m >= n) continue;
         const double val = acc[t][u][rr];
         const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
@@ -677,7 +678,7 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int 
   }
 }
 
-static void fft3d(etd_ctx *c, double *data, int npair, int sgn) {
+void fft3d(etd_ctx *c, double *data, int npair, int sgn) {
   fft_plan pl;
   pl.N = c->N;
   pl.npass = c->fft_npass;
@@ -709,39 +710,31 @@ __global__ void k_el_pack(const double *__restrict__ u, double2 *__restrict__ Z,
 }
 
 // k-space contraction acc_p = sum_q B_pq(n) F_q with B computed on the fly
-// (elastic_calculate_BN, elastic.c:460-605: n from signed wavenumbers, k = 0
-// uses n = (0,1,0); Omega = (C n n)^-1 by the adjugate as inverse_3x3).
+// (elastic_calculate_BN, elastic.c:460-605: n from signed wavenumbers
+// g < N/2 ? g : g - N, k = 0 uses n = (0,1,0); Omega = (C n n)^-1 by the
+// adjugate as inverse_3x3).  The reference keeps Re IFFT(B F)
+// (fft3d.c:133-136).  That signed-wavenumber rule is not odd under k -> -k
+// on the Nyquist planes (and next to them for odd N), so B F is not
+// Hermitian there; Re IFFT(X) = IFFT((X(k) + conj X(-k)) / 2), so the
+// contraction uses the Hermitian part B~ = (B(k) + B(-k)) / 2 wherever the
+// mirror's signed wavenumber is not the negative, and the two packed
+// variants of a pair then come back exactly as the real and imaginary parts.
 #define EL_MAXV 16
-__global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
-                                                     int nac, int npair, const etd_dev_consts *__restrict__ K) {
-  const size_t N3 = (size_t)N * N * N;
-  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= N3) return;
-  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
-  const size_t mm = ((size_t)((N - z) % N) * N + (N - y) % N) * N + (N - x) % N;
-  double2 F[EL_MAXV];
-  for (int r = 0; r < npair; r++) {
-    const double2 a = Z[N3 * r + m], b = Z[N3 * r + mm];
-    // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
-    F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-    if (2 * r + 1 < nac) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
-  }
-  const int cnt = N / 2;
-  const double n1 = x < cnt ? 1.0 * x : -1.0 * (N - x);
-  const double n2 = y < cnt ? 1.0 * y : -1.0 * (N - y);
-  const double n3_ = z < cnt ? 1.0 * z : -1.0 * (N - z);
-  double nv[3];
-  if (x == 0 && y == 0 && z == 0) {
+__device__ __forceinline__ int signed_wave(int x, int N) { return x < N / 2 ? x : x - N; }
+
+__device__ void b_geometry(int sx, int sy, int sz, const etd_dev_consts *__restrict__ K, double *nv, double *om) {
+  if (sx == 0 && sy == 0 && sz == 0) {
     nv[0] = 0.0;
     nv[1] = 1.0;
     nv[2] = 0.0;
   } else {
-    const double rr = sqrt(n1 * n1 + n2 * n2 + n3_ * n3_);
+    const double n1 = sx, n2 = sy, n3 = sz;
+    const double rr = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
     nv[0] = n1 / rr;
     nv[1] = n2 / rr;
-    nv[2] = n3_ / rr;
+    nv[2] = n3 / rr;
   }
-  double oi[9], om[9];
+  double oi[9];
   for (int i = 0; i < 3; ++i)
     for (int k = 0; k < 3; ++k) {
       double s = 0.0;
@@ -749,21 +742,49 @@ __global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__
         for (int l = 0; l < 3; ++l) s += K->C4[((i * 3 + j) * 3 + k) * 3 + l] * nv[j] * nv[l];
       oi[i * 3 + k] = s;
     }
-  {
-    double nr = 0.0;
-    nr += oi[0] * (oi[4] * oi[8] - oi[5] * oi[7]);
-    nr -= oi[1] * (oi[3] * oi[8] - oi[5] * oi[6]);
-    nr += oi[2] * (oi[3] * oi[7] - oi[4] * oi[6]);
-    om[0] = (oi[4] * oi[8] - oi[5] * oi[7]) / nr;
-    om[1] = -(oi[3] * oi[8] - oi[6] * oi[5]) / nr;
-    om[2] = (oi[3] * oi[7] - oi[6] * oi[4]) / nr;
-    om[3] = -(oi[1] * oi[8] - oi[7] * oi[2]) / nr;
-    om[4] = (oi[0] * oi[8] - oi[2] * oi[6]) / nr;
-    om[5] = -(oi[0] * oi[7] - oi[6] * oi[1]) / nr;
-    om[6] = (oi[1] * oi[5] - oi[4] * oi[2]) / nr;
-    om[7] = -(oi[0] * oi[5] - oi[3] * oi[2]) / nr;
-    om[8] = (oi[0] * oi[4] - oi[1] * oi[3]) / nr;
+  double nr = 0.0;
+  nr += oi[0] * (oi[4] * oi[8] - oi[5] * oi[7]);
+  nr -= oi[1] * (oi[3] * oi[8] - oi[5] * oi[6]);
+  nr += oi[2] * (oi[3] * oi[7] - oi[4] * oi[6]);
+  om[0] = (oi[4] * oi[8] - oi[5] * oi[7]) / nr;
+  om[1] = -(oi[3] * oi[8] - oi[6] * oi[5]) / nr;
+  om[2] = (oi[3] * oi[7] - oi[6] * oi[4]) / nr;
+  om[3] = -(oi[1] * oi[8] - oi[7] * oi[2]) / nr;
+  om[4] = (oi[0] * oi[8] - oi[2] * oi[6]) / nr;
+  om[5] = -(oi[0] * oi[7] - oi[6] * oi[1]) / nr;
+  om[6] = (oi[1] * oi[5] - oi[4] * oi[2]) / nr;
+  om[7] = -(oi[0] * oi[5] - oi[3] * oi[2]) / nr;
+  om[8] = (oi[0] * oi[4] - oi[1] * oi[3]) / nr;
+}
+
+// w = (sigma_p^T n) Omega for variant p
+__device__ __forceinline__ void b_wrow(const double *sp, const double *nv, const double *om, double *w) {
+  double av[3];
+  for (int j = 0; j < 3; j++) av[j] = nv[0] * sp[0 * 3 + j] + nv[1] * sp[1 * 3 + j] + nv[2] * sp[2 * 3 + j];
+  for (int k = 0; k < 3; k++) w[k] = av[0] * om[0 * 3 + k] + av[1] * om[1 * 3 + k] + av[2] * om[2 * 3 + k];
+}
+
+__global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+                                                     int nac, int npair, const etd_dev_consts *__restrict__ K) {
+  const size_t N3 = (size_t)N * N * N;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N3) return;
+  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
+  const size_t mm = ((size_t)zm * N + ym) * N + xm;
+  double2 F[EL_MAXV];
+  for (int r = 0; r < npair; r++) {
+    const double2 a = Z[N3 * r + m], b = Z[N3 * r + mm];
+    // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
+    F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    if (2 * r + 1 < nac) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
   }
+  const int sx = signed_wave(x, N), sy = signed_wave(y, N), sz = signed_wave(z, N);
+  const int sxm = signed_wave(xm, N), sym = signed_wave(ym, N), szm = signed_wave(zm, N);
+  const bool herm = (sxm == -sx) && (sym == -sy) && (szm == -sz);
+  double nv[3], om[9], nv2[3], om2[9];
+  b_geometry(sx, sy, sz, K, nv, om);
+  if (!herm) b_geometry(sxm, sym, szm, K, nv2, om2);
   double bq[EL_MAXV][3];
   for (int qq = 0; qq < nac; qq++) {
     const double *s = K->sig + 9 * qq;
@@ -772,13 +793,20 @@ __global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__
   double2 acc_prev = make_double2(0.0, 0.0);
   for (int pp = 0; pp < nac; pp++) {
     const double *sp = K->sig + 9 * pp;
-    double av[3], w[3];
-    for (int j = 0; j < 3; j++) av[j] = nv[0] * sp[0 * 3 + j] + nv[1] * sp[1 * 3 + j] + nv[2] * sp[2 * 3 + j];
-    for (int k = 0; k < 3; k++) w[k] = av[0] * om[0 * 3 + k] + av[1] * om[1 * 3 + k] + av[2] * om[2 * 3 + k];
+    double w[3], w2[3];
+    b_wrow(sp, nv, om, w);
+    if (!herm) b_wrow(sp, nv2, om2, w2);
     double re = 0.0, im = 0.0;
     for (int qq = 0; qq < nac; qq++) {
+      const double t0 = K->t0[pp * ETD_MAX_VAR + qq];
       const double t1 = w[0] * bq[qq][0] + w[1] * bq[qq][1] + w[2] * bq[qq][2];
-      const double B = K->t0[pp * ETD_MAX_VAR + qq] - t1;
+      double B = t0 - t1;
+      if (!herm) {
+        const double *s = K->sig + 9 * qq;
+        double t1m = 0.0;
+        for (int k = 0; k < 3; k++) t1m += w2[k] * (s[k * 3 + 0] * nv2[0] + s[k * 3 + 1] * nv2[1] + s[k * 3 + 2] * nv2[2]);
+        B = 0.5 * (B + (t0 - t1m));
+      }
       re += F[qq].x * B;
       im += F[qq].y * B;
     }

@@ -89,6 +89,7 @@ def lib():
             "etd_k_transfer": (C.c_int, [ctxp]),
             "etd_k_elastic": (C.c_int, [ctxp, _dp]),
             "etd_k_nonlinear": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_k_fft3d": (C.c_int, [ctxp, _dp, C.c_int]),
             "etd_k_modeprod": (C.c_int, [ctxp, C.c_int, _dp, _dp, _dp, C.c_int]),
             "etd_k_phiscale": (C.c_int, [ctxp, C.c_int, _dp, _dp]),
             "etd_time_steps": (C.c_int, [ctxp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]),
@@ -279,6 +280,13 @@ class Etd:
             return out.reshape(self.p.nac, self.n, self.n, self.n)
         _check(lib().etd_k_elastic(self.ctx, None), self.ctx, "elastic")
         return None
+
+    def fft3d(self, x, direction):
+        """In-place 3D FFT of an N^3 complex array on the GPU (unscaled)."""
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+        buf = a.view(np.float64).reshape(-1)
+        _check(lib().etd_k_fft3d(self.ctx, _ptr(buf), int(direction)), self.ctx, "fft3d")
+        return a
 
     def nonlinear(self, var):
         out = np.empty(self.n ** 3)

@@ -158,6 +158,11 @@ int etd_k_transfer(etd_ctx *c);
 /* elastic_calculate (elastic.c:689-762); felas_p copied to host_felas
  * (nac * n^3 doubles) when non-NULL. */
 int etd_k_elastic(etd_ctx *c, double *host_felas);
+/* The 3D FFT of the elastic path (replaces fft_forward / fft_backward,
+ * fft3d.c:98-138) on N^3 interleaved complex values, N = n - 2 nghost, in
+ * place; dir = +1 forward exp(-i..), -1 backward exp(+i..), both unscaled.
+ * Needs a context created with ELASTIC = 1. */
+int etd_k_fft3d(etd_ctx *c, double *data, int dir);
 /* ac_calc_FU / ch_calc_FU (calc.c:8-68,475-487) for variable var on the
  * current device state (after etd_k_transfer / etd_k_elastic). */
 int etd_k_nonlinear(etd_ctx *c, int var, double *host_out);

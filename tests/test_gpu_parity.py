@@ -84,6 +84,21 @@ def test_nonlinear_ch_matches_oracle():
     assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n", [10, 12, 20, 36, 52])
+def test_fft3d_matches_numpy(n):
+    """Kernel (d): own mixed-radix FFT (N = 6, 8, 16, 32, 48) vs numpy."""
+    N = n - 4
+    say(f"fft3d N={N}")
+    g = pyetd.Etd.from_dict(small_params(n=n, nac=2, elastic=1))
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    f = g.fft3d(x, 1)
+    ref = np.fft.fftn(x)
+    assert np.abs(f - ref).max() < 1e-12 * np.abs(ref).max()
+    b = g.fft3d(ref, -1)
+    assert np.abs(b / N ** 3 - x).max() < 1e-13
+
+
 @pytest.mark.parametrize("nac,n", [(2, 12), (3, 16), (5, 22)])
 def test_elastic_matches_oracle_over_two_evaluations(nac, n):
     """felas on every extended cell, including the ghost cells that inherit
