@@ -129,6 +129,15 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   } else if ((rc = etd_eigbasis(n, 3, c->hV, c->hVi, c->hD)) != ETD_OK) {
     return seterr(c, rc, "eigenbasis");
   }
+  c->mp_variant = 1;  // direct-to-LDS pipelined 4x4x4 kernel, two workgroups per CU
+  const char *mv = getenv("ETD_MFMA");
+  if (mv) {
+    const int v = atoi(mv);
+    if (v == 0 || v == 1 || v == 2 || v == 4 || v == 16) c->mp_variant = v;
+  }
+  c->mp_nb = 8;
+  const char *nbv = getenv("ETD_MFMA_NB");
+  if (nbv && atoi(nbv) == 4) c->mp_nb = 4;
   int dev = 0;
   const char *ev = getenv("ETD_DEVICE");
   if (ev) dev = atoi(ev);
@@ -215,7 +224,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   }
   // energy partials
   const int ms = n - 2 * c->g, me = n - 2 * c->g - 1;
-  c->npart = 3 * ms * V + me * (2 + c->nch);
+  c->npart = 3 * ms * V + me * (2 + c->nch) * 8;  // 8 row blocks per energy plane
   HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
   HCHK(c, hipHostMalloc(&c->hpart, sizeof(double) * c->npart, hipHostMallocDefault));
   HCHK(c, hipStreamSynchronize(c->stream));

@@ -53,6 +53,8 @@ struct etd_ctx {
   double *hpart;           /* pinned host copy */
   int npart;
   /* profiling */
+  int mp_variant;          /* 0..2: direct-to-LDS 4x4x4 (1 is the default), 4: register-staged 4x4x4, 16: 16x16x4 */
+  int mp_nb;               /* column groups per wave of the 4x4x4 kernel (4 or 8) */
   int prof;                /* record per-class event pairs */
   hipEvent_t *ev;
   int *ev_class;

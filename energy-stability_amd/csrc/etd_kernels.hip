@@ -7,6 +7,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <stdint.h>
+#include <type_traits>
 #include <utility>
 
 #include "etd_internal.h"
@@ -77,10 +79,10 @@ void launch_ghost_fill(etd_ctx *c, double *base, int nfields, size_t fstride) {
 // FP64 MFMA 16x16x4: a wave owns 16*NB columns and all MT = ceil(n/16) row
 // tiles.  K runs in chunks of 16; inside a chunk lane group q = lane/16
 // supplies k = 16 kc + 4 q + s at MFMA step s, so each lane's four B values
-// are contiguous in memory (one 32-byte read) and its A values are a
-// pre-arranged double4.  A chunks are staged through LDS (double-buffered)
-// and shared by the workgroup's 4 waves; B is read straight from HBM one
-// chunk ahead.
+// are contiguous in memory (one 32-byte read).  A chunks, pre-arranged as
+// [s][t][lane] rows of 64 doubles (conflict-free LDS reads), are staged
+// through LDS (double-buffered) and shared by the workgroup's 4 waves; B is
+// read straight from HBM one chunk ahead.
 template <int MT, int NB, bool VEC>
 __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__ A4, const double *__restrict__ in,
                                                       size_t in_fs, double *__restrict__ out, size_t out_fs, int n,
@@ -126,14 +128,21 @@ __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__
   };
   // register staging of the next A chunk (fold expressions keep every index a
   // compile-time constant so the array lives in VGPRs)
-  double2 aprf[APT];
+  // staging of the next A chunk in a vector register tuple (a plain array
+  // here was kept in scratch memory, which serialised every prefetch)
+  typedef double stage_t __attribute__((ext_vector_type(2 * APT)));
+  stage_t aprf;
   constexpr auto AIDX = std::make_index_sequence<APT>{};
   auto fetch_a = [&](int kc, auto seq) {
     const double2 *g2 = reinterpret_cast<const double2 *>(A4) + (size_t)kc * (APT * 256);
-    [&]<size_t... I>(std::index_sequence<I...>) { ((aprf[I] = g2[threadIdx.x + 256 * I]), ...); }(seq);
+    [&]<size_t... I>(std::index_sequence<I...>) {
+      ((aprf[2 * I] = g2[threadIdx.x + 256 * I].x, aprf[2 * I + 1] = g2[threadIdx.x + 256 * I].y), ...);
+    }(seq);
   };
   auto store_a = [&](int b, auto seq) {
-    [&]<size_t... I>(std::index_sequence<I...>) { ((sA2[b][threadIdx.x + 256 * I] = aprf[I]), ...); }(seq);
+    [&]<size_t... I>(std::index_sequence<I...>) {
+      ((sA2[b][threadIdx.x + 256 * I] = make_double2(aprf[2 * I], aprf[2 * I + 1])), ...);
+    }(seq);
   };
 
   v4d bcur[NB], bnext[NB];
@@ -150,16 +159,16 @@ __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__
 #pragma unroll
       for (int u = 0; u < NB; u++) bnext[u] = loadB(kc + 1, u);
     }
+    // step s outermost: the 4 * MT * NB MFMAs of a chunk cycle through MT * NB
+    // independent accumulators, so no MFMA waits on its predecessor
+    const double *sa = reinterpret_cast<const double *>(&sA2[buf][0]);
 #pragma unroll
-    for (int t = 0; t < MT; t++) {
-      const double2 alo = sA2[buf][2 * (t * 64 + lane)], ahi = sA2[buf][2 * (t * 64 + lane) + 1];
-      const v4d a = (v4d){alo.x, alo.y, ahi.x, ahi.y};
+    for (int st = 0; st < 4; st++) {
 #pragma unroll
-      for (int u = 0; u < NB; u++) {
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bcur[u].x, acc[t][u], 0, 0, 0);
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bcur[u].y, acc[t][u], 0, 0, 0);
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.z, bcur[u].z, acc[t][u], 0, 0, 0);
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.w, bcur[u].w, acc[t][u], 0, 0, 0);
+      for (int t = 0; t < MT; t++) {
+        const double a = sa[(st * MT + t) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < NB; u++) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[u][st], acc[t][u], 0, 0, 0);
       }
     }
     if (more) {
@@ -189,7 +198,270 @@ __global__ __launch_bounds__(256) void k_modeprod_f64(const double *__restrict__
     }
 }
 
-// host: A(m,k) column-major -> A4[kc*CH + (t*64 + lane)*4 + s]
+// Same contraction on v_mfma_f64_4x4x4f64, which this part issues at about
+// 1.5x the FP64 rate of the 16x16x4 form.  The instruction holds four
+// independent 4x4x4 products; lane = x + 4 blk + 16 y addresses A[x][y],
+// B[y][x] and D[y][x] of block blk.  Blocks are stacked along rows, so one
+// instruction covers rows 16t + lane%16 (the same A staging as above) and the
+// 4 columns colbase + 4u + lane%4; a wave owns 4*NB4 columns.
+template <int MT, int NB4, bool VEC>
+__global__ __launch_bounds__(256) void k_modeprod_f64m4(const double *__restrict__ A4, const double *__restrict__ in,
+                                                        size_t in_fs, double *__restrict__ out, size_t out_fs, int n,
+                                                        int rotate, int accumulate) {
+  constexpr int A2 = MT * 128;
+  constexpr int APT = (A2 + 255) / 256;
+  __shared__ double2 sA2[2][APT * 256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, j4 = lane & 3;
+  const double *src = in + in_fs * blockIdx.y;
+  double *dst = out + out_fs * blockIdx.y;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 15) / 16;
+  // 4 waves x 4*NB4 columns per workgroup
+  const long wcol = (long)blockIdx.x * (16 * NB4) + wave * (4 * NB4);
+  double acc[MT][NB4];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB4; u++) acc[t][u] = 0.0;
+
+  auto loadB = [&](int kc, int u) -> v4d {
+    const long col = wcol + 4 * u + j4;
+    const int k0 = 16 * kc + 4 * q;
+    v4d v = (v4d){0.0, 0.0, 0.0, 0.0};
+    if (col < n2) {
+      const double *p = src + k0 + (long)n * col;
+      if (VEC && k0 + 3 < n) {
+        const double2 lo = *reinterpret_cast<const double2 *>(p);
+        const double2 hi = *reinterpret_cast<const double2 *>(p + 2);
+        v = (v4d){lo.x, lo.y, hi.x, hi.y};
+      } else {
+        if (k0 + 0 < n) v.x = p[0];
+        if (k0 + 1 < n) v.y = p[1];
+        if (k0 + 2 < n) v.z = p[2];
+        if (k0 + 3 < n) v.w = p[3];
+      }
+    }
+    return v;
+  };
+  // staging of the next A chunk in a vector register tuple (a plain array
+  // here was kept in scratch memory, which serialised every prefetch)
+  typedef double stage_t __attribute__((ext_vector_type(2 * APT)));
+  stage_t aprf;
+  constexpr auto AIDX = std::make_index_sequence<APT>{};
+  auto fetch_a = [&](int kc, auto seq) {
+    const double2 *g2 = reinterpret_cast<const double2 *>(A4) + (size_t)kc * (APT * 256);
+    [&]<size_t... I>(std::index_sequence<I...>) {
+      ((aprf[2 * I] = g2[threadIdx.x + 256 * I].x, aprf[2 * I + 1] = g2[threadIdx.x + 256 * I].y), ...);
+    }(seq);
+  };
+  auto store_a = [&](int b, auto seq) {
+    [&]<size_t... I>(std::index_sequence<I...>) {
+      ((sA2[b][threadIdx.x + 256 * I] = make_double2(aprf[2 * I], aprf[2 * I + 1])), ...);
+    }(seq);
+  };
+
+  v4d bcur[NB4], bnext[NB4];
+  fetch_a(0, AIDX);
+#pragma unroll
+  for (int u = 0; u < NB4; u++) bcur[u] = loadB(0, u);
+  store_a(0, AIDX);
+  __syncthreads();
+  for (int kc = 0; kc < nkc; kc++) {
+    const int buf = kc & 1;
+    // the last iteration re-reads chunk 0 (in bounds, unused) so the staging
+    // registers are written and read unconditionally
+    const int kn = kc + 1 < nkc ? kc + 1 : 0;
+    fetch_a(kn, AIDX);
+#pragma unroll
+    for (int u = 0; u < NB4; u++) bnext[u] = loadB(kn, u);
+    const double *sa = reinterpret_cast<const double *>(&sA2[buf][0]);
+    // A operands of step st+1 are read from LDS while step st's MFMAs run
+    double acur[MT], anext[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) acur[t] = sa[t * 64 + lane];
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      if (st < 3) {
+#pragma unroll
+        for (int t = 0; t < MT; t++) anext[t] = sa[((st + 1) * MT + t) * 64 + lane];
+      }
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < NB4; u++)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_4x4x4f64(acur[t], bcur[u][st], acc[t][u], 0, 0, 0);
+      if (st < 3) {
+#pragma unroll
+        for (int t = 0; t < MT; t++) acur[t] = anext[t];
+      }
+    }
+    store_a(buf ^ 1, AIDX);
+#pragma unroll
+    for (int u = 0; u < NB4; u++) bcur[u] = bnext[u];
+    __syncthreads();
+  }
+  // D of block blk = (lane/4)%4: lane holds row 16t + 4 blk + lane/16,
+  // column wcol + 4u + lane%4
+  const int mrow = 4 * ((lane >> 2) & 3) + q;
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB4; u++) {
+      const long col = wcol + 4 * u + j4;
+      const int m = 16 * t + mrow;
+      if (col >= n2 || m >= n) continue;
+      const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
+      if (accumulate) dst[o] += acc[t][u];
+      else dst[o] = acc[t][u];
+    }
+}
+
+// ---- 3-stage direct-to-LDS pipeline (default) ----
+// A and B chunks move global -> LDS with buffer_load ... lds (no VGPR
+// staging), two chunks ahead of the MFMAs; the buffer descriptor's range
+// check returns 0 for out-of-range offsets, which also supplies the K tail.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ i32x4 make_rsrc(const void *base, uint32_t bytes) {
+  const uint64_t a = (uint64_t)base;
+  i32x4 r;
+  r.x = (int)(uint32_t)a;
+  r.y = (int)(uint32_t)(a >> 32);  // stride 0
+  r.z = (int)bytes;                // num_records (bytes)
+  r.w = 0x00020000;                // gfx9 data format word
+  return r;
+}
+
+// One 16-byte global -> LDS copy per lane: lane L's bytes land at
+// lds_wave_base + 16 L.  Written as asm so the compiler does not treat the
+// LDS write as aliasing every later LDS read (which would force a wait for
+// all outstanding copies before each read); completion is tracked by the
+// explicit vmcnt waits in the pipeline.  M0 needs one wait state before the
+// copy reads it.  The instruction has no destination VGPR; its address
+// operands are read at issue.
+__device__ __forceinline__ void dma16(i32x4 rs, uint32_t lds_wave_base, int voff) {
+  const uint32_t b = __builtin_amdgcn_readfirstlane(lds_wave_base);
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
+               :
+               : "s"(b), "v"(voff), "s"(rs)
+               : "memory", "m0");
+}
+
+// Wait until at most VM copies of this wave are in flight, then barrier.
+template <int VM>
+__device__ __forceinline__ void wait_copies_barrier() {
+  static_assert(VM >= 0 && VM < 64, "vmcnt");
+  asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(VM) : "memory");
+}
+
+template <int MT, int NB4, int NSTAGE>
+struct mp_dl_cfg {
+  static constexpr int APT = (MT * 128 + 255) / 256;   // 16-byte A pieces per thread per chunk
+  static constexpr int ASZ = APT * 256 * 16;           // A chunk bytes
+  static constexpr int WCOLS = 4 * NB4;                // columns per wave
+  static constexpr int BSZ = 4 * WCOLS * 128;          // B chunk bytes (16 k x 8 B per column)
+  static constexpr int BPW = WCOLS / 8;                // B pieces (1 KB) per wave per chunk
+  static constexpr int STAGE = ASZ + BSZ;
+  static constexpr int NST = NSTAGE;                   // LDS stages (chunks in flight + 1)
+  static_assert(NST == 2 || NST == 3, "stages");
+  static constexpr int LDS = NST * STAGE;
+  static constexpr int LPC = APT + BPW;                // loads per thread per chunk
+};
+
+template <int MT, int NB4, int NSTAGE, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_modeprod_dl(const double *__restrict__ A4, const double *__restrict__ in,
+                                                          size_t in_fs, double *__restrict__ out, size_t out_fs, int n,
+                                                          int rotate, int accumulate) {
+  using CF = mp_dl_cfg<MT, NB4, NSTAGE>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, j4 = lane & 3;
+  const double *src = in + in_fs * blockIdx.y;
+  double *dst = out + out_fs * blockIdx.y;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 15) / 16;
+  const long wgcol = (long)blockIdx.x * (4 * CF::WCOLS);
+  const long wcol = wgcol + wave * CF::WCOLS;
+  const i32x4 rsA = make_rsrc(A4, (uint32_t)((size_t)nkc * CF::APT * 256 * 16));
+  const i32x4 rsB = make_rsrc(src, (uint32_t)((size_t)n * n2 * 8));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue = [&](int kc, int stg) {
+    const uint32_t sbase = lds0 + stg * CF::STAGE;
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      dma16(rsA, sbase + (wave * 64 + 256 * e) * 16, (int)(((size_t)kc * CF::APT * 256 + threadIdx.x + 256 * e) * 16));
+#pragma unroll
+    for (int i = 0; i < CF::BPW; i++) {
+      const long col = wcol + 8 * i + (lane >> 3);
+      const int k = 16 * kc + 2 * (lane & 7);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 8) : (int)0x80000000;
+      dma16(rsB, sbase + CF::ASZ + (wave * CF::WCOLS + 8 * i) * 128, voff);
+    }
+  };
+
+  double acc[MT][NB4];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB4; u++) acc[t][u] = 0.0;
+
+  issue(0, 0);
+  if (CF::NST == 3 && nkc > 1) issue(1, 1);
+  for (int kc = 0; kc < nkc; kc++) {
+    // chunk kc is in LDS for every wave, and every wave is past chunk kc-1,
+    // whose stage the next copy overwrites
+    if (CF::NST == 3 && kc + 1 < nkc) wait_copies_barrier<(CF::NST - 2) * CF::LPC>();
+    else wait_copies_barrier<0>();
+    if (kc + CF::NST - 1 < nkc) issue(kc + CF::NST - 1, (kc + CF::NST - 1) % CF::NST);
+    const char *stg = smem + (kc % CF::NST) * CF::STAGE;
+    const double *sa = reinterpret_cast<const double *>(stg);
+    const double *sb = reinterpret_cast<const double *>(stg + CF::ASZ);
+    v4d bv[NB4];
+#pragma unroll
+    for (int u = 0; u < NB4; u++) {
+      const double *bp = sb + (wave * CF::WCOLS + 4 * u + j4) * 16 + 4 * q;
+      const double2 lo = *reinterpret_cast<const double2 *>(bp);
+      const double2 hi = *reinterpret_cast<const double2 *>(bp + 2);
+      bv[u] = (v4d){lo.x, lo.y, hi.x, hi.y};
+    }
+    double acur[MT], anext[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) acur[t] = sa[t * 64 + lane];
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      if (st < 3) {
+#pragma unroll
+        for (int t = 0; t < MT; t++) anext[t] = sa[((st + 1) * MT + t) * 64 + lane];
+      }
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < NB4; u++)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_4x4x4f64(acur[t], bv[u][st], acc[t][u], 0, 0, 0);
+      if (st < 3) {
+#pragma unroll
+        for (int t = 0; t < MT; t++) acur[t] = anext[t];
+      }
+    }
+  }
+  const int mrow = 4 * ((lane >> 2) & 3) + q;
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NB4; u++) {
+      const long col = wcol + 4 * u + j4;
+      const int m = 16 * t + mrow;
+      if (col >= n2 || m >= n) continue;
+      const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
+      if (accumulate) dst[o] += acc[t][u];
+      else dst[o] = acc[t][u];
+    }
+}
+
+// host: A(m,k) column-major -> A4[kc*CH + (s*MT + t)*64 + lane]
 //       = A(16t + lane%16, 16kc + 4(lane/16) + s), zero outside n x n;
 // CH = per-chunk stride (MT*256 doubles rounded up to 512)
 size_t arrange_A_chunk(int MT) { return (size_t)((MT * 128 + 255) / 256) * 256 * 2; }
@@ -201,21 +473,67 @@ void arrange_A(int n, const double *Acm, double *A4, int MT, int nkc) {
       for (int lane = 0; lane < 64; lane++)
         for (int s = 0; s < 4; s++) {
           int m = 16 * t + (lane & 15), k = 16 * kc + 4 * (lane >> 4) + s;
-          A4[(size_t)kc * CH + ((size_t)t * 64 + lane) * 4 + s] = (m < n && k < n) ? Acm[m + (size_t)n * k] : 0.0;
+          A4[(size_t)kc * CH + ((size_t)s * MT + t) * 64 + lane] = (m < n && k < n) ? Acm[m + (size_t)n * k] : 0.0;
         }
 }
 
 template <int MT>
 static void modeprod_mt(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
                         int nfields, int rotate, int accumulate) {
-  constexpr int NB = 2;
   const int n = c->n;
   const long n2 = (long)n * n;
-  dim3 grid((unsigned)((n2 + 64 * NB - 1) / (64 * NB)), nfields);
-  if (n % 2 == 0)
-    k_modeprod_f64<MT, NB, true><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
-  else
-    k_modeprod_f64<MT, NB, false><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+  auto dl = [&](auto nb4, auto nst, auto occ) {
+    constexpr int NB4 = decltype(nb4)::value, NST = decltype(nst)::value, OCC = decltype(occ)::value;
+    using CF = mp_dl_cfg<MT, NB4, NST>;
+    static_assert(CF::LDS * OCC <= 160 * 1024, "LDS budget");
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void *)k_modeprod_dl<MT, NB4, NST, OCC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
+      attr_set = true;
+    }
+    dim3 grid((unsigned)((n2 + 4 * CF::WCOLS - 1) / (4 * CF::WCOLS)), nfields);
+    k_modeprod_dl<MT, NB4, NST, OCC><<<grid, 256, CF::LDS, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate,
+                                                                        accumulate);
+  };
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I1 = std::integral_constant<int, 1>;
+  using I4 = std::integral_constant<int, 4>;
+  using I8 = std::integral_constant<int, 8>;
+  if constexpr (MT <= 13) {
+    if (n % 2 == 0 && c->mp_variant <= 2) {
+      if (c->mp_variant == 0)
+        dl(I8{}, I3{}, I1{});  // 32 columns per wave, 3 stages, one workgroup per CU
+      else if (c->mp_variant == 1)
+        dl(I4{}, I2{}, I2{});  // 16 columns per wave, 2 stages, two workgroups per CU
+      else
+        dl(I4{}, I3{}, I1{});
+      return;
+    }
+  }
+  if (c->mp_variant == 16) {
+    constexpr int NB = 2;
+    dim3 grid((unsigned)((n2 + 64 * NB - 1) / (64 * NB)), nfields);
+    if (n % 2 == 0)
+      k_modeprod_f64<MT, NB, true><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+    else
+      k_modeprod_f64<MT, NB, false><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+  } else {
+    // 4 column groups per wave keep VGPR + AGPR at ~224 (two waves per SIMD,
+    // so one workgroup's prologue / epilogue overlaps another's MFMAs);
+    // 8 groups halve the LDS reads per MFMA at one wave per SIMD
+    auto go = [&](auto nbtag) {
+      constexpr int NB4 = decltype(nbtag)::value;
+      dim3 grid((unsigned)((n2 + 16 * NB4 - 1) / (16 * NB4)), nfields);
+      if (n % 2 == 0)
+        k_modeprod_f64m4<MT, NB4, true><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+      else
+        k_modeprod_f64m4<MT, NB4, false><<<grid, 256, 0, c->stream>>>(A4, in, in_fs, out, out_fs, n, rotate, accumulate);
+    };
+    if (c->mp_nb == 8 && MT <= 13) go(std::integral_constant<int, 8>{});
+    else go(std::integral_constant<int, 4>{});
+  }
 }
 
 void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
@@ -558,20 +876,20 @@ __device__ __forceinline__ double2 cmi(double2 a, int sgn) {
   return sgn > 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
-template <int SGN>
-__device__ void dft_r(int R, double2 *a) {
-  if (R == 2) {
+template <int R, int SGN>
+__device__ __forceinline__ void dft_r(double2 *a) {
+  if constexpr (R == 2) {
     double2 y0 = cadd(a[0], a[1]), y1 = csub(a[0], a[1]);
     a[0] = y0;
     a[1] = y1;
-  } else if (R == 4) {
+  } else if constexpr (R == 4) {
     double2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
     double2 s13 = cadd(a[1], a[3]), d13 = cmi(csub(a[1], a[3]), SGN);
     a[0] = cadd(s02, s13);
     a[1] = cadd(d02, d13);
     a[2] = csub(s02, s13);
     a[3] = csub(d02, d13);
-  } else if (R == 3) {
+  } else if constexpr (R == 3) {
     const double h = 0.86602540378443864676;  // sin(2 pi / 3)
     double2 s = cadd(a[1], a[2]), d = csub(a[1], a[2]);
     double2 y0 = cadd(a[0], s);
@@ -599,6 +917,36 @@ __device__ void dft_r(int R, double2 *a) {
   }
 }
 
+// one Stockham pass of radix R over the FFT_LINES lines held in LDS
+template <int R, int SGN>
+__device__ __forceinline__ void fft_pass(const double2 *__restrict__ src, double2 *__restrict__ dst, int N, int p,
+                                         const double2 *__restrict__ tw) {
+  const unsigned M = (unsigned)(N / R);
+  const unsigned stw = (unsigned)(N / (p * R));
+  const unsigned up = (unsigned)p;
+  for (unsigned b = threadIdx.x; b < FFT_LINES * M; b += blockDim.x) {
+    const unsigned ln = b / M, i = b - ln * M;
+    const unsigned k = i % up;
+    double2 a[R];
+    const double2 *s = src + ln * (unsigned)N;
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      double2 x = s[i + t * M];
+      if (t > 0 && k > 0) {
+        double2 w = tw[t * k * stw];  // t k stw < N
+        if (SGN < 0) w.y = -w.y;
+        x = cmul(x, w);
+      }
+      a[t] = x;
+    }
+    dft_r<R, SGN>(a);
+    double2 *o = dst + ln * (unsigned)N;
+    const unsigned base = (i - k) * R + k;
+#pragma unroll
+    for (int t = 0; t < R; t++) o[base + t * up] = a[t];
+  }
+}
+
 template <int SGN>
 __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int axis, fft_plan pl,
                                                   const double2 *__restrict__ tw) {
@@ -606,53 +954,36 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int 
   const int N = pl.N;
   double2 *buf0 = sbuf, *buf1 = sbuf + FFT_LINES * N;
   double2 *d = data + fs * blockIdx.y;
-  const size_t N2 = (size_t)N * N;
-  const size_t line0 = (size_t)blockIdx.x * FFT_LINES;  // first line of the group
-  const int tid = threadIdx.x;
-  const int tot = FFT_LINES * N;
-  const size_t nlines = N2;
-  // gather
-  for (int e = tid; e < tot; e += blockDim.x) {
-    int ln, t;
+  const unsigned uN = (unsigned)N, N2 = uN * uN;
+  const unsigned line0 = blockIdx.x * FFT_LINES;  // first line of the group
+  const unsigned tot = FFT_LINES * uN;
+  // line L = x + N * o: axis 0 runs along x of row L; axis 1 along y at
+  // (x, z = o); axis 2 along z at (x, y = o)
+  for (unsigned e = threadIdx.x; e < tot; e += blockDim.x) {
+    unsigned ln, t;
     size_t addr;
     if (axis == 0) {
-      t = e % N;
-      ln = e / N;
-      addr = (line0 + ln) * N + t;
+      t = e % uN;
+      ln = e / uN;
+      addr = (size_t)(line0 + ln) * uN + t;
     } else {
       ln = e % FFT_LINES;
       t = e / FFT_LINES;
-      size_t L = line0 + ln;  // L = x + N * (other)
-      size_t x = L % N, o = L / N;
-      addr = axis == 1 ? x + (size_t)N * t + N2 * o : x + (size_t)N * o + N2 * t;
+      const unsigned L = line0 + ln, x = L % uN, o = L / uN;
+      addr = axis == 1 ? x + (size_t)uN * t + (size_t)N2 * o : x + (size_t)uN * o + (size_t)N2 * t;
     }
-    buf0[ln * N + t] = line0 + ln < nlines ? d[addr] : make_double2(0.0, 0.0);
+    buf0[ln * uN + t] = line0 + ln < N2 ? d[addr] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   double2 *src = buf0, *dst = buf1;
   int p = 1;
   for (int ps = 0; ps < pl.npass; ps++) {
     const int R = pl.radix[ps];
-    const int M = N / R;
-    const int stw = N / (p * R);
-    for (int b = tid; b < FFT_LINES * M; b += blockDim.x) {
-      const int ln = b / M, i = b % M;
-      const int k = i % p;
-      double2 a[5];
-      const double2 *s = src + ln * N;
-      for (int t = 0; t < R; t++) {
-        double2 x = s[i + t * M];
-        if (t > 0 && k > 0) {
-          double2 w = tw[(t * k * stw) % N];
-          if (SGN < 0) w.y = -w.y;
-          x = cmul(x, w);
-        }
-        a[t] = x;
-      }
-      dft_r<SGN>(R, a);
-      double2 *o = dst + ln * N;
-      const int base = (i - k) * R + k;
-      for (int t = 0; t < R; t++) o[base + t * p] = a[t];
+    switch (R) {
+      case 4: fft_pass<4, SGN>(src, dst, N, p, tw); break;
+      case 2: fft_pass<2, SGN>(src, dst, N, p, tw); break;
+      case 3: fft_pass<3, SGN>(src, dst, N, p, tw); break;
+      default: fft_pass<5, SGN>(src, dst, N, p, tw); break;
     }
     __syncthreads();
     double2 *tmp = src;
@@ -660,21 +991,20 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int 
     dst = tmp;
     p *= R;
   }
-  for (int e = tid; e < tot; e += blockDim.x) {
-    int ln, t;
+  for (unsigned e = threadIdx.x; e < tot; e += blockDim.x) {
+    unsigned ln, t;
     size_t addr;
     if (axis == 0) {
-      t = e % N;
-      ln = e / N;
-      addr = (line0 + ln) * N + t;
+      t = e % uN;
+      ln = e / uN;
+      addr = (size_t)(line0 + ln) * uN + t;
     } else {
       ln = e % FFT_LINES;
       t = e / FFT_LINES;
-      size_t L = line0 + ln;
-      size_t x = L % N, o = L / N;
-      addr = axis == 1 ? x + (size_t)N * t + N2 * o : x + (size_t)N * o + N2 * t;
+      const unsigned L = line0 + ln, x = L % uN, o = L / uN;
+      addr = axis == 1 ? x + (size_t)uN * t + (size_t)N2 * o : x + (size_t)uN * o + (size_t)N2 * t;
     }
-    if (line0 + ln < nlines) d[addr] = src[ln * N + t];
+    if (line0 + ln < N2) d[addr] = src[ln * uN + t];
   }
 }
 
@@ -764,60 +1094,105 @@ __device__ __forceinline__ void b_wrow(const double *sp, const double *nv, const
   for (int k = 0; k < 3; k++) w[k] = av[0] * om[0 * 3 + k] + av[1] * om[1 * 3 + k] + av[2] * om[2 * 3 + k];
 }
 
-__global__ __launch_bounds__(256) void k_el_contract(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
-                                                     int nac, int npair, const etd_dev_consts *__restrict__ K) {
+// One k-point of the contraction for NAC variants held in registers.
+// b_q = sigma_q n, c_p = Omega b_p, B_pq = t0_pq - c_p . b_q (equal to the
+// reference's n_i s_p,ij Omega_jk s_q,kl n_l for symmetric Omega and
+// sigma); with a symmetric Voigt matrix B is symmetric and each unordered
+// pair is formed once.  MIRROR: B is replaced by (B(k) + B(-k)) / 2.
+template <int NAC, bool MIRROR>
+__device__ __forceinline__ void contract_point(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+                                               int npair, size_t m, const etd_dev_consts *__restrict__ K) {
   const size_t N3 = (size_t)N * N * N;
-  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= N3) return;
   const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
   const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
   const size_t mm = ((size_t)zm * N + ym) * N + xm;
-  double2 F[EL_MAXV];
-  for (int r = 0; r < npair; r++) {
+  double2 F[NAC];
+#pragma unroll
+  for (int r = 0; r < (NAC + 1) / 2; r++) {
     const double2 a = Z[N3 * r + m], b = Z[N3 * r + mm];
     // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
     F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-    if (2 * r + 1 < nac) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+    if (2 * r + 1 < NAC) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
   }
-  const int sx = signed_wave(x, N), sy = signed_wave(y, N), sz = signed_wave(z, N);
-  const int sxm = signed_wave(xm, N), sym = signed_wave(ym, N), szm = signed_wave(zm, N);
-  const bool herm = (sxm == -sx) && (sym == -sy) && (szm == -sz);
-  double nv[3], om[9], nv2[3], om2[9];
-  b_geometry(sx, sy, sz, K, nv, om);
-  if (!herm) b_geometry(sxm, sym, szm, K, nv2, om2);
-  double bq[EL_MAXV][3];
-  for (int qq = 0; qq < nac; qq++) {
-    const double *s = K->sig + 9 * qq;
-    for (int k = 0; k < 3; k++) bq[qq][k] = s[k * 3 + 0] * nv[0] + s[k * 3 + 1] * nv[1] + s[k * 3 + 2] * nv[2];
+  double nv[3], om[9];
+  b_geometry(signed_wave(x, N), signed_wave(y, N), signed_wave(z, N), K, nv, om);
+  double bv[NAC][3], cv[NAC][3];
+#pragma unroll
+  for (int q = 0; q < NAC; q++) {
+    const double *sg = K->sig + 9 * q;
+#pragma unroll
+    for (int k = 0; k < 3; k++) bv[q][k] = sg[k * 3 + 0] * nv[0] + sg[k * 3 + 1] * nv[1] + sg[k * 3 + 2] * nv[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) cv[q][k] = bv[q][0] * om[0 * 3 + k] + bv[q][1] * om[1 * 3 + k] + bv[q][2] * om[2 * 3 + k];
   }
-  double2 acc_prev = make_double2(0.0, 0.0);
-  for (int pp = 0; pp < nac; pp++) {
-    const double *sp = K->sig + 9 * pp;
-    double w[3], w2[3];
-    b_wrow(sp, nv, om, w);
-    if (!herm) b_wrow(sp, nv2, om2, w2);
-    double re = 0.0, im = 0.0;
-    for (int qq = 0; qq < nac; qq++) {
-      const double t0 = K->t0[pp * ETD_MAX_VAR + qq];
-      const double t1 = w[0] * bq[qq][0] + w[1] * bq[qq][1] + w[2] * bq[qq][2];
-      double B = t0 - t1;
-      if (!herm) {
-        const double *s = K->sig + 9 * qq;
-        double t1m = 0.0;
-        for (int k = 0; k < 3; k++) t1m += w2[k] * (s[k * 3 + 0] * nv2[0] + s[k * 3 + 1] * nv2[1] + s[k * 3 + 2] * nv2[2]);
-        B = 0.5 * (B + (t0 - t1m));
+  double bm[MIRROR ? NAC : 1][3], cm[MIRROR ? NAC : 1][3];
+  if constexpr (MIRROR) {
+    double nv2[3], om2[9];
+    b_geometry(signed_wave(xm, N), signed_wave(ym, N), signed_wave(zm, N), K, nv2, om2);
+#pragma unroll
+    for (int q = 0; q < NAC; q++) {
+      const double *sg = K->sig + 9 * q;
+#pragma unroll
+      for (int k = 0; k < 3; k++) bm[q][k] = sg[k * 3 + 0] * nv2[0] + sg[k * 3 + 1] * nv2[1] + sg[k * 3 + 2] * nv2[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        cm[q][k] = bm[q][0] * om2[0 * 3 + k] + bm[q][1] * om2[1 * 3 + k] + bm[q][2] * om2[2 * 3 + k];
+    }
+  }
+  double2 acc[NAC];
+#pragma unroll
+  for (int p = 0; p < NAC; p++) acc[p] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int p = 0; p < NAC; p++)
+#pragma unroll
+    for (int q = p; q < NAC; q++) {
+      const double t0 = K->t0[p * ETD_MAX_VAR + q];
+      double B = t0 - (cv[p][0] * bv[q][0] + cv[p][1] * bv[q][1] + cv[p][2] * bv[q][2]);
+      if constexpr (MIRROR) B = 0.5 * (B + (t0 - (cm[p][0] * bm[q][0] + cm[p][1] * bm[q][1] + cm[p][2] * bm[q][2])));
+      acc[p].x += F[q].x * B;
+      acc[p].y += F[q].y * B;
+      if (q != p) {
+        acc[q].x += F[p].x * B;
+        acc[q].y += F[p].y * B;
       }
-      re += F[qq].x * B;
-      im += F[qq].y * B;
     }
-    if ((pp & 1) == 0) {
-      acc_prev = make_double2(re, im);
-      if (pp + 1 == nac) Y[N3 * (pp / 2) + m] = acc_prev;
-    } else {
-      // Y_r = acc_2r + i acc_2r+1
-      Y[N3 * (pp / 2) + m] = make_double2(acc_prev.x - im, acc_prev.y + re);
-    }
+#pragma unroll
+  for (int r = 0; r < (NAC + 1) / 2; r++) {
+    if (2 * r + 1 < NAC)  // Y_r = acc_2r + i acc_2r+1
+      Y[N3 * r + m] = make_double2(acc[2 * r].x - acc[2 * r + 1].y, acc[2 * r].y + acc[2 * r + 1].x);
+    else
+      Y[N3 * r + m] = acc[2 * r];
   }
+}
+
+template <int NAC>
+__global__ __launch_bounds__(256) void k_el_contract_t(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+                                                       int npair, const etd_dev_consts *__restrict__ K) {
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= (size_t)N * N * N) return;
+  contract_point<NAC, false>(Z, Y, N, npair, m, K);
+}
+
+// points whose mirror's signed wavenumber is not the negative: the index
+// planes c = N/2 (and c = N/2 + 1 for odd N) in each direction; one thread
+// per (plane, position), duplicates across the planes write equal values
+template <int NAC>
+__global__ __launch_bounds__(256) void k_el_contract_mirror(const double2 *__restrict__ Z, double2 *__restrict__ Y,
+                                                            int N, int npair, const etd_dev_consts *__restrict__ K) {
+  const int nplane = (N & 1) ? 2 : 1;
+  const size_t N2 = (size_t)N * N;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nplane * N2) return;
+  const int axis = (int)(t / (nplane * N2));
+  const int which = (int)((t / N2) % nplane);
+  const size_t r = t % N2;
+  const int c = N / 2 + which;
+  const int a = (int)(r % N), b = (int)(r / N);
+  int x, y, z;
+  if (axis == 0) { x = c; y = a; z = b; }
+  else if (axis == 1) { x = a; y = c; z = b; }
+  else { x = a; y = b; z = c; }
+  contract_point<NAC, true>(Z, Y, N, npair, ((size_t)z * N + y) * N + x, K);
 }
 
 __global__ void k_el_unpack(const double2 *__restrict__ Y, double *__restrict__ E, int N, int nac, int ring,
@@ -840,7 +1215,19 @@ void launch_elastic(etd_ctx *c) {
   etd_prof_begin(c, PC_ELASTIC);
   k_el_pack<<<dim3(nb, c->npair), 256, 0, c->stream>>>(c->u, (double2 *)c->Z, c->n, c->g, N, c->nac);
   fft3d(c, c->Z, c->npair, +1);
-  k_el_contract<<<nb, 256, 0, c->stream>>>((const double2 *)c->Z, (double2 *)c->Y, N, c->nac, c->npair, c->dconst);
+  {
+    const int nplane = (N & 1) ? 2 : 1;
+    const unsigned nbm = (unsigned)((3 * nplane * (size_t)N * N + 255) / 256);
+    const double2 *Zc = (const double2 *)c->Z;
+    double2 *Yc = (double2 *)c->Y;
+    switch (c->nac) {
+#define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X><<<nb, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
+      NACC(1) NACC(2) NACC(3) NACC(4) NACC(5) NACC(6) NACC(7) NACC(8) NACC(9) NACC(10) NACC(11) NACC(12) NACC(13)
+      NACC(14) NACC(15) NACC(16)
+#undef NACC
+      default: break;
+    }
+  }
   fft3d(c, c->Y, c->npair, -1);
   const long long Lbase = c->elastic_evals * (long long)c->nac;
   k_el_unpack<<<dim3(nb, c->npair), 256, 0, c->stream>>>((const double2 *)c->Y, c->E, N, c->nac, c->ring, Lbase);
@@ -929,15 +1316,18 @@ __device__ __forceinline__ void cell_grad(const double *u, int n, int k, int j, 
   uz = ((f4 - f0) + (f5 - f1) + (f6 - f2) + (f7 - f3)) / 4 / hz;
 }
 
-// grid (planes = n - 2g - 1, 1 + nch + couple): y = 0 AC energy, 1..nch CH
-// energy of field nac + y - 1, nch + 1 couple gradient terms
+// grid (planes = n - 2g - 1, 1 + nch + couple, ECHUNK): y = 0 AC energy,
+// 1..nch CH energy of field nac + y - 1, nch + 1 couple gradient terms; each
+// plane is split into ECHUNK row blocks
+#define ECHUNK 8
 __global__ __launch_bounds__(256) void k_diag_energy(const double *__restrict__ u, const double *__restrict__ lam,
                                                      double *__restrict__ part, diag_args a) {
   const int n = a.n, g = a.g, m = n - 2 * g - 1;
   const int k = g + blockIdx.x, kind = blockIdx.y;
   const size_t n3 = (size_t)n * n * n;
+  const int r0 = (int)(((long)m * blockIdx.z) / ECHUNK), r1 = (int)(((long)m * (blockIdx.z + 1)) / ECHUNK);
   double e = 0.0;
-  for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+  for (int t = r0 * m + threadIdx.x; t < r1 * m; t += blockDim.x) {
     const int j = g + t / m, i = g + t % m;
     if (kind == 0) {
       double s2 = 0.0, s3 = 0.0;
@@ -975,7 +1365,7 @@ __global__ __launch_bounds__(256) void k_diag_energy(const double *__restrict__ 
   e = wave_sum(e);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
   __syncthreads();
-  if (threadIdx.x == 0) part[(size_t)kind * m + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  if (threadIdx.x == 0) part[((size_t)kind * m + blockIdx.x) * ECHUNK + blockIdx.z] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 int launch_diag(etd_ctx *c, double *stats) {
@@ -997,9 +1387,9 @@ int launch_diag(etd_ctx *c, double *stats) {
   double *pst = c->part, *pen = c->part + (size_t)3 * ms * V;
   etd_prof_begin(c, PC_ENERGY);
   k_diag_stats<<<dim3(ms, V), 256, 0, c->stream>>>(c->u, pst, a);
-  k_diag_energy<<<dim3(me, kinds), 256, 0, c->stream>>>(c->u, c->lam, pen, a);
+  k_diag_energy<<<dim3(me, kinds, ECHUNK), 256, 0, c->stream>>>(c->u, c->lam, pen, a);
   etd_prof_end(c, PC_ENERGY);
-  const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds;
+  const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds * ECHUNK;
   if (hipMemcpyAsync(c->hpart, c->part, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     return ETD_ERR_DEVICE;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return ETD_ERR_DEVICE;
@@ -1020,21 +1410,21 @@ int launch_diag(etd_ctx *c, double *stats) {
   }
   if (c->nac > 0) {
     double e = 0.0;
-    for (int k = 0; k < me; k++) e += he[k];
+    for (int k = 0; k < me * ECHUNK; k++) e += he[k];
     e = e * h3;
     for (int v = 0; v < c->nac; v++) stats[4 * v + 3] = e;
   }
   double engtmp = 0.0;
   for (int q = 0; q < c->nch; q++) {
     engtmp = 0.0;
-    for (int k = 0; k < me; k++) engtmp += he[(size_t)(1 + q) * me + k];
+    for (int k = 0; k < me * ECHUNK; k++) engtmp += he[(size_t)(1 + q) * me * ECHUNK + k];
     engtmp = engtmp * h3;
     stats[4 * (c->nac + q) + 3] = engtmp;
   }
   double couple = 0.0;
   if (c->nac > 0 && c->nch > 0) {
     couple = engtmp;
-    for (int k = 0; k < me; k++) couple += he[(size_t)(1 + c->nch) * me + k];
+    for (int k = 0; k < me * ECHUNK; k++) couple += he[(size_t)(1 + c->nch) * me * ECHUNK + k];
     couple = couple * h3;
   }
   stats[4 * V] = couple;
