@@ -21,6 +21,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "energy-stability_amd"))
 
 METRIC = "sim-time/s at 192^3x12 vars: mixed-prec+learned dt vs plain FP64 at fixed err"
+# HBM bytes per mode-product launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled against a copy kernel of known size) for the default workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_modeprod_dl_192cube_13fields_hbm_counters.md")
+
+
+def measured_traffic(workload):
+    if workload != "beta_alpha_192" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if "traffic_bytes_per_launch_avg:" in line:
+            return float(line.split("traffic_bytes_per_launch_avg:")[1].strip(" *\n"))
+    return None
 PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
 
 
@@ -53,12 +65,7 @@ def main():
     if args.warmup:
         g.time_steps(args.warmup, with_diag=True)
     if args.calib_copy_mib:
-        import torch
-        a = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda:0")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        g.calib_copy(args.calib_copy_mib << 20)  # plain copy kernel, exactly this many bytes each way
     ms, launches = g.time_steps(args.steps, with_diag=True)
     total_s = ms[0] / 1e3
     K = args.steps
@@ -100,7 +107,8 @@ def main():
             "peak": PEAK_FP64_MFMA / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, see profiles/)",
             "launches": launches[0],
             "flops_per_step": flops_step,
         },

@@ -493,6 +493,19 @@ extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *n
   return ETD_OK;
 }
 
+// ------------------------------------------------------------------ counter calibration
+extern "C" int etd_calib_copy(etd_ctx *c, size_t bytes) {
+  if (!c) return ETD_ERR_ARG;
+  const size_t cap = sizeof(double) * c->n3 * 2 * c->V;
+  if (bytes == 0 || bytes % 16 || bytes > cap)
+    return seterr(c, ETD_ERR_ARG, "calibration copy of %zu bytes: must be a positive multiple of 16 up to %zu", bytes,
+                  cap);
+  launch_copy16(c, c->spec, c->w1, bytes / 16);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
 // ------------------------------------------------------------------ timing
 extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches) {
   if (!c || nsteps < 1 || !ms) return ETD_ERR_ARG;

@@ -84,6 +84,7 @@ void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_f
 void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var, int nvars, int order2);
 void launch_nonlinear(etd_ctx *c, double *out, const double *sub);
 void launch_elastic(etd_ctx *c);
+void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16);
 void fft3d(etd_ctx *c, double *data, int npair, int sgn);
 void launch_felas_materialize(etd_ctx *c, double *felas);
 int launch_diag(etd_ctx *c, double *stats);

@@ -1430,3 +1430,15 @@ int launch_diag(etd_ctx *c, double *stats) {
   stats[4 * V] = couple;
   return ETD_OK;
 }
+
+// ------------------------------------------------------------------ counter calibration
+// A plain 16-byte-per-lane copy: reads and writes exactly 16 * n16 bytes.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ in, uint4 *__restrict__ out, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) out[i] = in[i];
+}
+
+void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16) {
+  const size_t blocks = (n16 + 255) / 256;
+  k_copy16<<<(unsigned)(blocks < 8192 ? blocks : 8192), 256, 0, c->stream>>>((const uint4 *)in, (uint4 *)out, n16);
+}

@@ -93,6 +93,7 @@ def lib():
             "etd_k_modeprod": (C.c_int, [ctxp, C.c_int, _dp, _dp, _dp, C.c_int]),
             "etd_k_phiscale": (C.c_int, [ctxp, C.c_int, _dp, _dp]),
             "etd_time_steps": (C.c_int, [ctxp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]),
+            "etd_calib_copy": (C.c_int, [ctxp, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -307,6 +308,10 @@ class Etd:
         nh = np.ascontiguousarray(np.asarray(nhat, dtype=np.float64).reshape(-1))
         _check(lib().etd_k_phiscale(self.ctx, var, _ptr(u), _ptr(nh)), self.ctx, "phiscale")
         return u
+
+    def calib_copy(self, nbytes):
+        """One plain device copy of nbytes (HBM counter calibration)."""
+        _check(lib().etd_calib_copy(self.ctx, nbytes), self.ctx, "calib_copy")
 
     def time_steps(self, nsteps, with_diag=True):
         ms = np.zeros(7)

@@ -21,6 +21,8 @@
 #ifndef ETD_H
 #define ETD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -178,6 +180,13 @@ int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat);
  * contraction), ms[3] = nonlinear, ms[4] = phi scaling, ms[5] = energy,
  * ms[6] = halo; launches[] counts the dominant kernel's launches. */
 int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches);
+
+/* Measurement support, no counterpart in the reference: one plain copy
+ * kernel that reads and writes exactly `bytes` (a multiple of 16, at most the
+ * context's 2 (nac+nch) n^3 doubles of scratch) between two of the context's
+ * own buffers, so HBM counters can be scaled against a known size.  Clobbers
+ * the spectral scratch only; the fields are untouched. */
+int etd_calib_copy(etd_ctx *c, size_t bytes);
 
 #ifdef __cplusplus
 }
