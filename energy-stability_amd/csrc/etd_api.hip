@@ -93,7 +93,10 @@ static int validate(etd_ctx *c, const etd_params *p) {
   if (p->nghost < 1 || p->nx < 2 * p->nghost + 2) return seterr(c, ETD_ERR_ARG, "nghost / nx");
   if (p->nx > 16 * 17) return seterr(c, ETD_ERR_UNSUPPORTED, "n > 272");
   if (p->elastic == 1 && p->nac > 16) return seterr(c, ETD_ERR_UNSUPPORTED, "elasticity supports nac <= 16");
-  if (p->precision != ETD_PREC_FP64) return seterr(c, ETD_ERR_UNSUPPORTED, "only the FP64 path is built");
+  if (p->precision != ETD_PREC_FP64 && p->precision != ETD_PREC_MIXED)
+    return seterr(c, ETD_ERR_ARG, "precision %d is neither FP64 (0) nor mixed (1)", p->precision);
+  if (p->precision == ETD_PREC_MIXED && p->nx % 4)
+    return seterr(c, ETD_ERR_UNSUPPORTED, "the mixed-precision mode products need nx %% 4 == 0 (nx = %d)", p->nx);
   return ETD_OK;
 }
 
@@ -144,25 +147,50 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   HCHK(c, hipSetDevice(dev));
   HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t n3 = c->n3;
-  HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
-  c->N1 = c->u + n3 * V;
-  HCHK(c, hipMemsetAsync(c->u, 0, sizeof(double) * n3 * 2 * V, c->stream));
-  if (p->etd2) HCHK(c, hipMalloc(&c->N2, sizeof(double) * n3 * V));
-  HCHK(c, hipMalloc(&c->w1, sizeof(double) * n3 * 2 * V));
-  HCHK(c, hipMalloc(&c->w2, sizeof(double) * n3 * 2 * V));
-  HCHK(c, hipMalloc(&c->spec, sizeof(double) * n3 * 2 * V));
-  HCHK(c, hipMalloc(&c->gtmp, sizeof(double) * n3 * 3));
-  // eigenbasis as MFMA A fragments
-  const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
-  const size_t a4n = arrange_A_chunk(MT) * nkc;
-  double *h4 = (double *)malloc(sizeof(double) * a4n);
-  HCHK(c, hipMalloc(&c->V4, sizeof(double) * a4n));
-  HCHK(c, hipMalloc(&c->Vi4, sizeof(double) * a4n));
-  arrange_A(n, c->hVi, h4, MT, nkc);
-  HCHK(c, hipMemcpy(c->Vi4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
-  arrange_A(n, c->hV, h4, MT, nkc);
-  HCHK(c, hipMemcpy(c->V4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
-  free(h4);
+  c->mixed = p->precision == ETD_PREC_MIXED;
+  c->split_terms = 3;
+  const char *sv = getenv("ETD_SPLIT");
+  if (sv && atoi(sv) == 2) c->split_terms = 2;
+  if (!c->mixed) {
+    HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
+    c->N1 = c->u + n3 * V;
+    HCHK(c, hipMemsetAsync(c->u, 0, sizeof(double) * n3 * 2 * V, c->stream));
+    if (p->etd2) HCHK(c, hipMalloc(&c->N2, sizeof(double) * n3 * V));
+    HCHK(c, hipMalloc(&c->w1, sizeof(double) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&c->w2, sizeof(double) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&c->spec, sizeof(double) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&c->gtmp, sizeof(double) * n3 * 3));
+    // eigenbasis as MFMA A fragments
+    const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
+    const size_t a4n = arrange_A_chunk(MT) * nkc;
+    double *h4 = (double *)malloc(sizeof(double) * a4n);
+    HCHK(c, hipMalloc(&c->V4, sizeof(double) * a4n));
+    HCHK(c, hipMalloc(&c->Vi4, sizeof(double) * a4n));
+    arrange_A(n, c->hVi, h4, MT, nkc);
+    HCHK(c, hipMemcpy(c->Vi4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
+    arrange_A(n, c->hV, h4, MT, nkc);
+    HCHK(c, hipMemcpy(c->V4, h4, sizeof(double) * a4n, hipMemcpyHostToDevice));
+    free(h4);
+  } else {
+    etd_fields<float> &F = c->f32;
+    HCHK(c, hipMalloc(&F.u, sizeof(float) * n3 * 2 * V));  // [u | N1]
+    F.N1 = F.u + n3 * V;
+    HCHK(c, hipMemsetAsync(F.u, 0, sizeof(float) * n3 * 2 * V, c->stream));
+    if (p->etd2) HCHK(c, hipMalloc(&F.N2, sizeof(float) * n3 * V));
+    HCHK(c, hipMalloc(&F.w1, sizeof(float) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&F.w2, sizeof(float) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&F.spec, sizeof(float) * n3 * 2 * V));
+    HCHK(c, hipMalloc(&F.gtmp, sizeof(float) * n3 * 3));
+    const size_t sb = split_A_bytes(n, c->split_terms);
+    uint16_t *h3 = (uint16_t *)malloc(sb);
+    HCHK(c, hipMalloc(&c->V3, sb));
+    HCHK(c, hipMalloc(&c->Vi3, sb));
+    arrange_split_A(n, c->hVi, h3, c->split_terms);
+    HCHK(c, hipMemcpy(c->Vi3, h3, sb, hipMemcpyHostToDevice));
+    arrange_split_A(n, c->hV, h3, c->split_terms);
+    HCHK(c, hipMemcpy(c->V3, h3, sb, hipMemcpyHostToDevice));
+    free(h3);
+  }
   // D / h^2 per axis (read_matrices, vars.c:512,544,576)
   double *dd = (double *)malloc(sizeof(double) * 3 * n);
   for (int l = 0; l < n; l++) {
@@ -183,10 +211,17 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     if (c->fft_npass < 0) return seterr(c, ETD_ERR_UNSUPPORTED, "N = %d has prime factors other than 2, 3, 5", c->N);
     c->npair = (p->nac + 1) / 2;
     c->ring = p->nac + 2;
-    HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->N3 * c->npair));
-    HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->N3 * c->npair));
-    HCHK(c, hipMalloc(&c->E, sizeof(double) * c->N3 * c->ring));
-    HCHK(c, hipMemsetAsync(c->E, 0, sizeof(double) * c->N3 * c->ring, c->stream));
+    if (!c->mixed) {
+      HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->N3 * c->npair));
+      HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->N3 * c->npair));
+      HCHK(c, hipMalloc(&c->E, sizeof(double) * c->N3 * c->ring));
+      HCHK(c, hipMemsetAsync(c->E, 0, sizeof(double) * c->N3 * c->ring, c->stream));
+    } else {
+      HCHK(c, hipMalloc(&c->f32.Z, sizeof(float) * 2 * c->N3 * c->npair));
+      HCHK(c, hipMalloc(&c->f32.Y, sizeof(float) * 2 * c->N3 * c->npair));
+      HCHK(c, hipMalloc(&c->f32.E, sizeof(float) * c->N3 * c->ring));
+      HCHK(c, hipMemsetAsync(c->f32.E, 0, sizeof(float) * c->N3 * c->ring, c->stream));
+    }
     double *tw = (double *)malloc(sizeof(double) * 2 * c->N);
     const double pi = 3.14159265358979323846;
     for (int e = 0; e < c->N; e++) {
@@ -251,6 +286,17 @@ extern "C" void etd_close(etd_ctx *c) {
   (void)hipFree(c->dconst);
   (void)hipFree(c->part);
   if (c->hpart) (void)hipHostFree(c->hpart);
+  (void)hipFree(c->f32.u);
+  (void)hipFree(c->f32.N2);
+  (void)hipFree(c->f32.w1);
+  (void)hipFree(c->f32.w2);
+  (void)hipFree(c->f32.spec);
+  (void)hipFree(c->f32.gtmp);
+  (void)hipFree(c->f32.Z);
+  (void)hipFree(c->f32.Y);
+  (void)hipFree(c->f32.E);
+  (void)hipFree(c->V3);
+  (void)hipFree(c->Vi3);
   for (int i = 0; i < c->cap_ev; i++) (void)hipEventDestroy(c->ev[i]);
   free(c->ev);
   free(c->ev_class);
@@ -310,8 +356,31 @@ extern "C" int etd_open(const char *dir, const char *suffix, etd_ctx **out) {
 }
 
 // ------------------------------------------------------------------ fields
+// host FP64 <-> device FP32 through a temporary host buffer (mixed path)
+static int put_f32(etd_ctx *c, float *dev, const double *host, size_t cnt) {
+  float *t = (float *)malloc(sizeof(float) * cnt);
+  if (!t) return ETD_ERR_NOMEM;
+  for (size_t i = 0; i < cnt; i++) t[i] = (float)host[i];
+  hipError_t e = hipMemcpy(dev, t, sizeof(float) * cnt, hipMemcpyHostToDevice);
+  free(t);
+  if (e != hipSuccess) return seterr(c, ETD_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+  return ETD_OK;
+}
+static int get_f32(etd_ctx *c, double *host, const float *dev, size_t cnt) {
+  float *t = (float *)malloc(sizeof(float) * cnt);
+  if (!t) return ETD_ERR_NOMEM;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(t, dev, sizeof(float) * cnt, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (size_t i = 0; i < cnt; i++) host[i] = t[i];
+  free(t);
+  if (e != hipSuccess) return seterr(c, ETD_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+  return ETD_OK;
+}
+
 extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "set_field: bad variable %d", var);
+  if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
   HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -319,6 +388,7 @@ extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
 
 extern "C" int etd_get_field(etd_ctx *c, int var, double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "get_field: bad variable %d", var);
+  if (c->mixed) return get_f32(c, host, c->f32.u + c->n3 * var, c->n3);
   HCHK(c, hipMemcpyAsync(host, c->u + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -335,42 +405,53 @@ extern "C" int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, doub
 }
 
 // ------------------------------------------------------------------ step
-static void fwd3(etd_ctx *c, const double *in, double *out, int nf) {
+// Three rotating mode products by V^-1 (forward) or V (backward) over nf
+// fields; FP64 MFMA on the plain path, split-BF16 MFMA on the mixed path.
+template <typename T>
+static void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumulate) {
   const size_t n3 = c->n3;
-  launch_modeprod(c, c->Vi4, in, n3, c->w1, n3, nf, 1, 0);
-  launch_modeprod(c, c->Vi4, c->w1, n3, c->w2, n3, nf, 1, 0);
-  launch_modeprod(c, c->Vi4, c->w2, n3, out, n3, nf, 1, 0);
-}
-static void bwd3(etd_ctx *c, const double *in, double *out, int nf, int accumulate) {
-  const size_t n3 = c->n3;
-  launch_modeprod(c, c->V4, in, n3, c->w1, n3, nf, 1, 0);
-  launch_modeprod(c, c->V4, c->w1, n3, c->w2, n3, nf, 1, 0);
-  launch_modeprod(c, c->V4, c->w2, n3, out, n3, nf, 1, accumulate);
+  const etd_fields<T> F = fields<T>(c);
+  if constexpr (sizeof(T) == 8) {
+    const double *A = forward ? c->Vi4 : c->V4;
+    launch_modeprod(c, A, in, n3, F.w1, n3, nf, 1, 0);
+    launch_modeprod(c, A, F.w1, n3, F.w2, n3, nf, 1, 0);
+    launch_modeprod(c, A, F.w2, n3, out, n3, nf, 1, accumulate);
+  } else {
+    const uint16_t *A = forward ? c->Vi3 : c->V3;
+    launch_modeprod_split(c, A, in, n3, F.w1, n3, nf, 1, 0);
+    launch_modeprod_split(c, A, F.w1, n3, F.w2, n3, nf, 1, 0);
+    launch_modeprod_split(c, A, F.w2, n3, out, n3, nf, 1, accumulate);
+  }
 }
 
 // ScLETD.c:87-189 for every field at once: u and N(u) share the forward
 // transforms (one launch over 2V fields), the phi update writes u-hat in
 // place, three inverse products return u in xyz.
-static int step_once(etd_ctx *c) {
+template <typename T>
+static int step_once_t(etd_ctx *c) {
   const int V = c->V;
   const size_t n3 = c->n3;
-  launch_ghost_fill(c, c->u, V, n3);
-  if (c->p.elastic == 1 && c->nac > 0) launch_elastic(c);
-  launch_nonlinear(c, c->N1, nullptr);
-  fwd3(c, c->u, c->spec, 2 * V);
-  launch_phiscale(c, c->spec, c->spec + n3 * V, 0, V, 0);
-  bwd3(c, c->spec, c->u, V, 0);
+  const etd_fields<T> F = fields<T>(c);
+  launch_ghost_fill<T>(c, F.u, V, n3);
+  if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
+  launch_nonlinear<T>(c, F.N1, nullptr);
+  modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
+  launch_phiscale<T>(c, F.spec, F.spec + n3 * V, 0, V, 0);
+  modeprod3<T>(c, 0, F.spec, F.u, V, 0);
   if (c->p.etd2) {
-    launch_ghost_fill(c, c->u, V, n3);
-    if (c->p.elastic == 1 && c->nac > 0) launch_elastic(c);
-    launch_nonlinear(c, c->N2, c->N1);  // N(a) - N(u_n)
-    fwd3(c, c->N2, c->spec, V);
-    launch_phiscale(c, c->spec, nullptr, 0, V, 1);
-    bwd3(c, c->spec, c->u, V, 1);  // u = a + S phi2 S^-1 dN
+    launch_ghost_fill<T>(c, F.u, V, n3);
+    if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
+    launch_nonlinear<T>(c, F.N2, F.N1);  // N(a) - N(u_n)
+    modeprod3<T>(c, 1, F.N2, F.spec, V, 0);
+    launch_phiscale<T>(c, F.spec, (const T *)nullptr, 0, V, 1);
+    modeprod3<T>(c, 0, F.spec, F.u, V, 1);  // u = a + S phi2 S^-1 dN
   }
   HCHK(c, hipGetLastError());
   return ETD_OK;
 }
+
+static int step_once(etd_ctx *c) { return c->mixed ? step_once_t<float>(c) : step_once_t<double>(c); }
+static int diag_any(etd_ctx *c, double *st) { return c->mixed ? launch_diag<float>(c, st) : launch_diag<double>(c, st); }
 
 extern "C" int etd_step(etd_ctx *c, int nsteps) {
   if (!c || nsteps < 0) return ETD_ERR_ARG;
@@ -384,7 +465,7 @@ extern "C" int etd_step(etd_ctx *c, int nsteps) {
 
 extern "C" int etd_diag(etd_ctx *c, double *stats) {
   if (!c || !stats) return ETD_ERR_ARG;
-  int rc = launch_diag(c, stats);
+  int rc = diag_any(c, stats);
   if (rc) return seterr(c, rc, "diag: %s", hipGetErrorString(hipGetLastError()));
   return ETD_OK;
 }
@@ -406,9 +487,37 @@ extern "C" int etd_write_field(etd_ctx *c, int var, int ioutput) {
 }
 
 // ------------------------------------------------------------------ per-kernel entry points
+// On a mixed-precision context these run the FP32 kernels (split-BF16 mode
+// products); host buffers stay FP64 and are converted at the copy.
 extern "C" int etd_k_transfer(etd_ctx *c) {
   if (!c) return ETD_ERR_ARG;
-  launch_ghost_fill(c, c->u, c->V, c->n3);
+  if (c->mixed)
+    launch_ghost_fill<float>(c, c->f32.u, c->V, c->n3);
+  else
+    launch_ghost_fill<double>(c, c->u, c->V, c->n3);
+  HCHK(c, hipGetLastError());
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+template <typename T>
+static int k_elastic_t(etd_ctx *c, double *host_felas) {
+  launch_elastic<T>(c);
+  if (host_felas) {
+    const size_t cnt = c->n3 * c->nac;
+    T *d;
+    HCHK(c, hipMalloc(&d, sizeof(T) * cnt));
+    launch_felas_materialize<T>(c, d);
+    int rc = ETD_OK;
+    if constexpr (sizeof(T) == 8) {
+      HCHK(c, hipMemcpyAsync(host_felas, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      rc = get_f32(c, host_felas, d, cnt);
+    }
+    HCHK(c, hipStreamSynchronize(c->stream));
+    HCHK(c, hipFree(d));
+    if (rc) return rc;
+  }
   HCHK(c, hipGetLastError());
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -417,38 +526,43 @@ extern "C" int etd_k_transfer(etd_ctx *c) {
 extern "C" int etd_k_elastic(etd_ctx *c, double *host_felas) {
   if (!c) return ETD_ERR_ARG;
   if (c->p.elastic != 1 || c->nac == 0) return seterr(c, ETD_ERR_ARG, "elasticity is off");
-  launch_elastic(c);
-  if (host_felas) {
-    double *d;
-    HCHK(c, hipMalloc(&d, sizeof(double) * c->n3 * c->nac));
-    launch_felas_materialize(c, d);
-    HCHK(c, hipMemcpyAsync(host_felas, d, sizeof(double) * c->n3 * c->nac, hipMemcpyDeviceToHost, c->stream));
-    HCHK(c, hipStreamSynchronize(c->stream));
-    HCHK(c, hipFree(d));
-  }
-  HCHK(c, hipGetLastError());
-  HCHK(c, hipStreamSynchronize(c->stream));
-  return ETD_OK;
+  return c->mixed ? k_elastic_t<float>(c, host_felas) : k_elastic_t<double>(c, host_felas);
 }
 
 extern "C" int etd_k_fft3d(etd_ctx *c, double *data, int dir) {
   if (!c || !data || (dir != 1 && dir != -1)) return ETD_ERR_ARG;
   if (!c->tw) return seterr(c, ETD_ERR_ARG, "fft3d: the context has no elastic FFT plan");
-  double *d;
-  const size_t bytes = sizeof(double) * 2 * c->N3;
-  HCHK(c, hipMalloc(&d, bytes));
-  HCHK(c, hipMemcpy(d, data, bytes, hipMemcpyHostToDevice));
-  fft3d(c, d, 1, dir);
-  HCHK(c, hipGetLastError());
-  HCHK(c, hipStreamSynchronize(c->stream));
-  HCHK(c, hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost));
+  const size_t cnt = 2 * c->N3;
+  if (!c->mixed) {
+    double *d;
+    HCHK(c, hipMalloc(&d, sizeof(double) * cnt));
+    HCHK(c, hipMemcpy(d, data, sizeof(double) * cnt, hipMemcpyHostToDevice));
+    fft3d<double>(c, d, 1, dir);
+    HCHK(c, hipGetLastError());
+    HCHK(c, hipStreamSynchronize(c->stream));
+    HCHK(c, hipMemcpy(data, d, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    (void)hipFree(d);
+    return ETD_OK;
+  }
+  float *d;
+  HCHK(c, hipMalloc(&d, sizeof(float) * cnt));
+  int rc = put_f32(c, d, data, cnt);
+  if (!rc) {
+    fft3d<float>(c, d, 1, dir);
+    rc = get_f32(c, data, d, cnt);
+  }
   (void)hipFree(d);
-  return ETD_OK;
+  return rc;
 }
 
 extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
   if (!c || !host_out || var < 0 || var >= c->V) return ETD_ERR_ARG;
-  launch_nonlinear(c, c->N1, nullptr);
+  if (c->mixed) {
+    launch_nonlinear<float>(c, c->f32.N1, nullptr);
+    HCHK(c, hipGetLastError());
+    return get_f32(c, host_out, c->f32.N1 + c->n3 * var, c->n3);
+  }
+  launch_nonlinear<double>(c, c->N1, nullptr);
   HCHK(c, hipGetLastError());
   HCHK(c, hipMemcpyAsync(host_out, c->N1 + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
@@ -458,6 +572,29 @@ extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
 extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *in, double *out, int rotate) {
   if (!c || !A || !in || !out) return ETD_ERR_ARG;
   if (n != c->n) return seterr(c, ETD_ERR_ARG, "modeprod: n must equal the context edge %d", c->n);
+  if (c->mixed) {
+    const size_t sb = split_A_bytes(n, c->split_terms);
+    uint16_t *h3 = (uint16_t *)malloc(sb);
+    if (!h3) return ETD_ERR_NOMEM;
+    arrange_split_A(n, A, h3, c->split_terms);
+    uint16_t *dA;
+    float *din, *dout;
+    HCHK(c, hipMalloc(&dA, sb));
+    HCHK(c, hipMalloc(&din, sizeof(float) * c->n3));
+    HCHK(c, hipMalloc(&dout, sizeof(float) * c->n3));
+    HCHK(c, hipMemcpy(dA, h3, sb, hipMemcpyHostToDevice));
+    free(h3);
+    int rc = put_f32(c, din, in, c->n3);
+    if (!rc) {
+      launch_modeprod_split(c, dA, din, c->n3, dout, c->n3, 1, rotate, 0);
+      HCHK(c, hipGetLastError());
+      rc = get_f32(c, out, dout, c->n3);
+    }
+    (void)hipFree(dA);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
+  }
   const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
   const size_t a4n = arrange_A_chunk(MT) * nkc;
   double *h4 = (double *)malloc(sizeof(double) * a4n);
@@ -481,11 +618,24 @@ extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *
 
 extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat) {
   if (!c || !uhat || !nhat || var < 0 || var >= c->V) return ETD_ERR_ARG;
+  if (c->mixed) {
+    float *d;
+    HCHK(c, hipMalloc(&d, sizeof(float) * 2 * c->n3));
+    int rc = put_f32(c, d, uhat, c->n3);
+    if (!rc) rc = put_f32(c, d + c->n3, nhat, c->n3);
+    if (!rc) {
+      launch_phiscale<float>(c, d, d + c->n3, var, 1, 0);
+      HCHK(c, hipGetLastError());
+      rc = get_f32(c, uhat, d, c->n3);
+    }
+    (void)hipFree(d);
+    return rc;
+  }
   double *d;
   HCHK(c, hipMalloc(&d, sizeof(double) * 2 * c->n3));
   HCHK(c, hipMemcpy(d, uhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
   HCHK(c, hipMemcpy(d + c->n3, nhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
-  launch_phiscale(c, d, d + c->n3, var, 1, 0);
+  launch_phiscale<double>(c, d, d + c->n3, var, 1, 0);
   HCHK(c, hipGetLastError());
   HCHK(c, hipStreamSynchronize(c->stream));
   HCHK(c, hipMemcpy(uhat, d, sizeof(double) * c->n3, hipMemcpyDeviceToHost));
@@ -496,11 +646,15 @@ extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *n
 // ------------------------------------------------------------------ counter calibration
 extern "C" int etd_calib_copy(etd_ctx *c, size_t bytes) {
   if (!c) return ETD_ERR_ARG;
-  const size_t cap = sizeof(double) * c->n3 * 2 * c->V;
+  const size_t esz = c->mixed ? sizeof(float) : sizeof(double);
+  const size_t cap = esz * c->n3 * 2 * c->V;
   if (bytes == 0 || bytes % 16 || bytes > cap)
     return seterr(c, ETD_ERR_ARG, "calibration copy of %zu bytes: must be a positive multiple of 16 up to %zu", bytes,
                   cap);
-  launch_copy16(c, c->spec, c->w1, bytes / 16);
+  if (c->mixed)
+    launch_copy16(c, c->f32.spec, c->f32.w1, bytes / 16);
+  else
+    launch_copy16(c, c->spec, c->w1, bytes / 16);
   HCHK(c, hipGetLastError());
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -534,7 +688,7 @@ extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms,
   HCHK(c, hipEventRecord(t0, c->stream));
   int rc = ETD_OK;
   for (int s = 0; s < nsteps && rc == ETD_OK; s++) {
-    if (with_diag) rc = launch_diag(c, st);
+    if (with_diag) rc = diag_any(c, st);
     if (rc == ETD_OK) rc = step_once(c);
   }
   HCHK(c, hipEventRecord(t1, c->stream));

@@ -14,8 +14,27 @@
 #define ETD_INTERNAL_H
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 #include "etd.h"
+
+template <typename T>
+struct cplx;
+template <>
+struct cplx<double> {
+  typedef double2 t;
+};
+template <>
+struct cplx<float> {
+  typedef float2 t;
+};
+
+/* The step's field buffers for one storage type (see the layout above);
+ * Z and Y hold interleaved complex values. */
+template <typename T>
+struct etd_fields {
+  T *u, *N1, *N2, *w1, *w2, *spec, *gtmp, *Z, *Y, *E;
+};
 
 #define ETD_MAXN 512
 
@@ -48,6 +67,12 @@ struct etd_ctx {
   int npair, ring;
   long long elastic_evals; /* completed elastic evaluations */
   int fft_radix[16], fft_npass;
+  /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
+   * split into split_terms BF16 terms in MFMA fragment order */
+  int mixed;
+  int split_terms;
+  etd_fields<float> f32;
+  uint16_t *V3, *Vi3;
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
@@ -78,17 +103,45 @@ int etd_field_file(const char *data_dir, int var, int ioutput, int n, const doub
 void etd_prof_begin(etd_ctx *c, int cls);
 void etd_prof_end(etd_ctx *c, int cls);
 
-void launch_ghost_fill(etd_ctx *c, double *base, int nfields, size_t fstride);
+template <typename T>
+inline etd_fields<T> fields(const etd_ctx *c);
+template <>
+inline etd_fields<double> fields<double>(const etd_ctx *c) {
+  return etd_fields<double>{c->u, c->N1, c->N2, c->w1, c->w2, c->spec, c->gtmp, c->Z, c->Y, c->E};
+}
+template <>
+inline etd_fields<float> fields<float>(const etd_ctx *c) {
+  return c->f32;
+}
+
+/* T = double (plain FP64 path) or float (FP32 storage of the mixed path;
+ * arithmetic in registers stays FP64 in these kernels) */
+template <typename T>
+void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride);
+template <typename T>
+void launch_phiscale(etd_ctx *c, T *uhat, const T *nhat, int first_var, int nvars, int order2);
+template <typename T>
+void launch_nonlinear(etd_ctx *c, T *out, const T *sub);
+template <typename T>
+void launch_elastic(etd_ctx *c);
+template <typename T>
+void fft3d(etd_ctx *c, T *data, int npair, int sgn);
+template <typename T>
+void launch_felas_materialize(etd_ctx *c, T *felas);
+template <typename T>
+int launch_diag(etd_ctx *c, double *stats);
+
 void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
                      int nfields, int rotate, int accumulate);
-void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var, int nvars, int order2);
-void launch_nonlinear(etd_ctx *c, double *out, const double *sub);
-void launch_elastic(etd_ctx *c);
 void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16);
-void fft3d(etd_ctx *c, double *data, int npair, int sgn);
-void launch_felas_materialize(etd_ctx *c, double *felas);
-int launch_diag(etd_ctx *c, double *stats);
 size_t arrange_A_chunk(int MT);
 void arrange_A(int n, const double *Acm, double *A4host, int MT, int nkc);
+
+/* mixed precision mode products: FP32 in / out, A split into `terms` BF16
+ * terms, FP32 accumulation (etd_kernels_mixed.hip) */
+void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
+                           int nfields, int rotate, int accumulate);
+size_t split_A_bytes(int n, int terms);
+void arrange_split_A(int n, const double *Acm, uint16_t *A3host, int terms);
 
 #endif

@@ -26,8 +26,9 @@ __device__ __forceinline__ size_t IDX(int n, int k, int j, int i) { return ((siz
 // comes from the pre-transfer state, z faces override y faces override x
 // faces.  In place: z phase, then y phase (reads x-ghost cells of interior
 // planes, not yet rewritten), then x phase (reads interior cells only).
-__global__ void k_ghost_z(double *base, size_t fs, int n, int g) {
-  double *u = base + fs * blockIdx.y;
+template <typename T>
+__global__ void k_ghost_z(T *base, size_t fs, int n, int g) {
+  T *u = base + fs * blockIdx.y;
   const size_t n2 = (size_t)n * n;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)2 * g * n2) return;
@@ -36,8 +37,9 @@ __global__ void k_ghost_z(double *base, size_t fs, int n, int g) {
   int k = pl < g ? pl : n - 2 * g + pl;
   u[(size_t)k * n2 + r] = u[(size_t)wrap_e(k, n, g) * n2 + r];
 }
-__global__ void k_ghost_y(double *base, size_t fs, int n, int g) {
-  double *u = base + fs * blockIdx.y;
+template <typename T>
+__global__ void k_ghost_y(T *base, size_t fs, int n, int g) {
+  T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = 2 * g;
   if (t >= (size_t)(n - 2 * g) * rows * n) return;
@@ -47,8 +49,9 @@ __global__ void k_ghost_y(double *base, size_t fs, int n, int g) {
   int j = rr < g ? rr : n - 2 * g + rr;
   u[IDX(n, k, j, i)] = u[IDX(n, k, wrap_e(j, n, g), i)];
 }
-__global__ void k_ghost_x(double *base, size_t fs, int n, int g) {
-  double *u = base + fs * blockIdx.y;
+template <typename T>
+__global__ void k_ghost_x(T *base, size_t fs, int n, int g) {
+  T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int cols = 2 * g, m = n - 2 * g;
   if (t >= (size_t)m * m * cols) return;
@@ -59,14 +62,15 @@ __global__ void k_ghost_x(double *base, size_t fs, int n, int g) {
   u[IDX(n, k, j, i)] = u[IDX(n, k, j, wrap_e(i, n, g))];
 }
 
-void launch_ghost_fill(etd_ctx *c, double *base, int nfields, size_t fstride) {
+template <typename T>
+void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
   const int n = c->n, g = c->g;
   const size_t n2 = (size_t)n * n;
   size_t tz = 2 * g * n2, ty = (size_t)(n - 2 * g) * 2 * g * n, tx = (size_t)(n - 2 * g) * (n - 2 * g) * 2 * g;
   etd_prof_begin(c, PC_HALO);
-  k_ghost_z<<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
-  k_ghost_y<<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
-  k_ghost_x<<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  k_ghost_z<T><<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  k_ghost_y<T><<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  k_ghost_x<T><<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
   etd_prof_end(c, PC_HALO);
 }
 
@@ -340,6 +344,10 @@ __device__ __forceinline__ i32x4 make_rsrc(const void *base, uint32_t bytes) {
 // explicit vmcnt waits in the pipeline.  M0 needs one wait state before the
 // copy reads it.  The instruction has no destination VGPR; its address
 // operands are read at issue.
+// M0 is reserved to the compiler; the compiled kernels of this file use it
+// only inside these asm blocks (checked in the ISA), so the clobber holds.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void dma16(i32x4 rs, uint32_t lds_wave_base, int voff) {
   const uint32_t b = __builtin_amdgcn_readfirstlane(lds_wave_base);
   asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
@@ -347,6 +355,7 @@ __device__ __forceinline__ void dma16(i32x4 rs, uint32_t lds_wave_base, int voff
                : "s"(b), "v"(voff), "s"(rs)
                : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // Wait until at most VM copies of this wave are in flight, then barrier.
 template <int VM>
@@ -560,7 +569,8 @@ struct phi_args {
   double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
 };
 
-__global__ void k_phiscale(double *uhat, const double *nhat, size_t fs, int first_var, const double *DDX,
+template <typename T>
+__global__ void k_phiscale(T *uhat, const T *nhat, size_t fs, int first_var, const double *DDX,
                            const double *DDY, const double *DDZ, phi_args a, int order2) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
@@ -585,18 +595,19 @@ __global__ void k_phiscale(double *uhat, const double *nhat, size_t fs, int firs
     phi1 = a.dt;
     phi2 = a.dt / 2;
   }
-  double *uh = uhat + fs * blockIdx.y;
+  T *uh = uhat + fs * blockIdx.y;
   if (order2) {
-    uh[l] = phi2 * uh[l];
+    uh[l] = (T)(phi2 * (double)uh[l]);
   } else {
-    const double *nh = nhat + fs * blockIdx.y;
+    const T *nh = nhat + fs * blockIdx.y;
     const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
     const double tmp = 1.0 - phi1 * Hp;
-    uh[l] = tmp * uh[l] + phi1 * nh[l];
+    uh[l] = (T)(tmp * (double)uh[l] + phi1 * (double)nh[l]);
   }
 }
 
-void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var, int nvars, int order2) {
+template <typename T>
+void launch_phiscale(etd_ctx *c, T *uhat, const T *nhat, int first_var, int nvars, int order2) {
   phi_args a;
   a.n = c->n;
   a.nac = c->nac;
@@ -610,7 +621,7 @@ void launch_phiscale(etd_ctx *c, double *uhat, const double *nhat, int first_var
   a.kky = c->p.kky;
   a.kkz = c->p.kkz;
   etd_prof_begin(c, PC_PHI);
-  k_phiscale<<<dim3((unsigned)((c->n3 + 255) / 256), nvars), 256, 0, c->stream>>>(uhat, nhat, c->n3, first_var, c->DDX,
+  k_phiscale<T><<<dim3((unsigned)((c->n3 + 255) / 256), nvars), 256, 0, c->stream>>>(uhat, nhat, c->n3, first_var, c->DDX,
                                                                                   c->DDY, c->DDZ, a, order2);
   etd_prof_end(c, PC_PHI);
 }
@@ -622,7 +633,8 @@ struct nl_args {
   double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
 };
 
-__device__ __forceinline__ double Pimg(const double *u, int n, int g, int z, int y, int x) {
+template <typename T>
+__device__ __forceinline__ double Pimg(const T *u, int n, int g, int z, int y, int x) {
   return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
 }
 
@@ -633,7 +645,8 @@ __device__ __forceinline__ double Pimg(const double *u, int n, int g, int z, int
 // a ghost of f carry values of earlier variants: with d = number of ghost
 // coordinates, the value is the interior of variant p - max(0, d-1) at the
 // fully wrapped position (variants before 0 come from the previous call).
-__device__ __forceinline__ double felas_at(const double *E, const nl_args &a, int p, int k, int j, int i) {
+template <typename T>
+__device__ __forceinline__ double felas_at(const T *E, const nl_args &a, int p, int k, int j, int i) {
   const int n = a.n, g = a.g, N = a.N;
   const int gk = (k < g || k >= n - g), gj = (j < g || j >= n - g), gi = (i < g || i >= n - g);
   int lag = gk + gj + gi - 1;
@@ -646,8 +659,9 @@ __device__ __forceinline__ double felas_at(const double *E, const nl_args &a, in
 
 // ac_calc_FU (calc.c:8-68) -> ac_calc_F1 (anisotropic.c:9-161): all AC fields
 // of one point per thread (sum u^2 couples them).
-__global__ void k_nonlin_ac(const double *__restrict__ u, double *__restrict__ out, const double *__restrict__ sub,
-                            const double *__restrict__ E, nl_args a, int with_felas) {
+template <typename T>
+__global__ void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out, const T *__restrict__ sub,
+                            const T *__restrict__ E, nl_args a, int with_felas) {
   const int n = a.n, g = a.g;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -659,7 +673,7 @@ __global__ void k_nonlin_ac(const double *__restrict__ u, double *__restrict__ o
     s = s + um * um;
   }
   for (int p = 0; p < a.nac; p++) {
-    const double *up_f = u + n3 * p;
+    const T *up_f = u + n3 * p;
     const double up = up_f[l];
     const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
     double f = a.LE * a.KE * up;
@@ -673,14 +687,14 @@ __global__ void k_nonlin_ac(const double *__restrict__ u, double *__restrict__ o
     if (k == n - 1) f += ce * Pimg(up_f, n, g, n, j, i) / a.hz / a.hz;
     if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
     if (sub) f -= sub[n3 * p + l];
-    out[n3 * p + l] = f;
+    out[n3 * p + l] = (T)f;
   }
 }
 
 // ac_calc_F2 (anisotropic.c:164-627), pass 1: G = Lambda (gx, gy, gz) with
 // gy, gz taken as (u[-1] - u[+1]) / 2h; out-of-box neighbours read 0.
-__global__ void k_f2_grad(const double *__restrict__ u, double *__restrict__ G, const double *__restrict__ lam,
-                          nl_args a) {
+template <typename T>
+__global__ void k_f2_grad(const T *__restrict__ u, T *__restrict__ G, const double *__restrict__ lam, nl_args a) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,21 +709,22 @@ __global__ void k_f2_grad(const double *__restrict__ u, double *__restrict__ G, 
   const double gx = (a_right - a_left) / (2.0 * a.hx);
   const double gy = (a_top - a_bottom) / (2.0 * a.hy);
   const double gz = (a_front - a_back) / (2.0 * a.hz);
-  G[l] = lam[0] * gx + lam[1] * gy + lam[2] * gz;
-  G[n3 + l] = lam[3] * gx + lam[4] * gy + lam[5] * gz;
-  G[2 * n3 + l] = lam[6] * gx + lam[7] * gy + lam[8] * gz;
+  G[l] = (T)(lam[0] * gx + lam[1] * gy + lam[2] * gz);
+  G[n3 + l] = (T)(lam[3] * gx + lam[4] * gy + lam[5] * gz);
+  G[2 * n3 + l] = (T)(lam[6] * gx + lam[7] * gy + lam[8] * gz);
 }
 
 // pass 2: out -= epn2 LE div(G) (then the elastic term, keeping the
 // reference's order f1 - epn2 LE F2 - LE ES felas)
-__global__ void k_f2_div(const double *__restrict__ G, double *__restrict__ out, const double *__restrict__ E,
-                         nl_args a, int p, int with_felas) {
+template <typename T>
+__global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *__restrict__ E, nl_args a, int p,
+                         int with_felas) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
   const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
-  const double *gx = G, *gy = G + n3, *gz = G + 2 * n3;
+  const T *gx = G, *gy = G + n3, *gz = G + 2 * n3;
   const double a_left = i > 0 ? gx[l - 1] : 0.0;
   const double a_right = i < n - 1 ? gx[l + 1] : 0.0;
   const double a_top = j > 0 ? gy[l - n] : 0.0;
@@ -723,14 +738,15 @@ __global__ void k_f2_div(const double *__restrict__ G, double *__restrict__ out,
   double f = out[l];
   f -= a.epn2 * a.LE * tmp;
   if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
-  out[l] = f;
+  out[l] = (T)f;
 }
 
 __device__ __forceinline__ double ch_f(double cc) { return cc * (cc * cc - 1.0); }
 
 // ch_calc_FU_ConstantMobility (calc.c:70-473), pass 1:
 // t = -LCI KCI c + LCI CH_f(c), minus LCI epn2 c_out / h^2 on the faces.
-__global__ void k_ch_t(const double *__restrict__ cfield, double *__restrict__ t, nl_args a) {
+template <typename T>
+__global__ void k_ch_t(const T *__restrict__ cfield, T *__restrict__ t, nl_args a) {
   const int n = a.n, g = a.g;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -746,12 +762,13 @@ __global__ void k_ch_t(const double *__restrict__ cfield, double *__restrict__ t
   if (j == n - 1) f -= ce * Pimg(cfield, n, g, k, n, i) / a.hy / a.hy;
   if (k == 0) f -= ce * Pimg(cfield, n, g, -1, j, i) / a.hz / a.hz;
   if (k == n - 1) f -= ce * Pimg(cfield, n, g, n, j, i) / a.hz / a.hz;
-  t[l] = f;
+  t[l] = (T)f;
 }
 
 // chemical potential LCI (epn2 Lap c - CH_f(c)) at an outside cell, from the
 // periodic image (ch_mu, trans.c:1433-1937)
-__device__ double ch_mu_out(const double *c, const nl_args &a, int z, int y, int x) {
+template <typename T>
+__device__ double ch_mu_out(const T *c, const nl_args &a, int z, int y, int x) {
   const int n = a.n, g = a.g;
   const double v0 = Pimg(c, n, g, z, y, x);
   double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) / a.hx / a.hx;
@@ -763,8 +780,9 @@ __device__ double ch_mu_out(const double *c, const nl_args &a, int z, int y, int
 }
 
 // pass 2: out = Lap_T t (out-of-box neighbours 0) - mu_out / h^2 on faces
-__global__ void k_ch_lap(const double *__restrict__ cfield, const double *__restrict__ t, double *__restrict__ out,
-                         const double *__restrict__ sub, nl_args a) {
+template <typename T>
+__global__ void k_ch_lap(const T *__restrict__ cfield, const T *__restrict__ t, T *__restrict__ out,
+                         const T *__restrict__ sub, nl_args a) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -787,15 +805,17 @@ __global__ void k_ch_lap(const double *__restrict__ cfield, const double *__rest
   if (k == 0) r -= ch_mu_out(cfield, a, -1, j, i) / a.hz / a.hz;
   if (k == n - 1) r -= ch_mu_out(cfield, a, n, j, i) / a.hz / a.hz;
   if (sub) r -= sub[l];
-  out[l] = r;
+  out[l] = (T)r;
 }
 
-__global__ void k_sub(double *out, const double *sub, size_t cnt) {
+template <typename T>
+__global__ void k_sub(T *out, const T *sub, size_t cnt) {
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l < cnt) out[l] = out[l] - sub[l];
 }
-void launch_sub(etd_ctx *c, double *out, const double *sub, size_t cnt) {
-  k_sub<<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(out, sub, cnt);
+template <typename T>
+void launch_sub(etd_ctx *c, T *out, const T *sub, size_t cnt) {
+  k_sub<T><<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(out, sub, cnt);
 }
 
 static nl_args make_nl(etd_ctx *c) {
@@ -826,8 +846,10 @@ static nl_args make_nl(etd_ctx *c) {
 
 // N for every variable into out (V x n3); when sub != NULL the result is
 // N - sub (the ETD2 difference prepare_U1_new, calc.c:537-551).
-void launch_nonlinear(etd_ctx *c, double *out, const double *sub) {
+template <typename T>
+void launch_nonlinear(etd_ctx *c, T *out, const T *sub) {
   nl_args a = make_nl(c);
+  const etd_fields<T> F = fields<T>(c);
   const size_t n3 = c->n3;
   const unsigned nb = (unsigned)((n3 + 255) / 256);
   const int felas = c->p.elastic == 1 && c->nac > 0;
@@ -835,18 +857,18 @@ void launch_nonlinear(etd_ctx *c, double *out, const double *sub) {
   if (c->nac > 0) {
     const int aniso = c->p.anisotropic;
     // with F2 the felas and ETD2 subtraction move to the divergence pass
-    k_nonlin_ac<<<nb, 256, 0, c->stream>>>(c->u, out, aniso ? nullptr : sub, c->E, a, felas && !aniso);
+    k_nonlin_ac<T><<<nb, 256, 0, c->stream>>>(F.u, out, aniso ? nullptr : sub, F.E, a, felas && !aniso);
     if (aniso)
       for (int p = 0; p < c->nac; p++) {
-        k_f2_grad<<<nb, 256, 0, c->stream>>>(c->u + n3 * p, c->gtmp, c->lam + 9 * p, a);
-        k_f2_div<<<nb, 256, 0, c->stream>>>(c->gtmp, out + n3 * p, c->E, a, p, felas);
+        k_f2_grad<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * p, F.gtmp, c->lam + 9 * p, a);
+        k_f2_div<T><<<nb, 256, 0, c->stream>>>(F.gtmp, out + n3 * p, F.E, a, p, felas);
         if (sub) launch_sub(c, out + n3 * p, sub + n3 * p, n3);  // after the F2 / felas terms
       }
   }
   for (int q = 0; q < c->nch; q++) {
     const int v = c->nac + q;
-    k_ch_t<<<nb, 256, 0, c->stream>>>(c->u + n3 * v, c->gtmp, a);
-    k_ch_lap<<<nb, 256, 0, c->stream>>>(c->u + n3 * v, c->gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr, a);
+    k_ch_t<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, a);
+    k_ch_lap<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr, a);
   }
   etd_prof_end(c, PC_NONLIN);
 }
@@ -947,13 +969,14 @@ __device__ __forceinline__ void fft_pass(const double2 *__restrict__ src, double
   }
 }
 
-template <int SGN>
-__global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int axis, fft_plan pl,
+// T2 = double2 or float2 in HBM; the lines are transformed in FP64 in LDS
+template <int SGN, typename T2>
+__global__ __launch_bounds__(256) void k_fft_axis(T2 *data, size_t fs, int axis, fft_plan pl,
                                                   const double2 *__restrict__ tw) {
   extern __shared__ double2 sbuf[];
   const int N = pl.N;
   double2 *buf0 = sbuf, *buf1 = sbuf + FFT_LINES * N;
-  double2 *d = data + fs * blockIdx.y;
+  T2 *d = data + fs * blockIdx.y;
   const unsigned uN = (unsigned)N, N2 = uN * uN;
   const unsigned line0 = blockIdx.x * FFT_LINES;  // first line of the group
   const unsigned tot = FFT_LINES * uN;
@@ -972,7 +995,12 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int 
       const unsigned L = line0 + ln, x = L % uN, o = L / uN;
       addr = axis == 1 ? x + (size_t)uN * t + (size_t)N2 * o : x + (size_t)uN * o + (size_t)N2 * t;
     }
-    buf0[ln * uN + t] = line0 + ln < N2 ? d[addr] : make_double2(0.0, 0.0);
+    double2 v = make_double2(0.0, 0.0);
+    if (line0 + ln < N2) {
+      const T2 x = d[addr];
+      v = make_double2(x.x, x.y);
+    }
+    buf0[ln * uN + t] = v;
   }
   __syncthreads();
   double2 *src = buf0, *dst = buf1;
@@ -1004,11 +1032,19 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *data, size_t fs, int 
       const unsigned L = line0 + ln, x = L % uN, o = L / uN;
       addr = axis == 1 ? x + (size_t)uN * t + (size_t)N2 * o : x + (size_t)uN * o + (size_t)N2 * t;
     }
-    if (line0 + ln < N2) d[addr] = src[ln * uN + t];
+    if (line0 + ln < N2) {
+      const double2 v = src[ln * uN + t];
+      T2 x;
+      x.x = v.x;
+      x.y = v.y;
+      d[addr] = x;
+    }
   }
 }
 
-void fft3d(etd_ctx *c, double *data, int npair, int sgn) {
+template <typename T>
+void fft3d(etd_ctx *c, T *data, int npair, int sgn) {
+  using T2 = typename cplx<T>::t;
   fft_plan pl;
   pl.N = c->N;
   pl.npass = c->fft_npass;
@@ -1018,14 +1054,15 @@ void fft3d(etd_ctx *c, double *data, int npair, int sgn) {
   size_t lds = 2 * FFT_LINES * (size_t)c->N * sizeof(double2);
   for (int ax = 0; ax < 3; ax++) {
     if (sgn > 0)
-      k_fft_axis<1><<<grid, 256, lds, c->stream>>>((double2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+      k_fft_axis<1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
     else
-      k_fft_axis<-1><<<grid, 256, lds, c->stream>>>((double2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+      k_fft_axis<-1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
   }
 }
 
 // pack interior eta_2r + i eta_2r+1
-__global__ void k_el_pack(const double *__restrict__ u, double2 *__restrict__ Z, int n, int g, int N, int nac) {
+template <typename T>
+__global__ void k_el_pack(const T *__restrict__ u, typename cplx<T>::t *__restrict__ Z, int n, int g, int N, int nac) {
   const size_t N3 = (size_t)N * N * N;
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= N3) return;
@@ -1034,9 +1071,10 @@ __global__ void k_el_pack(const double *__restrict__ u, double2 *__restrict__ Z,
   const size_t n3 = (size_t)n * n * n;
   const size_t src = IDX(n, z + g, y + g, x + g);
   const int pa = 2 * r, pb = 2 * r + 1;
-  double re = u[n3 * pa + src];
-  double im = pb < nac ? u[n3 * pb + src] : 0.0;
-  Z[N3 * r + m] = make_double2(re, im);
+  typename cplx<T>::t zz;
+  zz.x = u[n3 * pa + src];
+  zz.y = pb < nac ? u[n3 * pb + src] : (T)0;
+  Z[N3 * r + m] = zz;
 }
 
 // k-space contraction acc_p = sum_q B_pq(n) F_q with B computed on the fly
@@ -1099,8 +1137,8 @@ __device__ __forceinline__ void b_wrow(const double *sp, const double *nv, const
 // reference's n_i s_p,ij Omega_jk s_q,kl n_l for symmetric Omega and
 // sigma); with a symmetric Voigt matrix B is symmetric and each unordered
 // pair is formed once.  MIRROR: B is replaced by (B(k) + B(-k)) / 2.
-template <int NAC, bool MIRROR>
-__device__ __forceinline__ void contract_point(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+template <int NAC, bool MIRROR, typename T2>
+__device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__restrict__ Y, int N,
                                                int npair, size_t m, const etd_dev_consts *__restrict__ K) {
   const size_t N3 = (size_t)N * N * N;
   const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
@@ -1109,7 +1147,8 @@ __device__ __forceinline__ void contract_point(const double2 *__restrict__ Z, do
   double2 F[NAC];
 #pragma unroll
   for (int r = 0; r < (NAC + 1) / 2; r++) {
-    const double2 a = Z[N3 * r + m], b = Z[N3 * r + mm];
+    const T2 za = Z[N3 * r + m], zb = Z[N3 * r + mm];
+    const double2 a = make_double2(za.x, za.y), b = make_double2(zb.x, zb.y);
     // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
     F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
     if (2 * r + 1 < NAC) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
@@ -1158,26 +1197,31 @@ __device__ __forceinline__ void contract_point(const double2 *__restrict__ Z, do
     }
 #pragma unroll
   for (int r = 0; r < (NAC + 1) / 2; r++) {
-    if (2 * r + 1 < NAC)  // Y_r = acc_2r + i acc_2r+1
-      Y[N3 * r + m] = make_double2(acc[2 * r].x - acc[2 * r + 1].y, acc[2 * r].y + acc[2 * r + 1].x);
-    else
-      Y[N3 * r + m] = acc[2 * r];
+    T2 y;
+    if (2 * r + 1 < NAC) {  // Y_r = acc_2r + i acc_2r+1
+      y.x = acc[2 * r].x - acc[2 * r + 1].y;
+      y.y = acc[2 * r].y + acc[2 * r + 1].x;
+    } else {
+      y.x = acc[2 * r].x;
+      y.y = acc[2 * r].y;
+    }
+    Y[N3 * r + m] = y;
   }
 }
 
-template <int NAC>
-__global__ __launch_bounds__(256) void k_el_contract_t(const double2 *__restrict__ Z, double2 *__restrict__ Y, int N,
+template <int NAC, typename T2>
+__global__ __launch_bounds__(256) void k_el_contract_t(const T2 *__restrict__ Z, T2 *__restrict__ Y, int N,
                                                        int npair, const etd_dev_consts *__restrict__ K) {
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= (size_t)N * N * N) return;
-  contract_point<NAC, false>(Z, Y, N, npair, m, K);
+  contract_point<NAC, false, T2>(Z, Y, N, npair, m, K);
 }
 
 // points whose mirror's signed wavenumber is not the negative: the index
 // planes c = N/2 (and c = N/2 + 1 for odd N) in each direction; one thread
 // per (plane, position), duplicates across the planes write equal values
-template <int NAC>
-__global__ __launch_bounds__(256) void k_el_contract_mirror(const double2 *__restrict__ Z, double2 *__restrict__ Y,
+template <int NAC, typename T2>
+__global__ __launch_bounds__(256) void k_el_contract_mirror(const T2 *__restrict__ Z, T2 *__restrict__ Y,
                                                             int N, int npair, const etd_dev_consts *__restrict__ K) {
   const int nplane = (N & 1) ? 2 : 1;
   const size_t N2 = (size_t)N * N;
@@ -1192,62 +1236,68 @@ __global__ __launch_bounds__(256) void k_el_contract_mirror(const double2 *__res
   if (axis == 0) { x = c; y = a; z = b; }
   else if (axis == 1) { x = a; y = c; z = b; }
   else { x = a; y = b; z = c; }
-  contract_point<NAC, true>(Z, Y, N, npair, ((size_t)z * N + y) * N + x, K);
+  contract_point<NAC, true, T2>(Z, Y, N, npair, ((size_t)z * N + y) * N + x, K);
 }
 
-__global__ void k_el_unpack(const double2 *__restrict__ Y, double *__restrict__ E, int N, int nac, int ring,
+template <typename T>
+__global__ void k_el_unpack(const typename cplx<T>::t *__restrict__ Y, T *__restrict__ E, int N, int nac, int ring,
                             long long Lbase) {
   const size_t N3 = (size_t)N * N * N;
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= N3) return;
   const int r = blockIdx.y;
   const double inv = 1.0 / ((double)N * N * N);
-  const double2 v = Y[N3 * r + m];
+  const typename cplx<T>::t v = Y[N3 * r + m];
   int pa = 2 * r, pb = 2 * r + 1;
-  E[(size_t)((Lbase + pa) % ring) * N3 + m] = v.x * inv;
-  if (pb < nac) E[(size_t)((Lbase + pb) % ring) * N3 + m] = v.y * inv;
+  E[(size_t)((Lbase + pa) % ring) * N3 + m] = (T)((double)v.x * inv);
+  if (pb < nac) E[(size_t)((Lbase + pb) % ring) * N3 + m] = (T)((double)v.y * inv);
 }
 
+template <typename T>
 void launch_elastic(etd_ctx *c) {
+  using T2 = typename cplx<T>::t;
+  const etd_fields<T> F = fields<T>(c);
   const int N = c->N;
   const size_t N3 = c->N3;
   const unsigned nb = (unsigned)((N3 + 255) / 256);
   etd_prof_begin(c, PC_ELASTIC);
-  k_el_pack<<<dim3(nb, c->npair), 256, 0, c->stream>>>(c->u, (double2 *)c->Z, c->n, c->g, N, c->nac);
-  fft3d(c, c->Z, c->npair, +1);
+  k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(F.u, (T2 *)F.Z, c->n, c->g, N, c->nac);
+  fft3d<T>(c, F.Z, c->npair, +1);
   {
     const int nplane = (N & 1) ? 2 : 1;
     const unsigned nbm = (unsigned)((3 * nplane * (size_t)N * N + 255) / 256);
-    const double2 *Zc = (const double2 *)c->Z;
-    double2 *Yc = (double2 *)c->Y;
+    const T2 *Zc = (const T2 *)F.Z;
+    T2 *Yc = (T2 *)F.Y;
     switch (c->nac) {
-#define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X><<<nb, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
+#define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X, T2><<<nb, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X, T2><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
       NACC(1) NACC(2) NACC(3) NACC(4) NACC(5) NACC(6) NACC(7) NACC(8) NACC(9) NACC(10) NACC(11) NACC(12) NACC(13)
       NACC(14) NACC(15) NACC(16)
 #undef NACC
       default: break;
     }
   }
-  fft3d(c, c->Y, c->npair, -1);
+  fft3d<T>(c, F.Y, c->npair, -1);
   const long long Lbase = c->elastic_evals * (long long)c->nac;
-  k_el_unpack<<<dim3(nb, c->npair), 256, 0, c->stream>>>((const double2 *)c->Y, c->E, N, c->nac, c->ring, Lbase);
+  k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, Lbase);
   c->elastic_evals++;
   etd_prof_end(c, PC_ELASTIC);
 }
 
-__global__ void k_felas_mat(double *felas, const double *E, nl_args a) {
+template <typename T>
+__global__ void k_felas_mat(T *felas, const T *E, nl_args a) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
   const int p = blockIdx.y;
   const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
-  felas[n3 * p + l] = felas_at(E, a, p, k, j, i);
+  felas[n3 * p + l] = (T)felas_at(E, a, p, k, j, i);
 }
 
-void launch_felas_materialize(etd_ctx *c, double *felas) {
+template <typename T>
+void launch_felas_materialize(etd_ctx *c, T *felas) {
   nl_args a = make_nl(c);
-  k_felas_mat<<<dim3((unsigned)((c->n3 + 255) / 256), c->nac), 256, 0, c->stream>>>(felas, c->E, a);
+  k_felas_mat<T><<<dim3((unsigned)((c->n3 + 255) / 256), c->nac), 256, 0, c->stream>>>(felas, fields<T>(c).E, a);
 }
 
 // ------------------------------------------------------------------ energy
@@ -1272,11 +1322,11 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // grid (planes = n - 2g, V): max, min, sum(u + 1) over the plane interior
-__global__ __launch_bounds__(256) void k_diag_stats(const double *__restrict__ u, double *__restrict__ part,
-                                                    diag_args a) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_stats(const T *__restrict__ u, double *__restrict__ part, diag_args a) {
   const int n = a.n, g = a.g, m = n - 2 * g;
   const int k = g + blockIdx.x, v = blockIdx.y;
-  const double *f = u + (size_t)n * n * n * v;
+  const T *f = u + (size_t)n * n * n * v;
   double mx = -1.0e30, mn = 1.0e30, s = 0.0;
   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
     const int j = g + t / m, i = g + t % m;
@@ -1304,7 +1354,8 @@ __global__ __launch_bounds__(256) void k_diag_stats(const double *__restrict__ u
   }
 }
 
-__device__ __forceinline__ void cell_grad(const double *u, int n, int k, int j, int i, double hx, double hy, double hz,
+template <typename T>
+__device__ __forceinline__ void cell_grad(const T *u, int n, int k, int j, int i, double hx, double hy, double hz,
                                           double &mean, double &ux, double &uy, double &uz) {
   double f0 = u[IDX(n, k, j, i)], f1 = u[IDX(n, k, j, i + 1)], f2 = u[IDX(n, k, j + 1, i)],
          f3 = u[IDX(n, k, j + 1, i + 1)];
@@ -1320,7 +1371,8 @@ __device__ __forceinline__ void cell_grad(const double *u, int n, int k, int j, 
 // 1..nch CH energy of field nac + y - 1, nch + 1 couple gradient terms; each
 // plane is split into ECHUNK row blocks
 #define ECHUNK 8
-__global__ __launch_bounds__(256) void k_diag_energy(const double *__restrict__ u, const double *__restrict__ lam,
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, const double *__restrict__ lam,
                                                      double *__restrict__ part, diag_args a) {
   const int n = a.n, g = a.g, m = n - 2 * g - 1;
   const int k = g + blockIdx.x, kind = blockIdx.y;
@@ -1368,7 +1420,9 @@ __global__ __launch_bounds__(256) void k_diag_energy(const double *__restrict__ 
   if (threadIdx.x == 0) part[((size_t)kind * m + blockIdx.x) * ECHUNK + blockIdx.z] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+template <typename T>
 int launch_diag(etd_ctx *c, double *stats) {
+  const T *U = fields<T>(c).u;
   diag_args a;
   a.n = c->n;
   a.g = c->g;
@@ -1386,8 +1440,8 @@ int launch_diag(etd_ctx *c, double *stats) {
   const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
   double *pst = c->part, *pen = c->part + (size_t)3 * ms * V;
   etd_prof_begin(c, PC_ENERGY);
-  k_diag_stats<<<dim3(ms, V), 256, 0, c->stream>>>(c->u, pst, a);
-  k_diag_energy<<<dim3(me, kinds, ECHUNK), 256, 0, c->stream>>>(c->u, c->lam, pen, a);
+  k_diag_stats<T><<<dim3(ms, V), 256, 0, c->stream>>>(U, pst, a);
+  k_diag_energy<T><<<dim3(me, kinds, ECHUNK), 256, 0, c->stream>>>(U, c->lam, pen, a);
   etd_prof_end(c, PC_ENERGY);
   const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds * ECHUNK;
   if (hipMemcpyAsync(c->hpart, c->part, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -1442,3 +1496,16 @@ void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16) {
   const size_t blocks = (n16 + 255) / 256;
   k_copy16<<<(unsigned)(blocks < 8192 ? blocks : 8192), 256, 0, c->stream>>>((const uint4 *)in, (uint4 *)out, n16);
 }
+
+// ------------------------------------------------------------------ instantiations (FP64 and FP32 storage)
+#define ETD_INST(T)                                                                   \
+  template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
+  template void launch_phiscale<T>(etd_ctx *, T *, const T *, int, int, int);        \
+  template void launch_nonlinear<T>(etd_ctx *, T *, const T *);                      \
+  template void launch_elastic<T>(etd_ctx *);                                        \
+  template void fft3d<T>(etd_ctx *, T *, int, int);                                  \
+  template void launch_felas_materialize<T>(etd_ctx *, T *);                         \
+  template int launch_diag<T>(etd_ctx *, double *);
+ETD_INST(double)
+ETD_INST(float)
+#undef ETD_INST

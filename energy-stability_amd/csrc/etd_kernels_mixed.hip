@@ -1,0 +1,290 @@
+// etd_kernels_mixed.hip -- mode products of the mixed-precision step on
+// gfx950: FP32 fields, V / V^-1 split into BF16 terms, FP32 MFMA
+// accumulation (v_mfma_f32_16x16x32_bf16).
+//
+// A value x is split exactly into BF16 terms x = x0 + x1 + x2 (x0 = bf16(x),
+// x1 = bf16(x - x0), x2 = x - x0 - x1; each residual is exact in FP32 and the
+// last one has at most 8 significant bits, so three terms hold every FP32
+// value).  With A = a0 + a1 + a2 (split from FP64) the product keeps the six
+// terms a_i x_j with i + j <= 2 ("split-3"); the dropped ones are 2^-24
+// relative, below FP32 rounding.  Two terms (i + j <= 1: three products,
+// "split-2") keep 2^-16.
+//
+// Same contraction as the FP64 kernel: C(m, col) = sum_a A(m,a) in[a + n col],
+// written rotated (out[col + n^2 m]) or in GEMM layout (out[m + n col]).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "etd_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4m __attribute__((ext_vector_type(4)));
+
+// ---- global -> LDS copies (buffer_load ... lds), as in the FP64 kernel
+__device__ __forceinline__ i32x4m mk_rsrc(const void *base, uint32_t bytes) {
+  const uint64_t a = (uint64_t)base;
+  i32x4m r;
+  r.x = (int)(uint32_t)a;
+  r.y = (int)(uint32_t)(a >> 32);
+  r.z = (int)bytes;
+  r.w = 0x00020000;
+  return r;
+}
+
+// lane L's 16 bytes land at lds_wave_base + 16 L; offsets outside the
+// descriptor's range read as zero.  asm so the compiler does not make every
+// later LDS read wait for all copies in flight; the explicit vmcnt wait below
+// orders them.  One wait state between the M0 write and the copy.  M0 is
+// reserved to the compiler; the compiled kernels of this file use it only
+// inside these asm blocks (checked in the ISA), so the clobber holds.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void copy16_to_lds(i32x4m rs, uint32_t lds_wave_base, int voff) {
+  const uint32_t b = __builtin_amdgcn_readfirstlane(lds_wave_base);
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
+               :
+               : "s"(b), "v"(voff), "s"(rs)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+__device__ __forceinline__ void wait_all_copies_barrier() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" : : : "memory"); }
+
+// ---- tiling
+// A workgroup of WAVES waves owns 32 * WAVES columns and all MT row tiles of
+// 16; K runs in chunks of 32 (one MFMA).  Per chunk the LDS stage holds
+//   A: terms x MT x 64 lanes x 16 B, pre-arranged in operand order, and
+//   X: 32 k x (32 WAVES) columns FP32 as [q = k/8][col][8 k] (a lane's 8
+//      consecutive k of one column are 32 contiguous bytes, and the 16 lanes
+//      of an operand read 16 consecutive 32-byte blocks).
+template <int MT, int NT, int WAVES>
+struct mps_cfg {
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int COLS = 32 * WAVES;
+  static constexpr int APIECES = NT * MT * 64;                // 16-byte pieces of one A chunk
+  static constexpr int APT = (APIECES + THREADS - 1) / THREADS;  // per thread
+  static constexpr int ASZ = APT * THREADS * 16;               // A chunk bytes (zero padded)
+  static constexpr int XSZ = 32 * COLS * 4;                    // X chunk bytes
+  static constexpr int XPT = XSZ / 16 / THREADS;               // = 4
+  static constexpr int STAGE = ASZ + XSZ;
+  static constexpr int LDS = 2 * STAGE;
+  static_assert(XPT == 4, "X pieces");
+};
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &t0, __bf16 &t1, __bf16 &t2) {
+  t0 = (__bf16)x;
+  const float r1 = x - (float)t0;
+  t1 = (__bf16)r1;
+  t2 = (__bf16)(r1 - (float)t1);
+}
+
+template <int MT, int NT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const uint16_t *__restrict__ A3, const float *__restrict__ in,
+                                                                  size_t in_fs, float *__restrict__ out, size_t out_fs,
+                                                                  int n, int rotate, int accumulate) {
+  using CF = mps_cfg<MT, NT, WAVES>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float *src = in + in_fs * blockIdx.y;
+  float *dst = out + out_fs * blockIdx.y;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const long wgcol = (long)blockIdx.x * CF::COLS;
+  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * CF::ASZ));
+  const i32x4m rsX = mk_rsrc(src, (uint32_t)((size_t)n * n2 * 4));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue = [&](int kc, int stg) {
+    const uint32_t sbase = lds0 + stg * CF::STAGE;
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
+                    (int)(((size_t)kc * CF::APT * CF::THREADS + e * CF::THREADS + threadIdx.x) * 16));
+    // X piece e of this lane: q = e, column 32 wave + lane/2, k half lane%2
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      const long col = wgcol + 32 * wave + (lane >> 1);
+      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + CF::ASZ + (e * CF::THREADS + wave * 64) * 16, voff);
+    }
+  };
+
+  f32x4 acc[MT][2];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue(0, 0);
+  for (int kc = 0; kc < nkc; kc++) {
+    // chunk kc is in LDS for every wave, and every wave is past chunk kc-1,
+    // whose stage the next copies overwrite
+    wait_all_copies_barrier();
+    if (kc + 1 < nkc) issue(kc + 1, (kc + 1) & 1);
+    const char *stg = smem + (kc & 1) * CF::STAGE;
+    const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(stg);
+    const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
+    bf16x8 b[2][NT];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+      const float4 lo = *reinterpret_cast<const float4 *>(bp);
+      const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+      const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        __bf16 t0, t1, t2;
+        split_bf16(x[e], t0, t1, t2);
+        b[u][0][e] = t0;
+        if constexpr (NT > 1) b[u][1][e] = t1;
+        if constexpr (NT > 2) b[u][2][e] = t2;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      bf16x8 a[NT];
+#pragma unroll
+      for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        // smallest terms first
+        if constexpr (NT > 2) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[t][u], 0, 0, 0);
+        }
+        if constexpr (NT > 1) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[t][u], 0, 0, 0);
+        }
+        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[t][u], 0, 0, 0);
+      }
+    }
+  }
+  // D: lane holds rows 4 q + i of column r16
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const long col = wgcol + 32 * wave + 16 * u + r16;
+      if (col >= n2) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int m = 16 * t + 4 * q + i;
+        if (m >= n) continue;
+        const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
+        if (accumulate) dst[o] += acc[t][u][i];
+        else dst[o] = acc[t][u][i];
+      }
+    }
+}
+
+// ---- host side
+static inline int waves_for(int MT) { return MT <= 14 ? 8 : 4; }
+
+template <int MT, int NT>
+static size_t chunk_bytes() {
+  if (waves_for(MT) == 8) return mps_cfg<MT, NT, 8>::ASZ;
+  return mps_cfg<MT, NT, 4>::ASZ;
+}
+
+static size_t chunk_bytes_rt(int MT, int NT) {
+  switch (MT) {
+#define C1(X) \
+  case X: return NT == 3 ? chunk_bytes<X, 3>() : chunk_bytes<X, 2>();
+    C1(1) C1(2) C1(3) C1(4) C1(5) C1(6) C1(7) C1(8) C1(9) C1(10) C1(11) C1(12) C1(13) C1(14) C1(15) C1(16) C1(17)
+#undef C1
+    default: return 0;
+  }
+}
+
+size_t split_A_bytes(int n, int terms) {
+  const int MT = (n + 15) / 16, nkc = (n + 31) / 32;
+  return chunk_bytes_rt(MT, terms) * nkc;
+}
+
+static uint16_t bf16_rn(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double bf16_val(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+// A (column-major FP64) -> per chunk kc: [term][row tile t][lane][8 bf16],
+// element e of lane l = term of A(16 t + l % 16, 32 kc + 8 (l / 16) + e),
+// zero outside n x n; chunks padded to the kernel's copy granularity.
+void arrange_split_A(int n, const double *Acm, uint16_t *A3, int terms) {
+  const int MT = (n + 15) / 16, nkc = (n + 31) / 32;
+  const size_t cb = chunk_bytes_rt(MT, terms);
+  memset(A3, 0, cb * nkc);
+  for (int kc = 0; kc < nkc; kc++) {
+    uint16_t *ch = A3 + (size_t)kc * cb / 2;
+    for (int t = 0; t < MT; t++)
+      for (int l = 0; l < 64; l++)
+        for (int e = 0; e < 8; e++) {
+          const int m = 16 * t + (l & 15), k = 32 * kc + 8 * (l >> 4) + e;
+          const double a = (m < n && k < n) ? Acm[m + (size_t)n * k] : 0.0;
+          double r = a;
+          for (int tm = 0; tm < terms; tm++) {
+            const uint16_t h = bf16_rn((float)r);
+            ch[(((size_t)tm * MT + t) * 64 + l) * 8 + e] = h;
+            r -= bf16_val(h);
+          }
+        }
+  }
+}
+
+template <int MT, int NT>
+static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
+                       int nfields, int rotate, int accumulate) {
+  const int n = c->n;
+  const long n2 = (long)n * n;
+  auto go = [&](auto wtag) {
+    constexpr int W = decltype(wtag)::value;
+    using CF = mps_cfg<MT, NT, W>;
+    static_assert(CF::LDS <= 160 * 1024, "LDS budget");
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void *)k_modeprod_split<MT, NT, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                CF::LDS);
+      attr_set = true;
+    }
+    dim3 grid((unsigned)((n2 + CF::COLS - 1) / CF::COLS), nfields);
+    k_modeprod_split<MT, NT, W><<<grid, CF::THREADS, CF::LDS, c->stream>>>(A3, in, in_fs, out, out_fs, n, rotate,
+                                                                          accumulate);
+  };
+  if constexpr (MT <= 14)
+    go(std::integral_constant<int, 8>{});
+  else
+    go(std::integral_constant<int, 4>{});
+}
+
+void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
+                           int nfields, int rotate, int accumulate) {
+  const int MT = (c->n + 15) / 16;
+  etd_prof_begin(c, PC_MODEPROD);
+  switch (MT) {
+#define MC(X)                                                                                      \
+  case X:                                                                                          \
+    if (c->split_terms == 2)                                                                       \
+      mps_launch<X, 2>(c, A3, in, in_fs, out, out_fs, nfields, rotate, accumulate);                \
+    else                                                                                           \
+      mps_launch<X, 3>(c, A3, in, in_fs, out, out_fs, nfields, rotate, accumulate);                \
+    break;
+    MC(1) MC(2) MC(3) MC(4) MC(5) MC(6) MC(7) MC(8) MC(9) MC(10) MC(11) MC(12) MC(13) MC(14) MC(15) MC(16) MC(17)
+#undef MC
+    default: break;
+  }
+  etd_prof_end(c, PC_MODEPROD);
+}

@@ -1,0 +1,137 @@
+"""GPU parity of the mixed-precision path (FP32 fields, split-BF16 MFMA mode
+products with FP32 accumulation, FP64 energy), called through the C ABI on a
+context created with precision = ETD_PREC_MIXED, against the FP64 CPU oracle.
+
+Tolerances follow from the formats: FP32 storage rounds at 6e-8 relative; the
+split-3 product keeps every term above 2^-24; split-2 keeps 2^-16 (2.5e-5
+relative per product, measured on the CPU in the split arithmetic).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pyetd
+from conftest import noise_fields, small_params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def mixed(prm):
+    q = dict(prm)
+    q["precision"] = 1
+    return q
+
+
+def pair(prm, fields):
+    o = O.Oracle(prm)
+    Vm, Vim, D = O.eigbasis(prm["n"], 3)
+    g = pyetd.Etd(pyetd.params_from_dict(mixed(prm)), Vm, Vim, D)
+    for v, f in enumerate(fields):
+        o.set_field(v, f)
+        g.set_field(v, f)
+    return o, g
+
+
+@pytest.mark.parametrize("n", [12, 36, 68, 100])
+@pytest.mark.parametrize("rotate", [True, False])
+def test_split3_modeprod_matches_oracle(n, rotate):
+    say(f"split-3 modeprod n={n} rotate={rotate}")
+    rng = np.random.default_rng(n + 1)
+    Vm, Vim, D = O.eigbasis(n, 3)
+    u = rng.standard_normal(n ** 3)
+    g = pyetd.Etd.from_dict(mixed(small_params(n=n, nac=1)))
+    for A in (Vim, Vm):
+        got = g.modeprod(A, u, rotate)
+        ref = O.modeprod(A, u, rotate)
+        # FP32 input rounding and FP32 accumulation over n terms
+        assert np.abs(got - ref).max() < 2e-6 * np.abs(ref).max()
+
+
+def test_split2_modeprod_matches_oracle():
+    say("split-2 modeprod n=68")
+    os.environ["ETD_SPLIT"] = "2"
+    try:
+        n = 68
+        rng = np.random.default_rng(5)
+        Vm, Vim, D = O.eigbasis(n, 3)
+        u = rng.standard_normal(n ** 3)
+        g = pyetd.Etd.from_dict(mixed(small_params(n=n, nac=1)))
+        got = g.modeprod(Vim, u, True)
+    finally:
+        del os.environ["ETD_SPLIT"]
+    ref = O.modeprod(Vim, u, True)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say(f"  split-2 rel err {err:.2e}")
+    assert err < 2e-4
+
+
+def test_mixed_fft3d_matches_numpy():
+    say("fp32 fft3d N=32")
+    n = 36
+    g = pyetd.Etd.from_dict(mixed(small_params(n=n, nac=2, elastic=1)))
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((32, 32, 32)) + 1j * rng.standard_normal((32, 32, 32))
+    f = g.fft3d(x, 1)
+    ref = np.fft.fftn(x)
+    assert np.abs(f - ref).max() < 2e-6 * np.abs(ref).max()
+
+
+def test_mixed_nonlinear_and_elastic_match_oracle():
+    say("fp32 nonlinear + elastic nac=3 n=16")
+    prm = small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, h=(0.9, 1.0, 1.1))
+    o, g = pair(prm, noise_fields(prm, amp=0.4, ch_amp=0.3))
+    o.transfer()
+    g.transfer()
+    o.elastic()
+    got = g.elastic()
+    for p in range(3):
+        ref = o.felas(p)
+        assert np.abs(got[p] - ref).max() < 1e-5 * max(1e-3, np.abs(ref).max())
+    for v in range(4):
+        ref = o.nonlinear(v)
+        gv = g.nonlinear(v)
+        assert np.abs(gv - ref).max() < 1e-5 * max(1.0, np.abs(ref).max()), v
+
+
+MIXED_CASES = [
+    dict(n=16, nac=3, nch=1, etd2=1, elastic=1, aniso=0, steps=5),
+    dict(n=36, nac=4, nch=1, etd2=0, elastic=1, aniso=1, steps=8),
+]
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: "n{n}_ac{nac}_ch{nch}_etd2{etd2}_el{elastic}_an{aniso}".format(**c))
+def test_mixed_step_matches_oracle(case):
+    say(f"mixed step {case}")
+    c = dict(case)
+    steps = c.pop("steps")
+    prm = small_params(dt=0.02, **c)
+    o, g = pair(prm, noise_fields(prm, amp=0.3))
+    for _ in range(steps):
+        o.step()
+        g.step()
+    worst = max(np.abs(g.get_field(v) - o.get_field(v)).max() for v in range(prm["nac"] + prm["nch"]))
+    say(f"  max |d| = {worst:.3e}")
+    # FP32 state: a few 1e-7 per step on O(0.3) fields
+    assert worst < 1e-5
+
+
+def test_mixed_diag_energy_in_fp64():
+    say("mixed diag")
+    prm = small_params(n=16, nac=3, nch=1, aniso=1)
+    fields = noise_fields(prm, amp=0.5)
+    fields = [f.astype(np.float32).astype(np.float64) for f in fields]  # exactly representable in FP32
+    o, g = pair(prm, fields)
+    st_o, cp_o = o.diag()
+    st_g, cp_g = g.diag()
+    # the inputs are exact in FP32 and the sums run in FP64: FP64 agreement
+    assert np.abs(st_o - st_g).max() < 1e-12 * np.abs(st_o).max()
+    assert abs(cp_o - cp_g) < 1e-12 * abs(cp_o)
+

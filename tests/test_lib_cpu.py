@@ -121,3 +121,13 @@ def test_unsupported_configurations_are_rejected():
         ctx = C.c_void_p()
         rc = pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx))
         assert rc == -3, field
+
+
+def test_precision_flag_is_checked_before_device_work():
+    """Mixed precision needs an edge divisible by 4 (16-byte FP32 copies);
+    unknown precision values are argument errors."""
+    p = pyetd.params_from_dict(dict(n=14, nac=1, nch=0, dt=0.01, epn2=1.0, precision=1))
+    ctx = C.c_void_p()
+    assert pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx)) == -3
+    p = pyetd.params_from_dict(dict(n=16, nac=1, nch=0, dt=0.01, epn2=1.0, precision=7))
+    assert pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx)) == -1
