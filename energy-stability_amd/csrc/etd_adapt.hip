@@ -1,0 +1,348 @@
+// etd_adapt.hip -- adaptive ETD with step doubling, the discrete-energy
+// acceptance test, and the plain / learned dt controllers.  No counterpart
+// in the reference, which steps with a fixed dt (ScLETD.c:84-191).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "etd_internal.h"
+
+#define RBLOCKS 1024
+#define RCOLS 4
+
+// ------------------------------------------------------------------ reductions
+// max |a - b| over cnt values; per-block maxima into part[block * RCOLS]
+template <typename T>
+__global__ __launch_bounds__(256) void k_maxdiff(const T *__restrict__ a, const T *__restrict__ b, size_t cnt,
+                                                 double *__restrict__ part) {
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs((double)a[i] - (double)b[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// Run features over the interior cells of the AC fields: max |u - u0|,
+// max |eta|, count of cells with max_p |eta_p| > 0.1, and the sum over p of
+// |grad eta_p| (central differences on the periodic images).
+template <typename T>
+__global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const T *__restrict__ u0, int n, int g,
+                                                  int nac, double hx, double hy, double hz,
+                                                  double *__restrict__ part) {
+  const int N = n - 2 * g;
+  const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
+  double dmax = 0.0, emax = 0.0, cnt = 0.0, gsum = 0.0;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+    const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
+    const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
+    const size_t xm = (((size_t)(z + g)) * n + (y + g)) * n + ((x + N - 1) % N + g);
+    const size_t yp = (((size_t)(z + g)) * n + ((y + 1) % N + g)) * n + (x + g);
+    const size_t ym = (((size_t)(z + g)) * n + ((y + N - 1) % N + g)) * n + (x + g);
+    const size_t zp = (((size_t)((z + 1) % N + g)) * n + (y + g)) * n + (x + g);
+    const size_t zm = (((size_t)((z + N - 1) % N + g)) * n + (y + g)) * n + (x + g);
+    double cell = 0.0;
+    for (int p = 0; p < nac; p++) {
+      const T *f = u + n3 * p;
+      const double v = f[l];
+      dmax = fmax(dmax, fabs(v - (double)u0[n3 * p + l]));
+      cell = fmax(cell, fabs(v));
+      const double gx = ((double)f[xp] - (double)f[xm]) / (2.0 * hx);
+      const double gy = ((double)f[yp] - (double)f[ym]) / (2.0 * hy);
+      const double gz = ((double)f[zp] - (double)f[zm]) / (2.0 * hz);
+      gsum += sqrt(gx * gx + gy * gy + gz * gz);
+    }
+    emax = fmax(emax, cell);
+    cnt += cell > 0.1 ? 1.0 : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dmax = fmax(dmax, __shfl_down(dmax, o, 64));
+    emax = fmax(emax, __shfl_down(emax, o, 64));
+    cnt += __shfl_down(cnt, o, 64);
+    gsum += __shfl_down(gsum, o, 64);
+  }
+  __shared__ double sh[4][4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = dmax;
+    sh[1][w] = emax;
+    sh[2][w] = cnt;
+    sh[3][w] = gsum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *o = part + blockIdx.x * RCOLS;
+    o[0] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    o[1] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+    o[2] = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
+    o[3] = ((sh[3][0] + sh[3][1]) + sh[3][2]) + sh[3][3];
+  }
+}
+
+// u = 2 u - b (local extrapolation of the step-doubling pair)
+template <typename T>
+__global__ void k_extrapolate(T *__restrict__ u, const T *__restrict__ b, size_t cnt) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+    u[i] = (T)(2.0 * (double)u[i] - (double)b[i]);
+}
+
+// ------------------------------------------------------------------ host helpers
+#define ACHK(call)                                                                              \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      snprintf(c->err, sizeof(c->err), "%s: %s", #call, hipGetErrorString(e_));                 \
+      return ETD_ERR_DEVICE;                                                                    \
+    }                                                                                           \
+  } while (0)
+
+static int fetch_part(etd_ctx *c, int cols) {
+  ACHK(hipMemcpyAsync(c->ad_hpart, c->ad_part, sizeof(double) * RBLOCKS * RCOLS, hipMemcpyDeviceToHost, c->stream));
+  ACHK(hipStreamSynchronize(c->stream));
+  (void)cols;
+  return ETD_OK;
+}
+
+template <typename T>
+static int max_abs_diff(etd_ctx *c, const T *a, const T *b, size_t cnt, double *out) {
+  k_maxdiff<T><<<RBLOCKS, 256, 0, c->stream>>>(a, b, cnt, c->ad_part);
+  int rc = fetch_part(c, 1);
+  if (rc) return rc;
+  double m = 0.0;
+  for (int i = 0; i < RBLOCKS; i++) m = fmax(m, c->ad_hpart[i * RCOLS]);
+  *out = m;
+  return ETD_OK;
+}
+
+struct run_feat {
+  double dmax, emax, ifrac, gmean;
+};
+
+template <typename T>
+static int features_of(etd_ctx *c, const T *u, const T *u0, run_feat *f) {
+  if (c->nac == 0) {
+    memset(f, 0, sizeof(*f));
+    return ETD_OK;
+  }
+  k_features<T><<<RBLOCKS, 256, 0, c->stream>>>(u, u0, c->n, c->g, c->nac, c->p.hx, c->p.hy, c->p.hz, c->ad_part);
+  int rc = fetch_part(c, 4);
+  if (rc) return rc;
+  double dm = 0.0, em = 0.0, cn = 0.0, gs = 0.0;
+  for (int i = 0; i < RBLOCKS; i++) {
+    const double *o = c->ad_hpart + i * RCOLS;
+    dm = fmax(dm, o[0]);
+    em = fmax(em, o[1]);
+    cn += o[2];
+    gs += o[3];
+  }
+  f->dmax = dm;
+  f->emax = em;
+  f->ifrac = cn / (double)c->N3;
+  f->gmean = gs / ((double)c->N3 * c->nac);
+  return ETD_OK;
+}
+
+// check_soln_new energy of AC plus CH fields, FP64
+template <typename T>
+static int energy_of(etd_ctx *c, double *E) {
+  double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
+  if (!st) return ETD_ERR_NOMEM;
+  int rc = launch_diag<T>(c, st);
+  if (!rc) {
+    double e = c->nac > 0 ? st[3] : 0.0;
+    for (int q = 0; q < c->nch; q++) e += st[4 * (c->nac + q) + 3];
+    *E = e;
+  }
+  free(st);
+  return rc;
+}
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static double lg(double x, double floor_) { return log10(fabs(x) + floor_); }
+
+extern "C" void etd_adapt_defaults(etd_adapt_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->t_end = 1.0;
+  o->dt0 = 1e-3;
+  o->dt_min = 1e-6;
+  o->dt_max = 1.0;
+  o->tol = 1e-4;
+  o->safety = 0.9;
+  o->grow = 2.0;
+  o->shrink = 0.2;
+  o->controller = ETD_CTRL_PLAIN;
+  o->extrapolate = 1;
+  o->energy_test = 1;
+  o->max_attempts = 1000000;
+}
+
+extern "C" int etd_set_controller(etd_ctx *c, etd_mlp *m) {
+  if (!c) return ETD_ERR_ARG;
+  c->mlp = m;
+  return ETD_OK;
+}
+
+template <typename T>
+static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st, double *log, int log_cap,
+                        int *log_rows) {
+  const int V = c->V;
+  const size_t n3 = c->n3, cnt = n3 * V;
+  const etd_fields<T> F = fields<T>(c);
+  if (!c->ad_S0) {
+    ACHK(hipMalloc(&c->ad_S0, sizeof(T) * cnt));
+    ACHK(hipMalloc(&c->ad_SB, sizeof(T) * cnt));
+    ACHK(hipMalloc(&c->ad_spec, sizeof(T) * cnt));
+    ACHK(hipMalloc(&c->ad_part, sizeof(double) * RBLOCKS * RCOLS));
+    ACHK(hipHostMalloc(&c->ad_hpart, sizeof(double) * RBLOCKS * RCOLS, hipHostMallocDefault));
+  }
+  T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec;
+  const double dt_fixed = c->p.dt;
+  const double slack = sizeof(T) == 4 ? 1e-7 : 1e-12;
+  const int learned = o->controller == ETD_CTRL_LEARNED && c->mlp;
+  memset(st, 0, sizeof(*st));
+  if (log_rows) *log_rows = 0;
+
+  // state of the run
+  double E;
+  int rc = energy_of<T>(c, &E);
+  if (rc) return rc;
+  run_feat rf;
+  if ((rc = features_of<T>(c, F.u, F.u, &rf))) return rc;
+  double hist[4] = {o->dt0, o->dt0, o->dt0, o->dt0};
+  double err_last = 0.1 * o->tol, dEdt = 0.0, plain_next = o->dt0;
+  int rej_since = 0, fallback = 0;
+  double dt = o->dt0;
+  int attempts = 0;
+  rc = ETD_OK;
+  while (c->t < o->t_end * (1.0 - 1e-12) && attempts < o->max_attempts) {
+    // features seen by the proposal
+    double feat[ETD_NFEAT];
+    for (int k = 0; k < 4; k++) feat[k] = log10(hist[k]);
+    feat[4] = log10(fmax(err_last, 1e-30) / o->tol);
+    feat[5] = lg(dEdt, 1e-12);
+    feat[6] = lg(E, 1e-30);
+    feat[7] = lg(rf.dmax, 1e-12);
+    feat[8] = rf.ifrac;
+    feat[9] = rf.gmean;
+    feat[10] = log10(c->t + 1e-6);
+    feat[11] = log10(1.0 + st->accepted);
+    feat[12] = rej_since;
+    feat[13] = lg(rf.emax, 1e-30);
+    feat[14] = log10(o->tol);
+    feat[15] = log10(plain_next);
+    if (rej_since == 0 && st->accepted + attempts > 0) {
+      dt = plain_next;
+      if (learned && !fallback) {
+        // the network predicts log10 of the error model's largest step over
+        // the plain proposal; at most a decade either way
+        double y;
+        if (etd_mlp_predict(c->mlp, feat, &y) == ETD_OK && isfinite(y)) {
+          dt = o->safety * plain_next * pow(10.0, clampd(y, -1.0, 1.0));
+          st->learned_proposals++;
+        } else {
+          st->learned_fallbacks++;
+        }
+      }
+    }
+    dt = clampd(dt, o->dt_min, o->dt_max);
+    const double remain = o->t_end - c->t;
+    if (dt > remain) dt = remain;
+    attempts++;
+    // attempt: full step and two half steps from u_n
+    ACHK(hipMemcpyAsync(S0, F.u, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+    if (!c->p.etd2) {
+      stage_forward<T>(c);  // N(u_n) and the forward transforms, shared
+      c->p.dt = dt;
+      launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
+      modeprod3<T>(c, 0, SP, SB, V, 0);
+      c->p.dt = 0.5 * dt;
+      launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
+      modeprod3<T>(c, 0, SP, F.u, V, 0);
+      if ((rc = step_once_t<T>(c))) break;
+    } else {
+      c->p.dt = dt;
+      if ((rc = step_once_t<T>(c))) break;
+      ACHK(hipMemcpyAsync(SB, F.u, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+      ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+      c->p.dt = 0.5 * dt;
+      if ((rc = step_once_t<T>(c))) break;
+      if ((rc = step_once_t<T>(c))) break;
+    }
+    st->step_evals += 3;
+    ACHK(hipGetLastError());
+    double err;
+    if ((rc = max_abs_diff<T>(c, F.u, SB, cnt, &err))) break;
+    if (o->extrapolate) k_extrapolate<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt);
+    double Enew;
+    if ((rc = energy_of<T>(c, &Enew))) break;
+    const double fac = err > 0.0 ? o->safety * sqrt(o->tol / err) : o->grow;
+    const int err_ok = err <= o->tol && isfinite(err);
+    const int energy_ok = !o->energy_test || Enew <= E + slack * fabs(E);
+    const int forced = dt <= o->dt_min * (1.0 + 1e-12) && isfinite(err);
+    if ((err_ok && energy_ok) || forced) {
+      run_feat nf;
+      if ((rc = features_of<T>(c, F.u, S0, &nf))) break;
+      c->t += dt;
+      st->accepted++;
+      if (log && log_rows && *log_rows < log_cap) {
+        double *row = log + (size_t)(*log_rows) * ETD_LOG_COLS;
+        row[0] = c->t;
+        row[1] = dt;
+        row[2] = err;
+        row[3] = Enew;
+        memcpy(row + 4, feat, sizeof(feat));
+        (*log_rows)++;
+      }
+      for (int k = 3; k > 0; k--) hist[k] = hist[k - 1];
+      hist[0] = dt;
+      dEdt = (Enew - E) / dt;
+      E = Enew;
+      err_last = err;
+      rf = nf;
+      plain_next = dt * clampd(fac, o->shrink, o->grow);
+      rej_since = 0;
+      fallback = 0;
+    } else {
+      ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+      if (!err_ok) {
+        st->rejected_err++;
+        dt = dt * clampd(fac, o->shrink, 1.0);
+      } else {
+        st->rejected_energy++;
+        dt = 0.5 * dt;
+      }
+      rej_since++;
+      if (learned) {
+        fallback = 1;  // retry with the plain controller
+        st->learned_fallbacks++;
+      }
+    }
+  }
+  c->p.dt = dt_fixed;
+  st->t = c->t;
+  st->dt_last = hist[0];
+  st->energy = E;
+  if (rc) return rc;
+  ACHK(hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st, double *log, int log_cap,
+                                int *log_rows) {
+  if (!c || !o || !st) return ETD_ERR_ARG;
+  if (!(o->tol > 0.0) || !(o->dt0 > 0.0) || !(o->dt_min > 0.0) || !(o->dt_max >= o->dt_min) || !(o->safety > 0.0) ||
+      !(o->grow >= 1.0) || !(o->shrink > 0.0 && o->shrink <= 1.0))
+    return ETD_ERR_ARG;
+  if (o->controller == ETD_CTRL_LEARNED && !c->mlp) {
+    snprintf(c->err, sizeof(c->err), "learned controller requested without etd_set_controller");
+    return ETD_ERR_ARG;
+  }
+  return c->mixed ? run_adaptive<float>(c, o, st, log, log_cap, log_rows)
+                  : run_adaptive<double>(c, o, st, log, log_cap, log_rows);
+}

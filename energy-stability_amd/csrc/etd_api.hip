@@ -139,6 +139,9 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     if (v == 0 || v == 1 || v == 2 || v == 4 || v == 16) c->mp_variant = v;
   }
   c->mp_nb = 8;
+  c->fft_variant = 1;
+  const char *fv = getenv("ETD_FFT");
+  if (fv && atoi(fv) == 0) c->fft_variant = 0;
   const char *nbv = getenv("ETD_MFMA_NB");
   if (nbv && atoi(nbv) == 4) c->mp_nb = 4;
   int dev = 0;
@@ -266,8 +269,18 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   return ETD_OK;
 }
 
+extern "C" double etd_get_time(const etd_ctx *c) { return c ? c->t : 0.0; }
+extern "C" void etd_set_time(etd_ctx *c, double t) {
+  if (c) c->t = t;
+}
+
 extern "C" void etd_close(etd_ctx *c) {
   if (!c) return;
+  (void)hipFree(c->ad_S0);
+  (void)hipFree(c->ad_SB);
+  (void)hipFree(c->ad_spec);
+  (void)hipFree(c->ad_part);
+  if (c->ad_hpart) (void)hipHostFree(c->ad_hpart);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->u);
   (void)hipFree(c->N2);
@@ -408,7 +421,7 @@ extern "C" int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, doub
 // Three rotating mode products by V^-1 (forward) or V (backward) over nf
 // fields; FP64 MFMA on the plain path, split-BF16 MFMA on the mixed path.
 template <typename T>
-static void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumulate) {
+void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumulate) {
   const size_t n3 = c->n3;
   const etd_fields<T> F = fields<T>(c);
   if constexpr (sizeof(T) == 8) {
@@ -427,28 +440,44 @@ static void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int 
 // ScLETD.c:87-189 for every field at once: u and N(u) share the forward
 // transforms (one launch over 2V fields), the phi update writes u-hat in
 // place, three inverse products return u in xyz.
+// ghost fill, elasticity, N1 = N(u), spec = V^-1 x3 [u | N1]: the part of a
+// step that does not depend on dt
 template <typename T>
-static int step_once_t(etd_ctx *c) {
+void stage_forward(etd_ctx *c) {
   const int V = c->V;
-  const size_t n3 = c->n3;
   const etd_fields<T> F = fields<T>(c);
-  launch_ghost_fill<T>(c, F.u, V, n3);
+  launch_ghost_fill<T>(c, F.u, V, c->n3);
   if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
   launch_nonlinear<T>(c, F.N1, nullptr);
   modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
-  launch_phiscale<T>(c, F.spec, F.spec + n3 * V, 0, V, 0);
+}
+
+template <typename T>
+int step_once_t(etd_ctx *c) {
+  const int V = c->V;
+  const size_t n3 = c->n3;
+  const etd_fields<T> F = fields<T>(c);
+  stage_forward<T>(c);
+  launch_phiscale<T>(c, F.spec, F.spec + n3 * V, F.spec, 0, V, 0);
   modeprod3<T>(c, 0, F.spec, F.u, V, 0);
   if (c->p.etd2) {
     launch_ghost_fill<T>(c, F.u, V, n3);
     if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
     launch_nonlinear<T>(c, F.N2, F.N1);  // N(a) - N(u_n)
     modeprod3<T>(c, 1, F.N2, F.spec, V, 0);
-    launch_phiscale<T>(c, F.spec, (const T *)nullptr, 0, V, 1);
+    launch_phiscale<T>(c, F.spec, (const T *)nullptr, F.spec, 0, V, 1);
     modeprod3<T>(c, 0, F.spec, F.u, V, 1);  // u = a + S phi2 S^-1 dN
   }
   HCHK(c, hipGetLastError());
   return ETD_OK;
 }
+
+template void modeprod3<double>(etd_ctx *, int, const double *, double *, int, int);
+template void modeprod3<float>(etd_ctx *, int, const float *, float *, int, int);
+template void stage_forward<double>(etd_ctx *);
+template void stage_forward<float>(etd_ctx *);
+template int step_once_t<double>(etd_ctx *);
+template int step_once_t<float>(etd_ctx *);
 
 static int step_once(etd_ctx *c) { return c->mixed ? step_once_t<float>(c) : step_once_t<double>(c); }
 static int diag_any(etd_ctx *c, double *st) { return c->mixed ? launch_diag<float>(c, st) : launch_diag<double>(c, st); }
@@ -458,6 +487,7 @@ extern "C" int etd_step(etd_ctx *c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     int rc = step_once(c);
     if (rc) return rc;
+    c->t += c->p.dt;
   }
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -624,7 +654,7 @@ extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *n
     int rc = put_f32(c, d, uhat, c->n3);
     if (!rc) rc = put_f32(c, d + c->n3, nhat, c->n3);
     if (!rc) {
-      launch_phiscale<float>(c, d, d + c->n3, var, 1, 0);
+      launch_phiscale<float>(c, d, d + c->n3, d, var, 1, 0);
       HCHK(c, hipGetLastError());
       rc = get_f32(c, uhat, d, c->n3);
     }
@@ -635,7 +665,7 @@ extern "C" int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *n
   HCHK(c, hipMalloc(&d, sizeof(double) * 2 * c->n3));
   HCHK(c, hipMemcpy(d, uhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
   HCHK(c, hipMemcpy(d + c->n3, nhat, sizeof(double) * c->n3, hipMemcpyHostToDevice));
-  launch_phiscale<double>(c, d, d + c->n3, var, 1, 0);
+  launch_phiscale<double>(c, d, d + c->n3, d, var, 1, 0);
   HCHK(c, hipGetLastError());
   HCHK(c, hipStreamSynchronize(c->stream));
   HCHK(c, hipMemcpy(uhat, d, sizeof(double) * c->n3, hipMemcpyDeviceToHost));
@@ -664,7 +694,7 @@ extern "C" int etd_calib_copy(etd_ctx *c, size_t bytes) {
 extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches) {
   if (!c || nsteps < 1 || !ms) return ETD_ERR_ARG;
   // enough event slots for every bracketed launch group of nsteps steps
-  const int per_step = 2 * (2 * 12 + 2 * 4 + 8);
+  const int per_step = 2 * (2 * 12 + 2 * 4 + 8 + 4);
   const int need = per_step * nsteps + 16;
   if (c->cap_ev < need) {
     for (int i = 0; i < c->cap_ev; i++) (void)hipEventDestroy(c->ev[i]);
@@ -698,7 +728,7 @@ extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms,
   if (rc) return rc;
   float tot;
   HCHK(c, hipEventElapsedTime(&tot, t0, t1));
-  for (int k = 0; k < 7; k++) ms[k] = 0.0;
+  for (int k = 0; k < 1 + PC_COUNT; k++) ms[k] = 0.0;
   ms[0] = tot;
   int cnt[PC_COUNT] = {0};
   for (int i = 0; i < c->nev; i++) {

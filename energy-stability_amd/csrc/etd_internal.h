@@ -67,12 +67,20 @@ struct etd_ctx {
   int npair, ring;
   long long elastic_evals; /* completed elastic evaluations */
   int fft_radix[16], fft_npass;
+  int fft_variant;         /* 1: two-stage register FFT for N % 16 == 0 (default), 0: radix passes in LDS */
   /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
    * split into split_terms BF16 terms in MFMA fragment order */
   int mixed;
   int split_terms;
   etd_fields<float> f32;
   uint16_t *V3, *Vi3;
+  /* simulated time; adaptive stepping scratch (allocated on first use):
+   * saved state, full-step result, phi-scaled spectrum (V x n3 each, in the
+   * storage type), reduction partials */
+  double t;
+  void *ad_S0, *ad_SB, *ad_spec;
+  double *ad_part, *ad_hpart;
+  struct etd_mlp *mlp;
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
@@ -86,7 +94,8 @@ struct etd_ctx {
   int nev, cap_ev;
 };
 
-enum { PC_MODEPROD = 0, PC_ELASTIC, PC_NONLIN, PC_PHI, PC_ENERGY, PC_HALO, PC_COUNT };
+/* profile classes; PC_FFT (the 3D FFT passes) is nested inside PC_ELASTIC */
+enum { PC_MODEPROD = 0, PC_ELASTIC, PC_NONLIN, PC_PHI, PC_ENERGY, PC_HALO, PC_FFT, PC_COUNT };
 
 #ifdef __cplusplus
 extern "C" {
@@ -119,7 +128,7 @@ inline etd_fields<float> fields<float>(const etd_ctx *c) {
 template <typename T>
 void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride);
 template <typename T>
-void launch_phiscale(etd_ctx *c, T *uhat, const T *nhat, int first_var, int nvars, int order2);
+void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2);
 template <typename T>
 void launch_nonlinear(etd_ctx *c, T *out, const T *sub);
 template <typename T>
@@ -130,6 +139,14 @@ template <typename T>
 void launch_felas_materialize(etd_ctx *c, T *felas);
 template <typename T>
 int launch_diag(etd_ctx *c, double *stats);
+
+/* step pieces (etd_api.hip) */
+template <typename T>
+void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumulate);
+template <typename T>
+void stage_forward(etd_ctx *c);
+template <typename T>
+int step_once_t(etd_ctx *c);
 
 void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_fs, double *out, size_t out_fs,
                      int nfields, int rotate, int accumulate);

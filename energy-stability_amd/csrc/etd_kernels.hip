@@ -569,45 +569,64 @@ struct phi_args {
   double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
 };
 
+// phi1, phi2 and 1 - phi1 H' of one point for one kind of field (AC or CH);
+// the formula and the |H| > 1e-8 switch are the reference's (vars.c:315-325)
+struct phi_pt {
+  double phi1, phi2, lin;
+};
+__device__ __forceinline__ phi_pt phi_at(double H, double dt) {
+  phi_pt r;
+  if (fabs(H) > 1.0e-8) {
+    const double e = exp(-dt * H);
+    r.phi1 = (1.0 - e) / H;
+    r.phi2 = (1.0 - r.phi1 / dt) / H;
+  } else {
+    r.phi1 = dt;
+    r.phi2 = dt / 2;
+  }
+  const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
+  r.lin = 1.0 - r.phi1 * Hp;
+  return r;
+}
+
+// One thread per spectral point, looping over the fields: every AC field
+// shares H (and so phi), as do the CH fields, so exp and the divisions run
+// once per point and kind instead of once per field.
 template <typename T>
-__global__ void k_phiscale(T *uhat, const T *nhat, size_t fs, int first_var, const double *DDX,
-                           const double *DDY, const double *DDZ, phi_args a, int order2) {
+__global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int first_var, int nvars,
+                           const double *DDX, const double *DDY, const double *DDZ, phi_args a, int order2) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int var = first_var + blockIdx.y;
   const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
-  double H;
-  if (var >= a.nac) {
-    const double tmp = DDZ[k] + DDY[j] + DDX[i];
-    H = a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp);
-  } else {
+  const int last = first_var + nvars;
+  phi_pt pac, pch;
+  if (first_var < a.nac) {
     const double tmp = a.kkz * DDZ[k] + a.kky * DDY[j] + a.kkx * DDX[i];
-    H = -a.LE * (tmp * a.epn2 - a.KE);
+    pac = phi_at(-a.LE * (tmp * a.epn2 - a.KE), a.dt);
   }
-  double phi1, phi2;
-  if (fabs(H) > 1.0e-8) {
-    const double e = exp(-a.dt * H);
-    phi1 = (1.0 - e) / H;
-    phi2 = (1.0 - phi1 / a.dt) / H;
-  } else {
-    phi1 = a.dt;
-    phi2 = a.dt / 2;
+  if (last > a.nac) {
+    const double tmp = DDZ[k] + DDY[j] + DDX[i];
+    pch = phi_at(a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp), a.dt);
   }
-  T *uh = uhat + fs * blockIdx.y;
-  if (order2) {
-    uh[l] = (T)(phi2 * (double)uh[l]);
-  } else {
-    const T *nh = nhat + fs * blockIdx.y;
-    const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
-    const double tmp = 1.0 - phi1 * Hp;
-    uh[l] = (T)(tmp * (double)uh[l] + phi1 * (double)nh[l]);
+  for (int v = first_var; v < last; v++) {
+    const phi_pt &ph = v < a.nac ? pac : pch;
+    const T *uh = uhat + fs * (v - first_var);
+    T *o = outp + fs * (v - first_var);
+    if (order2) {
+      o[l] = (T)(ph.phi2 * (double)uh[l]);
+    } else {
+      const T *nh = nhat + fs * (v - first_var);
+      o[l] = (T)(ph.lin * (double)uh[l] + ph.phi1 * (double)nh[l]);
+    }
   }
 }
 
+// out = (1 - phi1 H) uhat + phi1 nhat (or phi2 uhat), phi at the context's
+// current dt; out may be uhat
 template <typename T>
-void launch_phiscale(etd_ctx *c, T *uhat, const T *nhat, int first_var, int nvars, int order2) {
+void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2) {
   phi_args a;
   a.n = c->n;
   a.nac = c->nac;
@@ -621,8 +640,8 @@ void launch_phiscale(etd_ctx *c, T *uhat, const T *nhat, int first_var, int nvar
   a.kky = c->p.kky;
   a.kkz = c->p.kkz;
   etd_prof_begin(c, PC_PHI);
-  k_phiscale<T><<<dim3((unsigned)((c->n3 + 255) / 256), nvars), 256, 0, c->stream>>>(uhat, nhat, c->n3, first_var, c->DDX,
-                                                                                  c->DDY, c->DDZ, a, order2);
+  k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, first_var, nvars,
+                                                                      c->DDX, c->DDY, c->DDZ, a, order2);
   etd_prof_end(c, PC_PHI);
 }
 
@@ -1042,6 +1061,138 @@ __global__ __launch_bounds__(256) void k_fft_axis(T2 *data, size_t fs, int axis,
   }
 }
 
+// ---- two-stage line FFT for N = 16 R2 (R2 in 2..16 with factors 2, 3, 4, 5)
+// x[n], n = n1 + 16 n2; Y[n1][k2] = sum_n2 x[n1 + 16 n2] W_R2^(n2 k2),
+// times W_N^(n1 k2); X[k2 + R2 k1] = sum_n1 Y[n1][k2] W_16^(n1 k1).  Both
+// stages are DFTs in registers (composite sizes by one more split); the only
+// LDS trip is the transpose between them, stored [line][k2][n1] with a
+// one-element pad per k2 row and per line so that neither stage conflicts.
+// A workgroup transforms 16 lines with 16 threads each; lanes run along the
+// line for the x axis and across lines (consecutive x) for y and z, so every
+// global access covers whole 128-byte segments.
+template <int R, int SGN>
+__device__ __forceinline__ void dft_any(double2 *a);
+
+template <int A, int B, int SGN>
+__device__ __forceinline__ void dft_split(double2 *x) {  // size A B, in place, natural order
+  double2 y[A][B];
+#pragma unroll
+  for (int n1 = 0; n1 < A; n1++) {
+    double2 t[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; n2++) t[n2] = x[n1 + A * n2];
+    dft_any<B, SGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) {
+      if (n1 * k2 == 0) {
+        y[n1][k2] = t[k2];
+      } else {
+        const double ang = 2.0 * 3.14159265358979323846 * (double)(n1 * k2) / (double)(A * B);
+        y[n1][k2] = cmul(t[k2], make_double2(cos(ang), -SGN * sin(ang)));
+      }
+    }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < B; k2++) {
+    double2 s[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) s[n1] = y[n1][k2];
+    dft_any<A, SGN>(s);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) x[k2 + B * k1] = s[k1];
+  }
+}
+
+template <int R, int SGN>
+__device__ __forceinline__ void dft_any(double2 *a) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2 || R == 3 || R == 4 || R == 5) {
+    dft_r<R, SGN>(a);
+  } else if constexpr (R == 16) {
+    dft_split<4, 4, SGN>(a);
+  } else if constexpr (R % 4 == 0) {
+    dft_split<4, R / 4, SGN>(a);
+  } else if constexpr (R % 3 == 0) {
+    dft_split<3, R / 3, SGN>(a);
+  } else if constexpr (R % 2 == 0) {
+    dft_split<2, R / 2, SGN>(a);
+  } else if constexpr (R % 5 == 0) {
+    dft_split<5, R / 5, SGN>(a);
+  }
+}
+
+template <int R2>
+struct fft16_cfg {
+  static constexpr int N = 16 * R2;
+  static constexpr int ROW = 16 + 1;          // one k2 row of Y (16 n1 values + pad)
+  static constexpr int LINE = R2 * ROW + 1;   // one line (+ pad)
+  static constexpr int LDS = 16 * LINE * 16;  // bytes (double2)
+};
+
+template <int SGN, int R2, typename T2>
+__global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, const double2 *__restrict__ tw) {
+  using CF = fft16_cfg<R2>;
+  constexpr int N = CF::N;
+  __shared__ double2 buf[16 * CF::LINE];
+  T2 *d = data + fs * blockIdx.y;
+  const int tid = threadIdx.x;
+  const int ll = axis == 0 ? tid >> 4 : tid & 15;  // line within the group
+  const int j = axis == 0 ? tid & 15 : tid >> 4;   // thread within the line
+  const unsigned L = blockIdx.x * 16 + ll;          // N^2 % 16 == 0: every line exists
+  const unsigned x = L % N, o = L / N;
+  const size_t base = axis == 0 ? (size_t)L * N : (axis == 1 ? x + (size_t)N * N * o : x + (size_t)N * o);
+  const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)N : (size_t)N * N);
+  double2 *bl = buf + ll * CF::LINE;
+  {  // stage 1: thread n1 = j, DFT over n2 of x[n1 + 16 n2], twiddle, to LDS [k2][n1]
+    const int n1 = j;
+    double2 a[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) {
+      const T2 v = d[base + (size_t)(n1 + 16 * n2) * stride];
+      a[n2] = make_double2(v.x, v.y);
+    }
+    dft_any<R2, SGN>(a);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) {
+      double2 w = tw[(n1 * k2) % N];  // exp(-2 pi i n1 k2 / N)
+      if (SGN < 0) w.y = -w.y;
+      bl[k2 * CF::ROW + n1] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w);
+    }
+  }
+  __syncthreads();
+  if (j < R2) {  // stage 2: thread k2 = j, DFT over n1, out X[k2 + R2 k1]
+    const int k2 = j;
+    double2 b[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) b[n1] = bl[k2 * CF::ROW + n1];
+    dft_any<16, SGN>(b);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+      T2 v;
+      v.x = b[k1].x;
+      v.y = b[k1].y;
+      d[base + (size_t)(k2 + R2 * k1) * stride] = v;
+    }
+  }
+}
+
+template <int SGN, typename T2>
+static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis) {
+  const int N = c->N;
+  if (N % 16) return false;
+  dim3 grid((unsigned)((size_t)N * N / 16), npair);
+  const double2 *tw = (const double2 *)c->tw;
+  switch (N / 16) {
+#define F16(R)                                                                                   \
+  case R:                                                                                        \
+    k_fft16<SGN, R, T2><<<grid, 256, 0, c->stream>>>(data, c->N3, axis, tw);                     \
+    return true;
+    F16(2) F16(3) F16(4) F16(5) F16(6) F16(8) F16(9) F16(10) F16(12) F16(15) F16(16)
+#undef F16
+    default: return false;
+  }
+}
+
 template <typename T>
 void fft3d(etd_ctx *c, T *data, int npair, int sgn) {
   using T2 = typename cplx<T>::t;
@@ -1050,14 +1201,19 @@ void fft3d(etd_ctx *c, T *data, int npair, int sgn) {
   pl.npass = c->fft_npass;
   for (int i = 0; i < 16; i++) pl.radix[i] = c->fft_radix[i];
   const size_t N2 = (size_t)c->N * c->N;
+  etd_prof_begin(c, PC_FFT);
   dim3 grid((unsigned)((N2 + FFT_LINES - 1) / FFT_LINES), npair);
   size_t lds = 2 * FFT_LINES * (size_t)c->N * sizeof(double2);
   for (int ax = 0; ax < 3; ax++) {
+    if (c->fft_variant == 1 && (sgn > 0 ? launch_fft16<1, T2>(c, (T2 *)data, npair, ax)
+                                        : launch_fft16<-1, T2>(c, (T2 *)data, npair, ax)))
+      continue;
     if (sgn > 0)
       k_fft_axis<1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
     else
       k_fft_axis<-1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
   }
+  etd_prof_end(c, PC_FFT);
 }
 
 // pack interior eta_2r + i eta_2r+1
@@ -1500,7 +1656,7 @@ void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16) {
 // ------------------------------------------------------------------ instantiations (FP64 and FP32 storage)
 #define ETD_INST(T)                                                                   \
   template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
-  template void launch_phiscale<T>(etd_ctx *, T *, const T *, int, int, int);        \
+  template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \
   template void launch_nonlinear<T>(etd_ctx *, T *, const T *);                      \
   template void launch_elastic<T>(etd_ctx *);                                        \
   template void fft3d<T>(etd_ctx *, T *, int, int);                                  \

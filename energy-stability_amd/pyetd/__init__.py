@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libetd.so")
 MAX_VAR = 64
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 
 class EtdError(RuntimeError):
@@ -51,6 +52,32 @@ class EtdParams(C.Structure):
         ("lam", C.c_double * (MAX_VAR * 9)),
         ("precision", C.c_int),
     ]
+
+
+NFEAT = 16
+LOG_COLS = 4 + NFEAT
+MLP_PARAMS = 64 * NFEAT + 64 + 64 * 64 + 64 + 64 + 1
+CTRL_PLAIN, CTRL_LEARNED = 0, 1
+
+
+class AdaptOpts(C.Structure):
+    """Mirror of etd_adapt_opts."""
+
+    _fields_ = [("t_end", C.c_double), ("dt0", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("tol", C.c_double), ("safety", C.c_double), ("grow", C.c_double), ("shrink", C.c_double),
+                ("controller", C.c_int), ("extrapolate", C.c_int), ("energy_test", C.c_int),
+                ("max_attempts", C.c_int)]
+
+
+class AdaptStats(C.Structure):
+    """Mirror of etd_adapt_stats."""
+
+    _fields_ = [("t", C.c_double), ("dt_last", C.c_double), ("energy", C.c_double),
+                ("accepted", C.c_int), ("rejected_err", C.c_int), ("rejected_energy", C.c_int),
+                ("learned_proposals", C.c_int), ("learned_fallbacks", C.c_int), ("step_evals", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _LIB = None
@@ -94,6 +121,21 @@ def lib():
             "etd_k_phiscale": (C.c_int, [ctxp, C.c_int, _dp, _dp]),
             "etd_time_steps": (C.c_int, [ctxp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]),
             "etd_calib_copy": (C.c_int, [ctxp, C.c_size_t]),
+            "etd_adapt_defaults": (None, [C.POINTER(AdaptOpts)]),
+            "etd_run_adaptive": (C.c_int, [ctxp, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), _dp, C.c_int,
+                                           C.POINTER(C.c_int)]),
+            "etd_get_time": (C.c_double, [ctxp]),
+            "etd_set_time": (None, [ctxp, C.c_double]),
+            "etd_mlp_create": (C.c_int, [C.c_int, C.POINTER(ctxp)]),
+            "etd_mlp_destroy": (None, [ctxp]),
+            "etd_mlp_init": (C.c_int, [ctxp, C.c_ulonglong]),
+            "etd_mlp_set": (C.c_int, [ctxp, _fp, _fp, _fp, C.c_float, C.c_float]),
+            "etd_mlp_get": (C.c_int, [ctxp, _fp, _fp, _fp, _fp, _fp]),
+            "etd_mlp_forward": (C.c_int, [ctxp, _fp, C.c_int, _fp]),
+            "etd_mlp_loss_grad": (C.c_int, [ctxp, _fp, _fp, C.c_int, _fp, _fp]),
+            "etd_mlp_train": (C.c_int, [ctxp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]),
+            "etd_mlp_predict": (C.c_int, [ctxp, _dp, _dp]),
+            "etd_set_controller": (C.c_int, [ctxp, ctxp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -105,6 +147,87 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def adapt_opts(**kw):
+    """etd_adapt_opts with the library's defaults, overridden by keywords."""
+    o = AdaptOpts()
+    lib().etd_adapt_defaults(C.byref(o))
+    for k, v in kw.items():
+        if k == "controller" and isinstance(v, str):
+            v = {"plain": CTRL_PLAIN, "learned": CTRL_LEARNED}[v]
+        setattr(o, k, v)
+    return o
+
+
+class Mlp:
+    """The learned dt controller's network (etd_mlp) on one GPU."""
+
+    def __init__(self, device=0, seed=1):
+        self.h = C.c_void_p()
+        _check(lib().etd_mlp_create(device, C.byref(self.h)), what="mlp_create")
+        if seed != 1:
+            _check(lib().etd_mlp_init(self.h, seed), what="mlp_init")
+
+    def close(self):
+        if self.h:
+            lib().etd_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, params, fmean, fstd, ymean, ystd):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        fm = np.ascontiguousarray(fmean, dtype=np.float32)
+        fs = np.ascontiguousarray(fstd, dtype=np.float32)
+        assert p.size == MLP_PARAMS and fm.size == NFEAT and fs.size == NFEAT
+        _check(lib().etd_mlp_set(self.h, _fptr(p), _fptr(fm), _fptr(fs), float(ymean), float(ystd)), what="mlp_set")
+
+    def get(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        fm = np.empty(NFEAT, np.float32)
+        fs = np.empty(NFEAT, np.float32)
+        ym = np.zeros(1, np.float32)
+        ys = np.zeros(1, np.float32)
+        _check(lib().etd_mlp_get(self.h, _fptr(p), _fptr(fm), _fptr(fs), _fptr(ym), _fptr(ys)), what="mlp_get")
+        return p, fm, fs, float(ym[0]), float(ys[0])
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEAT)
+        y = np.empty(x.shape[0], np.float32)
+        _check(lib().etd_mlp_forward(self.h, _fptr(x), x.shape[0], _fptr(y)), what="mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEAT)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        loss = np.zeros(1, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        _check(lib().etd_mlp_loss_grad(self.h, _fptr(x), _fptr(t), x.shape[0], _fptr(loss), _fptr(g)),
+               what="mlp_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, x, t, epochs, batch, lr, seed=7):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEAT)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        el = np.zeros(epochs, np.float32)
+        _check(lib().etd_mlp_train(self.h, _fptr(x), _fptr(t), x.shape[0], epochs, batch, float(lr), seed, _fptr(el)),
+               what="mlp_train")
+        return el
+
+    def predict(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float64).reshape(NFEAT)
+        d = np.zeros(1)
+        _check(lib().etd_mlp_predict(self.h, _ptr(f), _ptr(d)), what="mlp_predict")
+        return float(d[0])
 
 
 def _check(rc, ctx=None, what=""):
@@ -313,9 +436,31 @@ class Etd:
         """One plain device copy of nbytes (HBM counter calibration)."""
         _check(lib().etd_calib_copy(self.ctx, nbytes), self.ctx, "calib_copy")
 
+    @property
+    def time(self):
+        return lib().etd_get_time(self.ctx)
+
+    @time.setter
+    def time(self, t):
+        lib().etd_set_time(self.ctx, t)
+
+    def set_controller(self, mlp):
+        _check(lib().etd_set_controller(self.ctx, mlp.h if mlp is not None else None), self.ctx, "set_controller")
+        self._mlp = mlp
+
+    def run_adaptive(self, log_cap=0, **kw):
+        """etd_run_adaptive; returns (stats dict, log rows [t, dt, err, E, features...])."""
+        o = adapt_opts(**kw)
+        st = AdaptStats()
+        log = np.zeros((max(log_cap, 1), LOG_COLS))
+        rows = C.c_int(0)
+        _check(lib().etd_run_adaptive(self.ctx, C.byref(o), C.byref(st), _ptr(log) if log_cap else None, log_cap,
+                                      C.byref(rows)), self.ctx, "run_adaptive")
+        return st.as_dict(), log[: rows.value].copy()
+
     def time_steps(self, nsteps, with_diag=True):
-        ms = np.zeros(7)
-        launches = (C.c_int * 6)()
+        ms = np.zeros(8)
+        launches = (C.c_int * 7)()
         _check(lib().etd_time_steps(self.ctx, nsteps, 1 if with_diag else 0, _ptr(ms), launches), self.ctx,
                "time_steps")
         return ms, list(launches)

@@ -178,8 +178,100 @@ int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat);
 /* ---- timing: runs nsteps on the context stream between HIP events ---- */
 /* ms[0] = total, ms[1] = mode-product kernels, ms[2] = elastic (FFT +
  * contraction), ms[3] = nonlinear, ms[4] = phi scaling, ms[5] = energy,
- * ms[6] = halo; launches[] counts the dominant kernel's launches. */
+ * ms[6] = halo, ms[7] = the 3D FFT passes alone (part of ms[2]); launches[k]
+ * counts the launch groups of class k (7 ints). */
 int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches);
+
+/* ---- adaptive ETD and the learned dt controller ----
+ * No counterpart in the reference, which steps with a fixed dt
+ * (ScLETD.c:84-191).  Each attempt from u_n with step dt computes one step of
+ * dt and two of dt/2 (the first half step shares N(u_n) and its forward
+ * transforms with the full step); err = max |u_half - u_full| over all fields.
+ * An attempt is accepted when err <= tol and, with energy_test, the FP64
+ * check_soln_new energy (AC + CH) does not increase (slack: 1e-12 relative on
+ * the FP64 path, 1e-7 on the mixed path, the FP32 rounding level).  Rejected
+ * attempts restore u_n.  The plain controller proposes
+ * dt * clamp(safety (tol/err)^(1/2), shrink, grow); the learned controller
+ * proposes safety * dt_plain * 10^clamp(y, -1, 1), where y = MLP(features)
+ * over ETD_NFEAT features of the run (below) predicts log10 of the error
+ * model's largest step over the plain proposal dt_plain (so the learned
+ * proposal is not held to the plain growth limit), and falls back to the
+ * plain proposal after a rejection or for a non-finite output. */
+#define ETD_NFEAT 16
+enum { ETD_CTRL_PLAIN = 0, ETD_CTRL_LEARNED = 1 };
+
+typedef struct etd_adapt_opts {
+  double t_end;        /* integrate from the context's time to t_end (last step clipped) */
+  double dt0;          /* first proposal */
+  double dt_min, dt_max;
+  double tol;          /* bound on the step-doubling estimate */
+  double safety, grow, shrink;
+  int controller;      /* ETD_CTRL_PLAIN or ETD_CTRL_LEARNED (needs etd_set_controller) */
+  int extrapolate;     /* 1: accept 2 u_half - u_full (local extrapolation) */
+  int energy_test;     /* 1: reject an energy increase */
+  int max_attempts;
+} etd_adapt_opts;
+
+typedef struct etd_adapt_stats {
+  double t, dt_last, energy;
+  int accepted, rejected_err, rejected_energy;
+  int learned_proposals, learned_fallbacks;
+  int step_evals;      /* ETD step evaluations: 3 per attempt (one shares N with another) */
+} etd_adapt_stats;
+
+/* Features of the proposal for step k (all from the run so far):
+ *  0..3  log10 of the last four accepted dt (newest first)
+ *  4     log10(err / tol) of the last accepted step
+ *  5     log10 |dE/dt| over the last accepted step
+ *  6     log10 |E|
+ *  7     log10 max |d eta| over the last accepted step (AC fields)
+ *  8     interface-volume fraction: interior cells with max_p |eta_p| > 0.1
+ *  9     mean |grad eta| over interior cells and AC fields
+ *  10    log10(t + 1e-6)      11  log10(1 + accepted steps)
+ *  12    rejections before the last acceptance
+ *  13    log10 max |eta|      14  log10(tol)
+ *  15    log10 of the plain controller's proposal */
+#define ETD_LOG_COLS (4 + ETD_NFEAT)
+
+void etd_adapt_defaults(etd_adapt_opts *o);
+/* Runs to o->t_end.  When log != NULL, each accepted step appends one row of
+ * ETD_LOG_COLS doubles (t after the step, dt, err, E, then the features seen
+ * when dt was proposed) while fewer than log_cap rows are stored. */
+int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st, double *log, int log_cap,
+                     int *log_rows);
+/* Simulated time of the context (advanced by etd_step and etd_run_adaptive). */
+double etd_get_time(const etd_ctx *c);
+void etd_set_time(etd_ctx *c, double t);
+
+/* The controller network: MLP ETD_NFEAT -> 64 -> 64 -> 1, GELU, FP32, with
+ * hand-written forward / backward / Adam kernels.  Parameters (ETD_MLP_PARAMS
+ * floats): W1 [64][16] row-major, b1 [64], W2 [64][64], b2 [64], w3 [64],
+ * b3.  Inputs and targets are normalised by the stored feature mean / std
+ * and target mean / std; the controller's network predicts
+ * log10(dt_best / dt_plain) for the next step (see etd_run_adaptive). */
+#define ETD_MLP_PARAMS (64 * ETD_NFEAT + 64 + 64 * 64 + 64 + 64 + 1)
+typedef struct etd_mlp etd_mlp;
+int etd_mlp_create(int device, etd_mlp **out);
+void etd_mlp_destroy(etd_mlp *m);
+/* Seeded init (uniform +-sqrt(6/(fan_in + fan_out)), zero biases) and unit normalisation. */
+int etd_mlp_init(etd_mlp *m, unsigned long long seed);
+int etd_mlp_set(etd_mlp *m, const float *params, const float *feat_mean, const float *feat_std, float y_mean,
+                float y_std);
+int etd_mlp_get(const etd_mlp *m, float *params, float *feat_mean, float *feat_std, float *y_mean, float *y_std);
+/* Network on normalised inputs x[batch][ETD_NFEAT] -> normalised outputs y[batch]. */
+int etd_mlp_forward(etd_mlp *m, const float *x, int batch, float *y);
+/* Mean squared error over the batch and its gradient (ETD_MLP_PARAMS floats),
+ * normalised inputs and targets. */
+int etd_mlp_loss_grad(etd_mlp *m, const float *x, const float *t, int batch, float *loss, float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over epochs of seeded shuffled batches of
+ * the normalised data set; epoch_loss[e] = mean loss of epoch e (may be NULL). */
+int etd_mlp_train(etd_mlp *m, const float *x, const float *t, int count, int epochs, int batch, float lr,
+                  unsigned long long seed, float *epoch_loss);
+/* Raw features -> the network's prediction in target units (de-normalised);
+ * what the learned controller calls once per accepted step. */
+int etd_mlp_predict(etd_mlp *m, const double *features, double *y);
+/* Attach (or detach with NULL) the controller network of a context. */
+int etd_set_controller(etd_ctx *c, etd_mlp *m);
 
 /* Measurement support, no counterpart in the reference: one plain copy
  * kernel that reads and writes exactly `bytes` (a multiple of 16, at most the

@@ -898,6 +898,43 @@ void orc_step(orc_ctx *c) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Adaptive stepping support (no counterpart in the reference; restates the
+ * GPU driver's step-doubling attempt so its decisions can be checked).  */
+void orc_set_dt(orc_ctx *c, double dt) {
+  c->p.dt = dt;
+  init_phi(c);
+}
+
+/* One ETD1 attempt's shared part: N(u_n) and the forward transforms once;
+ * ufull = u_n advanced by dt, field = u_n advanced by dt/2 (the first half
+ * step).  Leaves the context's dt at dt/2.  Pinned by the identity with
+ * orc_step at dt and at dt/2 (tests). */
+void orc_step_pair(orc_ctx *c, double dt, double *ufull) {
+  const size_t n3 = c->n3;
+  double *uh = malloc(sizeof(double) * n3), *nh = malloc(sizeof(double) * n3);
+  double *u0h = malloc(sizeof(double) * n3 * c->V), *n0h = malloc(sizeof(double) * n3 * c->V);
+  double *w1 = malloc(sizeof(double) * n3), *w2 = malloc(sizeof(double) * n3);
+  orc_transfer(c);
+  if (c->p.elastic == 1) orc_elastic(c);
+  for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N1 + n3 * v);
+  for (int v = 0; v < c->V; v++) {
+    fwd3(c, c->field + n3 * v, u0h + n3 * v, w1, w2);
+    fwd3(c, c->N1 + n3 * v, n0h + n3 * v, w1, w2);
+  }
+  const double dts[2] = {dt, 0.5 * dt};
+  for (int s = 0; s < 2; s++) {
+    orc_set_dt(c, dts[s]);
+    for (int v = 0; v < c->V; v++) {
+      memcpy(uh, u0h + n3 * v, sizeof(double) * n3);
+      memcpy(nh, n0h + n3 * v, sizeof(double) * n3);
+      orc_phi_update(c, v, uh, nh);
+      bwd3(c, uh, s == 0 ? ufull + n3 * v : c->field + n3 * v, w1, w2);
+    }
+  }
+  free(uh); free(nh); free(u0h); free(n0h); free(w1); free(w2);
+}
+
+/* ------------------------------------------------------------------ */
 /* check_soln_new (field.c:407-678).  Per AC field: max, min, vol over the
  * interior [g, n-g)^3; eng over cells [g, n-g-1)^3 with the 8-corner mean
  * and 4-edge-averaged gradient, summed over all AC fields m:

@@ -69,6 +69,11 @@ void orc_nonlinear(orc_ctx *c, int v, double *out);
 void orc_modeprod(int n, const double *A, const double *in, double *out, int rotate);
 void orc_phi_update(orc_ctx *c, int v, double *uhat, const double *nhat);  /* zxy layout */
 void orc_step(orc_ctx *c);
+/* adaptive stepping support: change dt (rebuilds the phi tables); the
+ * step-doubling pair of one ETD1 attempt (ufull: V x n3, u_n + dt; the
+ * context's fields: u_n + dt/2; dt left at dt/2) */
+void orc_set_dt(orc_ctx *c, double dt);
+void orc_step_pair(orc_ctx *c, double dt, double *ufull);
 /* stats: per field v (nac+nch) 4 doubles max,min,vol,eng; then couple eng */
 void orc_diag(orc_ctx *c, double *stats);
 

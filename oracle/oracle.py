@@ -51,6 +51,8 @@ def lib():
         L.orc_modeprod.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int]
         L.orc_phi_update.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_step.argtypes = [C.c_void_p]
+        L.orc_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.orc_step_pair.argtypes = [C.c_void_p, C.c_double, _dp]
         L.orc_diag.argtypes = [C.c_void_p, _dp]
         L.orc_eigbasis.restype = C.c_int
         L.orc_eigbasis.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
@@ -194,6 +196,16 @@ class Oracle:
     def step(self, nsteps=1):
         for _ in range(nsteps):
             lib().orc_step(self.ctx)
+
+    def set_dt(self, dt):
+        lib().orc_set_dt(self.ctx, dt)
+
+    def step_pair(self, dt):
+        """Shared-N step-doubling pair: returns u_n + dt (V x n^3) and leaves
+        the fields at u_n + dt/2, dt at dt/2."""
+        full = np.empty(self.V * self.n ** 3)
+        lib().orc_step_pair(self.ctx, dt, _ptr(full))
+        return full.reshape(self.V, self.n ** 3)
 
     def diag(self):
         st = np.zeros(4 * self.V + 1)

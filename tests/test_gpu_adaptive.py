@@ -1,0 +1,155 @@
+"""GPU tests of adaptive ETD and the learned dt controller, through the C ABI.
+
+- The driver with every attempt accepted and no extrapolation is plain
+  stepping at dt/2, bit for bit.
+- Its accept / reject decisions, error estimates, energies and states match
+  the CPU restatement on the FP64 oracle (oracle/adaptive.py).
+- The network's forward pass and gradient match the numpy oracle
+  (oracle/mlp_oracle.py, itself pinned by finite differences); training is
+  deterministic and reduces the loss.
+- A network trained on plain adaptive runs keeps the guarantees (every
+  accepted step within tolerance, monotone energy, t_end reached) and its
+  gain over the plain controller is reported.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import adaptive as AD
+import mlp_oracle as M
+import oracle as O
+import pyetd
+from conftest import noise_fields, small_params
+from pyetd import controller as CT
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu(prm, fields):
+    Vm, Vim, D = O.eigbasis(prm["n"], 3)
+    g = pyetd.Etd(pyetd.params_from_dict(prm), Vm, Vim, D)
+    for v, f in enumerate(fields):
+        g.set_field(v, f)
+    return g
+
+
+def test_all_accepted_without_extrapolation_is_half_steps():
+    say("adaptive == half steps")
+    dt = 2.0 ** -6
+    prm = small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, dt=dt)
+    f = noise_fields(prm, amp=0.3)
+    a = gpu(prm, f)
+    b = gpu(dict(prm, dt=dt / 2), f)
+    st, log = a.run_adaptive(log_cap=16, t_end=3 * dt, dt0=dt, tol=1e9, grow=1.0, shrink=1.0, extrapolate=0,
+                             energy_test=0)
+    assert st["accepted"] == 3 and st["rejected_err"] == 0 and st["step_evals"] == 9
+    assert a.time == 3 * dt and len(log) == 3
+    b.step(6)
+    for v in range(4):
+        assert np.array_equal(a.get_field(v), b.get_field(v))
+
+
+@pytest.mark.parametrize("elastic", [0, 1])
+def test_plain_controller_matches_oracle_restatement(elastic):
+    say(f"adaptive vs oracle restatement elastic={elastic}")
+    prm = small_params(n=16, nac=3, nch=1, elastic=elastic, dt=0.01, KE=3.2, landau=(0.2, 3.0, 3.0))
+    f = noise_fields(prm, amp=0.05, ch_amp=0.05)
+    o = O.Oracle(prm)
+    for v, x in enumerate(f):
+        o.set_field(v, x)
+    kw = dict(t_end=0.6, dt0=0.2, dt_min=1e-5, dt_max=0.3, tol=2e-4)
+    t_o, rows_o, cnt_o = AD.run(o, **kw)
+    g = gpu(prm, f)
+    st, log = g.run_adaptive(log_cap=1000, **kw)
+    say(f"  oracle {cnt_o}  gpu {st}")
+    assert cnt_o["rejected_err"] > 0 and cnt_o["accepted"] > 3  # both branches exercised
+    assert st["accepted"] == cnt_o["accepted"]
+    assert st["rejected_err"] == cnt_o["rejected_err"] and st["rejected_energy"] == cnt_o["rejected_energy"]
+    ro = np.array(rows_o)
+    assert np.abs(log[:, 0] - ro[:, 0]).max() < 1e-12
+    assert np.abs(log[:, 1] - ro[:, 1]).max() < 1e-12
+    assert np.abs(log[:, 2] - ro[:, 2]).max() < 1e-9 * ro[:, 2].max()
+    assert np.abs(log[:, 3] - ro[:, 3]).max() < 1e-10 * np.abs(ro[:, 3]).max()
+    for v in range(4):
+        assert np.abs(g.get_field(v) - o.get_field(v)).max() < 1e-10
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    say("mlp forward / grad vs numpy oracle")
+    p = M.init(11)
+    p[-1] = 0.25
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((37, M.NF))
+    t = rng.standard_normal(37)
+    m = pyetd.Mlp()
+    m.set(p.astype(np.float32), np.zeros(16), np.ones(16), 0.0, 1.0)
+    p32 = p.astype(np.float32).astype(np.float64)
+    x32 = x.astype(np.float32).astype(np.float64)
+    t32 = t.astype(np.float32).astype(np.float64)
+    y_ref, _ = M.forward(p32, x32)
+    y = m.forward(x)
+    assert np.abs(y - y_ref).max() < 1e-5 * max(1.0, np.abs(y_ref).max())
+    loss_ref, g_ref = M.loss_grad(p32, x32, t32)
+    loss, g = m.loss_grad(x, t)
+    assert abs(loss - loss_ref) < 1e-5 * loss_ref
+    assert np.abs(g - g_ref).max() < 1e-5 * np.abs(g_ref).max()
+    m.close()
+
+
+def test_mlp_training_is_deterministic_and_fits():
+    say("mlp training")
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((2048, 16)).astype(np.float32)
+    y = (np.sin(x[:, 0]) + 0.5 * x[:, 1] ** 2 - 0.3 * x[:, 2] * x[:, 3]).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        m = pyetd.Mlp(seed=3)
+        el = m.train(x, y, epochs=40, batch=128, lr=3e-3, seed=9)
+        runs.append((el, m.get()[0]))
+        m.close()
+    say(f"  loss {runs[0][0][0]:.4f} -> {runs[0][0][-1]:.4f}")
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][0][-1] < 0.1 * runs[0][0][0]
+
+
+def test_learned_controller_keeps_guarantees_and_reports_gain():
+    say("learned controller on 32^3 x 4")
+    base = dict(small_params(n=36, nac=4, nch=0, elastic=1, dt=0.005, KE=3.2, landau=(0.2, 3.0, 3.0)),
+                C=pyetd.workloads.cubic_voigt(134.0, 110.0, 36.0))
+    kw = dict(t_end=3.0, tol=1e-4, dt0=0.005, dt_min=1e-5, dt_max=0.5)
+    X, Y = [], []
+    for seed in range(6):
+        prm = dict(base, eps0=pyetd.workloads.eigenstrains(4, 0.03, seed), seed=seed)
+        st, log = CT.plain_run(prm, **kw)
+        x, y = CT.targets_from_log(log, kw["tol"], kw["dt_min"], kw["dt_max"])
+        X.append(x)
+        Y.append(y)
+    X, Y = np.concatenate(X), np.concatenate(Y)
+    m = pyetd.Mlp(seed=5)
+    el = CT.fit(m, X, Y, epochs=300, batch=64, lr=2e-3)
+    say(f"  {len(Y)} pairs, loss {el[0]:.3f} -> {el[-1]:.3f}")
+    prm = dict(base, eps0=pyetd.workloads.eigenstrains(4, 0.03, 99), seed=99)  # held out
+    st_p, log_p = CT.plain_run(prm, **kw)
+    g = pyetd.Etd.from_dict(prm)
+    for v, f in enumerate(pyetd.workloads.initial_fields(prm)):
+        g.set_field(v, f)
+    g.set_controller(m)
+    st_l, log_l = g.run_adaptive(log_cap=100000, controller="learned", **kw)
+    say(f"  plain   {st_p}")
+    say(f"  learned {st_l}")
+    say(f"  step evaluations: plain {st_p['step_evals']}, learned {st_l['step_evals']} "
+        f"(gain {st_p['step_evals'] / st_l['step_evals']:.2f}x)")
+    for st, log in ((st_p, log_p), (st_l, log_l)):
+        assert abs(st["t"] - kw["t_end"]) < 1e-12
+        assert (log[:, 2] <= kw["tol"]).all() or (log[:, 1] <= kw["dt_min"] * (1 + 1e-12)).any()
+        E = log[:, 3]
+        assert (np.diff(E) <= 1e-12 * np.abs(E[:-1])).all()
+    assert st_l["learned_proposals"] > 0
+    g.close()
+    m.close()

@@ -385,3 +385,61 @@ def test_diag_single_cell_by_hand():
         inner = f[v][2:4, 2:4, 2:4]
         assert st[v, 0] == inner.max() and st[v, 1] == inner.min()
         assert abs(st[v, 2] - (inner + 1.0).sum() * h3) < 1e-12
+
+
+def test_step_pair_equals_full_and_half_steps():
+    """orc_step_pair (the adaptive attempt's shared-N pair) is orc_step at dt
+    and orc_step at dt/2 from the same state, bit for bit."""
+    prm = small_params(n=12, nac=2, nch=1, elastic=1, aniso=1, dt=0.02)
+    f = noise_fields(prm, amp=0.3)
+    a, b, c = O.Oracle(prm), O.Oracle(dict(prm)), O.Oracle(dict(prm, dt=0.01))
+    for v, x in enumerate(f):
+        a.set_field(v, x)
+        b.set_field(v, x)
+        c.set_field(v, x)
+    full = a.step_pair(0.02)
+    b.step()
+    c.step()
+    for v in range(3):
+        assert np.array_equal(full[v], b.get_field(v).reshape(-1))
+        assert np.array_equal(a.get_field(v), c.get_field(v))
+
+
+def test_mlp_oracle_gradient_by_finite_differences():
+    import mlp_oracle as M
+    rng = np.random.default_rng(3)
+    p = M.init(5)
+    p[M.NPAR - 1] = 0.3
+    x = rng.standard_normal((7, M.NF))
+    t = rng.standard_normal(7)
+    loss, g = M.loss_grad(p, x, t)
+    idx = list(rng.choice(M.NPAR, 60, replace=False)) + [M.NPAR - 1, 0, 1024, 1024 + 64]
+    for i in idx:
+        e = np.zeros(M.NPAR)
+        e[i] = 1e-6
+        fd = (M.loss_grad(p + e, x, t)[0] - M.loss_grad(p - e, x, t)[0]) / 2e-6
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(g[i])), i
+    # GELU against its definition at a few points (erf-based, exact)
+    a = np.array([-3.0, -0.5, 0.0, 0.7, 2.5])
+    from math import erf, sqrt
+    assert np.allclose(M.gelu(a), [0.5 * v * (1 + erf(v / sqrt(2))) for v in a], rtol=0, atol=1e-15)
+
+
+def test_adaptive_restatement_with_everything_accepted_is_half_steps():
+    """With a huge tolerance, no extrapolation, no energy test and fixed
+    dt the adaptive restatement is plain stepping at dt/2."""
+    import adaptive as AD
+    dt = 2.0 ** -6  # binary-exact times, so no clipped last step
+    prm = small_params(n=12, nac=2, nch=0, elastic=1, dt=dt)
+    f = noise_fields(prm, amp=0.3)
+    a, b = O.Oracle(prm), O.Oracle(dict(prm, dt=dt / 2))
+    for v, x in enumerate(f):
+        a.set_field(v, x)
+        b.set_field(v, x)
+    t, rows, cnt = AD.run(a, t_end=3 * dt, dt0=dt, dt_min=1e-6, dt_max=1.0, tol=1e9, grow=1.0, shrink=1.0,
+                          extrapolate=False, energy_test=False)
+    assert cnt["accepted"] == 3 and cnt["attempts"] == 3 and t == 3 * dt
+    for _ in range(6):
+        b.step()
+    for v in range(2):
+        assert np.abs(a.get_field(v) - b.get_field(v)).max() == 0.0
