@@ -17,8 +17,9 @@ resident in HBM before every timed region.
   plain FP64 path after its 1000 steps (the plain FP64 GPU path equals the FP64
   oracle within 1e-10: tests/test_gpu_parity.py), the relative energy error,
   monotone energy; and the mixed-precision fixed-dt report every 100 steps.
-- roofline: the dominant kernel of the headline path, timed with HIP events
-  on the library's stream; traffic from the committed rocprofv3 counters.
+- roofline: the dominant kernel of the headline path (mixed precision, timed
+  at fixed dt with HIP events on the library's stream); traffic from the
+  committed rocprofv3 counters.
 - cpu_baseline: the FP64 C oracle on this host, one full step.
 Prints one JSON line.
 """
@@ -140,10 +141,10 @@ def main():
         mp_fields_avg = (3 * 2 * V + 3 * V) / 6.0
         mp_bytes = 2 * 4.0 * n ** 3 * mp_fields_avg  # read + write FP32 per field
         mp_mfma = 6 * 2.0 * n ** 4 * mp_fields_avg  # six BF16 products per split-3 product
-        r_fft = {"kernel": "k_fft_axis (own mixed-radix FFT pass, FP32 in HBM)", "bound": "hbm",
+        r_fft = {"kernel": "k_fft16<12> (own two-stage register FFT pass, FP32 in HBM)", "bound": "hbm",
                  "achieved": fft_bytes / (fft_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                  "frac": fft_bytes / (fft_launch_ms / 1e3) / PEAK_HBM,
-                 "traffic": measured_traffic("k_fft_axis_f32_192cube_hbm_counters.md"), "launch_ms": fft_launch_ms,
+                 "traffic": measured_traffic("k_fft16_f32_192cube_hbm_counters.md"), "launch_ms": fft_launch_ms,
                  "ms_per_step": ms[7] / K}
         r_mp = {"kernel": "k_modeprod_split (split-BF16 x3 MFMA, FP32 accumulate)", "bound": "hbm",
                 "achieved": mp_bytes / (mp_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -184,6 +185,20 @@ def main():
         g.close()
         if mlp is not None:
             mlp.close()
+            # the same horizon with the plain controller: what the network gains
+            g = fresh(pyetd, W, q)
+            kw_p = dict(kw, controller="plain")
+            g.run_adaptive(**dict(kw_p, t_end=2 * dt))
+            for v, f in enumerate(W.initial_fields(q)):
+                g.set_field(v, f)
+            g.time = 0.0
+            t0 = time.perf_counter()
+            st_p, _ = g.run_adaptive(**kw_p)
+            wall_p = time.perf_counter() - t0
+            g.close()
+            out["adaptive"]["plain_controller"] = dict(st_p, wall_s=wall_p, sim_time_per_s=t_end / wall_p)
+            out["adaptive"]["learned_gain_step_evals"] = st_p["step_evals"] / st["step_evals"]
+            out["adaptive"]["learned_gain_wall"] = wall_p / wall
         out["config"] = {"workload": f"{prm['name']}: {N}^3 interior (n={n}), {nac} AC + {nch} CH fields, cubic "
                                      f"elasticity, ETD1, horizon t = {t_end:g} ({args.horizon_steps} steps of dt={dt})",
                          "grid": N, "extended_edge": n, "fields": V, "t_end": t_end,
