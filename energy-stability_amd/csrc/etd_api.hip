@@ -148,6 +148,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   const char *ev = getenv("ETD_DEVICE");
   if (ev) dev = atoi(ev);
   HCHK(c, hipSetDevice(dev));
+  c->device = dev;
   HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t n3 = c->n3;
   c->mixed = p->precision == ETD_PREC_MIXED;

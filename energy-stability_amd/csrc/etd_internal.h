@@ -47,6 +47,7 @@ struct etd_dev_consts {
 
 struct etd_ctx {
   etd_params p;
+  int device;
   int n, g, N, V, nac, nch;
   size_t n3, N3;
   hipStream_t stream;
@@ -68,6 +69,7 @@ struct etd_ctx {
   long long elastic_evals; /* completed elastic evaluations */
   int fft_radix[16], fft_npass;
   int fft_variant;         /* 1: two-stage register FFT for N % 16 == 0 (default), 0: radix passes in LDS */
+  int split_waves;         /* mixed mode products: 8 or 4 waves per workgroup, 0 by size */
   /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
    * split into split_terms BF16 terms in MFMA fragment order */
   int mixed;

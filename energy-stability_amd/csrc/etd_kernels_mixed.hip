@@ -83,7 +83,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &t0, __bf16 &t1, __bf
 template <int MT, int NT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const uint16_t *__restrict__ A3, const float *__restrict__ in,
                                                                   size_t in_fs, float *__restrict__ out, size_t out_fs,
-                                                                  int n, int rotate, int accumulate) {
+                                                                  int n, int rotate, int accumulate, int a_chunk) {
   using CF = mps_cfg<MT, NT, WAVES>;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -92,7 +92,8 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
   const long n2 = (long)n * n;
   const int nkc = (n + 31) / 32;
   const long wgcol = (long)blockIdx.x * CF::COLS;
-  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * CF::ASZ));
+  // a_chunk: byte stride of the arranged operand's chunks (>= CF::ASZ)
+  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * a_chunk));
   const i32x4m rsX = mk_rsrc(src, (uint32_t)((size_t)n * n2 * 4));
   const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
 
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
 #pragma unroll
     for (int e = 0; e < CF::APT; e++)
       copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
-                    (int)(((size_t)kc * CF::APT * CF::THREADS + e * CF::THREADS + threadIdx.x) * 16));
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
     // X piece e of this lane: q = e, column 32 wave + lane/2, k half lane%2
 #pragma unroll
     for (int e = 0; e < CF::XPT; e++) {
@@ -185,12 +186,12 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
 }
 
 // ---- host side
-static inline int waves_for(int MT) { return MT <= 14 ? 8 : 4; }
-
+// the arranged operand's chunk stride serves both workgroup shapes: the
+// larger of the two copy granularities
 template <int MT, int NT>
 static size_t chunk_bytes() {
-  if (waves_for(MT) == 8) return mps_cfg<MT, NT, 8>::ASZ;
-  return mps_cfg<MT, NT, 4>::ASZ;
+  const size_t a = mps_cfg<MT, NT, 8>::ASZ, b = mps_cfg<MT, NT, 4>::ASZ;
+  return a > b ? a : b;
 }
 
 static size_t chunk_bytes_rt(int MT, int NT) {
@@ -262,12 +263,16 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
     }
     dim3 grid((unsigned)((n2 + CF::COLS - 1) / CF::COLS), nfields);
     k_modeprod_split<MT, NT, W><<<grid, CF::THREADS, CF::LDS, c->stream>>>(A3, in, in_fs, out, out_fs, n, rotate,
-                                                                          accumulate);
+                                                                          accumulate, (int)chunk_bytes<MT, NT>());
   };
-  if constexpr (MT <= 14)
-    go(std::integral_constant<int, 8>{});
-  else
+  if constexpr (MT <= 14) {
+    if (c->split_waves == 4)
+      go(std::integral_constant<int, 4>{});
+    else
+      go(std::integral_constant<int, 8>{});
+  } else {
     go(std::integral_constant<int, 4>{});
+  }
 }
 
 void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,

@@ -80,6 +80,16 @@ class AdaptStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """Mirror of etd_tuning."""
+
+    _fields_ = [("mp_variant", C.c_int), ("split_waves", C.c_int), ("fft_variant", C.c_int),
+                ("ms_per_step", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _LIB = None
 
 
@@ -136,6 +146,13 @@ def lib():
             "etd_mlp_train": (C.c_int, [ctxp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]),
             "etd_mlp_predict": (C.c_int, [ctxp, _dp, _dp]),
             "etd_set_controller": (C.c_int, [ctxp, ctxp]),
+            "etd_ensemble_create": (C.c_int, [C.POINTER(ctxp), C.c_int, C.c_int, C.POINTER(ctxp)]),
+            "etd_ensemble_run": (C.c_int, [ctxp, C.c_int]),
+            "etd_ensemble_destroy": (None, [ctxp]),
+            "etd_ensemble_error": (C.c_char_p, [ctxp]),
+            "etd_autotune": (C.c_int, [P, _dp, _dp, _dp, C.c_int, C.POINTER(Tuning)]),
+            "etd_set_tuning": (C.c_int, [ctxp, C.POINTER(Tuning)]),
+            "etd_get_tuning": (C.c_int, [ctxp, C.POINTER(Tuning)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -162,6 +179,45 @@ def adapt_opts(**kw):
             v = {"plain": CTRL_PLAIN, "learned": CTRL_LEARNED}[v]
         setattr(o, k, v)
     return o
+
+
+def autotune(prm, steps=3):
+    """etd_autotune for a parameter dict: the fastest kernel choice (Tuning)."""
+    t = Tuning()
+    _check(lib().etd_autotune(C.byref(params_from_dict(prm)), None, None, None, steps, C.byref(t)), what="autotune")
+    return t
+
+
+class Ensemble:
+    """One hipGraph stepping many contexts (etd_ensemble)."""
+
+    def __init__(self, members, steps_per_launch):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.h = C.c_void_p()
+        rc = lib().etd_ensemble_create(arr, len(self.members), steps_per_launch, C.byref(self.h))
+        if rc != 0:
+            msg = lib().etd_ensemble_error(self.h).decode() if self.h else ""
+            if self.h:
+                lib().etd_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+            raise EtdError(f"ensemble_create: {lib().etd_error_string(rc).decode()}: {msg}")
+
+    def run(self, launches=1):
+        rc = lib().etd_ensemble_run(self.h, launches)
+        if rc != 0:
+            raise EtdError(f"ensemble_run: {lib().etd_ensemble_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().etd_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Mlp:
@@ -443,6 +499,14 @@ class Etd:
     @time.setter
     def time(self, t):
         lib().etd_set_time(self.ctx, t)
+
+    def set_tuning(self, t):
+        _check(lib().etd_set_tuning(self.ctx, C.byref(t)), self.ctx, "set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(lib().etd_get_tuning(self.ctx, C.byref(t)), self.ctx, "get_tuning")
+        return t
 
     def set_controller(self, mlp):
         _check(lib().etd_set_controller(self.ctx, mlp.h if mlp is not None else None), self.ctx, "set_controller")

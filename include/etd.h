@@ -273,6 +273,40 @@ int etd_mlp_predict(etd_mlp *m, const double *features, double *y);
 /* Attach (or detach with NULL) the controller network of a context. */
 int etd_set_controller(etd_ctx *c, etd_mlp *m);
 
+/* ---- batched ensembles and the tile autotuner ----
+ * No counterpart in the reference (one simulation per MPI job).  An ensemble
+ * steps many independent contexts of uneven sizes on one GPU: one hipGraph,
+ * built once, holds steps_per_launch fixed-dt steps of every member, each
+ * member on its own stream forked from and joined to the ensemble's stream,
+ * so members run concurrently; etd_ensemble_run replays it.  A replay is bit
+ * for bit the same as stepping the members one by one.  With elasticity the
+ * graph repeats the captured felas ring positions, so steps_per_launch *
+ * (elastic evaluations per step) * nac must be a multiple of the ring length
+ * nac + 2 for every member (etd_ensemble_create says which multiple).
+ * Members must live on one device and must not be stepped elsewhere while a
+ * replay runs. */
+typedef struct etd_ensemble etd_ensemble;
+int etd_ensemble_create(etd_ctx *const *members, int count, int steps_per_launch, etd_ensemble **out);
+/* launches replays; each advances every member by steps_per_launch steps. */
+int etd_ensemble_run(etd_ensemble *e, int launches);
+void etd_ensemble_destroy(etd_ensemble *e);
+const char *etd_ensemble_error(const etd_ensemble *e);
+
+/* Kernel choices of a context (the autotuner's search space). */
+typedef struct etd_tuning {
+  int mp_variant;      /* FP64 mode products: 0, 1, 2 (direct-to-LDS), 4 (register-staged), 16 (16x16x4) */
+  int split_waves;     /* mixed mode products: 8 or 4 waves per workgroup (0: by size) */
+  int fft_variant;     /* 1: two-stage register FFT (16 | N), 0: radix passes */
+  double ms_per_step;  /* measured by etd_autotune */
+} etd_tuning;
+/* Times `steps` fixed-dt steps of every candidate on a scratch context with
+ * the same parameters (the caller's contexts are untouched) and returns the
+ * fastest choice. */
+int etd_autotune(const etd_params *p, const double *V, const double *Vi, const double *D, int steps,
+                 etd_tuning *best);
+int etd_set_tuning(etd_ctx *c, const etd_tuning *t);
+int etd_get_tuning(const etd_ctx *c, etd_tuning *t);
+
 /* Measurement support, no counterpart in the reference: one plain copy
  * kernel that reads and writes exactly `bytes` (a multiple of 16, at most the
  * context's 2 (nac+nch) n^3 doubles of scratch) between two of the context's
