@@ -84,6 +84,110 @@ __global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const
   }
 }
 
+// block sums of two per-thread values into part[block * RCOLS + 0 / 1]
+__device__ inline void block_sum2(double a, double b, double *__restrict__ part) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  __shared__ double sh[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * RCOLS] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+    part[blockIdx.x * RCOLS + 1] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+  }
+}
+
+// sum over AC fields and interior cells of eta_p felas_p (column 0) and of
+// |eta_p felas_p| (column 1), felas from the ring slots of the latest elastic
+// evaluation (its base linear index Lbase)
+template <typename T>
+__global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u, const T *__restrict__ E, int n, int g,
+                                                        int nac, int ring, long long Lbase,
+                                                        double *__restrict__ part) {
+  const int N = n - 2 * g;
+  const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
+  double acc = 0.0, aabs = 0.0;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+    const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
+    for (int p = 0; p < nac; p++) {
+      const int slot = (int)((Lbase + p) % ring);
+      const double t = (double)u[n3 * p + l] * (double)E[(size_t)slot * N3 + m];
+      acc += t;
+      aabs += fabs(t);
+    }
+  }
+  block_sum2(acc, aabs, part);
+}
+
+// The scheme's own discrete free energy over the interior (periodic single
+// subdomain: the truncated operators plus the explicit outside coupling are
+// the periodic 7-point Laplacian), per cell h^3 times
+//   AC: sum_p epn2/2 (kkx (D+x eta_p)^2 + kky (D+y)^2 + kkz (D+z)^2)
+//       + epn2/2 grad_c eta_p . Lambda_p grad_c eta_p (anisotropic, central)
+//       + a2/2 S2 - a3/3 S3 + a4/4 S2^2,   S2 = sum eta^2, S3 = sum eta^3
+//   CH: epn2/2 |D+ c|^2 + (c^2 - 1)^2 / 4
+// The semi-discrete dynamics is the gradient flow of this energy (plus the
+// elastic energy), unlike the 8-corner check_soln_new diagnostic.  Column 0
+// holds the energy, column 1 the sum of the absolute values of its terms (the
+// scale of its rounding).
+struct scheme_args {
+  int n, g, nac, nch, aniso;
+  double hx, hy, hz, epn2, kkx, kky, kkz, a2, a3, a4;
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, const double *__restrict__ lam,
+                                                       scheme_args a, double *__restrict__ part) {
+  const int n = a.n, g = a.g, N = n - 2 * g;
+  const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
+  double acc = 0.0, aabs = 0.0;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+    const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
+    const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
+    const size_t yp = (((size_t)(z + g)) * n + ((y + 1) % N + g)) * n + (x + g);
+    const size_t zp = (((size_t)((z + 1) % N + g)) * n + (y + g)) * n + (x + g);
+    const size_t xm = (((size_t)(z + g)) * n + (y + g)) * n + ((x + N - 1) % N + g);
+    const size_t ym = (((size_t)(z + g)) * n + ((y + N - 1) % N + g)) * n + (x + g);
+    const size_t zm = (((size_t)((z + N - 1) % N + g)) * n + (y + g)) * n + (x + g);
+    double s2 = 0.0, s3 = 0.0, e = 0.0, ea = 0.0;
+    for (int p = 0; p < a.nac + a.nch; p++) {
+      const T *f = u + n3 * p;
+      const double v = f[l];
+      const double dx = ((double)f[xp] - v) / a.hx, dy = ((double)f[yp] - v) / a.hy, dz = ((double)f[zp] - v) / a.hz;
+      if (p < a.nac) {
+        e += 0.5 * a.epn2 * (a.kkx * dx * dx + a.kky * dy * dy + a.kkz * dz * dz);
+        if (a.aniso) {
+          const double gx = ((double)f[xp] - (double)f[xm]) / (2.0 * a.hx);
+          const double gy = ((double)f[ym] - (double)f[yp]) / (2.0 * a.hy);
+          const double gz = ((double)f[zm] - (double)f[zp]) / (2.0 * a.hz);
+          const double *L = lam + 9 * p;
+          const double lx = L[0] * gx + L[1] * gy + L[2] * gz, ly = L[3] * gx + L[4] * gy + L[5] * gz,
+                       lz = L[6] * gx + L[7] * gy + L[8] * gz;
+          e += 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+          ea += fabs(0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz)) - 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+        }
+        s2 += v * v;
+        s3 += v * v * v;
+      } else {
+        e += 0.5 * a.epn2 * (dx * dx + dy * dy + dz * dz) + 0.25 * (v * v - 1.0) * (v * v - 1.0);
+      }
+    }
+    // the gradient and CH terms are non-negative: their absolute sum is e
+    // itself (ea corrects an indefinite Lambda's term)
+    ea += e + fabs(0.5 * a.a2 * s2) + fabs(a.a3 / 3.0 * s3) + 0.25 * fabs(a.a4) * s2 * s2;
+    e += 0.5 * a.a2 * s2 - a.a3 / 3.0 * s3 + 0.25 * a.a4 * s2 * s2;
+    acc += e;
+    aabs += ea;
+  }
+  block_sum2(acc, aabs, part);
+}
+
 // u = 2 u - b (local extrapolation of the step-doubling pair)
 template <typename T>
 __global__ void k_extrapolate(T *__restrict__ u, const T *__restrict__ b, size_t cnt) {
@@ -147,19 +251,76 @@ static int features_of(etd_ctx *c, const T *u, const T *u0, run_feat *f) {
   return ETD_OK;
 }
 
-// check_soln_new energy of AC plus CH fields, FP64
+// The acceptance energy of the current fields, FP64.  kind 0: the
+// check_soln_new energy of the AC and CH fields (the reference's diagnostic);
+// kind 1: the scheme's discrete free energy (k_scheme_energy).  With
+// with_elastic, plus the elastic energy ES/2 h^3 sum_p eta_p felas_p from a
+// fresh elastic evaluation of these fields, which the next step reuses
+// (c->elastic_fresh) if they are kept.  *Eabs is the sum of the absolute
+// values of the energy's terms (kind 1) or |E| (kind 0).
 template <typename T>
-static int energy_of(etd_ctx *c, double *E) {
-  double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
-  if (!st) return ETD_ERR_NOMEM;
-  int rc = launch_diag<T>(c, st);
-  if (!rc) {
-    double e = c->nac > 0 ? st[3] : 0.0;
-    for (int q = 0; q < c->nch; q++) e += st[4 * (c->nac + q) + 3];
-    *E = e;
+static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *Eabs) {
+  double e = 0.0, ea = 0.0;
+  int rc = ETD_OK;
+  if (kind == 0) {
+    double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
+    if (!st) return ETD_ERR_NOMEM;
+    rc = launch_diag<T>(c, st);
+    if (!rc) {
+      e = c->nac > 0 ? st[3] : 0.0;
+      for (int q = 0; q < c->nch; q++) e += st[4 * (c->nac + q) + 3];
+      ea = fabs(e);
+    }
+    free(st);
+    if (rc) return rc;
+  } else {
+    scheme_args a;
+    a.n = c->n;
+    a.g = c->g;
+    a.nac = c->nac;
+    a.nch = c->nch;
+    a.aniso = c->p.anisotropic;
+    a.hx = c->p.hx;
+    a.hy = c->p.hy;
+    a.hz = c->p.hz;
+    a.epn2 = c->p.epn2;
+    a.kkx = c->p.kkx;
+    a.kky = c->p.kky;
+    a.kkz = c->p.kkz;
+    a.a2 = c->p.a2;
+    a.a3 = c->p.a3;
+    a.a4 = c->p.a4;
+    k_scheme_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(fields<T>(c).u, c->lam, a, c->ad_part);
+    if ((rc = fetch_part(c, 2))) return rc;
+    double s = 0.0, sa = 0.0;
+    for (int i = 0; i < RBLOCKS; i++) {
+      s += c->ad_hpart[i * RCOLS];
+      sa += c->ad_hpart[i * RCOLS + 1];
+    }
+    e = s * c->p.hx * c->p.hy * c->p.hz;
+    ea = sa * c->p.hx * c->p.hy * c->p.hz;
   }
-  free(st);
-  return rc;
+  if (with_elastic && c->p.elastic == 1 && c->nac > 0) {
+    const etd_fields<T> F = fields<T>(c);
+    // the evaluation packs interior cells only: the fields' ghost cells are
+    // left as the step wrote them (plain stepping refreshes them next step)
+    launch_elastic<T>(c);
+    const long long Lbase = (c->elastic_evals - 1) * (long long)c->nac;
+    k_elastic_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, F.E, c->n, c->g, c->nac, c->ring, Lbase, c->ad_part);
+    if ((rc = fetch_part(c, 2))) return rc;
+    double s = 0.0, sa = 0.0;
+    for (int i = 0; i < RBLOCKS; i++) {
+      s += c->ad_hpart[i * RCOLS];
+      sa += c->ad_hpart[i * RCOLS + 1];
+    }
+    const double w = 0.5 * c->p.elastic_scale * c->p.hx * c->p.hy * c->p.hz;
+    e += w * s;
+    ea = kind == 0 ? fabs(e) : ea + fabs(w) * sa;
+    c->elastic_fresh = 1;  // the ring's latest evaluation is of these fields
+  }
+  *E = e;
+  *Eabs = ea;
+  return ETD_OK;
 }
 
 static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
@@ -179,6 +340,8 @@ extern "C" void etd_adapt_defaults(etd_adapt_opts *o) {
   o->controller = ETD_CTRL_PLAIN;
   o->extrapolate = 1;
   o->energy_test = 1;
+  o->energy_elastic = 1;
+  o->energy_kind = 1;
   o->max_attempts = 1000000;
 }
 
@@ -203,14 +366,21 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   }
   T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec;
   const double dt_fixed = c->p.dt;
+  // energy-increase slack: the rounding level of the storage type times the
+  // energy's scale, for the scheme energy the larger absolute term sum of
+  // the two states (the same for one run or for the run cut into pieces),
+  // for check_soln_new the largest |E| of the run
   const double slack = sizeof(T) == 4 ? 1e-7 : 1e-12;
+  const int with_el = o->energy_elastic;
   const int learned = o->controller == ETD_CTRL_LEARNED && c->mlp;
   memset(st, 0, sizeof(*st));
   if (log_rows) *log_rows = 0;
 
   // state of the run
-  double E;
-  int rc = energy_of<T>(c, &E);
+  double E, Eabs;
+  const int ekind = o->energy_kind;
+  int rc = energy_of<T>(c, ekind, with_el, &E, &Eabs);
+  double Escale = fabs(E);
   if (rc) return rc;
   run_feat rf;
   if ((rc = features_of<T>(c, F.u, F.u, &rf))) return rc;
@@ -243,7 +413,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
         // the plain proposal; at most a decade either way
         double y;
         if (etd_mlp_predict(c->mlp, feat, &y) == ETD_OK && isfinite(y)) {
-          dt = o->safety * plain_next * pow(10.0, clampd(y, -1.0, 1.0));
+          // no more than a decade above the plain proposal; never below the
+          // smaller of it and the last step, so a network that errs low
+          // cannot shrink the step where the plain rule would not
+          dt = clampd(o->safety * plain_next * pow(10.0, y), fmin(plain_next, hist[0]), 10.0 * plain_next);
           st->learned_proposals++;
         } else {
           st->learned_fallbacks++;
@@ -279,11 +452,12 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     double err;
     if ((rc = max_abs_diff<T>(c, F.u, SB, cnt, &err))) break;
     if (o->extrapolate) k_extrapolate<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt);
-    double Enew;
-    if ((rc = energy_of<T>(c, &Enew))) break;
+    double Enew, Eabs_new;
+    if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new))) break;
+    const double scale = ekind == 0 ? fmax(Escale, fabs(E)) : fmax(Eabs, Eabs_new);
     const double fac = err > 0.0 ? o->safety * sqrt(o->tol / err) : o->grow;
     const int err_ok = err <= o->tol && isfinite(err);
-    const int energy_ok = !o->energy_test || Enew <= E + slack * fabs(E);
+    const int energy_ok = !o->energy_test || Enew <= E + slack * scale;
     const int forced = dt <= o->dt_min * (1.0 + 1e-12) && isfinite(err);
     if ((err_ok && energy_ok) || forced) {
       run_feat nf;
@@ -303,6 +477,8 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       hist[0] = dt;
       dEdt = (Enew - E) / dt;
       E = Enew;
+      Eabs = Eabs_new;
+      Escale = fmax(Escale, fabs(E));
       err_last = err;
       rf = nf;
       plain_next = dt * clampd(fac, o->shrink, o->grow);
@@ -310,6 +486,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       fallback = 0;
     } else {
       ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+      c->elastic_fresh = 0;  // the latest evaluation is of the rejected candidate
       if (!err_ok) {
         st->rejected_err++;
         dt = dt * clampd(fac, o->shrink, 1.0);

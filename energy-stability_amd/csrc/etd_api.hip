@@ -149,6 +149,10 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   if (ev) dev = atoi(ev);
   HCHK(c, hipSetDevice(dev));
   c->device = dev;
+  HCHK(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  c->split_persistent = 1;
+  const char *spv = getenv("ETD_SPLIT_PERSIST");
+  if (spv && atoi(spv) == 0) c->split_persistent = 0;
   HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t n3 = c->n3;
   c->mixed = p->precision == ETD_PREC_MIXED;
@@ -394,6 +398,7 @@ static int get_f32(etd_ctx *c, double *host, const float *dev, size_t cnt) {
 
 extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "set_field: bad variable %d", var);
+  c->elastic_fresh = 0;
   if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
   HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
@@ -448,7 +453,10 @@ void stage_forward(etd_ctx *c) {
   const int V = c->V;
   const etd_fields<T> F = fields<T>(c);
   launch_ghost_fill<T>(c, F.u, V, c->n3);
-  if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
+  // an adaptive acceptance test may already have evaluated felas of exactly
+  // these fields (same ghost fill, same kernels): reuse it
+  if (c->p.elastic == 1 && c->nac > 0 && !c->elastic_fresh) launch_elastic<T>(c);
+  c->elastic_fresh = 0;
   launch_nonlinear<T>(c, F.N1, nullptr);
   modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
 }

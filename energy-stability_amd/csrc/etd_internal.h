@@ -67,9 +67,12 @@ struct etd_ctx {
   etd_dev_consts *dconst;
   int npair, ring;
   long long elastic_evals; /* completed elastic evaluations */
+  int elastic_fresh;       /* the latest evaluation is of the current fields (adaptive energy test) */
   int fft_radix[16], fft_npass;
   int fft_variant;         /* 1: two-stage register FFT for N % 16 == 0 (default), 0: radix passes in LDS */
   int split_waves;         /* mixed mode products: 8 or 4 waves per workgroup, 0 by size */
+  int split_persistent;    /* 1: persistent workgroups looping over tiles (default) */
+  int num_cu;              /* compute units of the device */
   /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
    * split into split_terms BF16 terms in MFMA fragment order */
   int mixed;

@@ -185,6 +185,134 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
     }
 }
 
+// Persistent variant: gridDim.x workgroups (one per CU) loop over the tiles
+// (field, 32 WAVES columns); the double-buffered chunk pipeline runs across
+// tile boundaries, so the next tile's first chunk loads during the current
+// tile's last chunk and no workgroup start-up is exposed.
+template <int MT, int NT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
+    const uint16_t *__restrict__ A3, const float *__restrict__ in, size_t in_fs, float *__restrict__ out,
+    size_t out_fs, int n, int rotate, int accumulate, int a_chunk, int nfields) {
+  using CF = mps_cfg<MT, NT, WAVES>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;  // uniform
+  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * a_chunk));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue = [&](int tl, int kc, int stg) {
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    i32x4m rsX = mk_rsrc(in + in_fs * f, (uint32_t)((size_t)n * n2 * 4));
+    // the tile is the same for the whole workgroup: keep the descriptor scalar
+    rsX.x = __builtin_amdgcn_readfirstlane(rsX.x);
+    rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
+    rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
+    const uint32_t sbase = lds0 + stg * CF::STAGE;
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      const long col = wgc + 32 * wave + (lane >> 1);
+      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + CF::ASZ + (e * CF::THREADS + wave * 64) * 16, voff);
+    }
+  };
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue(tile, 0, 0);
+  int g = 0;  // chunks consumed so far: stage g & 1
+#pragma unroll 1
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int f = tile / ncb;
+    const long wgcol = (long)(tile % ncb) * CF::COLS;
+    float *dst = out + out_fs * f;
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int kc = 0; kc < nkc; kc++, g++) {
+      // this chunk is in LDS for every wave, and every wave is past the
+      // previous one, whose stage the next copies overwrite
+      wait_all_copies_barrier();
+      if (kc + 1 < nkc)
+        issue(tile, kc + 1, (g + 1) & 1);
+      else if (tile + (int)gridDim.x < ntiles)
+        issue(tile + gridDim.x, 0, (g + 1) & 1);
+      const char *stg = smem + (g & 1) * CF::STAGE;
+      const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(stg);
+      const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
+      bf16x8 b[2][NT];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+        const float4 lo = *reinterpret_cast<const float4 *>(bp);
+        const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+        const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          __bf16 t0, t1, t2;
+          split_bf16(x[e], t0, t1, t2);
+          b[u][0][e] = t0;
+          if constexpr (NT > 1) b[u][1][e] = t1;
+          if constexpr (NT > 2) b[u][2][e] = t2;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MT; t++) {
+        bf16x8 a[NT];
+#pragma unroll
+        for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          if constexpr (NT > 2) {
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[t][u], 0, 0, 0);
+          }
+          if constexpr (NT > 1) {
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[t][u], 0, 0, 0);
+          }
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[t][u], 0, 0, 0);
+        }
+      }
+    }
+    // 32-bit offsets (n^3 < 2^31) from strides made opaque per tile: the
+    // compiler would otherwise hoist all MT * 4 row offsets out of the tile
+    // loop and spill them
+    uint32_t s_row = rotate ? (uint32_t)n2 : 1u, s_col = rotate ? 1u : (uint32_t)n;
+    asm volatile("" : "+s"(s_row), "+s"(s_col));
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const long col = wgcol + 32 * wave + 16 * u + r16;
+        if (col >= n2) continue;
+        const uint32_t cbase = (uint32_t)col * s_col;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int m = 16 * t + 4 * q + i;
+          if (m >= n) continue;
+          const uint32_t o = cbase + (uint32_t)m * s_row;
+          if (accumulate) dst[o] += acc[t][u][i];
+          else dst[o] = acc[t][u][i];
+        }
+      }
+  }
+}
+
 // ---- host side
 // the arranged operand's chunk stride serves both workgroup shapes: the
 // larger of the two copy granularities
@@ -259,11 +387,21 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
     if (!attr_set) {
       (void)hipFuncSetAttribute((const void *)k_modeprod_split<MT, NT, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 CF::LDS);
+      (void)hipFuncSetAttribute((const void *)k_modeprod_split_p<MT, NT, W>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
       attr_set = true;
     }
-    dim3 grid((unsigned)((n2 + CF::COLS - 1) / CF::COLS), nfields);
-    k_modeprod_split<MT, NT, W><<<grid, CF::THREADS, CF::LDS, c->stream>>>(A3, in, in_fs, out, out_fs, n, rotate,
-                                                                          accumulate, (int)chunk_bytes<MT, NT>());
+    const long ncb = (n2 + CF::COLS - 1) / CF::COLS;
+    if (c->split_persistent) {
+      const long ntiles = ncb * nfields;
+      const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;  // one workgroup per CU (LDS-bound)
+      k_modeprod_split_p<MT, NT, W><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+          A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+    } else {
+      dim3 grid((unsigned)ncb, nfields);
+      k_modeprod_split<MT, NT, W><<<grid, CF::THREADS, CF::LDS, c->stream>>>(A3, in, in_fs, out, out_fs, n, rotate,
+                                                                            accumulate, (int)chunk_bytes<MT, NT>());
+    }
   };
   if constexpr (MT <= 14) {
     if (c->split_waves == 4)

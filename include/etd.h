@@ -188,15 +188,25 @@ int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launc
  * dt and two of dt/2 (the first half step shares N(u_n) and its forward
  * transforms with the full step); err = max |u_half - u_full| over all fields.
  * An attempt is accepted when err <= tol and, with energy_test, the FP64
- * check_soln_new energy (AC + CH) does not increase (slack: 1e-12 relative on
- * the FP64 path, 1e-7 on the mixed path, the FP32 rounding level).  Rejected
+ * energy does not increase.  The energy is the scheme's discrete free energy
+ * (energy_kind 1: periodic forward-difference gradient energy with kk, the
+ * central-difference anisotropic term, Landau bulk, CH double well; the
+ * semi-discrete dynamics is its gradient flow) or check_soln_new's AC + CH
+ * energy (energy_kind 0, the reference's 8-corner diagnostic, which is not
+ * a Lyapunov function of the scheme); with energy_elastic plus the elastic
+ * energy ES/2 h^3 sum_p eta_p felas_p (one more elastic evaluation per
+ * attempt, reused by the next step when accepted).
+ * Slack: 1e-12 (FP64 path) or 1e-7 (mixed path, the FP32 rounding level)
+ * times the largest |E| of the run.  Rejected
  * attempts restore u_n.  The plain controller proposes
  * dt * clamp(safety (tol/err)^(1/2), shrink, grow); the learned controller
- * proposes safety * dt_plain * 10^clamp(y, -1, 1), where y = MLP(features)
- * over ETD_NFEAT features of the run (below) predicts log10 of the error
- * model's largest step over the plain proposal dt_plain (so the learned
- * proposal is not held to the plain growth limit), and falls back to the
- * plain proposal after a rejection or for a non-finite output. */
+ * proposes safety * dt_plain * 10^y, where y = MLP(features) over
+ * ETD_NFEAT features of the run (below) predicts log10 of the error model's
+ * largest step over the plain proposal dt_plain, bounded to
+ * [min(dt_plain, last dt), 10 dt_plain] (so the learned step is not held to
+ * the plain growth limit but cannot shrink where the plain rule would not),
+ * and falls back to the plain proposal after a rejection or for a
+ * non-finite output. */
 #define ETD_NFEAT 16
 enum { ETD_CTRL_PLAIN = 0, ETD_CTRL_LEARNED = 1 };
 
@@ -209,6 +219,8 @@ typedef struct etd_adapt_opts {
   int controller;      /* ETD_CTRL_PLAIN or ETD_CTRL_LEARNED (needs etd_set_controller) */
   int extrapolate;     /* 1: accept 2 u_half - u_full (local extrapolation) */
   int energy_test;     /* 1: reject an energy increase */
+  int energy_elastic;  /* 1: the tested energy adds the elastic energy ES/2 h^3 sum eta_p felas_p */
+  int energy_kind;     /* 1: the scheme's discrete free energy (default); 0: check_soln_new's */
   int max_attempts;
 } etd_adapt_opts;
 

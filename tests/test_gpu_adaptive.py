@@ -66,7 +66,8 @@ def test_plain_controller_matches_oracle_restatement(elastic):
     kw = dict(t_end=0.6, dt0=0.2, dt_min=1e-5, dt_max=0.3, tol=2e-4)
     t_o, rows_o, cnt_o = AD.run(o, **kw)
     g = gpu(prm, f)
-    st, log = g.run_adaptive(log_cap=1000, **kw)
+    # the restatement tests the check_soln_new energy (no elastic term)
+    st, log = g.run_adaptive(log_cap=1000, energy_elastic=0, energy_kind=0, **kw)
     say(f"  oracle {cnt_o}  gpu {st}")
     assert cnt_o["rejected_err"] > 0 and cnt_o["accepted"] > 3  # both branches exercised
     assert st["accepted"] == cnt_o["accepted"]
@@ -78,6 +79,72 @@ def test_plain_controller_matches_oracle_restatement(elastic):
     assert np.abs(log[:, 3] - ro[:, 3]).max() < 1e-10 * np.abs(ro[:, 3]).max()
     for v in range(4):
         assert np.abs(g.get_field(v) - o.get_field(v)).max() < 1e-10
+
+
+def scheme_energy(prm, fields):
+    """numpy restatement of the scheme's discrete free energy (interior,
+    periodic forward differences; isotropic AC and CH)."""
+    g = prm["g"]
+    hx, hy, hz = prm["h"]
+    kk = prm["kk"]
+    a2, a3, a4 = prm["landau"]
+    ep = prm["epn2"]
+    inner = [f[g:-g, g:-g, g:-g] for f in fields]  # [z][y][x]
+    e = 0.0
+    s2 = np.zeros_like(inner[0])
+    s3 = np.zeros_like(inner[0])
+    for p, f in enumerate(inner):
+        dx = (np.roll(f, -1, 2) - f) / hx
+        dy = (np.roll(f, -1, 1) - f) / hy
+        dz = (np.roll(f, -1, 0) - f) / hz
+        if p < prm["nac"]:
+            e += 0.5 * ep * (kk[0] * dx ** 2 + kk[1] * dy ** 2 + kk[2] * dz ** 2).sum()
+            s2 += f ** 2
+            s3 += f ** 3
+        else:
+            e += (0.5 * ep * (dx ** 2 + dy ** 2 + dz ** 2) + 0.25 * (f ** 2 - 1) ** 2).sum()
+    e += (0.5 * a2 * s2 - a3 / 3.0 * s3 + 0.25 * a4 * s2 ** 2).sum()
+    return e * hx * hy * hz
+
+
+def test_acceptance_energy_scheme_and_elastic_terms():
+    """The acceptance energy of an accepted state: the scheme energy plus
+    ES/2 h^3 sum_p eta_p felas_p, against a numpy restatement and felas from
+    etd_k_elastic on a second context holding the same fields (FP64); with
+    energy_kind 0 the check_soln_new energy replaces the scheme energy."""
+    say("acceptance energy: scheme + elastic")
+    prm = small_params(n=16, nac=3, nch=1, elastic=1, dt=0.01, h=(0.9, 1.0, 1.1), kk=(1.0, 1.2, 0.8))
+    f = noise_fields(prm, amp=0.2)
+    for kind in (1, 0):
+        g = gpu(prm, f)
+        st, log = g.run_adaptive(log_cap=4, t_end=0.01, dt0=0.01, tol=1e9, energy_test=0, energy_kind=kind)
+        assert st["accepted"] == 1
+        u1 = [g.get_field(v) for v in range(4)]
+        h = gpu(prm, u1)
+        h.transfer()
+        fel = h.elastic()
+        sd, _ = h.diag()
+        gg = prm["g"]
+        inner = (slice(gg, -gg),) * 3
+        e_el = 0.5 * 0.9 * 1.0 * 1.1 * sum(float((u1[p][inner] * fel[p][inner]).sum()) for p in range(3))
+        e_base = scheme_energy(prm, u1) if kind == 1 else sd[0, 3] + sd[3, 3]
+        say(f"  kind {kind}: E = {log[0, 3]:.12e}  (base {e_base:.6e} + elastic {e_el:.6e})")
+        assert abs(log[0, 3] - (e_base + e_el)) < 1e-10 * abs(e_base + e_el)
+
+
+def test_scheme_energy_decreases_along_small_steps():
+    """The scheme energy is a Lyapunov function of the discrete dynamics: it
+    does not increase over small steps, even from rough states where the
+    check_soln_new diagnostic is a different quadratic form."""
+    say("scheme energy monotone")
+    prm = dict(small_params(n=20, nac=4, nch=1, elastic=1, dt=0.002, KE=3.2, landau=(0.2, 3.0, 3.0)),
+               eps0=pyetd.workloads.eigenstrains(4, 0.03, 3))
+    f = noise_fields(prm, amp=0.25, ch_amp=0.2)
+    g = gpu(prm, f)
+    st, log = g.run_adaptive(log_cap=100, t_end=0.2, dt0=0.002, tol=1e9, grow=1.0, shrink=1.0, energy_test=0)
+    E = log[:, 3]
+    say(f"  E {E[0]:.6e} -> {E[-1]:.6e} over {len(E)} steps, max rise {np.diff(E).max():.3e}")
+    assert (np.diff(E) <= 1e-12 * np.abs(E[:-1])).all()
 
 
 def test_mlp_forward_and_gradient_match_oracle():

@@ -131,3 +131,25 @@ def test_precision_flag_is_checked_before_device_work():
     assert pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx)) == -3
     p = pyetd.params_from_dict(dict(n=16, nac=1, nch=0, dt=0.01, epn2=1.0, precision=7))
     assert pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx)) == -1
+
+
+def test_seeded_nuclei_initial_state():
+    """The coarsening workload's nuclei: deterministic for a seed, periodic
+    ghosts, values within [-a, 1], and ordered volume close to the sum of the
+    nuclei's spheres (overlaps only reduce it)."""
+    from pyetd import workloads as W
+    prm = W.scaled(W.beta_omega_256_coarsening(), 68)
+    a = W.initial_fields(prm)
+    b = W.initial_fields(prm)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    g, N = prm["g"], 64
+    for x in a:
+        assert np.array_equal(x[:g], x[N:N + g]) and np.array_equal(x[:, :, -g:], x[:, :, g:2 * g])
+        assert x.min() >= -0.01 and x.max() <= 1.0
+    inner = (slice(g, -g),) * 3
+    frac = sum(float((x[inner] > 0.5).mean()) for x in a)
+    count = round(prm["nuclei_per_cell"] * N ** 3)
+    r3 = (3.0 ** 4 - 6.0 ** 4) / (4 * (3.0 - 6.0))  # E[r^3] for r ~ U[3, 6]
+    expect = count * 4.0 / 3.0 * np.pi * r3 / N ** 3
+    assert 0.6 * expect < frac <= 1.05 * expect

@@ -443,3 +443,27 @@ def test_adaptive_restatement_with_everything_accepted_is_half_steps():
         b.step()
     for v in range(2):
         assert np.abs(a.get_field(v) - b.get_field(v)).max() == 0.0
+
+
+def test_seeded_nuclei_grow_and_energy_falls():
+    """Physics of the coarsening workload on the FP64 oracle at 32^3: nuclei
+    above the critical radius grow (ordered volume up), the scheme lowers the
+    check_soln_new energy, and no variant decays to the beta state."""
+    from pyetd import workloads as W
+    prm = W.scaled(W.beta_omega_256_coarsening(), 36, precision=0, dt=0.1)
+    f = W.initial_fields(prm)
+    o = O.Oracle(prm)
+    for v, x in enumerate(f):
+        o.set_field(v, x)
+    g = prm["g"]
+    inner = (slice(g, -g),) * 3
+    vol0 = sum(float((x[inner] > 0.5).mean()) for x in f)
+    e = []
+    for _ in range(4):
+        o.step(25)
+        sd, _ = o.diag()
+        e.append(sd[0, 3])
+    vol = sum(float((o.get_field(v)[inner] > 0.5).mean()) for v in range(4))
+    assert vol > 2.0 * vol0
+    assert all(b < a for a, b in zip(e, e[1:]))
+    assert max(float(o.get_field(v).max()) for v in range(4)) > 0.85
