@@ -27,6 +27,14 @@ __global__ __launch_bounds__(256) void k_maxdiff(const T *__restrict__ a, const 
   if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
+// Interior cells visited one (z, y) row per wave, lanes along x: one 32-bit
+// division per row instead of a 64-bit one per cell.  Fixed mapping, so the
+// partial sums are deterministic.
+#define FOR_INTERIOR_CELLS(N, x, y, z)                                                          \
+  for (int r_ = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), z = r_ / (N), y = r_ - z * (N); \
+       r_ < (N) * (N); r_ += gridDim.x * (blockDim.x >> 6), z = r_ / (N), y = r_ - z * (N))        \
+    for (int x = threadIdx.x & 63; x < (N); x += 64)
+
 // Run features over the interior cells of the AC fields: max |u - u0|,
 // max |eta|, count of cells with max_p |eta_p| > 0.1, and the sum over p of
 // |grad eta_p| (central differences on the periodic images).
@@ -37,8 +45,8 @@ __global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const
   const int N = n - 2 * g;
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
   double dmax = 0.0, emax = 0.0, cnt = 0.0, gsum = 0.0;
-  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  (void)N3;
+  FOR_INTERIOR_CELLS(N, x, y, z) {
     const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
     const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
     const size_t xm = (((size_t)(z + g)) * n + (y + g)) * n + ((x + N - 1) % N + g);
@@ -112,9 +120,9 @@ __global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u,
   const int N = n - 2 * g;
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
   double acc = 0.0, aabs = 0.0;
-  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  FOR_INTERIOR_CELLS(N, x, y, z) {
     const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
+    const size_t m = ((size_t)z * N + y) * N + x;
     for (int p = 0; p < nac; p++) {
       const int slot = (int)((Lbase + p) % ring);
       const double t = (double)u[n3 * p + l] * (double)E[(size_t)slot * N3 + m];
@@ -146,8 +154,8 @@ __global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, 
   const int n = a.n, g = a.g, N = n - 2 * g;
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
   double acc = 0.0, aabs = 0.0;
-  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < N3; m += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  (void)N3;
+  FOR_INTERIOR_CELLS(N, x, y, z) {
     const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
     const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
     const size_t yp = (((size_t)(z + g)) * n + ((y + 1) % N + g)) * n + (x + g);

@@ -313,6 +313,174 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
   }
 }
 
+// Register-prefetch variant (persistent): the field operand goes straight
+// from HBM into registers with compiler-visible buffer loads issued two K
+// chunks ahead, so HBM latency is covered by two chunks of MFMA work; only
+// the split V / V^-1 operand (L2-resident, shared by the waves) passes
+// through LDS, staged through registers one chunk ahead in a two-stage ring
+// (2 x 40 KB for MT = 13).  A workgroup's chunk sequence runs over its tiles
+// without a break: chunk g is K chunk g % nkc of the workgroup's tile
+// blockIdx.x + (g / nkc) gridDim.x.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int MT, int NT>
+struct msr_cfg {
+  static constexpr int WAVES = 8;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int COLS = 32 * WAVES;
+  static constexpr int APIECES = NT * MT * 64;
+  static constexpr int APT = (APIECES + THREADS - 1) / THREADS;
+  static constexpr int ASZ = APT * THREADS * 16;
+  static constexpr int LDS = 2 * ASZ;
+};
+
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" : : : "memory"); }
+
+template <int MT, int NT>
+__global__ __launch_bounds__(512, 1) void k_modeprod_sr(const uint16_t *__restrict__ A3, const float *__restrict__ in,
+                                                        size_t in_fs, float *__restrict__ out, size_t out_fs, int n,
+                                                        int rotate, int accumulate, int a_chunk, int nfields) {
+  using CF = msr_cfg<MT, NT>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  if ((int)blockIdx.x >= ntiles) return;  // uniform
+  const int my_tiles = (ntiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int total = my_tiles * nkc;  // chunks of this workgroup
+  const __amdgpu_buffer_rsrc_t rA =
+      __builtin_amdgcn_make_buffer_rsrc((void *)A3, 0, (int)((size_t)nkc * a_chunk), 0x00020000);
+
+  // A chunk kc -> registers (this thread's pieces)
+  auto load_a = [&](int kc, u32x4 (&ra)[CF::APT]) {
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      ra[e] = __builtin_amdgcn_raw_buffer_load_b128(rA, (int)((size_t)kc * a_chunk + (size_t)(e * CF::THREADS + threadIdx.x) * 16), 0, 0);
+  };
+  auto store_a = [&](int stg, const u32x4 (&ra)[CF::APT]) {
+    u32x4 *d = reinterpret_cast<u32x4 *>(smem + stg * CF::ASZ);
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++) d[e * CF::THREADS + threadIdx.x] = ra[e];
+  };
+  // field chunk g -> registers: per column tile u, k = 32 kc + 8 q + [0, 8)
+  auto load_x = [&](int g, u32x4 (&rx)[4]) {
+    if (g >= total) return;
+    const int tl = (int)blockIdx.x + (g / nkc) * (int)gridDim.x, kc = g % nkc;
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    const __amdgpu_buffer_rsrc_t rX =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(in + in_fs * f), 0, (int)((size_t)n * n2 * 4), 0x00020000);
+    const int k = 32 * kc + 8 * q;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const long col = wgc + 32 * wave + 16 * u + r16;
+      const bool ok = col < n2 && k < n;
+      const int off = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      rx[2 * u] = __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0);
+      rx[2 * u + 1] = __builtin_amdgcn_raw_buffer_load_b128(rX, ok && k + 4 < n ? off + 16 : (int)0x80000000, 0, 0);
+    }
+  };
+
+  f32x4 acc[MT][2];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  u32x4 ra[CF::APT];
+  u32x4 x0[4], x1[4], x2[4];
+  load_a(0, ra);
+  load_x(0, x0);
+  load_x(1, x1);
+  store_a(0, ra);
+  if (total > 1) load_a(1 % nkc, ra);
+
+  // one chunk: consumes field slot xs (chunk g), refills slot xl with g + 2
+  auto chunk = [&](int g, u32x4 (&xs)[4], u32x4 (&xl)[4]) {
+    const int kc = g % nkc;
+    // stage g & 1 is complete, and every wave is past chunk g - 1, whose
+    // stage the next store overwrites
+    lds_barrier();
+    if (g + 1 < total) {
+      store_a((g + 1) & 1, ra);
+      if (g + 2 < total) load_a((g + 2) % nkc, ra);
+    }
+    load_x(g + 2, xl);
+    const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(smem + (g & 1) * CF::ASZ);
+    bf16x8 b[2][NT];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const u32x4 lo = xs[2 * u], hi = xs[2 * u + 1];
+      const float x[8] = {__uint_as_float(lo.x), __uint_as_float(lo.y), __uint_as_float(lo.z), __uint_as_float(lo.w),
+                          __uint_as_float(hi.x), __uint_as_float(hi.y), __uint_as_float(hi.z), __uint_as_float(hi.w)};
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        __bf16 t0, t1, t2;
+        split_bf16(x[e], t0, t1, t2);
+        b[u][0][e] = t0;
+        if constexpr (NT > 1) b[u][1][e] = t1;
+        if constexpr (NT > 2) b[u][2][e] = t2;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      bf16x8 a[NT];
+#pragma unroll
+      for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if constexpr (NT > 2) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[t][u], 0, 0, 0);
+        }
+        if constexpr (NT > 1) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[t][u], 0, 0, 0);
+        }
+        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[t][u], 0, 0, 0);
+      }
+    }
+    if (kc == nkc - 1) {
+      const int tl = (int)blockIdx.x + (g / nkc) * (int)gridDim.x;
+      const int f = tl / ncb;
+      const long wgcol = (long)(tl % ncb) * CF::COLS;
+      float *dst = out + out_fs * f;
+      uint32_t s_row = rotate ? (uint32_t)n2 : 1u, s_col = rotate ? 1u : (uint32_t)n;
+      asm volatile("" : "+s"(s_row), "+s"(s_col));
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const long col = wgcol + 32 * wave + 16 * u + r16;
+          if (col < n2) {
+            const uint32_t cbase = (uint32_t)col * s_col;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              const int m = 16 * t + 4 * q + i;
+              if (m < n) {
+                const uint32_t o = cbase + (uint32_t)m * s_row;
+                if (accumulate) dst[o] += acc[t][u][i];
+                else dst[o] = acc[t][u][i];
+              }
+            }
+          }
+          acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  };
+
+#pragma unroll 1
+  for (int g = 0; g < total; g += 3) {
+    chunk(g, x0, x2);
+    if (g + 1 < total) chunk(g + 1, x1, x0);
+    if (g + 2 < total) chunk(g + 2, x2, x1);
+  }
+}
+
 // ---- host side
 // the arranged operand's chunk stride serves both workgroup shapes: the
 // larger of the two copy granularities
@@ -392,6 +560,23 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
       attr_set = true;
     }
     const long ncb = (n2 + CF::COLS - 1) / CF::COLS;
+    if constexpr (W == 8) {
+      if (c->split_kernel == 1) {
+        using CR = msr_cfg<MT, NT>;
+        static_assert(CR::COLS == CF::COLS && CR::ASZ == CF::ASZ, "same tiling");
+        static bool attr_r = false;
+        if (!attr_r) {
+          (void)hipFuncSetAttribute((const void *)k_modeprod_sr<MT, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    CR::LDS);
+          attr_r = true;
+        }
+        const long ntiles = ncb * nfields;
+        const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
+        k_modeprod_sr<MT, NT><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
+            A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+        return;
+      }
+    }
     if (c->split_persistent) {
       const long ntiles = ncb * nfields;
       const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;  // one workgroup per CU (LDS-bound)
