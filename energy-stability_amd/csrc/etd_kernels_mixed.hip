@@ -80,6 +80,52 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &t0, __bf16 &t1, __bf
   t2 = (__bf16)(r1 - (float)t1);
 }
 
+// Epilogue of a workgroup tile: lane (q, r16) holds rows 16 t + 4 q + i of
+// columns col0 + 16 u.  Raw buffer stores with 32-bit byte offsets (n^3 * 4 <
+// 2^31), advanced by one add per row; a lane whose column or row is outside
+// the matrix gets an offset past the buffer's end, which the hardware drops
+// (rows m >= n would alias the next column in the GEMM layout).  No branches
+// per element, no 64-bit address arithmetic.
+template <int MT, bool ACCUMULATE>
+__device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][2], float *dst, int n, long col0, int q,
+                                            int rotate) {
+  const long n2 = (long)n * n;
+  const __amdgpu_buffer_rsrc_t rD =
+      __builtin_amdgcn_make_buffer_rsrc((void *)dst, 0, (int)((size_t)n * n2 * 4), 0x00020000);
+  const uint32_t s_row = rotate ? (uint32_t)n2 * 4u : 4u, s_col = rotate ? 4u : (uint32_t)n * 4u;
+  const uint32_t OOB = 0x80000000u;
+  // MT = ceil(n / 16): only the last row tile holds rows >= n
+  const int last_rows = n - 16 * (MT - 1) - 4 * q;  // valid i in that tile: i < last_rows
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const long col = col0 + 16 * u;
+    const bool cok = col < n2;
+    uint32_t o = (uint32_t)col * s_col + (uint32_t)(4 * q) * s_row;
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const bool rok = t < MT - 1 || i < last_rows;
+        const int off = (int)((cok && rok) ? o : OOB);
+        float v = acc[t][u][i];
+        if constexpr (ACCUMULATE) v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rD, off, 0, 0)) + v;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rD, off, 0, 0);
+        o += s_row;
+      }
+      o += 12u * s_row;
+    }
+  }
+}
+
+template <int MT>
+__device__ __forceinline__ void store_tile(const f32x4 (&acc)[MT][2], float *dst, int n, long col0, int q,
+                                           int rotate, int accumulate) {
+  if (accumulate)
+    store_tile_<MT, true>(acc, dst, n, col0, q, rotate);
+  else
+    store_tile_<MT, false>(acc, dst, n, col0, q, rotate);
+}
+
 template <int MT, int NT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const uint16_t *__restrict__ A3, const float *__restrict__ in,
                                                                   size_t in_fs, float *__restrict__ out, size_t out_fs,
@@ -168,21 +214,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
     }
   }
   // D: lane holds rows 4 q + i of column r16
-#pragma unroll
-  for (int t = 0; t < MT; t++)
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const long col = wgcol + 32 * wave + 16 * u + r16;
-      if (col >= n2) continue;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int m = 16 * t + 4 * q + i;
-        if (m >= n) continue;
-        const size_t o = rotate ? (size_t)col + (size_t)n2 * m : (size_t)m + (size_t)n * col;
-        if (accumulate) dst[o] += acc[t][u][i];
-        else dst[o] = acc[t][u][i];
-      }
-    }
+  store_tile<MT>(acc, dst, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
 }
 
 // Persistent variant: gridDim.x workgroups (one per CU) loop over the tiles
@@ -289,27 +321,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
         }
       }
     }
-    // 32-bit offsets (n^3 < 2^31) from strides made opaque per tile: the
-    // compiler would otherwise hoist all MT * 4 row offsets out of the tile
-    // loop and spill them
-    uint32_t s_row = rotate ? (uint32_t)n2 : 1u, s_col = rotate ? 1u : (uint32_t)n;
-    asm volatile("" : "+s"(s_row), "+s"(s_col));
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const long col = wgcol + 32 * wave + 16 * u + r16;
-        if (col >= n2) continue;
-        const uint32_t cbase = (uint32_t)col * s_col;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int m = 16 * t + 4 * q + i;
-          if (m >= n) continue;
-          const uint32_t o = cbase + (uint32_t)m * s_row;
-          if (accumulate) dst[o] += acc[t][u][i];
-          else dst[o] = acc[t][u][i];
-        }
-      }
+    store_tile<MT>(acc, dst, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
   }
 }
 
@@ -448,28 +460,11 @@ __global__ __launch_bounds__(512, 1) void k_modeprod_sr(const uint16_t *__restri
       const int tl = (int)blockIdx.x + (g / nkc) * (int)gridDim.x;
       const int f = tl / ncb;
       const long wgcol = (long)(tl % ncb) * CF::COLS;
-      float *dst = out + out_fs * f;
-      uint32_t s_row = rotate ? (uint32_t)n2 : 1u, s_col = rotate ? 1u : (uint32_t)n;
-      asm volatile("" : "+s"(s_row), "+s"(s_col));
+      store_tile<MT>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
 #pragma unroll
       for (int t = 0; t < MT; t++)
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const long col = wgcol + 32 * wave + 16 * u + r16;
-          if (col < n2) {
-            const uint32_t cbase = (uint32_t)col * s_col;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-              const int m = 16 * t + 4 * q + i;
-              if (m < n) {
-                const uint32_t o = cbase + (uint32_t)m * s_row;
-                if (accumulate) dst[o] += acc[t][u][i];
-                else dst[o] = acc[t][u][i];
-              }
-            }
-          }
-          acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
-        }
+        for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
     }
   };
 
