@@ -9,24 +9,10 @@
 
 #include "etd_internal.h"
 
-#define RBLOCKS 1024
+#define RBLOCKS 2048
 #define RCOLS 4
 
 // ------------------------------------------------------------------ reductions
-// max |a - b| over cnt values; per-block maxima into part[block * RCOLS]
-template <typename T>
-__global__ __launch_bounds__(256) void k_maxdiff(const T *__restrict__ a, const T *__restrict__ b, size_t cnt,
-                                                 double *__restrict__ part) {
-  double m = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
-    m = fmax(m, fabs((double)a[i] - (double)b[i]));
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
-  __shared__ double sh[4];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-}
-
 // Interior cells visited one (z, y) row per wave, lanes along x: one 32-bit
 // division per row instead of a 64-bit one per cell.  Fixed mapping, so the
 // partial sums are deterministic.
@@ -37,33 +23,66 @@ __global__ __launch_bounds__(256) void k_maxdiff(const T *__restrict__ a, const 
 
 // Run features over the interior cells of the AC fields: max |u - u0|,
 // max |eta|, count of cells with max_p |eta_p| > 0.1, and the sum over p of
-// |grad eta_p| (central differences on the periodic images).
+// |grad eta_p| (central differences on the periodic images).  Fields go in
+// batches of FB whose loads are all issued before any is used (a batch past
+// the last field re-reads the last field and ignores it), so a thread waits
+// for memory once per batch, not once per field.
+#define FB 4
+struct cell_nb {
+  size_t l, xp, xm, yp, ym, zp, zm;
+};
+__device__ __forceinline__ cell_nb neighbours(int n, int g, int N, int x, int y, int z) {
+  const int xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+  const int yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+  const int zp = z + 1 == N ? 0 : z + 1, zm = z == 0 ? N - 1 : z - 1;
+  auto at = [&](int zz, int yy, int xx) { return (((size_t)(zz + g)) * n + (yy + g)) * n + (xx + g); };
+  cell_nb c;
+  c.l = at(z, y, x);
+  c.xp = at(z, y, xp);
+  c.xm = at(z, y, xm);
+  c.yp = at(z, yp, x);
+  c.ym = at(z, ym, x);
+  c.zp = at(zp, y, x);
+  c.zm = at(zm, y, x);
+  return c;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const T *__restrict__ u0, int n, int g,
                                                   int nac, double hx, double hy, double hz,
                                                   double *__restrict__ part) {
   const int N = n - 2 * g;
-  const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
+  const size_t n3 = (size_t)n * n * n;
   double dmax = 0.0, emax = 0.0, cnt = 0.0, gsum = 0.0;
-  (void)N3;
   FOR_INTERIOR_CELLS(N, x, y, z) {
-    const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
-    const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
-    const size_t xm = (((size_t)(z + g)) * n + (y + g)) * n + ((x + N - 1) % N + g);
-    const size_t yp = (((size_t)(z + g)) * n + ((y + 1) % N + g)) * n + (x + g);
-    const size_t ym = (((size_t)(z + g)) * n + ((y + N - 1) % N + g)) * n + (x + g);
-    const size_t zp = (((size_t)((z + 1) % N + g)) * n + (y + g)) * n + (x + g);
-    const size_t zm = (((size_t)((z + N - 1) % N + g)) * n + (y + g)) * n + (x + g);
+    const cell_nb c = neighbours(n, g, N, x, y, z);
     double cell = 0.0;
-    for (int p = 0; p < nac; p++) {
-      const T *f = u + n3 * p;
-      const double v = f[l];
-      dmax = fmax(dmax, fabs(v - (double)u0[n3 * p + l]));
-      cell = fmax(cell, fabs(v));
-      const double gx = ((double)f[xp] - (double)f[xm]) / (2.0 * hx);
-      const double gy = ((double)f[yp] - (double)f[ym]) / (2.0 * hy);
-      const double gz = ((double)f[zp] - (double)f[zm]) / (2.0 * hz);
-      gsum += sqrt(gx * gx + gy * gy + gz * gz);
+    for (int p0 = 0; p0 < nac; p0 += FB) {
+      T v[FB], v0[FB], vxp[FB], vxm[FB], vyp[FB], vym[FB], vzp[FB], vzm[FB];
+#pragma unroll
+      for (int e = 0; e < FB; e++) {
+        const int p = p0 + e < nac ? p0 + e : nac - 1;
+        const T *f = u + n3 * p;
+        v[e] = f[c.l];
+        v0[e] = u0[n3 * p + c.l];
+        vxp[e] = f[c.xp];
+        vxm[e] = f[c.xm];
+        vyp[e] = f[c.yp];
+        vym[e] = f[c.ym];
+        vzp[e] = f[c.zp];
+        vzm[e] = f[c.zm];
+      }
+#pragma unroll
+      for (int e = 0; e < FB; e++) {
+        if (p0 + e >= nac) break;
+        const double vv = v[e];
+        dmax = fmax(dmax, fabs(vv - (double)v0[e]));
+        cell = fmax(cell, fabs(vv));
+        const double gx = ((double)vxp[e] - (double)vxm[e]) / (2.0 * hx);
+        const double gy = ((double)vyp[e] - (double)vym[e]) / (2.0 * hy);
+        const double gz = ((double)vzp[e] - (double)vzm[e]) / (2.0 * hz);
+        gsum += sqrt(gx * gx + gy * gy + gz * gz);
+      }
     }
     emax = fmax(emax, cell);
     cnt += cell > 0.1 ? 1.0 : 0.0;
@@ -115,7 +134,7 @@ __device__ inline void block_sum2(double a, double b, double *__restrict__ part)
 // evaluation (its base linear index Lbase)
 template <typename T>
 __global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u, const T *__restrict__ E, int n, int g,
-                                                        int nac, int ring, long long Lbase,
+                                                        int nac, int ring, int slot0,
                                                         double *__restrict__ part) {
   const int N = n - 2 * g;
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
@@ -123,11 +142,23 @@ __global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u,
   FOR_INTERIOR_CELLS(N, x, y, z) {
     const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
     const size_t m = ((size_t)z * N + y) * N + x;
-    for (int p = 0; p < nac; p++) {
-      const int slot = (int)((Lbase + p) % ring);
-      const double t = (double)u[n3 * p + l] * (double)E[(size_t)slot * N3 + m];
-      acc += t;
-      aabs += fabs(t);
+    for (int p0 = 0; p0 < nac; p0 += FB) {
+      T a[FB], b[FB];
+#pragma unroll
+      for (int e = 0; e < FB; e++) {
+        const int p = p0 + e < nac ? p0 + e : nac - 1;
+        int slot = slot0 + p;  // p < nac < ring
+        if (slot >= ring) slot -= ring;
+        a[e] = u[n3 * p + l];
+        b[e] = E[(size_t)slot * N3 + m];
+      }
+#pragma unroll
+      for (int e = 0; e < FB; e++) {
+        if (p0 + e >= nac) break;
+        const double t = (double)a[e] * (double)b[e];
+        acc += t;
+        aabs += fabs(t);
+      }
     }
   }
   block_sum2(acc, aabs, part);
@@ -155,35 +186,50 @@ __global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, 
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
   double acc = 0.0, aabs = 0.0;
   (void)N3;
+  const int V = a.nac + a.nch;
   FOR_INTERIOR_CELLS(N, x, y, z) {
-    const size_t l = (((size_t)(z + g)) * n + (y + g)) * n + (x + g);
-    const size_t xp = (((size_t)(z + g)) * n + (y + g)) * n + ((x + 1) % N + g);
-    const size_t yp = (((size_t)(z + g)) * n + ((y + 1) % N + g)) * n + (x + g);
-    const size_t zp = (((size_t)((z + 1) % N + g)) * n + (y + g)) * n + (x + g);
-    const size_t xm = (((size_t)(z + g)) * n + (y + g)) * n + ((x + N - 1) % N + g);
-    const size_t ym = (((size_t)(z + g)) * n + ((y + N - 1) % N + g)) * n + (x + g);
-    const size_t zm = (((size_t)((z + N - 1) % N + g)) * n + (y + g)) * n + (x + g);
+    const cell_nb c = neighbours(n, g, N, x, y, z);
     double s2 = 0.0, s3 = 0.0, e = 0.0, ea = 0.0;
-    for (int p = 0; p < a.nac + a.nch; p++) {
-      const T *f = u + n3 * p;
-      const double v = f[l];
-      const double dx = ((double)f[xp] - v) / a.hx, dy = ((double)f[yp] - v) / a.hy, dz = ((double)f[zp] - v) / a.hz;
-      if (p < a.nac) {
-        e += 0.5 * a.epn2 * (a.kkx * dx * dx + a.kky * dy * dy + a.kkz * dz * dz);
+    for (int p0 = 0; p0 < V; p0 += FB) {
+      T v[FB], vxp[FB], vyp[FB], vzp[FB], vxm[FB], vym[FB], vzm[FB];
+#pragma unroll
+      for (int q = 0; q < FB; q++) {
+        const int p = p0 + q < V ? p0 + q : V - 1;
+        const T *f = u + n3 * p;
+        v[q] = f[c.l];
+        vxp[q] = f[c.xp];
+        vyp[q] = f[c.yp];
+        vzp[q] = f[c.zp];
         if (a.aniso) {
-          const double gx = ((double)f[xp] - (double)f[xm]) / (2.0 * a.hx);
-          const double gy = ((double)f[ym] - (double)f[yp]) / (2.0 * a.hy);
-          const double gz = ((double)f[zm] - (double)f[zp]) / (2.0 * a.hz);
-          const double *L = lam + 9 * p;
-          const double lx = L[0] * gx + L[1] * gy + L[2] * gz, ly = L[3] * gx + L[4] * gy + L[5] * gz,
-                       lz = L[6] * gx + L[7] * gy + L[8] * gz;
-          e += 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
-          ea += fabs(0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz)) - 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+          vxm[q] = f[c.xm];
+          vym[q] = f[c.ym];
+          vzm[q] = f[c.zm];
         }
-        s2 += v * v;
-        s3 += v * v * v;
-      } else {
-        e += 0.5 * a.epn2 * (dx * dx + dy * dy + dz * dz) + 0.25 * (v * v - 1.0) * (v * v - 1.0);
+      }
+#pragma unroll
+      for (int q = 0; q < FB; q++) {
+        const int p = p0 + q;
+        if (p >= V) break;
+        const double vv = v[q];
+        const double dx = ((double)vxp[q] - vv) / a.hx, dy = ((double)vyp[q] - vv) / a.hy,
+                     dz = ((double)vzp[q] - vv) / a.hz;
+        if (p < a.nac) {
+          e += 0.5 * a.epn2 * (a.kkx * dx * dx + a.kky * dy * dy + a.kkz * dz * dz);
+          if (a.aniso) {
+            const double gx = ((double)vxp[q] - (double)vxm[q]) / (2.0 * a.hx);
+            const double gy = ((double)vym[q] - (double)vyp[q]) / (2.0 * a.hy);
+            const double gz = ((double)vzm[q] - (double)vzp[q]) / (2.0 * a.hz);
+            const double *L = lam + 9 * p;
+            const double lx = L[0] * gx + L[1] * gy + L[2] * gz, ly = L[3] * gx + L[4] * gy + L[5] * gz,
+                         lz = L[6] * gx + L[7] * gy + L[8] * gz;
+            e += 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+            ea += fabs(0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz)) - 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+          }
+          s2 += vv * vv;
+          s3 += vv * vv * vv;
+        } else {
+          e += 0.5 * a.epn2 * (dx * dx + dy * dy + dz * dz) + 0.25 * (vv * vv - 1.0) * (vv * vv - 1.0);
+        }
       }
     }
     // the gradient and CH terms are non-negative: their absolute sum is e
@@ -196,12 +242,25 @@ __global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, 
   block_sum2(acc, aabs, part);
 }
 
-// u = 2 u - b (local extrapolation of the step-doubling pair)
+// max |u - b| per block into part[block * RCOLS] and, with extrapolate,
+// u = 2 u - b (local extrapolation of the step-doubling pair) in the same pass
 template <typename T>
-__global__ void k_extrapolate(T *__restrict__ u, const T *__restrict__ b, size_t cnt) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
-    u[i] = (T)(2.0 * (double)u[i] - (double)b[i]);
+__global__ __launch_bounds__(256) void k_maxdiff_extrap(T *__restrict__ u, const T *__restrict__ b, size_t cnt,
+                                                        int extrapolate, double *__restrict__ part) {
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x) {
+    const double x = u[i], y = b[i];
+    m = fmax(m, fabs(x - y));
+    if (extrapolate) u[i] = (T)(2.0 * x - y);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
+
+
 
 // ------------------------------------------------------------------ host helpers
 #define ACHK(call)                                                                              \
@@ -217,17 +276,6 @@ static int fetch_part(etd_ctx *c, int cols) {
   ACHK(hipMemcpyAsync(c->ad_hpart, c->ad_part, sizeof(double) * RBLOCKS * RCOLS, hipMemcpyDeviceToHost, c->stream));
   ACHK(hipStreamSynchronize(c->stream));
   (void)cols;
-  return ETD_OK;
-}
-
-template <typename T>
-static int max_abs_diff(etd_ctx *c, const T *a, const T *b, size_t cnt, double *out) {
-  k_maxdiff<T><<<RBLOCKS, 256, 0, c->stream>>>(a, b, cnt, c->ad_part);
-  int rc = fetch_part(c, 1);
-  if (rc) return rc;
-  double m = 0.0;
-  for (int i = 0; i < RBLOCKS; i++) m = fmax(m, c->ad_hpart[i * RCOLS]);
-  *out = m;
   return ETD_OK;
 }
 
@@ -314,7 +362,8 @@ static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *
     // left as the step wrote them (plain stepping refreshes them next step)
     launch_elastic<T>(c);
     const long long Lbase = (c->elastic_evals - 1) * (long long)c->nac;
-    k_elastic_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, F.E, c->n, c->g, c->nac, c->ring, Lbase, c->ad_part);
+    k_elastic_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, F.E, c->n, c->g, c->nac, c->ring, (int)(Lbase % c->ring),
+                                                        c->ad_part);
     if ((rc = fetch_part(c, 2))) return rc;
     double s = 0.0, sa = 0.0;
     for (int i = 0; i < RBLOCKS; i++) {
@@ -457,9 +506,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     }
     st->step_evals += 3;
     ACHK(hipGetLastError());
-    double err;
-    if ((rc = max_abs_diff<T>(c, F.u, SB, cnt, &err))) break;
-    if (o->extrapolate) k_extrapolate<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt);
+    double err = 0.0;
+    k_maxdiff_extrap<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt, o->extrapolate, c->ad_part);
+    if ((rc = fetch_part(c, 1))) break;
+    for (int i = 0; i < RBLOCKS; i++) err = fmax(err, c->ad_hpart[i * RCOLS]);
     double Enew, Eabs_new;
     if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new))) break;
     const double scale = ekind == 0 ? fmax(Escale, fabs(E)) : fmax(Eabs, Eabs_new);

@@ -21,6 +21,35 @@ __device__ __forceinline__ int wrap_e(int e, int n, int g) {
 }
 __device__ __forceinline__ size_t IDX(int n, int k, int j, int i) { return ((size_t)k * n + j) * n + i; }
 
+// l = (k n + j) n + i, for l < 2^32: each quotient from a double-precision
+// reciprocal (within one of the true quotient: the relative error is about
+// 2^-52) and one correction step, instead of 64-bit integer division.
+__device__ __forceinline__ void idx3(size_t l, int n, int &i, int &j, int &k) {
+  const double inv = 1.0 / n;
+  const unsigned un = (unsigned)n, lu = (unsigned)l;
+  unsigned r = (unsigned)((double)lu * inv);
+  int rem = (int)(lu - r * un);
+  if (rem < 0) {
+    r--;
+    rem += n;
+  } else if (rem >= n) {
+    r++;
+    rem -= n;
+  }
+  unsigned kk = (unsigned)((double)r * inv);
+  int jj = (int)(r - kk * un);
+  if (jj < 0) {
+    kk--;
+    jj += n;
+  } else if (jj >= n) {
+    kk++;
+    jj -= n;
+  }
+  i = rem;
+  j = jj;
+  k = (int)kk;
+}
+
 // ------------------------------------------------------------------ halo
 // Periodic self-exchange of transfer() (trans.c:1939-1996): every ghost value
 // comes from the pre-transfer state, z faces override y faces override x
@@ -599,7 +628,8 @@ __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   const int last = first_var + nvars;
   phi_pt pac, pch;
   if (first_var < a.nac) {
@@ -610,15 +640,26 @@ __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int
     const double tmp = DDZ[k] + DDY[j] + DDX[i];
     pch = phi_at(a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp), a.dt);
   }
-  for (int v = first_var; v < last; v++) {
-    const phi_pt &ph = v < a.nac ? pac : pch;
-    const T *uh = uhat + fs * (v - first_var);
-    T *o = outp + fs * (v - first_var);
-    if (order2) {
-      o[l] = (T)(ph.phi2 * (double)uh[l]);
-    } else {
-      const T *nh = nhat + fs * (v - first_var);
-      o[l] = (T)(ph.lin * (double)uh[l] + ph.phi1 * (double)nh[l]);
+  // fields in batches of FB whose loads are issued before any is used
+  constexpr int FB = 4;
+  for (int v0 = first_var; v0 < last; v0 += FB) {
+    T uv[FB], nv[FB];
+#pragma unroll
+    for (int e = 0; e < FB; e++) {
+      const int v = v0 + e < last ? v0 + e : last - 1;
+      uv[e] = uhat[fs * (v - first_var) + l];
+      if (!order2) nv[e] = nhat[fs * (v - first_var) + l];
+    }
+#pragma unroll
+    for (int e = 0; e < FB; e++) {
+      const int v = v0 + e;
+      if (v >= last) break;
+      const phi_pt &ph = v < a.nac ? pac : pch;
+      T *o = outp + fs * (v - first_var);
+      if (order2)
+        o[l] = (T)(ph.phi2 * (double)uv[e]);
+      else
+        o[l] = (T)(ph.lin * (double)uv[e] + ph.phi1 * (double)nv[e]);
     }
   }
 }
@@ -649,6 +690,7 @@ void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first
 struct nl_args {
   int n, g, N, nac, nch, aniso, elastic, ring;
   long long Lbase;  // linear index of AC field 0 of the latest elastic evaluation
+  int slot0;        // Lbase mod ring
   double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
 };
 
@@ -670,8 +712,10 @@ __device__ __forceinline__ double felas_at(const T *E, const nl_args &a, int p, 
   const int gk = (k < g || k >= n - g), gj = (j < g || j >= n - g), gi = (i < g || i >= n - g);
   int lag = gk + gj + gi - 1;
   lag = lag < 0 ? 0 : lag;
-  long long L = a.Lbase + p - lag;
-  int slot = (int)(((L % a.ring) + a.ring) % a.ring);
+  // (Lbase + p - lag) mod ring, with p < nac <= ring - 2 and lag <= 2
+  int slot = a.slot0 + p - lag;
+  if (slot >= a.ring) slot -= a.ring;
+  if (slot < 0) slot += a.ring;
   const int wk = wrap_e(k, n, g) - g, wj = wrap_e(j, n, g) - g, wi = wrap_e(i, n, g) - g;
   return E[(size_t)slot * N * N * N + ((size_t)wk * N + wj) * N + wi];
 }
@@ -685,28 +729,49 @@ __global__ void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out, const 
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
+  // fields in batches of FB whose loads are issued before any is used (past
+  // the last field a batch re-reads it and ignores it); sums in field order
+  constexpr int FB = 4;
   double s = 0.0;
-  for (int m = 0; m < a.nac; m++) {
-    const double um = u[n3 * m + l];
-    s = s + um * um;
+  for (int m0 = 0; m0 < a.nac; m0 += FB) {
+    T um[FB];
+#pragma unroll
+    for (int e = 0; e < FB; e++) um[e] = u[n3 * (m0 + e < a.nac ? m0 + e : a.nac - 1) + l];
+#pragma unroll
+    for (int e = 0; e < FB; e++)
+      if (m0 + e < a.nac) s = s + (double)um[e] * (double)um[e];
   }
-  for (int p = 0; p < a.nac; p++) {
-    const T *up_f = u + n3 * p;
-    const double up = up_f[l];
-    const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
-    double f = a.LE * a.KE * up;
-    f -= a.LE * fp;
-    const double ce = a.LE * a.epn2;
-    if (i == 0) f += ce * Pimg(up_f, n, g, k, j, -1) / a.hx / a.hx;
-    if (i == n - 1) f += ce * Pimg(up_f, n, g, k, j, n) / a.hx / a.hx;
-    if (j == 0) f += ce * Pimg(up_f, n, g, k, -1, i) / a.hy / a.hy;
-    if (j == n - 1) f += ce * Pimg(up_f, n, g, k, n, i) / a.hy / a.hy;
-    if (k == 0) f += ce * Pimg(up_f, n, g, -1, j, i) / a.hz / a.hz;
-    if (k == n - 1) f += ce * Pimg(up_f, n, g, n, j, i) / a.hz / a.hz;
-    if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
-    if (sub) f -= sub[n3 * p + l];
-    out[n3 * p + l] = (T)f;
+  for (int p0 = 0; p0 < a.nac; p0 += FB) {
+    T upv[FB], fel[FB], sbv[FB];
+#pragma unroll
+    for (int e = 0; e < FB; e++) {
+      const int p = p0 + e < a.nac ? p0 + e : a.nac - 1;
+      upv[e] = u[n3 * p + l];
+      if (with_felas) fel[e] = (T)felas_at(E, a, p, k, j, i);
+      if (sub) sbv[e] = sub[n3 * p + l];
+    }
+#pragma unroll
+    for (int e = 0; e < FB; e++) {
+      const int p = p0 + e;
+      if (p >= a.nac) break;
+      const T *up_f = u + n3 * p;
+      const double up = upv[e];
+      const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
+      double f = a.LE * a.KE * up;
+      f -= a.LE * fp;
+      const double ce = a.LE * a.epn2;
+      if (i == 0) f += ce * Pimg(up_f, n, g, k, j, -1) / a.hx / a.hx;
+      if (i == n - 1) f += ce * Pimg(up_f, n, g, k, j, n) / a.hx / a.hx;
+      if (j == 0) f += ce * Pimg(up_f, n, g, k, -1, i) / a.hy / a.hy;
+      if (j == n - 1) f += ce * Pimg(up_f, n, g, k, n, i) / a.hy / a.hy;
+      if (k == 0) f += ce * Pimg(up_f, n, g, -1, j, i) / a.hz / a.hz;
+      if (k == n - 1) f += ce * Pimg(up_f, n, g, n, j, i) / a.hz / a.hz;
+      if (with_felas) f -= a.LE * a.es * (double)fel[e];
+      if (sub) f -= sbv[e];
+      out[n3 * p + l] = (T)f;
+    }
   }
 }
 
@@ -718,7 +783,8 @@ __global__ void k_f2_grad(const T *__restrict__ u, T *__restrict__ G, const doub
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   const double a_left = i > 0 ? u[l - 1] : 0.0;
   const double a_right = i < n - 1 ? u[l + 1] : 0.0;
   const double a_top = j > 0 ? u[l - n] : 0.0;
@@ -742,7 +808,8 @@ __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   const T *gx = G, *gy = G + n3, *gz = G + 2 * n3;
   const double a_left = i > 0 ? gx[l - 1] : 0.0;
   const double a_right = i < n - 1 ? gx[l + 1] : 0.0;
@@ -770,7 +837,8 @@ __global__ void k_ch_t(const T *__restrict__ cfield, T *__restrict__ t, nl_args 
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   const double cc = cfield[l];
   double f = -a.LCI * a.KCI * cc;
   f += a.LCI * ch_f(cc);
@@ -806,7 +874,8 @@ __global__ void k_ch_lap(const T *__restrict__ cfield, const T *__restrict__ t, 
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   const double tc = t[l];
   const double f_left = i > 0 ? t[l - 1] : 0.0;
   const double f_right = i < n - 1 ? t[l + 1] : 0.0;
@@ -848,6 +917,7 @@ static nl_args make_nl(etd_ctx *c) {
   a.elastic = c->p.elastic;
   a.ring = c->ring > 0 ? c->ring : 1;
   a.Lbase = (c->elastic_evals - 1) * (long long)c->nac;
+  a.slot0 = (int)(((a.Lbase % a.ring) + a.ring) % a.ring);
   a.LE = c->p.LE;
   a.KE = c->p.KE;
   a.a2 = c->p.a2;
@@ -1223,7 +1293,8 @@ __global__ void k_el_pack(const T *__restrict__ u, typename cplx<T>::t *__restri
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= N3) return;
   const int r = blockIdx.y;
-  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  int x, y, z;
+  idx3(m, N, x, y, z);
   const size_t n3 = (size_t)n * n * n;
   const size_t src = IDX(n, z + g, y + g, x + g);
   const int pa = 2 * r, pb = 2 * r + 1;
@@ -1293,11 +1364,17 @@ __device__ __forceinline__ void b_wrow(const double *sp, const double *nv, const
 // reference's n_i s_p,ij Omega_jk s_q,kl n_l for symmetric Omega and
 // sigma); with a symmetric Voigt matrix B is symmetric and each unordered
 // pair is formed once.  MIRROR: B is replaced by (B(k) + B(-k)) / 2.
-template <int NAC, bool MIRROR, typename T2>
+// WRITE_NEG (with MIRROR false): also write the point -k, whose values are
+// the conjugates: F_q(-k) = conj F_q(k) for real fields and B(-k) = B(k) (the
+// geometry at -n is an exact sign flip of the one at n, squared away), so
+// acc_p(-k) = conj acc_p(k) bit for bit and Y_r(-k) = (a_x + b_y, b_x - a_y)
+// with a = acc_2r, b = acc_2r+1.
+template <int NAC, bool MIRROR, typename T2, bool WRITE_NEG = false>
 __device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__restrict__ Y, int N,
                                                int npair, size_t m, const etd_dev_consts *__restrict__ K) {
   const size_t N3 = (size_t)N * N * N;
-  const int x = (int)(m % N), y = (int)((m / N) % N), z = (int)(m / ((size_t)N * N));
+  int x, y, z;
+  idx3(m, N, x, y, z);
   const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
   const size_t mm = ((size_t)zm * N + ym) * N + xm;
   double2 F[NAC];
@@ -1362,15 +1439,35 @@ __device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__r
       y.y = acc[2 * r].y;
     }
     Y[N3 * r + m] = y;
+    if constexpr (WRITE_NEG) {
+      if (mm != m) {
+        T2 w;
+        if (2 * r + 1 < NAC) {
+          w.x = acc[2 * r].x + acc[2 * r + 1].y;
+          w.y = acc[2 * r + 1].x - acc[2 * r].y;
+        } else {
+          w.x = acc[2 * r].x;
+          w.y = -acc[2 * r].y;
+        }
+        Y[N3 * r + mm] = w;
+      }
+    }
   }
 }
 
+// One thread per pair {k, -k}: planes z = 0 .. N/2 hold a member of every
+// pair; the point with the smaller index computes both (within the planes
+// z = 0 and z = N/2, which map to themselves, half the threads return).
 template <int NAC, typename T2>
 __global__ __launch_bounds__(256) void k_el_contract_t(const T2 *__restrict__ Z, T2 *__restrict__ Y, int N,
                                                        int npair, const etd_dev_consts *__restrict__ K) {
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= (size_t)N * N * N) return;
-  contract_point<NAC, false, T2>(Z, Y, N, npair, m, K);
+  if (m >= (size_t)N * N * (N / 2 + 1)) return;
+  int x, y, z;
+  idx3(m, N, x, y, z);
+  const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
+  if (((size_t)zm * N + ym) * N + xm < m) return;
+  contract_point<NAC, false, T2, true>(Z, Y, N, npair, m, K);
 }
 
 // points whose mirror's signed wavenumber is not the negative: the index
@@ -1397,16 +1494,19 @@ __global__ __launch_bounds__(256) void k_el_contract_mirror(const T2 *__restrict
 
 template <typename T>
 __global__ void k_el_unpack(const typename cplx<T>::t *__restrict__ Y, T *__restrict__ E, int N, int nac, int ring,
-                            long long Lbase) {
+                            int slot0) {
   const size_t N3 = (size_t)N * N * N;
   const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= N3) return;
   const int r = blockIdx.y;
   const double inv = 1.0 / ((double)N * N * N);
   const typename cplx<T>::t v = Y[N3 * r + m];
-  int pa = 2 * r, pb = 2 * r + 1;
-  E[(size_t)((Lbase + pa) % ring) * N3 + m] = (T)((double)v.x * inv);
-  if (pb < nac) E[(size_t)((Lbase + pb) % ring) * N3 + m] = (T)((double)v.y * inv);
+  // slots (slot0 + p) mod ring, p < nac < ring
+  int sa = slot0 + 2 * r, sb = sa + 1;
+  if (sa >= ring) sa -= ring;
+  if (sb >= ring) sb -= ring;
+  E[(size_t)sa * N3 + m] = (T)((double)v.x * inv);
+  if (2 * r + 1 < nac) E[(size_t)sb * N3 + m] = (T)((double)v.y * inv);
 }
 
 template <typename T>
@@ -1422,10 +1522,11 @@ void launch_elastic(etd_ctx *c) {
   {
     const int nplane = (N & 1) ? 2 : 1;
     const unsigned nbm = (unsigned)((3 * nplane * (size_t)N * N + 255) / 256);
+    const unsigned nbh = (unsigned)(((size_t)N * N * (N / 2 + 1) + 255) / 256);
     const T2 *Zc = (const T2 *)F.Z;
     T2 *Yc = (T2 *)F.Y;
     switch (c->nac) {
-#define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X, T2><<<nb, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X, T2><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
+#define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X, T2><<<nbh, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X, T2><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
       NACC(1) NACC(2) NACC(3) NACC(4) NACC(5) NACC(6) NACC(7) NACC(8) NACC(9) NACC(10) NACC(11) NACC(12) NACC(13)
       NACC(14) NACC(15) NACC(16)
 #undef NACC
@@ -1434,7 +1535,8 @@ void launch_elastic(etd_ctx *c) {
   }
   fft3d<T>(c, F.Y, c->npair, -1);
   const long long Lbase = c->elastic_evals * (long long)c->nac;
-  k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, Lbase);
+  const int slot0 = (int)(Lbase % c->ring);
+  k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, slot0);
   c->elastic_evals++;
   etd_prof_end(c, PC_ELASTIC);
 }
@@ -1446,7 +1548,8 @@ __global__ void k_felas_mat(T *felas, const T *E, nl_args a) {
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
   const int p = blockIdx.y;
-  const int i = (int)(l % n), j = (int)((l / n) % n), k = (int)(l / ((size_t)n * n));
+  int i, j, k;
+  idx3(l, n, i, j, k);
   felas[n3 * p + l] = (T)felas_at(E, a, p, k, j, i);
 }
 
