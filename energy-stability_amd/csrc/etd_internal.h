@@ -72,7 +72,8 @@ struct etd_ctx {
   int fft_variant;         /* 1: two-stage register FFT for N % 16 == 0 (default), 0: radix passes in LDS */
   int split_waves;         /* mixed mode products: 8 or 4 waves per workgroup, 0 by size */
   int split_persistent;    /* 1: persistent workgroups looping over tiles (default) */
-  int split_kernel;        /* 0: field chunks staged through LDS, 1: field chunks prefetched into registers */
+  int split_kernel;        /* 0: field chunks staged through LDS, 8 waves x 32 columns; 1: field chunks
+                              prefetched into registers; 2: LDS-staged, 4 waves x 64 columns */
   int num_cu;              /* compute units of the device */
   /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
    * split into split_terms BF16 terms in MFMA fragment order */

@@ -720,6 +720,30 @@ __device__ __forceinline__ double felas_at(const T *E, const nl_args &a, int p, 
   return E[(size_t)slot * N * N * N + ((size_t)wk * N + wj) * N + wi];
 }
 
+// The field-independent part of felas_at: the ring lag of cell (k,j,i) and
+// its offset within a ring slot; felas of AC field p is then
+// E[felas_slot(a, p, lag) N^3 + off].
+struct felas_cell {
+  int lag;
+  size_t off;
+};
+__device__ __forceinline__ felas_cell felas_locate(const nl_args &a, int k, int j, int i) {
+  const int n = a.n, g = a.g, N = a.N;
+  const int gk = (k < g || k >= n - g), gj = (j < g || j >= n - g), gi = (i < g || i >= n - g);
+  int lag = gk + gj + gi - 1;
+  felas_cell c;
+  c.lag = lag < 0 ? 0 : lag;
+  const int wk = wrap_e(k, n, g) - g, wj = wrap_e(j, n, g) - g, wi = wrap_e(i, n, g) - g;
+  c.off = ((size_t)wk * N + wj) * N + wi;
+  return c;
+}
+__device__ __forceinline__ int felas_slot(const nl_args &a, int p, int lag) {
+  int slot = a.slot0 + p - lag;
+  if (slot >= a.ring) slot -= a.ring;
+  if (slot < 0) slot += a.ring;
+  return slot;
+}
+
 // ac_calc_FU (calc.c:8-68) -> ac_calc_F1 (anisotropic.c:9-161): all AC fields
 // of one point per thread (sum u^2 couples them).
 template <typename T>
@@ -743,13 +767,15 @@ __global__ void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out, const 
     for (int e = 0; e < FB; e++)
       if (m0 + e < a.nac) s = s + (double)um[e] * (double)um[e];
   }
+  const felas_cell fc = felas_locate(a, k, j, i);
+  const size_t N3 = (size_t)a.N * a.N * a.N;
   for (int p0 = 0; p0 < a.nac; p0 += FB) {
     T upv[FB], fel[FB], sbv[FB];
 #pragma unroll
     for (int e = 0; e < FB; e++) {
       const int p = p0 + e < a.nac ? p0 + e : a.nac - 1;
       upv[e] = u[n3 * p + l];
-      if (with_felas) fel[e] = (T)felas_at(E, a, p, k, j, i);
+      if (with_felas) fel[e] = E[(size_t)felas_slot(a, p, fc.lag) * N3 + fc.off];
       if (sub) sbv[e] = sub[n3 * p + l];
     }
 #pragma unroll

@@ -53,24 +53,26 @@ __device__ __forceinline__ void copy16_to_lds(i32x4m rs, uint32_t lds_wave_base,
 __device__ __forceinline__ void wait_all_copies_barrier() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" : : : "memory"); }
 
 // ---- tiling
-// A workgroup of WAVES waves owns 32 * WAVES columns and all MT row tiles of
-// 16; K runs in chunks of 32 (one MFMA).  Per chunk the LDS stage holds
+// A workgroup of WAVES waves owns 16 UT WAVES columns (UT column tiles of 16
+// per wave) and all MT row tiles of 16; K runs in chunks of 32 (one MFMA).
+// Per chunk the LDS stage holds
 //   A: terms x MT x 64 lanes x 16 B, pre-arranged in operand order, and
-//   X: 32 k x (32 WAVES) columns FP32 as [q = k/8][col][8 k] (a lane's 8
+//   X: 32 k x COLS columns FP32 as [q = k/8][col][8 k] (a lane's 8
 //      consecutive k of one column are 32 contiguous bytes, and the 16 lanes
 //      of an operand read 16 consecutive 32-byte blocks).
-template <int MT, int NT, int WAVES>
+// Each A fragment read from LDS feeds the MFMAs of UT column tiles.
+template <int MT, int NT, int WAVES, int UT = 2>
 struct mps_cfg {
   static constexpr int THREADS = 64 * WAVES;
-  static constexpr int COLS = 32 * WAVES;
+  static constexpr int COLS = 16 * UT * WAVES;
   static constexpr int APIECES = NT * MT * 64;                // 16-byte pieces of one A chunk
   static constexpr int APT = (APIECES + THREADS - 1) / THREADS;  // per thread
   static constexpr int ASZ = APT * THREADS * 16;               // A chunk bytes (zero padded)
   static constexpr int XSZ = 32 * COLS * 4;                    // X chunk bytes
-  static constexpr int XPT = XSZ / 16 / THREADS;               // = 4
+  static constexpr int XPT = XSZ / 16 / THREADS;               // = 2 UT
   static constexpr int STAGE = ASZ + XSZ;
   static constexpr int LDS = 2 * STAGE;
-  static_assert(XPT == 4, "X pieces");
+  static_assert(XPT == 2 * UT && UT % 2 == 0, "X pieces");
 };
 
 __device__ __forceinline__ void split_bf16(float x, __bf16 &t0, __bf16 &t1, __bf16 &t2) {
@@ -86,8 +88,8 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &t0, __bf16 &t1, __bf
 // the matrix gets an offset past the buffer's end, which the hardware drops
 // (rows m >= n would alias the next column in the GEMM layout).  No branches
 // per element, no 64-bit address arithmetic.
-template <int MT, bool ACCUMULATE>
-__device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][2], float *dst, int n, long col0, int q,
+template <int MT, int UT, bool ACCUMULATE>
+__device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][UT], float *dst, int n, long col0, int q,
                                             int rotate) {
   const long n2 = (long)n * n;
   const __amdgpu_buffer_rsrc_t rD =
@@ -97,7 +99,7 @@ __device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][2], float *ds
   // MT = ceil(n / 16): only the last row tile holds rows >= n
   const int last_rows = n - 16 * (MT - 1) - 4 * q;  // valid i in that tile: i < last_rows
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
+  for (int u = 0; u < UT; u++) {
     const long col = col0 + 16 * u;
     const bool cok = col < n2;
     uint32_t o = (uint32_t)col * s_col + (uint32_t)(4 * q) * s_row;
@@ -117,13 +119,13 @@ __device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][2], float *ds
   }
 }
 
-template <int MT>
-__device__ __forceinline__ void store_tile(const f32x4 (&acc)[MT][2], float *dst, int n, long col0, int q,
+template <int MT, int UT>
+__device__ __forceinline__ void store_tile(const f32x4 (&acc)[MT][UT], float *dst, int n, long col0, int q,
                                            int rotate, int accumulate) {
   if (accumulate)
-    store_tile_<MT, true>(acc, dst, n, col0, q, rotate);
+    store_tile_<MT, UT, true>(acc, dst, n, col0, q, rotate);
   else
-    store_tile_<MT, false>(acc, dst, n, col0, q, rotate);
+    store_tile_<MT, UT, false>(acc, dst, n, col0, q, rotate);
 }
 
 template <int MT, int NT, int WAVES>
@@ -214,18 +216,18 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
     }
   }
   // D: lane holds rows 4 q + i of column r16
-  store_tile<MT>(acc, dst, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+  store_tile<MT, 2>(acc, dst, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
 }
 
 // Persistent variant: gridDim.x workgroups (one per CU) loop over the tiles
 // (field, 32 WAVES columns); the double-buffered chunk pipeline runs across
 // tile boundaries, so the next tile's first chunk loads during the current
 // tile's last chunk and no workgroup start-up is exposed.
-template <int MT, int NT, int WAVES>
+template <int MT, int NT, int WAVES, int UT = 2>
 __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     const uint16_t *__restrict__ A3, const float *__restrict__ in, size_t in_fs, float *__restrict__ out,
     size_t out_fs, int n, int rotate, int accumulate, int a_chunk, int nfields) {
-  using CF = mps_cfg<MT, NT, WAVES>;
+  using CF = mps_cfg<MT, NT, WAVES, UT>;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long n2 = (long)n * n;
@@ -250,13 +252,16 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     for (int e = 0; e < CF::APT; e++)
       copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
                     (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+    // X piece e: k group q = e % 4 of 32-column block cb; lane L brings
+    // column 32 cb + L / 2, k half L % 2, to [q][col][8 k]
 #pragma unroll
     for (int e = 0; e < CF::XPT; e++) {
-      const long col = wgc + 32 * wave + (lane >> 1);
-      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const int qe = e & 3, cb = wave * (UT / 2) + (e >> 2);
+      const long col = wgc + 32 * cb + (lane >> 1);
+      const int k = 32 * kc + 8 * qe + 4 * (lane & 1);
       const bool ok = col < n2 && k < n;
       const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
-      copy16_to_lds(rsX, sbase + CF::ASZ + (e * CF::THREADS + wave * 64) * 16, voff);
+      copy16_to_lds(rsX, sbase + CF::ASZ + (qe * CF::COLS + 32 * cb) * 32, voff);
     }
   };
 
@@ -268,11 +273,11 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     const int f = tile / ncb;
     const long wgcol = (long)(tile % ncb) * CF::COLS;
     float *dst = out + out_fs * f;
-    f32x4 acc[MT][2];
+    f32x4 acc[MT][UT];
 #pragma unroll
     for (int t = 0; t < MT; t++)
 #pragma unroll
-      for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      for (int u = 0; u < UT; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
     for (int kc = 0; kc < nkc; kc++, g++) {
       // this chunk is in LDS for every wave, and every wave is past the
@@ -285,10 +290,10 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
       const char *stg = smem + (g & 1) * CF::STAGE;
       const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(stg);
       const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
-      bf16x8 b[2][NT];
+      bf16x8 b[UT][NT];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+      for (int u = 0; u < UT; u++) {
+        const float *bp = sx + ((size_t)q * CF::COLS + 16 * UT * wave + 16 * u + r16) * 8;
         const float4 lo = *reinterpret_cast<const float4 *>(bp);
         const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
         const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
@@ -307,7 +312,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
 #pragma unroll
         for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < UT; u++) {
           if constexpr (NT > 2) {
             acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
             acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
         }
       }
     }
-    store_tile<MT>(acc, dst, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+    store_tile<MT, UT>(acc, dst, n, wgcol + 16 * UT * wave + r16, q, rotate, accumulate);
   }
 }
 
@@ -460,7 +465,7 @@ __global__ __launch_bounds__(512, 1) void k_modeprod_sr(const uint16_t *__restri
       const int tl = (int)blockIdx.x + (g / nkc) * (int)gridDim.x;
       const int f = tl / ncb;
       const long wgcol = (long)(tl % ncb) * CF::COLS;
-      store_tile<MT>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+      store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
 #pragma unroll
       for (int t = 0; t < MT; t++)
 #pragma unroll
@@ -583,6 +588,24 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
                                                                             accumulate, (int)chunk_bytes<MT, NT>());
     }
   };
+  if constexpr (mps_cfg<MT, NT, 4, 4>::LDS <= 160 * 1024) {
+    if (c->split_kernel == 2) {
+      // 4 waves x 64 columns: each A fragment feeds four column tiles
+      using CF = mps_cfg<MT, NT, 4, 4>;
+      static bool attr4 = false;
+      if (!attr4) {
+        (void)hipFuncSetAttribute((const void *)k_modeprod_split_p<MT, NT, 4, 4>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
+        attr4 = true;
+      }
+      const long ncb = ((long)n * n + CF::COLS - 1) / CF::COLS;
+      const long ntiles = ncb * nfields;
+      const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
+      k_modeprod_split_p<MT, NT, 4, 4><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+          A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+      return;
+    }
+  }
   if constexpr (MT <= 14) {
     if (c->split_waves == 4)
       go(std::integral_constant<int, 4>{});
