@@ -487,6 +487,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     // attempt: full step and two half steps from u_n
     ACHK(hipMemcpyAsync(S0, F.u, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
     if (!c->p.etd2) {
+      if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
       stage_forward<T>(c);  // N(u_n) and the forward transforms, shared
       c->p.dt = dt;
       launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
@@ -494,6 +495,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       c->p.dt = 0.5 * dt;
       launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
       modeprod3<T>(c, 0, SP, F.u, V, 0);
+      if constexpr (sizeof(T) == 4) {
+        ch_mean_restore(c, SB);
+        ch_mean_restore(c, F.u);
+      }
       if ((rc = step_once_t<T>(c))) break;
     } else {
       c->p.dt = dt;

@@ -272,6 +272,8 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   const int ms = n - 2 * c->g, me = n - 2 * c->g - 1;
   c->npart = 3 * ms * V + me * (2 + c->nch) * 8;  // 8 row blocks per energy plane
   HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
+  if (c->mixed && c->nch > 0) HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
+  c->ch_stale = 1;
   HCHK(c, hipHostMalloc(&c->hpart, sizeof(double) * c->npart, hipHostMallocDefault));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
@@ -288,6 +290,7 @@ extern "C" void etd_close(etd_ctx *c) {
   (void)hipFree(c->ad_SB);
   (void)hipFree(c->ad_spec);
   (void)hipFree(c->ad_part);
+  (void)hipFree(c->ch_sum);
   if (c->ad_hpart) (void)hipHostFree(c->ad_hpart);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->u);
@@ -402,6 +405,7 @@ static int get_f32(etd_ctx *c, double *host, const float *dev, size_t cnt) {
 extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "set_field: bad variable %d", var);
   c->elastic_fresh = 0;
+  c->ch_stale = 1;
   if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
   HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
@@ -469,6 +473,7 @@ int step_once_t(etd_ctx *c) {
   const int V = c->V;
   const size_t n3 = c->n3;
   const etd_fields<T> F = fields<T>(c);
+  if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
   stage_forward<T>(c);
   launch_phiscale<T>(c, F.spec, F.spec + n3 * V, F.spec, 0, V, 0);
   modeprod3<T>(c, 0, F.spec, F.u, V, 0);
@@ -480,6 +485,7 @@ int step_once_t(etd_ctx *c) {
     launch_phiscale<T>(c, F.spec, (const T *)nullptr, F.spec, 0, V, 1);
     modeprod3<T>(c, 0, F.spec, F.u, V, 1);  // u = a + S phi2 S^-1 dN
   }
+  if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
   HCHK(c, hipGetLastError());
   return ETD_OK;
 }

@@ -85,6 +85,10 @@ extern "C" int etd_ensemble_create(etd_ctx *const *members, int count, int steps
   for (int i = 0; i < count; i++) ECHK(hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming));
   // every kernel's one-time host set-up happens outside the capture: finish
   // pending work first
+  // the CH mean targets are taken outside the capture, so that replays do
+  // not retake them
+  for (int i = 0; i < count; i++)
+    if (e->m[i]->mixed) ch_mean_mark(e->m[i], e->m[i]->f32.u);
   for (int i = 0; i < count; i++) ECHK(hipStreamSynchronize(e->m[i]->stream));
   long long *saved = (long long *)malloc(sizeof(long long) * count);
   for (int i = 0; i < count; i++) saved[i] = e->m[i]->elastic_evals;

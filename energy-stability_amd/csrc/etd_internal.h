@@ -85,6 +85,12 @@ struct etd_ctx {
    * saved state, full-step result, phi-scaled spectrum (V x n3 each, in the
    * storage type), reduction partials */
   double t;
+  /* mixed path with CH fields: the conserved interior sum of each CH field
+   * (nch targets, then nch x CH_BLOCKS per-block partials), restoring the mean
+   * that the split-BF16 V V^-1 does not conserve exactly; ch_stale: the
+   * fields were set from outside, the targets are taken at the next step */
+  double *ch_sum;
+  int ch_stale;
   void *ad_S0, *ad_SB, *ad_spec;
   double *ad_part, *ad_hpart;
   struct etd_mlp *mlp;
@@ -146,6 +152,15 @@ template <typename T>
 void launch_felas_materialize(etd_ctx *c, T *felas);
 template <typename T>
 int launch_diag(etd_ctx *c, double *stats);
+/* CH mean conservation of the mixed path: ch_mean_mark takes the interior
+ * sums of the CH fields of the state u (V fields, stride n3) as the targets
+ * if the fields were set since the last mark; ch_mean_restore shifts the CH
+ * fields of u so that their interior sums are the targets (a shift below the
+ * fields' rounding is carried to a later step, the targets being fixed).
+ * No-ops unless the context is mixed and has CH fields. */
+#define CH_BLOCKS 256
+void ch_mean_mark(etd_ctx *c, const float *u);
+void ch_mean_restore(etd_ctx *c, float *u);
 
 /* step pieces (etd_api.hip) */
 template <typename T>

@@ -1225,8 +1225,22 @@ struct fft16_cfg {
   static constexpr int LDS = 16 * LINE * 16;  // bytes (double2)
 };
 
-template <int SGN, int R2, typename T2>
-__global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, const double2 *__restrict__ tw) {
+// IO = 0: in place on data.  IO = 1 (forward x pass of an elastic
+// evaluation): the input is read from the fields, eta_2r + i eta_2r+1 on the
+// interior of the extended layout (the pack).  IO = 2 (inverse z pass): the
+// output goes to the felas ring, real and imaginary parts to the slots of
+// fields 2r and 2r+1, times 1/N^3 (the unpack); each value is rounded as the
+// separate pack / unpack kernels round it.
+struct fft_io {
+  const void *u;
+  void *E;
+  int n, g, nac, ring, slot0;
+};
+
+template <int SGN, int R2, typename T2, int IO = 0>
+__global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, const double2 *__restrict__ tw,
+                                               fft_io io = fft_io{}) {
+  using T = decltype(T2{}.x);
   using CF = fft16_cfg<R2>;
   constexpr int N = CF::N;
   __shared__ double2 buf[16 * CF::LINE];
@@ -1242,10 +1256,23 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   {  // stage 1: thread n1 = j, DFT over n2 of x[n1 + 16 n2], twiddle, to LDS [k2][n1]
     const int n1 = j;
     double2 a[R2];
+    if constexpr (IO == 1) {  // axis 0: line L = (y, z) = (x, o)
+      const T *u = (const T *)io.u;
+      const size_t n3 = (size_t)io.n * io.n * io.n;
+      const int pa = 2 * (int)blockIdx.y, pb = pa + 1;
 #pragma unroll
-    for (int n2 = 0; n2 < R2; n2++) {
-      const T2 v = d[base + (size_t)(n1 + 16 * n2) * stride];
-      a[n2] = make_double2(v.x, v.y);
+      for (int n2 = 0; n2 < R2; n2++) {
+        const size_t src = IDX(io.n, (int)o + io.g, (int)x + io.g, n1 + 16 * n2 + io.g);
+        const T re = u[n3 * pa + src];
+        const T im = pb < io.nac ? u[n3 * pb + src] : (T)0;
+        a[n2] = make_double2(re, im);
+      }
+    } else {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; n2++) {
+        const T2 v = d[base + (size_t)(n1 + 16 * n2) * stride];
+        a[n2] = make_double2(v.x, v.y);
+      }
     }
     dft_any<R2, SGN>(a);
 #pragma unroll
@@ -1267,13 +1294,26 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
       T2 v;
       v.x = b[k1].x;
       v.y = b[k1].y;
-      d[base + (size_t)(k2 + R2 * k1) * stride] = v;
+      const size_t m = base + (size_t)(k2 + R2 * k1) * stride;
+      if constexpr (IO == 2) {
+        T *E = (T *)io.E;
+        const size_t N3 = (size_t)N * N * N;
+        const double inv = 1.0 / ((double)N * N * N);
+        const int r = (int)blockIdx.y;
+        int sa = io.slot0 + 2 * r, sb = sa + 1;  // (slot0 + p) mod ring, p < nac < ring
+        if (sa >= io.ring) sa -= io.ring;
+        if (sb >= io.ring) sb -= io.ring;
+        E[(size_t)sa * N3 + m] = (T)((double)v.x * inv);
+        if (2 * r + 1 < io.nac) E[(size_t)sb * N3 + m] = (T)((double)v.y * inv);
+      } else {
+        d[m] = v;
+      }
     }
   }
 }
 
-template <int SGN, typename T2>
-static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis) {
+template <int SGN, typename T2, int IO = 0>
+static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis, fft_io io = fft_io{}) {
   const int N = c->N;
   if (N % 16) return false;
   dim3 grid((unsigned)((size_t)N * N / 16), npair);
@@ -1281,7 +1321,7 @@ static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis) {
   switch (N / 16) {
 #define F16(R)                                                                                   \
   case R:                                                                                        \
-    k_fft16<SGN, R, T2><<<grid, 256, 0, c->stream>>>(data, c->N3, axis, tw);                     \
+    k_fft16<SGN, R, T2, IO><<<grid, 256, 0, c->stream>>>(data, c->N3, axis, tw, io);             \
     return true;
     F16(2) F16(3) F16(4) F16(5) F16(6) F16(8) F16(9) F16(10) F16(12) F16(15) F16(16)
 #undef F16
@@ -1543,8 +1583,22 @@ void launch_elastic(etd_ctx *c) {
   const size_t N3 = c->N3;
   const unsigned nb = (unsigned)((N3 + 255) / 256);
   etd_prof_begin(c, PC_ELASTIC);
-  k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(F.u, (T2 *)F.Z, c->n, c->g, N, c->nac);
-  fft3d<T>(c, F.Z, c->npair, +1);
+  const long long Lbase = c->elastic_evals * (long long)c->nac;
+  const int slot0 = (int)(Lbase % c->ring);
+  // two-stage FFT: the pack is the forward x pass's input and the unpack the
+  // inverse z pass's output
+  const bool fused = c->fft_variant == 1 && N % 16 == 0;
+  fft_io io{F.u, F.E, c->n, c->g, c->nac, c->ring, slot0};
+  if (fused) {
+    etd_prof_begin(c, PC_FFT);
+    launch_fft16<1, T2, 1>(c, (T2 *)F.Z, c->npair, 0, io);
+    launch_fft16<1, T2>(c, (T2 *)F.Z, c->npair, 1);
+    launch_fft16<1, T2>(c, (T2 *)F.Z, c->npair, 2);
+    etd_prof_end(c, PC_FFT);
+  } else {
+    k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(F.u, (T2 *)F.Z, c->n, c->g, N, c->nac);
+    fft3d<T>(c, F.Z, c->npair, +1);
+  }
   {
     const int nplane = (N & 1) ? 2 : 1;
     const unsigned nbm = (unsigned)((3 * nplane * (size_t)N * N + 255) / 256);
@@ -1559,10 +1613,16 @@ void launch_elastic(etd_ctx *c) {
       default: break;
     }
   }
-  fft3d<T>(c, F.Y, c->npair, -1);
-  const long long Lbase = c->elastic_evals * (long long)c->nac;
-  const int slot0 = (int)(Lbase % c->ring);
-  k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, slot0);
+  if (fused) {
+    etd_prof_begin(c, PC_FFT);
+    launch_fft16<-1, T2>(c, (T2 *)F.Y, c->npair, 0);
+    launch_fft16<-1, T2>(c, (T2 *)F.Y, c->npair, 1);
+    launch_fft16<-1, T2, 2>(c, (T2 *)F.Y, c->npair, 2, io);
+    etd_prof_end(c, PC_FFT);
+  } else {
+    fft3d<T>(c, F.Y, c->npair, -1);
+    k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, slot0);
+  }
   c->elastic_evals++;
   etd_prof_end(c, PC_ELASTIC);
 }
@@ -1783,6 +1843,72 @@ void launch_copy16(etd_ctx *c, const void *in, void *out, size_t n16) {
 }
 
 // ------------------------------------------------------------------ instantiations (FP64 and FP32 storage)
+// ------------------------------------------------------------------ CH mean
+// Interior sum of CH field blockIdx.y per block (one (z, y) row per wave,
+// lanes along x), into part[blockIdx.y * CH_BLOCKS + blockIdx.x].
+__device__ __forceinline__ double block_sum256(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  __shared__ double sh[4];
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_ch_sum(const float *__restrict__ u, size_t fs, int n, int g,
+                                                double *__restrict__ part) {
+  const float *f = u + fs * blockIdx.y;
+  const int N = n - 2 * g;
+  double acc = 0.0;
+  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < N * N; r += gridDim.x * 4) {
+    const int z = r / N, y = r - z * N;
+    for (int x = threadIdx.x & 63; x < N; x += 64) acc += (double)f[IDX(n, z + g, y + g, x + g)];
+  }
+  const double t = block_sum256(acc);
+  if (threadIdx.x == 0) part[blockIdx.y * CH_BLOCKS + blockIdx.x] = t;
+}
+
+// the partials of field q = blockIdx.x reduced in a fixed order into target[q]
+__global__ __launch_bounds__(256) void k_ch_target(const double *__restrict__ part, double *__restrict__ target) {
+  const double s = block_sum256(threadIdx.x < CH_BLOCKS ? part[blockIdx.x * CH_BLOCKS + threadIdx.x] : 0.0);
+  if (threadIdx.x == 0) target[blockIdx.x] = s;
+}
+
+// u_q += (target_q - sum_q) / N^3 on the interior of CH field q = blockIdx.y;
+// every block reduces the CH_BLOCKS partials of sum_q in the same order
+__global__ __launch_bounds__(256) void k_ch_shift(float *__restrict__ u, size_t fs, int n, int g,
+                                                  const double *__restrict__ target,
+                                                  const double *__restrict__ part) {
+  float *f = u + fs * blockIdx.y;
+  const int N = n - 2 * g;
+  const int q = blockIdx.y;
+  const double a = block_sum256(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
+  const double shift = (target[q] - a) / ((double)N * N * N);
+  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < N * N; r += gridDim.x * 4) {
+    const int z = r / N, y = r - z * N;
+    for (int x = threadIdx.x & 63; x < N; x += 64) {
+      const size_t l = IDX(n, z + g, y + g, x + g);
+      f[l] = (float)((double)f[l] + shift);
+    }
+  }
+}
+
+void ch_mean_mark(etd_ctx *c, const float *u) {
+  if (!c->ch_sum || !c->ch_stale) return;
+  double *part = c->ch_sum + c->nch;
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_target<<<c->nch, 256, 0, c->stream>>>(part, c->ch_sum);
+  c->ch_stale = 0;
+}
+
+void ch_mean_restore(etd_ctx *c, float *u) {
+  if (!c->ch_sum) return;
+  double *part = c->ch_sum + c->nch;
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_shift<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, c->ch_sum,
+                                                             part);
+}
+
 #define ETD_INST(T)                                                                   \
   template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
   template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \

@@ -135,3 +135,25 @@ def test_mixed_diag_energy_in_fp64():
     assert np.abs(st_o - st_g).max() < 1e-12 * np.abs(st_o).max()
     assert abs(cp_o - cp_g) < 1e-12 * abs(cp_o)
 
+
+
+@pytest.mark.parametrize("etd2", [0, 1])
+def test_mixed_path_conserves_the_ch_mean(etd2):
+    """The CH concentration's interior mean is conserved by the scheme (its
+    k = 0 mode has H = 0); the mixed path restores it after every step, so it
+    stays at its initial value to FP32 rounding over many steps, as the FP64
+    path keeps it to FP64 rounding."""
+    say(f"mixed CH mean conservation etd2={etd2}")
+    prm = small_params(n=20, nac=2, nch=1, etd2=etd2, elastic=1, dt=0.02)
+    f = noise_fields(prm, amp=0.2, ch_amp=0.05)
+    g = pyetd.Etd.from_dict(mixed(prm))
+    for v, x in enumerate(f):
+        g.set_field(v, x)
+    gg = prm["g"]
+    inner = (slice(gg, -gg),) * 3
+    m0 = float(np.mean(g.get_field(2)[inner], dtype=np.float64))
+    g.step(60)
+    m1 = float(np.mean(g.get_field(2)[inner], dtype=np.float64))
+    say(f"  mean {m0:.12f} -> {m1:.12f} (diff {m1 - m0:.2e})")
+    assert abs(m1 - m0) < 2e-7 * abs(m0)
+    g.close()
