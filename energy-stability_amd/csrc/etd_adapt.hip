@@ -496,6 +496,8 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
       modeprod3<T>(c, 0, SP, F.u, V, 0);
       if constexpr (sizeof(T) == 4) {
+        ch_center(c, SB, nullptr, +1);
+        ch_center(c, F.u, nullptr, +1);
         ch_mean_restore(c, SB);
         ch_mean_restore(c, F.u);
       }

@@ -272,7 +272,10 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   const int ms = n - 2 * c->g, me = n - 2 * c->g - 1;
   c->npart = 3 * ms * V + me * (2 + c->nch) * 8;  // 8 row blocks per energy plane
   HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
-  if (c->mixed && c->nch > 0) HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
+  if (c->mixed && c->nch > 0) {
+    HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
+    HCHK(c, hipMalloc(&c->ch_const, sizeof(float) * c->nch * c->n3));
+  }
   c->ch_stale = 1;
   HCHK(c, hipHostMalloc(&c->hpart, sizeof(double) * c->npart, hipHostMallocDefault));
   HCHK(c, hipStreamSynchronize(c->stream));
@@ -291,6 +294,7 @@ extern "C" void etd_close(etd_ctx *c) {
   (void)hipFree(c->ad_spec);
   (void)hipFree(c->ad_part);
   (void)hipFree(c->ch_sum);
+  (void)hipFree(c->ch_const);
   if (c->ad_hpart) (void)hipHostFree(c->ad_hpart);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->u);
@@ -465,7 +469,9 @@ void stage_forward(etd_ctx *c) {
   if (c->p.elastic == 1 && c->nac > 0 && !c->elastic_fresh) launch_elastic<T>(c);
   c->elastic_fresh = 0;
   launch_nonlinear<T>(c, F.N1, nullptr);
+  if constexpr (sizeof(T) == 4) ch_center(c, F.u, F.N1, -1);
   modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
+  if constexpr (sizeof(T) == 4) ch_center(c, F.u, F.N1, +1);
 }
 
 template <typename T>
@@ -477,6 +483,7 @@ int step_once_t(etd_ctx *c) {
   stage_forward<T>(c);
   launch_phiscale<T>(c, F.spec, F.spec + n3 * V, F.spec, 0, V, 0);
   modeprod3<T>(c, 0, F.spec, F.u, V, 0);
+  if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
   if (c->p.etd2) {
     launch_ghost_fill<T>(c, F.u, V, n3);
     if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);

@@ -91,6 +91,7 @@ struct etd_ctx {
    * fields were set from outside, the targets are taken at the next step */
   double *ch_sum;
   int ch_stale;
+  float *ch_const;         /* nch x n3: N(c_mean), the CH nonlinear term of the uniform field at the target mean */
   void *ad_S0, *ad_SB, *ad_spec;
   double *ad_part, *ad_hpart;
   struct etd_mlp *mlp;
@@ -161,6 +162,13 @@ int launch_diag(etd_ctx *c, double *stats);
 #define CH_BLOCKS 256
 void ch_mean_mark(etd_ctx *c, const float *u);
 void ch_mean_restore(etd_ctx *c, float *u);
+/* The mixed path transforms the CH fields' deviation from their conserved
+ * mean: a uniform field is a fixed point of the scheme, so the step evolves
+ * u - c_mean with N(u) - N(c_mean) and adds c_mean back.  The split-BF16
+ * products then round the small deviation instead of the large mean, whose
+ * rounding would otherwise leak into the barely damped long-wave CH modes
+ * every step.  sign -1 centres (u and/or N, either may be null), +1 undoes. */
+void ch_center(etd_ctx *c, float *u, float *N, int sign);
 
 /* step pieces (etd_api.hip) */
 template <typename T>

@@ -1893,12 +1893,55 @@ __global__ __launch_bounds__(256) void k_ch_shift(float *__restrict__ u, size_t 
   }
 }
 
+// f = target[q] / N^3 on the whole extended box
+__global__ void k_ch_fill(float *__restrict__ f, size_t n3, const double *__restrict__ target, int q, double invN3) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < n3) f[l] = (float)(target[q] * invN3);
+}
+
+// u += sign c_mean (whole box), N += sign N(c_mean), CH field blockIdx.y
+__global__ void k_ch_center(float *__restrict__ u, float *__restrict__ N, size_t n3, const double *__restrict__ target,
+                            const float *__restrict__ nconst, double invN3, int sign) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  const int q = blockIdx.y;
+  if (u) {
+    float *f = u + n3 * q;
+    const float m = (float)(target[q] * invN3);
+    f[l] = sign > 0 ? f[l] + m : f[l] - m;
+  }
+  if (N) {
+    float *f = N + n3 * q;
+    const float k = nconst[n3 * q + l];
+    f[l] = sign > 0 ? f[l] + k : f[l] - k;
+  }
+}
+
 void ch_mean_mark(etd_ctx *c, const float *u) {
   if (!c->ch_sum || !c->ch_stale) return;
   double *part = c->ch_sum + c->nch;
   k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
   k_ch_target<<<c->nch, 256, 0, c->stream>>>(part, c->ch_sum);
+  // N(c_mean) of each CH field, through the same kernels as the step, on a
+  // uniform field in the mode-product scratch
+  const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
+  nl_args a = make_nl(c);
+  const unsigned nb = (unsigned)((c->n3 + 255) / 256);
+  float *uni = c->f32.w1, *t = c->f32.gtmp;
+  for (int q = 0; q < c->nch; q++) {
+    k_ch_fill<<<nb, 256, 0, c->stream>>>(uni, c->n3, c->ch_sum, q, invN3);
+    k_ch_t<float><<<nb, 256, 0, c->stream>>>(uni, t, a);
+    k_ch_lap<float><<<nb, 256, 0, c->stream>>>(uni, t, c->ch_const + c->n3 * q, nullptr, a);
+  }
   c->ch_stale = 0;
+}
+
+void ch_center(etd_ctx *c, float *u, float *N, int sign) {
+  if (!c->ch_sum) return;
+  const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
+  const size_t off = c->n3 * c->nac;
+  k_ch_center<<<dim3((unsigned)((c->n3 + 255) / 256), c->nch), 256, 0, c->stream>>>(
+      u ? u + off : nullptr, N ? N + off : nullptr, c->n3, c->ch_sum, c->ch_const, invN3, sign);
 }
 
 void ch_mean_restore(etd_ctx *c, float *u) {

@@ -291,8 +291,9 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
       const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(stg);
       const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
       bf16x8 b[UT][NT];
-#pragma unroll
-      for (int u = 0; u < UT; u++) {
+      // column tile u's split; tiles after the first are split between the
+      // MFMAs of row tile 0, so that their VALU work overlaps MFMA work
+      auto split_tile = [&](int u) {
         const float *bp = sx + ((size_t)q * CF::COLS + 16 * UT * wave + 16 * u + r16) * 8;
         const float4 lo = *reinterpret_cast<const float4 *>(bp);
         const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
@@ -305,7 +306,8 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
           if constexpr (NT > 1) b[u][1][e] = t1;
           if constexpr (NT > 2) b[u][2][e] = t2;
         }
-      }
+      };
+      split_tile(0);
 #pragma unroll
       for (int t = 0; t < MT; t++) {
         bf16x8 a[NT];
@@ -313,6 +315,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
         for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
 #pragma unroll
         for (int u = 0; u < UT; u++) {
+          if (t == 0 && u > 0) split_tile(u);
           if constexpr (NT > 2) {
             acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
             acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
