@@ -1256,17 +1256,43 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   {  // stage 1: thread n1 = j, DFT over n2 of x[n1 + 16 n2], twiddle, to LDS [k2][n1]
     const int n1 = j;
     double2 a[R2];
-    if constexpr (IO == 1) {  // axis 0: line L = (y, z) = (x, o)
+    if constexpr (IO == 1) {
+      // axis 0, pack: the group's 16 x rows of fields 2r and 2r+1 go to LDS
+      // with consecutive threads on consecutive x (coalesced; the rows of the
+      // extended layout start two ghost cells in), padded so that the strided
+      // reads below take no extra bank passes
+      constexpr int SP = N + (sizeof(T2) == 8 ? 16 : R2);
+      static_assert(16 * SP * sizeof(T2) <= sizeof(buf), "staging fits the transpose buffer");
+      T2 *stg = reinterpret_cast<T2 *>(buf);
       const T *u = (const T *)io.u;
       const size_t n3 = (size_t)io.n * io.n * io.n;
       const int pa = 2 * (int)blockIdx.y, pb = pa + 1;
+      // 16 N / 256 = R2 elements per thread, all loads issued before any
+      // store (the loop is unrolled)
+      T va[R2], vb[R2];
+#pragma unroll
+      for (int k = 0; k < R2; k++) {
+        const int e = tid + 256 * k, l2 = e / N, xx = e - l2 * N;
+        const unsigned Lg = blockIdx.x * 16 + l2;
+        const size_t src = IDX(io.n, (int)(Lg / N) + io.g, (int)(Lg % N) + io.g, xx + io.g);
+        va[k] = u[n3 * pa + src];
+        vb[k] = pb < io.nac ? u[n3 * pb + src] : (T)0;
+      }
+#pragma unroll
+      for (int k = 0; k < R2; k++) {
+        const int e = tid + 256 * k, l2 = e / N, xx = e - l2 * N;
+        T2 v;
+        v.x = va[k];
+        v.y = vb[k];
+        stg[l2 * SP + xx] = v;
+      }
+      __syncthreads();
 #pragma unroll
       for (int n2 = 0; n2 < R2; n2++) {
-        const size_t src = IDX(io.n, (int)o + io.g, (int)x + io.g, n1 + 16 * n2 + io.g);
-        const T re = u[n3 * pa + src];
-        const T im = pb < io.nac ? u[n3 * pb + src] : (T)0;
-        a[n2] = make_double2(re, im);
+        const T2 v = stg[ll * SP + n1 + 16 * n2];
+        a[n2] = make_double2(v.x, v.y);
       }
+      __syncthreads();  // the transpose below reuses the buffer
     } else {
 #pragma unroll
       for (int n2 = 0; n2 < R2; n2++) {

@@ -1,6 +1,6 @@
 """CPU restatement of libetd's adaptive ETD driver (etd_run_adaptive with the
-plain controller) on the FP64 oracle -- test infrastructure only, the checker
-for the GPU driver's accept / reject decisions and its accepted states.
+plain controller) on the FP64 oracle -- test infrastructure only: the GPU
+driver's accept / reject decisions and accepted states are checked against it.
 
 Attempt from u_n with step dt (ETD1): one shared N(u_n) gives u_full = u_n +
 dt and the first half step; a second half step gives u_half;

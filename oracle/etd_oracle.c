@@ -2,7 +2,7 @@
  * etd_oracle.c -- FP64 CPU oracle for the ScLETD step (test infrastructure).
  *
  * Not the product: only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg use it, as the checker.  Function headers cite the
+ * cpu_baseline leg use it, to check results.  Function headers cite the
  * reference lines restated (paths relative to the reference root) and what
  * pins each one:
  *

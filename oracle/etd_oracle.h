@@ -2,7 +2,7 @@
  * etd_oracle.h -- FP64 CPU oracle for the ScLETD exponential-time-differencing
  * step (test infrastructure only).
  *
- * This is the checker, not the product.  Only tests/, __graft_entry__.smoke()
+ * It checks results; it is not the product.  Only tests/, __graft_entry__.smoke()
  * and the cpu_baseline leg of bench.py may load it.  The product library
  * (libetd, HIP) never links or calls it.
  *

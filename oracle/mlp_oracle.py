@@ -1,5 +1,5 @@
 """FP64 numpy restatement of the controller network (test infrastructure only;
-the checker for libetd's etd_mlp forward / backward kernels).
+what libetd's etd_mlp forward / backward kernels are checked against).
 
 MLP 16 -> 64 -> 64 -> 1, GELU(a) = a/2 (1 + erf(a / sqrt 2)), loss = mean
 squared error.  Parameter vector layout as in include/etd.h: W1 [64][16]

@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
 
 Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it as the checker.  The product (libetd) never imports it.
+cpu_baseline leg use it, to check results.  The product (libetd) never imports it.
 """
 import ctypes as C
 import os
