@@ -1243,7 +1243,11 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   using T = decltype(T2{}.x);
   using CF = fft16_cfg<R2>;
   constexpr int N = CF::N;
-  __shared__ double2 buf[16 * CF::LINE];
+  // the transpose between the stages holds FP32 pairs for FP32 data (one
+  // rounding of the intermediate, within the FP32 transform's accuracy; half
+  // the LDS, so twice the workgroups per CU) and FP64 pairs for FP64 data
+  using LT = typename std::conditional<sizeof(T2) == 8, float2, double2>::type;
+  __shared__ LT buf[16 * CF::LINE];
   T2 *d = data + fs * blockIdx.y;
   const int tid = threadIdx.x;
   const int ll = axis == 0 ? tid >> 4 : tid & 15;  // line within the group
@@ -1252,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   const unsigned x = L % N, o = L / N;
   const size_t base = axis == 0 ? (size_t)L * N : (axis == 1 ? x + (size_t)N * N * o : x + (size_t)N * o);
   const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)N : (size_t)N * N);
-  double2 *bl = buf + ll * CF::LINE;
+  LT *bl = buf + ll * CF::LINE;
   {  // stage 1: thread n1 = j, DFT over n2 of x[n1 + 16 n2], twiddle, to LDS [k2][n1]
     const int n1 = j;
     double2 a[R2];
@@ -1261,7 +1265,7 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
       // with consecutive threads on consecutive x (coalesced; the rows of the
       // extended layout start two ghost cells in), padded so that the strided
       // reads below take no extra bank passes
-      constexpr int SP = N + (sizeof(T2) == 8 ? 16 : R2);
+      constexpr int SP = N + R2;  // fits: the buffer holds 16 (17 R2 + 1) pairs
       static_assert(16 * SP * sizeof(T2) <= sizeof(buf), "staging fits the transpose buffer");
       T2 *stg = reinterpret_cast<T2 *>(buf);
       const T *u = (const T *)io.u;
@@ -1305,7 +1309,11 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
     for (int k2 = 0; k2 < R2; k2++) {
       double2 w = tw[(n1 * k2) % N];  // exp(-2 pi i n1 k2 / N)
       if (SGN < 0) w.y = -w.y;
-      bl[k2 * CF::ROW + n1] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w);
+      const double2 y = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w);
+      LT v;
+      v.x = y.x;
+      v.y = y.y;
+      bl[k2 * CF::ROW + n1] = v;
     }
   }
   __syncthreads();
@@ -1313,7 +1321,10 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
     const int k2 = j;
     double2 b[16];
 #pragma unroll
-    for (int n1 = 0; n1 < 16; n1++) b[n1] = bl[k2 * CF::ROW + n1];
+    for (int n1 = 0; n1 < 16; n1++) {
+      const LT v = bl[k2 * CF::ROW + n1];
+      b[n1] = make_double2(v.x, v.y);
+    }
     dft_any<16, SGN>(b);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) {

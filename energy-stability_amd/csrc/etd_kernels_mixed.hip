@@ -119,6 +119,39 @@ __device__ __forceinline__ void store_tile_(const f32x4 (&acc)[MT][UT], float *d
   }
 }
 
+// Epilogue for a wave that owns row tiles t0 .. t0 + MTH - 1 of MT (tiles at
+// MT and beyond are skipped; only tile MT - 1 holds rows >= n)
+template <int MTH, int UT, bool ACCUMULATE>
+__device__ __forceinline__ void store_rows_(const f32x4 (&acc)[MTH][UT], float *dst, int n, long col0, int q,
+                                           int rotate, int t0, int MT) {
+  const long n2 = (long)n * n;
+  const __amdgpu_buffer_rsrc_t rD =
+      __builtin_amdgcn_make_buffer_rsrc((void *)dst, 0, (int)((size_t)n * n2 * 4), 0x00020000);
+  const uint32_t s_row = rotate ? (uint32_t)n2 * 4u : 4u, s_col = rotate ? 4u : (uint32_t)n * 4u;
+  const uint32_t OOB = 0x80000000u;
+  const int last_rows = n - 16 * (MT - 1) - 4 * q;
+#pragma unroll
+  for (int u = 0; u < UT; u++) {
+    const long col = col0 + 16 * u;
+    const bool cok = col < n2;
+    uint32_t o = (uint32_t)col * s_col + (uint32_t)(16 * t0 + 4 * q) * s_row;
+#pragma unroll
+    for (int tt = 0; tt < MTH; tt++) {
+      const int t = t0 + tt;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const bool rok = t < MT - 1 || (t == MT - 1 && i < last_rows);
+        const int off = (int)((cok && rok) ? o : OOB);
+        float v = acc[tt][u][i];
+        if constexpr (ACCUMULATE) v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rD, off, 0, 0)) + v;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rD, off, 0, 0);
+        o += s_row;
+      }
+      o += 12u * s_row;
+    }
+  }
+}
+
 template <int MT, int UT>
 __device__ __forceinline__ void store_tile(const f32x4 (&acc)[MT][UT], float *dst, int n, long col0, int q,
                                            int rotate, int accumulate) {
@@ -307,29 +340,280 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
           if constexpr (NT > 2) b[u][2][e] = t2;
         }
       };
+      for (int u = 0; u < UT; u++) split_tile(u);
+      // row tiles in pairs: 2 UT independent accumulation chains in flight,
+      // so consecutive MFMAs on one accumulator are 2 UT issues apart
+#pragma unroll
+      for (int t = 0; t < MT; t += 2) {
+        constexpr int TP = 2;
+        bf16x8 a[TP][NT];
+#pragma unroll
+        for (int d = 0; d < TP; d++)
+          if (t + d < MT)
+#pragma unroll
+            for (int tm = 0; tm < NT; tm++) a[d][tm] = sa[(tm * MT + t + d) * 64 + lane];
+        // smallest terms first, as in the other kernels
+        constexpr int NP = NT == 3 ? 6 : (NT == 2 ? 3 : 1);
+        constexpr int ia[6] = {2, 1, 0, 1, 0, 0}, ib[6] = {0, 1, 2, 0, 1, 0};
+        constexpr int off = NT == 3 ? 0 : (NT == 2 ? 3 : 5);
+#pragma unroll
+        for (int pi = 0; pi < NP; pi++)
+#pragma unroll
+          for (int d = 0; d < TP; d++)
+#pragma unroll
+            for (int u = 0; u < UT; u++)
+              if (t + d < MT)
+                acc[t + d][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[d][ia[off + pi]], b[u][ib[off + pi]],
+                                                                        acc[t + d][u], 0, 0, 0);
+      }
+    }
+    store_tile<MT, UT>(acc, dst, n, wgcol + 16 * UT * wave + r16, q, rotate, accumulate);
+  }
+}
+
+// Two-workgroups-per-CU variant (persistent): 4 waves x 32 columns, ONE
+// V / V^-1 stage (40 KB) and two field stages (16 KB each), 72 KB of LDS, so
+// two workgroups share a CU.  Each SIMD then holds waves of two workgroups
+// that do not meet at the same barriers: while one waits for its next
+// operand chunk (issued only after every wave of the workgroup is past the
+// previous one) or its epilogue stores, the other issues MFMAs.
+template <int N_>
+__device__ __forceinline__ void wait_vm_barrier() {
+  if constexpr (N_ == 0)
+    asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" : : : "memory");
+  else
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(N_) : "memory");
+}
+__device__ __forceinline__ void plain_barrier() { asm volatile("s_barrier" : : : "memory"); }
+
+template <int MT, int NT>
+__global__ __launch_bounds__(256, 2) void k_modeprod_split_2wg(const uint16_t *__restrict__ A3,
+                                                               const float *__restrict__ in, size_t in_fs,
+                                                               float *__restrict__ out, size_t out_fs, int n,
+                                                               int rotate, int accumulate, int a_chunk, int nfields) {
+  using CF = mps_cfg<MT, NT, 4, 2>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  if ((int)blockIdx.x >= ntiles) return;  // uniform
+  const int my_tiles = (ntiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int total = my_tiles * nkc;
+  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * a_chunk));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+  auto tile_of = [&](int g) { return (int)blockIdx.x + (g / nkc) * (int)gridDim.x; };
+
+  auto issue_a = [&](int kc) {
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      copy16_to_lds(rsA, lds0 + (e * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+  };
+  auto issue_x = [&](int g, int stg) {
+    const int tl = tile_of(g), kc = g % nkc;
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    i32x4m rsX = mk_rsrc(in + in_fs * f, (uint32_t)((size_t)n * n2 * 4));
+    rsX.x = __builtin_amdgcn_readfirstlane(rsX.x);
+    rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
+    rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
+    const uint32_t sbase = lds0 + CF::ASZ + stg * CF::XSZ;
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      const long col = wgc + 32 * wave + (lane >> 1);
+      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + (e * CF::COLS + 32 * wave) * 32, voff);
+    }
+  };
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue_a(0);
+  issue_x(0, 0);
+  const bool two = total > 1;
+  if (two) issue_x(1, 1);
+  f32x4 acc[MT][2];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int g = 0; g < total; g++) {
+    // A(g) and X(g) are in LDS for every wave; only X(g + 1), issued after
+    // them, may still be in flight
+    if (g + 1 < total)
+      wait_vm_barrier<CF::XPT>();
+    else
+      wait_vm_barrier<0>();
+    const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(smem);
+    const float *sx = reinterpret_cast<const float *>(smem + CF::ASZ + (g & 1) * CF::XSZ);
+    bf16x8 b[2][NT];
+    auto split_tile = [&](int u) {
+      const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+      const float4 lo = *reinterpret_cast<const float4 *>(bp);
+      const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+      const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        __bf16 t0, t1, t2;
+        split_bf16(x[e], t0, t1, t2);
+        b[u][0][e] = t0;
+        if constexpr (NT > 1) b[u][1][e] = t1;
+        if constexpr (NT > 2) b[u][2][e] = t2;
+      }
+    };
+    split_tile(0);
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      bf16x8 a[NT];
+#pragma unroll
+      for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (t == 0 && u > 0) split_tile(u);
+        if constexpr (NT > 2) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[t][u], 0, 0, 0);
+        }
+        if constexpr (NT > 1) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[t][u], 0, 0, 0);
+        }
+        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[t][u], 0, 0, 0);
+      }
+    }
+    if (g % nkc == nkc - 1) {
+      const int tl = tile_of(g);
+      store_tile<MT, 2>(acc, out + out_fs * (tl / ncb), n, (long)(tl % ncb) * CF::COLS + 32 * wave + r16, q, rotate,
+                        accumulate);
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
+    // every wave is past chunk g: its A stage and X stage may be refilled
+    plain_barrier();
+    if (g + 1 < total) issue_a((g + 1) % nkc);
+    if (g + 2 < total) issue_x(g + 2, g & 1);
+  }
+}
+
+// Wave-grid variant (persistent): 8 waves as 2 row halves x 4 column groups
+// of 64 columns.  A wave reads only its half of the split V / V^-1 chunk
+// from LDS and feeds each fragment to four column tiles, which cuts the LDS
+// reads per chunk by a third; the two waves of a column group split the same
+// field values (twice the split VALU).  Same LDS stages as the default.
+template <int MT, int NT>
+__global__ __launch_bounds__(512, 2) void k_modeprod_split_rc(const uint16_t *__restrict__ A3,
+                                                              const float *__restrict__ in, size_t in_fs,
+                                                              float *__restrict__ out, size_t out_fs, int n,
+                                                              int rotate, int accumulate, int a_chunk, int nfields) {
+  using CF = mps_cfg<MT, NT, 8, 2>;  // COLS = 256, same staging as the default kernel
+  constexpr int MTH = (MT + 1) / 2;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rh = wave & 1, cg = wave >> 1;
+  const int t0 = rh * MTH;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;  // uniform
+  const i32x4m rsA = mk_rsrc(A3, (uint32_t)((size_t)nkc * a_chunk));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue = [&](int tl, int kc, int stg) {
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    i32x4m rsX = mk_rsrc(in + in_fs * f, (uint32_t)((size_t)n * n2 * 4));
+    rsX.x = __builtin_amdgcn_readfirstlane(rsX.x);
+    rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
+    rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
+    const uint32_t sbase = lds0 + stg * CF::STAGE;
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      const long col = wgc + 32 * wave + (lane >> 1);
+      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + CF::ASZ + (e * CF::THREADS + wave * 64) * 16, voff);
+    }
+  };
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue(tile, 0, 0);
+  int g = 0;
+#pragma unroll 1
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int f = tile / ncb;
+    const long wgcol = (long)(tile % ncb) * CF::COLS;
+    f32x4 acc[MTH][4];
+#pragma unroll
+    for (int t = 0; t < MTH; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int kc = 0; kc < nkc; kc++, g++) {
+      wait_all_copies_barrier();
+      if (kc + 1 < nkc)
+        issue(tile, kc + 1, (g + 1) & 1);
+      else if (tile + (int)gridDim.x < ntiles)
+        issue(tile + gridDim.x, 0, (g + 1) & 1);
+      const char *stg = smem + (g & 1) * CF::STAGE;
+      const bf16x8 *sa = reinterpret_cast<const bf16x8 *>(stg);
+      const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
+      bf16x8 b[4][NT];
+      auto split_tile = [&](int u) {
+        const float *bp = sx + ((size_t)q * CF::COLS + 64 * cg + 16 * u + r16) * 8;
+        const float4 lo = *reinterpret_cast<const float4 *>(bp);
+        const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+        const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          __bf16 t0_, t1_, t2_;
+          split_bf16(x[e], t0_, t1_, t2_);
+          b[u][0][e] = t0_;
+          if constexpr (NT > 1) b[u][1][e] = t1_;
+          if constexpr (NT > 2) b[u][2][e] = t2_;
+        }
+      };
       split_tile(0);
 #pragma unroll
-      for (int t = 0; t < MT; t++) {
+      for (int tt = 0; tt < MTH; tt++) {
+        const int t = t0 + tt;
+        if (t >= MT) break;  // wave-uniform
         bf16x8 a[NT];
 #pragma unroll
         for (int tm = 0; tm < NT; tm++) a[tm] = sa[(tm * MT + t) * 64 + lane];
 #pragma unroll
-        for (int u = 0; u < UT; u++) {
-          if (t == 0 && u > 0) split_tile(u);
+        for (int u = 0; u < 4; u++) {
+          if (tt == 0 && u > 0) split_tile(u);
           if constexpr (NT > 2) {
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[t][u], 0, 0, 0);
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[t][u], 0, 0, 0);
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[t][u], 0, 0, 0);
+            acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[u][0], acc[tt][u], 0, 0, 0);
+            acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][1], acc[tt][u], 0, 0, 0);
+            acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][2], acc[tt][u], 0, 0, 0);
           }
           if constexpr (NT > 1) {
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[t][u], 0, 0, 0);
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[t][u], 0, 0, 0);
+            acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[u][0], acc[tt][u], 0, 0, 0);
+            acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][1], acc[tt][u], 0, 0, 0);
           }
-          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[t][u], 0, 0, 0);
+          acc[tt][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[u][0], acc[tt][u], 0, 0, 0);
         }
       }
     }
-    store_tile<MT, UT>(acc, dst, n, wgcol + 16 * UT * wave + r16, q, rotate, accumulate);
+    if (accumulate)
+      store_rows_<MTH, 4, true>(acc, out + out_fs * f, n, wgcol + 64 * cg + r16, q, rotate, t0, MT);
+    else
+      store_rows_<MTH, 4, false>(acc, out + out_fs * f, n, wgcol + 64 * cg + r16, q, rotate, t0, MT);
   }
 }
 
@@ -591,6 +875,43 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
                                                                             accumulate, (int)chunk_bytes<MT, NT>());
     }
   };
+  if constexpr (mps_cfg<MT, NT, 8, 2>::LDS <= 160 * 1024 && MT >= 2) {
+    if (c->split_kernel == 4) {
+      // 2 row halves x 4 column groups of waves
+      using CF = mps_cfg<MT, NT, 8, 2>;
+      static bool attr5 = false;
+      if (!attr5) {
+        (void)hipFuncSetAttribute((const void *)k_modeprod_split_rc<MT, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
+        attr5 = true;
+      }
+      const long ncb = ((long)n * n + CF::COLS - 1) / CF::COLS;
+      const long ntiles = ncb * nfields;
+      const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
+      k_modeprod_split_rc<MT, NT><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+          A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+      return;
+    }
+  }
+  if constexpr (mps_cfg<MT, NT, 4, 2>::ASZ + 2 * mps_cfg<MT, NT, 4, 2>::XSZ <= 80 * 1024) {
+    if (c->split_kernel == 3) {
+      // two 4-wave workgroups per CU, one operand stage each
+      using CF = mps_cfg<MT, NT, 4, 2>;
+      constexpr int lds = CF::ASZ + 2 * CF::XSZ;
+      static bool attr3 = false;
+      if (!attr3) {
+        (void)hipFuncSetAttribute((const void *)k_modeprod_split_2wg<MT, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        attr3 = true;
+      }
+      const long ncb = ((long)n * n + CF::COLS - 1) / CF::COLS;
+      const long ntiles = ncb * nfields;
+      const long nwg = ntiles < 2L * c->num_cu ? ntiles : 2L * c->num_cu;
+      k_modeprod_split_2wg<MT, NT><<<(unsigned)nwg, CF::THREADS, lds, c->stream>>>(
+          A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+      return;
+    }
+  }
   if constexpr (mps_cfg<MT, NT, 4, 4>::LDS <= 160 * 1024) {
     if (c->split_kernel == 2) {
       // 4 waves x 64 columns: each A fragment feeds four column tiles
