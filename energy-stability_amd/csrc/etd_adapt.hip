@@ -10,7 +10,7 @@
 #include "etd_internal.h"
 
 #define RBLOCKS 2048
-#define RCOLS 4
+#define RCOLS 8
 
 // ------------------------------------------------------------------ reductions
 // Interior cells visited one (z, y) row per wave, lanes along x: one 32-bit
@@ -172,26 +172,35 @@ __global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u,
 //       + a2/2 S2 - a3/3 S3 + a4/4 S2^2,   S2 = sum eta^2, S3 = sum eta^3
 //   CH: epn2/2 |D+ c|^2 + (c^2 - 1)^2 / 4
 // The semi-discrete dynamics is the gradient flow of this energy (plus the
-// elastic energy), unlike the 8-corner check_soln_new diagnostic.  Column 0
-// holds the energy, column 1 the sum of the absolute values of its terms (the
-// scale of its rounding).
+// elastic energy), unlike the 8-corner check_soln_new diagnostic.  The sum of
+// the absolute values of its terms is the scale of its rounding.
 struct scheme_args {
   int n, g, nac, nch, aniso;
   double hx, hy, hz, epn2, kkx, kky, kkz, a2, a3, a4;
 };
+
+// The acceptance statistics of a candidate in one pass over its fields
+// (scheme energy, elastic energy, run features; columns of part):
+//   0, 1  the scheme energy above and the sum of its terms' absolute values
+//   2, 3  sum_p eta_p felas_p and sum |eta_p felas_p| (with_el)
+//   4, 5  max |u - u0| and max |eta| over the AC fields
+//   6, 7  count of cells with max_p |eta_p| > 0.1, sum_p |grad eta_p|
+// (the features as k_features computes them).
 template <typename T>
-__global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, const double *__restrict__ lam,
-                                                       scheme_args a, double *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_accept_stats(const T *__restrict__ u, const T *__restrict__ u0,
+                                                      const T *__restrict__ E, const double *__restrict__ lam,
+                                                      scheme_args a, int with_el, int ring, int slot0,
+                                                      double *__restrict__ part) {
   const int n = a.n, g = a.g, N = n - 2 * g;
   const size_t N3 = (size_t)N * N * N, n3 = (size_t)n * n * n;
-  double acc = 0.0, aabs = 0.0;
-  (void)N3;
   const int V = a.nac + a.nch;
+  double acc = 0.0, aabs = 0.0, el = 0.0, elabs = 0.0, dmax = 0.0, emax = 0.0, cnt = 0.0, gsum = 0.0;
   FOR_INTERIOR_CELLS(N, x, y, z) {
     const cell_nb c = neighbours(n, g, N, x, y, z);
-    double s2 = 0.0, s3 = 0.0, e = 0.0, ea = 0.0;
+    const size_t m = ((size_t)z * N + y) * N + x;
+    double s2 = 0.0, s3 = 0.0, e = 0.0, ea = 0.0, cell = 0.0;
     for (int p0 = 0; p0 < V; p0 += FB) {
-      T v[FB], vxp[FB], vyp[FB], vzp[FB], vxm[FB], vym[FB], vzm[FB];
+      T v[FB], v0[FB], vxp[FB], vyp[FB], vzp[FB], vxm[FB], vym[FB], vzm[FB], fe[FB];
 #pragma unroll
       for (int q = 0; q < FB; q++) {
         const int p = p0 + q < V ? p0 + q : V - 1;
@@ -200,10 +209,15 @@ __global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, 
         vxp[q] = f[c.xp];
         vyp[q] = f[c.yp];
         vzp[q] = f[c.zp];
-        if (a.aniso) {
-          vxm[q] = f[c.xm];
-          vym[q] = f[c.ym];
-          vzm[q] = f[c.zm];
+        vxm[q] = f[c.xm];
+        vym[q] = f[c.ym];
+        vzm[q] = f[c.zm];
+        const int pa = p < a.nac ? p : 0;  // AC-only reads stay inside the AC fields
+        v0[q] = u0[n3 * pa + c.l];
+        if (with_el) {
+          int slot = slot0 + pa;  // pa < nac < ring
+          if (slot >= ring) slot -= ring;
+          fe[q] = E[(size_t)slot * N3 + m];
         }
       }
 #pragma unroll
@@ -215,31 +229,60 @@ __global__ __launch_bounds__(256) void k_scheme_energy(const T *__restrict__ u, 
                      dz = ((double)vzp[q] - vv) / a.hz;
         if (p < a.nac) {
           e += 0.5 * a.epn2 * (a.kkx * dx * dx + a.kky * dy * dy + a.kkz * dz * dz);
+          const double gxc = ((double)vxp[q] - (double)vxm[q]) / (2.0 * a.hx);
+          const double gyc = ((double)vyp[q] - (double)vym[q]) / (2.0 * a.hy);
+          const double gzc = ((double)vzp[q] - (double)vzm[q]) / (2.0 * a.hz);
           if (a.aniso) {
-            const double gx = ((double)vxp[q] - (double)vxm[q]) / (2.0 * a.hx);
-            const double gy = ((double)vym[q] - (double)vyp[q]) / (2.0 * a.hy);
-            const double gz = ((double)vzm[q] - (double)vzp[q]) / (2.0 * a.hz);
+            // the reference's F2 convention: gy, gz as (u[-1] - u[+1]) / 2h
+            const double gx = gxc, gy = -gyc, gz = -gzc;
             const double *L = lam + 9 * p;
             const double lx = L[0] * gx + L[1] * gy + L[2] * gz, ly = L[3] * gx + L[4] * gy + L[5] * gz,
                          lz = L[6] * gx + L[7] * gy + L[8] * gz;
-            e += 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
-            ea += fabs(0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz)) - 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+            const double t = 0.5 * a.epn2 * (gx * lx + gy * ly + gz * lz);
+            e += t;
+            ea += fabs(t) - t;
           }
           s2 += vv * vv;
           s3 += vv * vv * vv;
+          dmax = fmax(dmax, fabs(vv - (double)v0[q]));
+          cell = fmax(cell, fabs(vv));
+          gsum += sqrt(gxc * gxc + gyc * gyc + gzc * gzc);
+          if (with_el) {
+            const double t = vv * (double)fe[q];
+            el += t;
+            elabs += fabs(t);
+          }
         } else {
           e += 0.5 * a.epn2 * (dx * dx + dy * dy + dz * dz) + 0.25 * (vv * vv - 1.0) * (vv * vv - 1.0);
         }
       }
     }
-    // the gradient and CH terms are non-negative: their absolute sum is e
-    // itself (ea corrects an indefinite Lambda's term)
     ea += e + fabs(0.5 * a.a2 * s2) + fabs(a.a3 / 3.0 * s3) + 0.25 * fabs(a.a4) * s2 * s2;
     e += 0.5 * a.a2 * s2 - a.a3 / 3.0 * s3 + 0.25 * a.a4 * s2 * s2;
     acc += e;
     aabs += ea;
+    if (a.nac > 0) {
+      emax = fmax(emax, cell);
+      cnt += cell > 0.1 ? 1.0 : 0.0;
+    }
   }
-  block_sum2(acc, aabs, part);
+  double vals[8] = {acc, aabs, el, elabs, dmax, emax, cnt, gsum};
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    for (int o = 32; o > 0; o >>= 1) {
+      const double w = __shfl_down(vals[k], o, 64);
+      vals[k] = (k == 4 || k == 5) ? fmax(vals[k], w) : vals[k] + w;
+    }
+  __shared__ double sh[8][4];
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 8; k++) sh[k][threadIdx.x >> 6] = vals[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      part[blockIdx.x * RCOLS + k] = (k == 4 || k == 5) ? fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]))
+                                                        : ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
 }
 
 // max |u - b| per block into part[block * RCOLS] and, with extrapolate,
@@ -309,15 +352,20 @@ static int features_of(etd_ctx *c, const T *u, const T *u0, run_feat *f) {
 
 // The acceptance energy of the current fields, FP64.  kind 0: the
 // check_soln_new energy of the AC and CH fields (the reference's diagnostic);
-// kind 1: the scheme's discrete free energy (k_scheme_energy).  With
+// kind 1: the scheme's discrete free energy (k_accept_stats).  With
 // with_elastic, plus the elastic energy ES/2 h^3 sum_p eta_p felas_p from a
 // fresh elastic evaluation of these fields, which the next step reuses
 // (c->elastic_fresh) if they are kept.  *Eabs is the sum of the absolute
 // values of the energy's terms (kind 1) or |E| (kind 0).
 template <typename T>
-static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *Eabs) {
+static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *Eabs, const T *u0, run_feat *feat,
+                     int *have_feat) {
   double e = 0.0, ea = 0.0;
   int rc = ETD_OK;
+  *have_feat = 0;
+  const bool el_on = with_elastic && c->p.elastic == 1 && c->nac > 0;
+  const double w = 0.5 * c->p.elastic_scale * c->p.hx * c->p.hy * c->p.hz;
+  const etd_fields<T> F = fields<T>(c);
   if (kind == 0) {
     double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
     if (!st) return ETD_ERR_NOMEM;
@@ -329,6 +377,20 @@ static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *
     }
     free(st);
     if (rc) return rc;
+    if (el_on) {
+      // the evaluation packs interior cells only: the fields' ghost cells are
+      // left as the step wrote them (plain stepping refreshes them next step)
+      launch_elastic<T>(c);
+      const long long Lbase = (c->elastic_evals - 1) * (long long)c->nac;
+      k_elastic_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, F.E, c->n, c->g, c->nac, c->ring,
+                                                          (int)(Lbase % c->ring), c->ad_part);
+      if ((rc = fetch_part(c, 2))) return rc;
+      double s = 0.0;
+      for (int i = 0; i < RBLOCKS; i++) s += c->ad_hpart[i * RCOLS];
+      e += w * s;
+      ea = fabs(e);
+      c->elastic_fresh = 1;  // the ring's latest evaluation is of these fields
+    }
   } else {
     scheme_args a;
     a.n = c->n;
@@ -346,34 +408,34 @@ static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *
     a.a2 = c->p.a2;
     a.a3 = c->p.a3;
     a.a4 = c->p.a4;
-    k_scheme_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(fields<T>(c).u, c->lam, a, c->ad_part);
-    if ((rc = fetch_part(c, 2))) return rc;
-    double s = 0.0, sa = 0.0;
-    for (int i = 0; i < RBLOCKS; i++) {
-      s += c->ad_hpart[i * RCOLS];
-      sa += c->ad_hpart[i * RCOLS + 1];
+    int slot0 = 0;
+    if (el_on) {
+      launch_elastic<T>(c);
+      slot0 = (int)(((c->elastic_evals - 1) * (long long)c->nac) % c->ring);
     }
-    e = s * c->p.hx * c->p.hy * c->p.hz;
-    ea = sa * c->p.hx * c->p.hy * c->p.hz;
-  }
-  if (with_elastic && c->p.elastic == 1 && c->nac > 0) {
-    const etd_fields<T> F = fields<T>(c);
-    // the evaluation packs interior cells only: the fields' ghost cells are
-    // left as the step wrote them (plain stepping refreshes them next step)
-    launch_elastic<T>(c);
-    const long long Lbase = (c->elastic_evals - 1) * (long long)c->nac;
-    k_elastic_energy<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, F.E, c->n, c->g, c->nac, c->ring, (int)(Lbase % c->ring),
-                                                        c->ad_part);
-    if ((rc = fetch_part(c, 2))) return rc;
-    double s = 0.0, sa = 0.0;
+    k_accept_stats<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, u0, F.E, c->lam, a, el_on ? 1 : 0, c->ring, slot0,
+                                                      c->ad_part);
+    if ((rc = fetch_part(c, 8))) return rc;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < RBLOCKS; i++) {
-      s += c->ad_hpart[i * RCOLS];
-      sa += c->ad_hpart[i * RCOLS + 1];
+      const double *o = c->ad_hpart + i * RCOLS;
+      for (int k = 0; k < 8; k++) s[k] = (k == 4 || k == 5) ? fmax(s[k], o[k]) : s[k] + o[k];
     }
-    const double w = 0.5 * c->p.elastic_scale * c->p.hx * c->p.hy * c->p.hz;
-    e += w * s;
-    ea = kind == 0 ? fabs(e) : ea + fabs(w) * sa;
-    c->elastic_fresh = 1;  // the ring's latest evaluation is of these fields
+    const double h3 = c->p.hx * c->p.hy * c->p.hz;
+    e = s[0] * h3;
+    ea = s[1] * h3;
+    if (el_on) {
+      e += w * s[2];
+      ea += fabs(w) * s[3];
+      c->elastic_fresh = 1;  // the ring's latest evaluation is of these fields
+    }
+    if (feat && c->nac > 0) {
+      feat->dmax = s[4];
+      feat->emax = s[5];
+      feat->ifrac = s[6] / (double)c->N3;
+      feat->gmean = s[7] / ((double)c->N3 * c->nac);
+      *have_feat = 1;
+    }
   }
   *E = e;
   *Eabs = ea;
@@ -436,11 +498,12 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   // state of the run
   double E, Eabs;
   const int ekind = o->energy_kind;
-  int rc = energy_of<T>(c, ekind, with_el, &E, &Eabs);
+  run_feat rf;
+  int hf = 0;
+  int rc = energy_of<T>(c, ekind, with_el, &E, &Eabs, F.u, &rf, &hf);
   double Escale = fabs(E);
   if (rc) return rc;
-  run_feat rf;
-  if ((rc = features_of<T>(c, F.u, F.u, &rf))) return rc;
+  if (!hf && (rc = features_of<T>(c, F.u, F.u, &rf))) return rc;
   double hist[4] = {o->dt0, o->dt0, o->dt0, o->dt0};
   double err_last = 0.1 * o->tol, dEdt = 0.0, plain_next = o->dt0;
   int rej_since = 0, fallback = 0;
@@ -518,15 +581,16 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     if ((rc = fetch_part(c, 1))) break;
     for (int i = 0; i < RBLOCKS; i++) err = fmax(err, c->ad_hpart[i * RCOLS]);
     double Enew, Eabs_new;
-    if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new))) break;
+    run_feat nf;
+    int nhf = 0;
+    if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new, S0, &nf, &nhf))) break;
     const double scale = ekind == 0 ? fmax(Escale, fabs(E)) : fmax(Eabs, Eabs_new);
     const double fac = err > 0.0 ? o->safety * sqrt(o->tol / err) : o->grow;
     const int err_ok = err <= o->tol && isfinite(err);
     const int energy_ok = !o->energy_test || Enew <= E + slack * scale;
     const int forced = dt <= o->dt_min * (1.0 + 1e-12) && isfinite(err);
     if ((err_ok && energy_ok) || forced) {
-      run_feat nf;
-      if ((rc = features_of<T>(c, F.u, S0, &nf))) break;
+      if (!nhf && (rc = features_of<T>(c, F.u, S0, &nf))) break;
       c->t += dt;
       st->accepted++;
       if (log && log_rows && *log_rows < log_cap) {
