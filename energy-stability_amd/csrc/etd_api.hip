@@ -155,7 +155,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   if (spv && atoi(spv) == 0) c->split_persistent = 0;
   c->split_kernel = 0;
   const char *skv = getenv("ETD_SPLIT_KERNEL");
-  if (skv && atoi(skv) >= 1 && atoi(skv) <= 5) c->split_kernel = atoi(skv);
+  if (skv && atoi(skv) >= 1 && atoi(skv) <= 4) c->split_kernel = atoi(skv);
   HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t n3 = c->n3;
   c->mixed = p->precision == ETD_PREC_MIXED;

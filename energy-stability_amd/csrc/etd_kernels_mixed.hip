@@ -48,15 +48,6 @@ __device__ __forceinline__ void copy16_to_lds(i32x4m rs, uint32_t lds_wave_base,
                : "s"(b), "v"(voff), "s"(rs)
                : "memory", "m0");
 }
-// the same copy marked non-temporal: streamed field chunks, so that they do
-// not displace the V / V^-1 chunks every workgroup re-reads from L2
-__device__ __forceinline__ void copy16_to_lds_nt(i32x4m rs, uint32_t lds_wave_base, int voff) {
-  const uint32_t b = __builtin_amdgcn_readfirstlane(lds_wave_base);
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen nt lds"
-               :
-               : "s"(b), "v"(voff), "s"(rs)
-               : "memory", "m0");
-}
 #pragma clang diagnostic pop
 
 __device__ __forceinline__ void wait_all_copies_barrier() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" : : : "memory"); }
@@ -265,7 +256,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
 // (field, 32 WAVES columns); the double-buffered chunk pipeline runs across
 // tile boundaries, so the next tile's first chunk loads during the current
 // tile's last chunk and no workgroup start-up is exposed.
-template <int MT, int NT, int WAVES, int UT = 2, bool NTX = false>
+template <int MT, int NT, int WAVES, int UT = 2>
 __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     const uint16_t *__restrict__ A3, const float *__restrict__ in, size_t in_fs, float *__restrict__ out,
     size_t out_fs, int n, int rotate, int accumulate, int a_chunk, int nfields) {
@@ -303,10 +294,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
       const int k = 32 * kc + 8 * qe + 4 * (lane & 1);
       const bool ok = col < n2 && k < n;
       const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
-      if constexpr (NTX)
-        copy16_to_lds_nt(rsX, sbase + CF::ASZ + (qe * CF::COLS + 32 * cb) * 32, voff);
-      else
-        copy16_to_lds(rsX, sbase + CF::ASZ + (qe * CF::COLS + 32 * cb) * 32, voff);
+      copy16_to_lds(rsX, sbase + CF::ASZ + (qe * CF::COLS + 32 * cb) * 32, voff);
     }
   };
 
@@ -872,21 +860,6 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
         const long ntiles = ncb * nfields;
         const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
         k_modeprod_sr<MT, NT><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
-            A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
-        return;
-      }
-    }
-    if constexpr (W == 8) {
-      if (c->split_kernel == 5) {
-        static bool attr_nt = false;
-        if (!attr_nt) {
-          (void)hipFuncSetAttribute((const void *)k_modeprod_split_p<MT, NT, 8, 2, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
-          attr_nt = true;
-        }
-        const long ntiles = ncb * nfields;
-        const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
-        k_modeprod_split_p<MT, NT, 8, 2, true><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
             A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
         return;
       }
