@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--controller", choices=["learned", "plain"], default="learned")
     ap.add_argument("--weights", default=CONTROLLER)
+    ap.add_argument("--learned-safety", type=float, default=None,
+                    help="margin of the learned proposal below the error model's largest step (library default)")
     ap.add_argument("--quick", action="store_true", help="skip the accuracy runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -169,6 +171,8 @@ def main():
             mlp = CT.load(args.weights)
             g.set_controller(mlp)
         kw = dict(t_end=t_end, tol=args.tol, dt0=dt, dt_min=1e-5, dt_max=1.0, controller=ctrl)
+        if args.learned_safety is not None:
+            kw["learned_safety"] = args.learned_safety
         g.run_adaptive(**dict(kw, t_end=2 * dt))  # warmup: first launches, scratch allocation
         for v, f in enumerate(W.initial_fields(q)):
             g.set_field(v, f)
@@ -180,7 +184,11 @@ def main():
         out.update({"value": t_end / wall, "steps": st["accepted"], "warmup": 1,
                     "ms_per_step": wall * 1e3 / st["accepted"], "dtype": "f32",
                     "adaptive": dict(st, wall_s=wall, energy_monotone=bool((np.diff(E) <= 1e-7 * np.abs(E[:-1])).all()),
-                                     dt_first=float(log[0, 1]), dt_largest=float(log[:, 1].max()))})
+                                     dt_first=float(log[0, 1]), dt_largest=float(log[:, 1].max()),
+                                     options={k: getattr(pyetd.adapt_opts(**kw), k)
+                                              for k in ("tol", "safety", "learned_safety", "grow", "shrink",
+                                                        "extrapolate", "energy_test", "energy_elastic",
+                                                        "energy_kind")})})
         headline_fields = [g.get_field(v) for v in range(V)]
         g.close()
         if mlp is not None:

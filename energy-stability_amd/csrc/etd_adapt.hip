@@ -454,6 +454,7 @@ extern "C" void etd_adapt_defaults(etd_adapt_opts *o) {
   o->dt_max = 1.0;
   o->tol = 1e-4;
   o->safety = 0.9;
+  o->learned_safety = 0.97;
   o->grow = 2.0;
   o->shrink = 0.2;
   o->controller = ETD_CTRL_PLAIN;
@@ -536,7 +537,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
           // no more than a decade above the plain proposal; never below the
           // smaller of it and the last step, so a network that errs low
           // cannot shrink the step where the plain rule would not
-          dt = clampd(o->safety * plain_next * pow(10.0, y), fmin(plain_next, hist[0]), 10.0 * plain_next);
+          // y predicts log10(dt_best / dt_plain), dt_best the error model's
+          // largest step; its accuracy allows a smaller margin than the
+          // plain rule's
+          dt = clampd(o->learned_safety * plain_next * pow(10.0, y), fmin(plain_next, hist[0]), 10.0 * plain_next);
           st->learned_proposals++;
         } else {
           st->learned_fallbacks++;
@@ -642,7 +646,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
 extern "C" int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st, double *log, int log_cap,
                                 int *log_rows) {
   if (!c || !o || !st) return ETD_ERR_ARG;
-  if (!(o->tol > 0.0) || !(o->dt0 > 0.0) || !(o->dt_min > 0.0) || !(o->dt_max >= o->dt_min) || !(o->safety > 0.0) ||
+  if (!(o->tol > 0.0) || !(o->dt0 > 0.0) || !(o->dt_min > 0.0) || !(o->dt_max >= o->dt_min) || !(o->safety > 0.0) || !(o->learned_safety > 0.0) ||
       !(o->grow >= 1.0) || !(o->shrink > 0.0 && o->shrink <= 1.0))
     return ETD_ERR_ARG;
   if (o->controller == ETD_CTRL_LEARNED && !c->mlp) {

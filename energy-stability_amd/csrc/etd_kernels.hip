@@ -1427,10 +1427,12 @@ __device__ void b_geometry(int sx, int sy, int sz, const etd_dev_consts *__restr
     nv[2] = 0.0;
   } else {
     const double n1 = sx, n2 = sy, n3 = sz;
-    const double rr = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-    nv[0] = n1 / rr;
-    nv[1] = n2 / rr;
-    nv[2] = n3 / rr;
+    // one division per point (a sign flip of n flips nv exactly, so the
+    // geometry at -k stays the exact mirror of the one at k)
+    const double ir = 1.0 / sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+    nv[0] = n1 * ir;
+    nv[1] = n2 * ir;
+    nv[2] = n3 * ir;
   }
   double oi[9];
   for (int i = 0; i < 3; ++i)
@@ -1444,15 +1446,16 @@ __device__ void b_geometry(int sx, int sy, int sz, const etd_dev_consts *__restr
   nr += oi[0] * (oi[4] * oi[8] - oi[5] * oi[7]);
   nr -= oi[1] * (oi[3] * oi[8] - oi[5] * oi[6]);
   nr += oi[2] * (oi[3] * oi[7] - oi[4] * oi[6]);
-  om[0] = (oi[4] * oi[8] - oi[5] * oi[7]) / nr;
-  om[1] = -(oi[3] * oi[8] - oi[6] * oi[5]) / nr;
-  om[2] = (oi[3] * oi[7] - oi[6] * oi[4]) / nr;
-  om[3] = -(oi[1] * oi[8] - oi[7] * oi[2]) / nr;
-  om[4] = (oi[0] * oi[8] - oi[2] * oi[6]) / nr;
-  om[5] = -(oi[0] * oi[7] - oi[6] * oi[1]) / nr;
-  om[6] = (oi[1] * oi[5] - oi[4] * oi[2]) / nr;
-  om[7] = -(oi[0] * oi[5] - oi[3] * oi[2]) / nr;
-  om[8] = (oi[0] * oi[4] - oi[1] * oi[3]) / nr;
+  const double inr = 1.0 / nr;
+  om[0] = (oi[4] * oi[8] - oi[5] * oi[7]) * inr;
+  om[1] = -(oi[3] * oi[8] - oi[6] * oi[5]) * inr;
+  om[2] = (oi[3] * oi[7] - oi[6] * oi[4]) * inr;
+  om[3] = -(oi[1] * oi[8] - oi[7] * oi[2]) * inr;
+  om[4] = (oi[0] * oi[8] - oi[2] * oi[6]) * inr;
+  om[5] = -(oi[0] * oi[7] - oi[6] * oi[1]) * inr;
+  om[6] = (oi[1] * oi[5] - oi[4] * oi[2]) * inr;
+  om[7] = -(oi[0] * oi[5] - oi[3] * oi[2]) * inr;
+  om[8] = (oi[0] * oi[4] - oi[1] * oi[3]) * inr;
 }
 
 // w = (sigma_p^T n) Omega for variant p

@@ -197,10 +197,12 @@ int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launc
  * energy ES/2 h^3 sum_p eta_p felas_p (one more elastic evaluation per
  * attempt, reused by the next step when accepted).
  * Slack: 1e-12 (FP64 path) or 1e-7 (mixed path, the FP32 rounding level)
- * times the largest |E| of the run.  Rejected
+ * times the energy's scale: the larger sum of the absolute values of its
+ * terms over the two states (scheme energy), or the largest |E| of the run
+ * (check_soln_new energy).  Rejected
  * attempts restore u_n.  The plain controller proposes
  * dt * clamp(safety (tol/err)^(1/2), shrink, grow); the learned controller
- * proposes safety * dt_plain * 10^y, where y = MLP(features) over
+ * proposes learned_safety * dt_plain * 10^y, where y = MLP(features) over
  * ETD_NFEAT features of the run (below) predicts log10 of the error model's
  * largest step over the plain proposal dt_plain, bounded to
  * [min(dt_plain, last dt), 10 dt_plain] (so the learned step is not held to
@@ -216,6 +218,7 @@ typedef struct etd_adapt_opts {
   double dt_min, dt_max;
   double tol;          /* bound on the step-doubling estimate */
   double safety, grow, shrink;
+  double learned_safety; /* the learned proposal's margin below the error model's largest step (0.97) */
   int controller;      /* ETD_CTRL_PLAIN or ETD_CTRL_LEARNED (needs etd_set_controller) */
   int extrapolate;     /* 1: accept 2 u_half - u_full (local extrapolation) */
   int energy_test;     /* 1: reject an energy increase */
