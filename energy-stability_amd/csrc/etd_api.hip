@@ -93,9 +93,9 @@ static int validate(etd_ctx *c, const etd_params *p) {
   if (p->nghost < 1 || p->nx < 2 * p->nghost + 2) return seterr(c, ETD_ERR_ARG, "nghost / nx");
   if (p->nx > 16 * 17) return seterr(c, ETD_ERR_UNSUPPORTED, "n > 272");
   if (p->elastic == 1 && p->nac > 16) return seterr(c, ETD_ERR_UNSUPPORTED, "elasticity supports nac <= 16");
-  if (p->precision != ETD_PREC_FP64 && p->precision != ETD_PREC_MIXED)
-    return seterr(c, ETD_ERR_ARG, "precision %d is neither FP64 (0) nor mixed (1)", p->precision);
-  if (p->precision == ETD_PREC_MIXED && p->nx % 4)
+  if (p->precision != ETD_PREC_FP64 && p->precision != ETD_PREC_MIXED && p->precision != ETD_PREC_MIXED_F16)
+    return seterr(c, ETD_ERR_ARG, "precision %d is not FP64 (0), mixed BF16 (1) or mixed FP16 (2)", p->precision);
+  if (p->precision != ETD_PREC_FP64 && p->nx % 4)
     return seterr(c, ETD_ERR_UNSUPPORTED, "the mixed-precision mode products need nx %% 4 == 0 (nx = %d)", p->nx);
   return ETD_OK;
 }
@@ -158,10 +158,12 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   if (skv && atoi(skv) >= 1 && atoi(skv) <= 4) c->split_kernel = atoi(skv);
   HCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t n3 = c->n3;
-  c->mixed = p->precision == ETD_PREC_MIXED;
+  c->mixed = p->precision == ETD_PREC_MIXED || p->precision == ETD_PREC_MIXED_F16;
+  c->split_f16 = p->precision == ETD_PREC_MIXED_F16;
   c->split_terms = 3;
   const char *sv = getenv("ETD_SPLIT");
   if (sv && atoi(sv) == 2) c->split_terms = 2;
+  if (c->split_f16) c->split_terms = 2;
   if (!c->mixed) {
     HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
     c->N1 = c->u + n3 * V;
@@ -196,10 +198,18 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     uint16_t *h3 = (uint16_t *)malloc(sb);
     HCHK(c, hipMalloc(&c->V3, sb));
     HCHK(c, hipMalloc(&c->Vi3, sb));
-    arrange_split_A(n, c->hVi, h3, c->split_terms);
-    HCHK(c, hipMemcpy(c->Vi3, h3, sb, hipMemcpyHostToDevice));
-    arrange_split_A(n, c->hV, h3, c->split_terms);
-    HCHK(c, hipMemcpy(c->V3, h3, sb, hipMemcpyHostToDevice));
+    if (c->split_f16) {
+      c->Vi3_inv = arrange_split_A_f16(n, c->hVi, h3);
+      HCHK(c, hipMemcpy(c->Vi3, h3, sb, hipMemcpyHostToDevice));
+      c->V3_inv = arrange_split_A_f16(n, c->hV, h3);
+      HCHK(c, hipMemcpy(c->V3, h3, sb, hipMemcpyHostToDevice));
+    } else {
+      c->V3_inv = c->Vi3_inv = 1.0f;
+      arrange_split_A(n, c->hVi, h3, c->split_terms);
+      HCHK(c, hipMemcpy(c->Vi3, h3, sb, hipMemcpyHostToDevice));
+      arrange_split_A(n, c->hV, h3, c->split_terms);
+      HCHK(c, hipMemcpy(c->V3, h3, sb, hipMemcpyHostToDevice));
+    }
     free(h3);
   }
   // D / h^2 per axis (read_matrices, vars.c:512,544,576)
@@ -448,9 +458,10 @@ void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumul
     launch_modeprod(c, A, F.w2, n3, out, n3, nf, 1, accumulate);
   } else {
     const uint16_t *A = forward ? c->Vi3 : c->V3;
-    launch_modeprod_split(c, A, in, n3, F.w1, n3, nf, 1, 0);
-    launch_modeprod_split(c, A, F.w1, n3, F.w2, n3, nf, 1, 0);
-    launch_modeprod_split(c, A, F.w2, n3, out, n3, nf, 1, accumulate);
+    const float ai = forward ? c->Vi3_inv : c->V3_inv;
+    launch_modeprod_split(c, A, in, n3, F.w1, n3, nf, 1, 0, ai);
+    launch_modeprod_split(c, A, F.w1, n3, F.w2, n3, nf, 1, 0, ai);
+    launch_modeprod_split(c, A, F.w2, n3, out, n3, nf, 1, accumulate, ai);
   }
 }
 
@@ -631,7 +642,11 @@ extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *
     const size_t sb = split_A_bytes(n, c->split_terms);
     uint16_t *h3 = (uint16_t *)malloc(sb);
     if (!h3) return ETD_ERR_NOMEM;
-    arrange_split_A(n, A, h3, c->split_terms);
+    float ai = 1.0f;
+    if (c->split_f16)
+      ai = arrange_split_A_f16(n, A, h3);
+    else
+      arrange_split_A(n, A, h3, c->split_terms);
     uint16_t *dA;
     float *din, *dout;
     HCHK(c, hipMalloc(&dA, sb));
@@ -641,7 +656,7 @@ extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *
     free(h3);
     int rc = put_f32(c, din, in, c->n3);
     if (!rc) {
-      launch_modeprod_split(c, dA, din, c->n3, dout, c->n3, 1, rotate, 0);
+      launch_modeprod_split(c, dA, din, c->n3, dout, c->n3, 1, rotate, 0, ai);
       HCHK(c, hipGetLastError());
       rc = get_f32(c, out, dout, c->n3);
     }

@@ -81,6 +81,8 @@ struct etd_ctx {
   int split_terms;
   etd_fields<float> f32;
   uint16_t *V3, *Vi3;
+  int split_f16;           /* ETD_PREC_MIXED_F16: V3 / Vi3 hold two FP16 terms of V / V^-1 scaled by 1 / *_inv */
+  float V3_inv, Vi3_inv;   /* the power of two that undoes that scaling (1 for BF16) */
   /* simulated time; adaptive stepping scratch (allocated on first use):
    * saved state, full-step result, phi-scaled spectrum (V x n3 each, in the
    * storage type), reduction partials */
@@ -187,8 +189,11 @@ void arrange_A(int n, const double *Acm, double *A4host, int MT, int nkc);
 /* mixed precision mode products: FP32 in / out, A split into `terms` BF16
  * terms, FP32 accumulation (etd_kernels_mixed.hip) */
 void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
-                           int nfields, int rotate, int accumulate);
+                           int nfields, int rotate, int accumulate, float a_inv = 1.0f);
 size_t split_A_bytes(int n, int terms);
 void arrange_split_A(int n, const double *Acm, uint16_t *A3host, int terms);
+/* FP16 split-2 arrangement of A / s (s a power of two bringing max |A| to
+ * [2^13, 2^14)); returns 1 / s. */
+float arrange_split_A_f16(int n, const double *Acm, uint16_t *A2host);
 
 #endif

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <math.h>
 
 #include "etd_internal.h"
 
@@ -617,6 +618,141 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_split_rc(const uint16_t *__
   }
 }
 
+// FP16 split-2 (persistent, the default kernel's tiling and staging): A is
+// stored as two FP16 terms of A / s, s a power of two (a_inv = 1 / s); each
+// lane's column chunk of 32 field values (its 8 and those of the three lanes
+// with the same column) is scaled by 2^(14 - e), e the exponent of its largest
+// magnitude, and split into x0 = fp16(xs), x1 = fp16(xs - x0).  Three
+// v_mfma_f32_16x16x32_f16 (a1 x0 + a0 x1 + a0 x0) per tile and chunk go into a
+// chunk-local FP32 tile, added to the accumulator times 2^(e - 14) / s (exact
+// powers of two).  Values below 2^-14 of their chunk's scale lose relative
+// precision, which is below FP32 rounding of the chunk's sum.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <int MT>
+__global__ __launch_bounds__(512, 2) void k_modeprod_f16_p(const uint16_t *__restrict__ A2, const float *__restrict__ in,
+                                                           size_t in_fs, float *__restrict__ out, size_t out_fs, int n,
+                                                           int rotate, int accumulate, int a_chunk, int nfields,
+                                                           float a_inv) {
+  constexpr int NT = 2;
+  using CF = mps_cfg<MT, NT, 8, 2>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;  // uniform
+  const i32x4m rsA = mk_rsrc(A2, (uint32_t)((size_t)nkc * a_chunk));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue = [&](int tl, int kc, int stg) {
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    i32x4m rsX = mk_rsrc(in + in_fs * f, (uint32_t)((size_t)n * n2 * 4));
+    rsX.x = __builtin_amdgcn_readfirstlane(rsX.x);
+    rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
+    rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
+    const uint32_t sbase = lds0 + stg * CF::STAGE;
+#pragma unroll
+    for (int e = 0; e < CF::APT; e++)
+      copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      const long col = wgc + 32 * wave + (lane >> 1);
+      const int k = 32 * kc + 8 * e + 4 * (lane & 1);
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + CF::ASZ + (e * CF::THREADS + wave * 64) * 16, voff);
+    }
+  };
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue(tile, 0, 0);
+  int g = 0;
+#pragma unroll 1
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int f = tile / ncb;
+    const long wgcol = (long)(tile % ncb) * CF::COLS;
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int kc = 0; kc < nkc; kc++, g++) {
+      wait_all_copies_barrier();
+      if (kc + 1 < nkc)
+        issue(tile, kc + 1, (g + 1) & 1);
+      else if (tile + (int)gridDim.x < ntiles)
+        issue(tile + gridDim.x, 0, (g + 1) & 1);
+      const char *stg = smem + (g & 1) * CF::STAGE;
+      const f16x8 *sa = reinterpret_cast<const f16x8 *>(stg);
+      const float *sx = reinterpret_cast<const float *>(stg + CF::ASZ);
+      f16x8 b[2][2];
+      float back[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+        const float4 lo = *reinterpret_cast<const float4 *>(bp);
+        const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+        const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        float m = 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; e++) m = fmaxf(m, fabsf(x[e]));
+        // the column's 32 values of this chunk sit in lanes r16 + 16 q
+        m = fmaxf(m, __shfl_xor(m, 16, 64));
+        m = fmaxf(m, __shfl_xor(m, 32, 64));
+        const int ex = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+        const float sc = __builtin_ldexpf(1.0f, 14 - ex);
+        back[u] = __builtin_ldexpf(a_inv, ex - 14);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float xs = x[e] * sc;
+          const _Float16 h0 = (_Float16)xs;
+          b[u][0][e] = h0;
+          b[u][1][e] = (_Float16)(xs - (float)h0);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MT; t += 2) {
+        f16x8 a[2][2];
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+          if (t + d < MT)
+#pragma unroll
+            for (int tm = 0; tm < 2; tm++) a[d][tm] = sa[(tm * MT + t + d) * 64 + lane];
+        f32x4 dch[2][2];
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][1], b[u][0], (f32x4){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][1], dch[d][u], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][0], dch[d][u], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT)
+#pragma unroll
+              for (int i = 0; i < 4; i++) acc[t + d][u][i] = fmaf(dch[d][u][i], back[u], acc[t + d][u][i]);
+      }
+    }
+    store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+  }
+}
+
 // Register-prefetch variant (persistent): the field operand goes straight
 // from HBM into registers with compiler-visible buffer loads issued two K
 // chunks ahead, so HBM latency is covered by two chunks of MFMA work; only
@@ -829,6 +965,54 @@ void arrange_split_A(int n, const double *Acm, uint16_t *A3, int terms) {
   }
 }
 
+static uint16_t f16_bits(double v) {
+  const _Float16 h = (_Float16)(float)v;
+  uint16_t b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+
+float arrange_split_A_f16(int n, const double *Acm, uint16_t *A2) {
+  const int MT = (n + 15) / 16, nkc = (n + 31) / 32;
+  const size_t cb = chunk_bytes_rt(MT, 2);
+  double amax = 0.0;
+  for (size_t i = 0; i < (size_t)n * n; i++) amax = fmax(amax, fabs(Acm[i]));
+  int ex = 0;
+  if (amax > 0.0) (void)frexp(amax, &ex);       // amax in [2^(ex-1), 2^ex)
+  const double s = ldexp(1.0, 14 - ex);         // max |A s| < 2^14
+  memset(A2, 0, cb * nkc);
+  for (int kc = 0; kc < nkc; kc++) {
+    uint16_t *ch = A2 + (size_t)kc * cb / 2;
+    for (int t = 0; t < MT; t++)
+      for (int l = 0; l < 64; l++)
+        for (int e = 0; e < 8; e++) {
+          const int m = 16 * t + (l & 15), k = 32 * kc + 8 * (l >> 4) + e;
+          const double a = (m < n && k < n) ? Acm[m + (size_t)n * k] * s : 0.0;
+          const _Float16 h0 = (_Float16)(float)a;
+          ch[(((size_t)0 * MT + t) * 64 + l) * 8 + e] = f16_bits((double)h0);
+          ch[(((size_t)1 * MT + t) * 64 + l) * 8 + e] = f16_bits(a - (double)h0);
+        }
+  }
+  return (float)(1.0 / s);
+}
+
+template <int MT>
+static void f16_launch(etd_ctx *c, const uint16_t *A2, const float *in, size_t in_fs, float *out, size_t out_fs,
+                       int nfields, int rotate, int accumulate, float a_inv) {
+  using CF = mps_cfg<MT, 2, 8, 2>;
+  static_assert(CF::LDS <= 160 * 1024, "LDS budget");
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k_modeprod_f16_p<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
+    attr = true;
+  }
+  const long ncb = ((long)c->n * c->n + CF::COLS - 1) / CF::COLS;
+  const long ntiles = ncb * nfields;
+  const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
+  k_modeprod_f16_p<MT><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+      A2, in, in_fs, out, out_fs, c->n, rotate, accumulate, (int)chunk_bytes<MT, 2>(), nfields, a_inv);
+}
+
 template <int MT, int NT>
 static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
                        int nfields, int rotate, int accumulate) {
@@ -941,9 +1125,22 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
 }
 
 void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
-                           int nfields, int rotate, int accumulate) {
+                           int nfields, int rotate, int accumulate, float a_inv) {
   const int MT = (c->n + 15) / 16;
   etd_prof_begin(c, PC_MODEPROD);
+  if (c->split_f16) {
+    switch (MT) {
+#define MF(X)                                                                             \
+  case X:                                                                                 \
+    f16_launch<X>(c, A3, in, in_fs, out, out_fs, nfields, rotate, accumulate, a_inv);     \
+    break;
+      MF(1) MF(2) MF(3) MF(4) MF(5) MF(6) MF(7) MF(8) MF(9) MF(10) MF(11) MF(12) MF(13) MF(14) MF(15) MF(16) MF(17)
+#undef MF
+      default: break;
+    }
+    etd_prof_end(c, PC_MODEPROD);
+    return;
+  }
   switch (MT) {
 #define MC(X)                                                                                      \
   case X:                                                                                          \

@@ -38,8 +38,12 @@ enum {
   ETD_ERR_NOMEM = -5
 };
 
-/* precision of the step */
-enum { ETD_PREC_FP64 = 0, ETD_PREC_MIXED = 1 };
+/* precision of the step: FP64 throughout; or FP32 fields with the mode
+ * products as MFMA products of split operands accumulated in FP32 and the
+ * energy in FP64 -- BF16 split-3 (six BF16 products, 2^-24 per term) or FP16
+ * split-2 (three FP16 products after power-of-two scaling of V / V^-1 and of
+ * every 32-long column chunk of the field, 2^-22 per term) */
+enum { ETD_PREC_FP64 = 0, ETD_PREC_MIXED = 1, ETD_PREC_MIXED_F16 = 2 };
 
 /* Everything the reference reads from input<s>.ini (ScLETD.c:18-34),
  * elastic_input.txt (elastic.c:375-418), anisotropic_input.txt

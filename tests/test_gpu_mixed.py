@@ -30,10 +30,10 @@ def mixed(prm):
     return q
 
 
-def pair(prm, fields):
+def pair(prm, fields, precision=1):
     o = O.Oracle(prm)
     Vm, Vim, D = O.eigbasis(prm["n"], 3)
-    g = pyetd.Etd(pyetd.params_from_dict(mixed(prm)), Vm, Vim, D)
+    g = pyetd.Etd(pyetd.params_from_dict(dict(prm, precision=precision)), Vm, Vim, D)
     for v, f in enumerate(fields):
         o.set_field(v, f)
         g.set_field(v, f)
@@ -157,3 +157,49 @@ def test_mixed_path_conserves_the_ch_mean(etd2):
     say(f"  mean {m0:.12f} -> {m1:.12f} (diff {m1 - m0:.2e})")
     assert abs(m1 - m0) < 2e-7 * abs(m0)
     g.close()
+
+
+@pytest.mark.parametrize("n", [12, 36, 68, 100])
+@pytest.mark.parametrize("rotate", [True, False])
+def test_f16_split2_modeprod_matches_oracle(n, rotate):
+    """FP16 split-2 mode products (ETD_PREC_MIXED_F16): power-of-two scaled V
+    and field column chunks, three FP16 MFMA products, FP32 accumulation; also
+    a field whose values span 12 decades (the per-chunk scaling keeps the small
+    columns' relative accuracy)."""
+    say(f"f16 split-2 modeprod n={n} rotate={rotate}")
+    rng = np.random.default_rng(n + 17)
+    Vm, Vim, D = O.eigbasis(n, 3)
+    g = pyetd.Etd.from_dict(dict(small_params(n=n, nac=1), precision=2))
+    for A in (Vim, Vm):
+        u = rng.standard_normal(n ** 3)
+        got = g.modeprod(A, u, rotate)
+        ref = O.modeprod(A, u, rotate)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        say(f"  rel err {err:.2e}")
+        assert err < 2e-6
+    # columns (lines along x) scaled over 12 decades: each column's own error
+    # stays relative to that column
+    scale = 10.0 ** rng.uniform(-12, 0, size=(n, n, 1))
+    u = (rng.standard_normal((n, n, n)) * scale).ravel()
+    got = g.modeprod(Vim, u, False)
+    ref = O.modeprod(Vim, u, False)
+    colmax = np.abs(ref.reshape(n, n, n)).max(axis=2, keepdims=True)
+    err = (np.abs(got - ref).reshape(n, n, n) / colmax).max()
+    say(f"  per-column rel err over 12 decades {err:.2e}")
+    assert err < 2e-6
+    g.close()
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: "n{n}_ac{nac}_ch{nch}_etd2{etd2}_el{elastic}_an{aniso}".format(**c))
+def test_f16_mixed_step_matches_oracle(case):
+    say(f"f16 mixed step {case}")
+    c = dict(case)
+    steps = c.pop("steps")
+    prm = small_params(dt=0.02, **c)
+    o, g = pair(prm, noise_fields(prm, amp=0.3), precision=2)
+    for _ in range(steps):
+        o.step()
+        g.step()
+    worst = max(np.abs(g.get_field(v) - o.get_field(v)).max() for v in range(prm["nac"] + prm["nch"]))
+    say(f"  max |d| = {worst:.3e}")
+    assert worst < 1e-5
