@@ -7,8 +7,9 @@ Workload: 192^3 interior (n = 196) x (12 AC + 1 CH) beta -> alpha fields,
 cubic elasticity, ETD1, dt = 0.005, the 1000-step horizon t = 5.  Inputs are
 resident in HBM before every timed region.
 
-- value: the headline path, mixed precision (FP32 fields, split-BF16 MFMA
-  mode products, FP64 energy) with adaptive steps proposed by the learned
+- value: the headline path, mixed precision (FP32 fields, MFMA mode products
+  as FP16 split-2 by default or BF16 split-3 with --precision 1, FP32
+  accumulation, FP64 energy) with adaptive steps proposed by the learned
   controller (step doubling, tolerance 1e-4, energy-decrease acceptance),
   integrating the whole horizon t = 0 -> 5: value = 5 / wall seconds.
 - plain_fp64: the drop-in path (FP64, fixed dt = 0.005): K timed steps after
@@ -34,7 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "energy-stability_amd"))
 
 METRIC = "sim-time/s at 192^3x12 vars: mixed-prec+learned dt vs plain FP64 at fixed err"
 PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
-PEAK_BF16_MFMA = 2.5e15   # dense BF16 matrix
+PEAK_BF16_MFMA = 2.5e15   # dense BF16 / FP16 matrix
 PEAK_HBM = 8.0e12         # HBM3E, B/s
 CONTROLLER = os.path.join(ROOT, "energy-stability_amd", "controller", "mlp.npz")
 PROFILES = os.path.join(ROOT, "profiles")
@@ -68,6 +69,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--controller", choices=["learned", "plain"], default="learned")
     ap.add_argument("--weights", default=CONTROLLER)
+    ap.add_argument("--precision", type=int, choices=[1, 2], default=2,
+                    help="mixed-precision mode products: 1 BF16 split-3, 2 FP16 split-2")
     ap.add_argument("--learned-safety", type=float, default=None,
                     help="margin of the learned proposal below the error model's largest step (library default)")
     ap.add_argument("--quick", action="store_true", help="skip the accuracy runs")
@@ -121,7 +124,7 @@ def main():
     # ---- mixed precision, fixed dt: per-kernel timing of the headline path's kernels
     mx = None
     if args.only in ("all", "mixed-fixed"):
-        q = dict(prm, precision=1)
+        q = dict(prm, precision=args.precision)
         g = fresh(pyetd, W, q)
         if args.calib_copy_mib and args.only == "mixed-fixed":
             g.calib_copy(args.calib_copy_mib << 20)
@@ -135,31 +138,34 @@ def main():
               "breakdown_ms_per_step": {"mode_products": ms[1] / K, "elastic_fft": ms[2] / K, "fft_passes": ms[7] / K,
                                         "nonlinear": ms[3] / K, "phi_scaling": ms[4] / K, "energy": ms[5] / K,
                                         "halo": ms[6] / K}}
-        # the two largest kernels of the mixed step: FFT axis passes and split-BF16 mode products
+        # the two largest kernels of the mixed step: FFT axis passes and split mode products
         fft_launch_ms = ms[7] / max(1, launches[6]) / 3.0  # 3 axis passes per fft3d group
         npair = (nac + 1) // 2
         fft_bytes = 2 * npair * N ** 3 * 8.0  # read + write one float2 per point and packed field
         mp_launch_ms = ms[1] / max(1, launches[0])
         mp_fields_avg = (3 * 2 * V + 3 * V) / 6.0
         mp_bytes = 2 * 4.0 * n ** 3 * mp_fields_avg  # read + write FP32 per field
-        mp_mfma = 6 * 2.0 * n ** 4 * mp_fields_avg  # six BF16 products per split-3 product
+        nterm = 6 if args.precision == 1 else 3  # BF16 split-3: six products; FP16 split-2: three
+        mp_mfma = nterm * 2.0 * n ** 4 * mp_fields_avg
         r_fft = {"kernel": "k_fft16<12> (own two-stage register FFT pass, FP32 in HBM)", "bound": "hbm",
                  "achieved": fft_bytes / (fft_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                  "frac": fft_bytes / (fft_launch_ms / 1e3) / PEAK_HBM,
                  "traffic": measured_traffic("k_fft16_f32_192cube_hbm_counters.md"), "launch_ms": fft_launch_ms,
                  "ms_per_step": ms[7] / K}
-        r_mp = {"kernel": "k_modeprod_split (split-BF16 x3 MFMA, FP32 accumulate)", "bound": "hbm",
+        r_mp = {"kernel": ("k_modeprod_split_p (split-BF16 x3 MFMA, FP32 accumulate)" if args.precision == 1 else
+                           "k_modeprod_f16_p (split-FP16 x2 MFMA, FP32 accumulate)"), "bound": "hbm",
                 "achieved": mp_bytes / (mp_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": mp_bytes / (mp_launch_ms / 1e3) / PEAK_HBM,
-                "traffic": measured_traffic("k_modeprod_split_192cube_hbm_counters.md"), "launch_ms": mp_launch_ms,
+                "traffic": measured_traffic("k_modeprod_split_192cube_hbm_counters.md" if args.precision == 1 else
+                                            "k_modeprod_f16_192cube_hbm_counters.md"), "launch_ms": mp_launch_ms,
                 "ms_per_step": ms[1] / K, "mfma_tflops": mp_mfma / (mp_launch_ms / 1e3) / 1e12,
-                "mfma_frac_bf16": mp_mfma / (mp_launch_ms / 1e3) / PEAK_BF16_MFMA}
+                "mfma_frac_16bit": mp_mfma / (mp_launch_ms / 1e3) / PEAK_BF16_MFMA}
         mx["rooflines"] = [r_fft, r_mp]
         out["roofline"] = dict(r_fft if r_fft["ms_per_step"] >= r_mp["ms_per_step"] else r_mp)
 
     # ---- headline: mixed precision + learned dt over the horizon
     if args.only in ("all", "headline"):
-        q = dict(prm, precision=1)
+        q = dict(prm, precision=args.precision)
         g = fresh(pyetd, W, q)
         mlp = None
         ctrl = args.controller
@@ -210,15 +216,17 @@ def main():
         out["config"] = {"workload": f"{prm['name']}: {N}^3 interior (n={n}), {nac} AC + {nch} CH fields, cubic "
                                      f"elasticity, ETD1, horizon t = {t_end:g} ({args.horizon_steps} steps of dt={dt})",
                          "grid": N, "extended_edge": n, "fields": V, "t_end": t_end,
-                         "precision": "mixed: FP32 fields, BF16 split-3 MFMA mode products with FP32 accumulation, "
-                                      "FP64 energy and acceptance",
+                         "precision": ("mixed: FP32 fields, " + ("BF16 split-3" if args.precision == 1 else
+                                                                  "FP16 split-2 (power-of-two scaled)") +
+                                       " MFMA mode products with FP32 accumulation, FP64 energy and acceptance"),
                          "dt_control": f"adaptive, step doubling, tol {args.tol:g}, {ctrl} controller"}
         if fp is not None:
             out["speedup_vs_plain_fp64"] = out["value"] / fp["sim_time_per_s"]
         # ---- fixed-error check against the plain FP64 path, and the mixed fixed-dt report
         if not args.quick:
             a = fresh(pyetd, W, prm)
-            b = fresh(pyetd, W, dict(prm, precision=1))
+            b = fresh(pyetd, W, dict(prm, precision=args.precision))
+            e_init = float(a.diag()[0][0, 3])
             rows = []
             for k in range(0, args.horizon_steps, 100):
                 m = min(100, args.horizon_steps - k)
@@ -234,7 +242,7 @@ def main():
             sa, _ = a.diag()
             a.close()
             b.close()
-            g2 = fresh(pyetd, W, dict(prm, precision=1))
+            g2 = fresh(pyetd, W, dict(prm, precision=args.precision))
             for v, f in enumerate(headline_fields):
                 g2.set_field(v, f)
             sh, _ = g2.diag()
@@ -244,6 +252,8 @@ def main():
                 "max_deta": max(float(np.abs(headline_fields[v] - ref[v]).max()) for v in range(nac)),
                 "max_dc": max([float(np.abs(headline_fields[v] - ref[v]).max()) for v in range(nac, V)] or [0.0]),
                 "rel_energy": abs(sh[0, 3] - sa[0, 3]) / abs(sa[0, 3]),
+                "energy_fp64": float(sa[0, 3]), "energy_headline": float(sh[0, 3]), "energy_initial": e_init,
+                "max_abs_eta_fp64": max(float(np.abs(ref[v]).max()) for v in range(nac)),
                 "mixed_fixed_dt_every_100_steps": rows}
 
     if fp is not None:

@@ -49,7 +49,7 @@ __device__ __forceinline__ cell_nb neighbours(int n, int g, int N, int x, int y,
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const T *__restrict__ u0, int n, int g,
-                                                  int nac, double hx, double hy, double hz,
+                                                  int nac, double ihx, double ihy, double ihz,
                                                   double *__restrict__ part) {
   const int N = n - 2 * g;
   const size_t n3 = (size_t)n * n * n;
@@ -78,10 +78,10 @@ __global__ __launch_bounds__(256) void k_features(const T *__restrict__ u, const
         const double vv = v[e];
         dmax = fmax(dmax, fabs(vv - (double)v0[e]));
         cell = fmax(cell, fabs(vv));
-        const double gx = ((double)vxp[e] - (double)vxm[e]) / (2.0 * hx);
-        const double gy = ((double)vyp[e] - (double)vym[e]) / (2.0 * hy);
-        const double gz = ((double)vzp[e] - (double)vzm[e]) / (2.0 * hz);
-        gsum += sqrt(gx * gx + gy * gy + gz * gz);
+        const double gx = ((double)vxp[e] - (double)vxm[e]) * 0.5 * ihx;
+        const double gy = ((double)vyp[e] - (double)vym[e]) * 0.5 * ihy;
+        const double gz = ((double)vzp[e] - (double)vzm[e]) * 0.5 * ihz;
+        gsum += (double)sqrtf((float)(gx * gx + gy * gy + gz * gz));  // a feature: FP32 suffices
       }
     }
     emax = fmax(emax, cell);
@@ -176,7 +176,8 @@ __global__ __launch_bounds__(256) void k_elastic_energy(const T *__restrict__ u,
 // the absolute values of its terms is the scale of its rounding.
 struct scheme_args {
   int n, g, nac, nch, aniso;
-  double hx, hy, hz, epn2, kkx, kky, kkz, a2, a3, a4;
+  double ihx, ihy, ihz;  // 1 / h: products, not FP64 divisions, in the cell loop
+  double epn2, kkx, kky, kkz, a2, a3, a4;
 };
 
 // The acceptance statistics of a candidate in one pass over its fields
@@ -225,13 +226,13 @@ __global__ __launch_bounds__(256) void k_accept_stats(const T *__restrict__ u, c
         const int p = p0 + q;
         if (p >= V) break;
         const double vv = v[q];
-        const double dx = ((double)vxp[q] - vv) / a.hx, dy = ((double)vyp[q] - vv) / a.hy,
-                     dz = ((double)vzp[q] - vv) / a.hz;
+        const double dx = ((double)vxp[q] - vv) * a.ihx, dy = ((double)vyp[q] - vv) * a.ihy,
+                     dz = ((double)vzp[q] - vv) * a.ihz;
         if (p < a.nac) {
           e += 0.5 * a.epn2 * (a.kkx * dx * dx + a.kky * dy * dy + a.kkz * dz * dz);
-          const double gxc = ((double)vxp[q] - (double)vxm[q]) / (2.0 * a.hx);
-          const double gyc = ((double)vyp[q] - (double)vym[q]) / (2.0 * a.hy);
-          const double gzc = ((double)vzp[q] - (double)vzm[q]) / (2.0 * a.hz);
+          const double gxc = ((double)vxp[q] - (double)vxm[q]) * 0.5 * a.ihx;
+          const double gyc = ((double)vyp[q] - (double)vym[q]) * 0.5 * a.ihy;
+          const double gzc = ((double)vzp[q] - (double)vzm[q]) * 0.5 * a.ihz;
           if (a.aniso) {
             // the reference's F2 convention: gy, gz as (u[-1] - u[+1]) / 2h
             const double gx = gxc, gy = -gyc, gz = -gzc;
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) void k_accept_stats(const T *__restrict__ u, c
           s3 += vv * vv * vv;
           dmax = fmax(dmax, fabs(vv - (double)v0[q]));
           cell = fmax(cell, fabs(vv));
-          gsum += sqrt(gxc * gxc + gyc * gyc + gzc * gzc);
+          gsum += (double)sqrtf((float)(gxc * gxc + gyc * gyc + gzc * gzc));
           if (with_el) {
             const double t = vv * (double)fe[q];
             el += t;
@@ -332,7 +333,8 @@ static int features_of(etd_ctx *c, const T *u, const T *u0, run_feat *f) {
     memset(f, 0, sizeof(*f));
     return ETD_OK;
   }
-  k_features<T><<<RBLOCKS, 256, 0, c->stream>>>(u, u0, c->n, c->g, c->nac, c->p.hx, c->p.hy, c->p.hz, c->ad_part);
+  k_features<T><<<RBLOCKS, 256, 0, c->stream>>>(u, u0, c->n, c->g, c->nac, 1.0 / c->p.hx, 1.0 / c->p.hy,
+                                                1.0 / c->p.hz, c->ad_part);
   int rc = fetch_part(c, 4);
   if (rc) return rc;
   double dm = 0.0, em = 0.0, cn = 0.0, gs = 0.0;
@@ -398,9 +400,9 @@ static int energy_of(etd_ctx *c, int kind, int with_elastic, double *E, double *
     a.nac = c->nac;
     a.nch = c->nch;
     a.aniso = c->p.anisotropic;
-    a.hx = c->p.hx;
-    a.hy = c->p.hy;
-    a.hz = c->p.hz;
+    a.ihx = 1.0 / c->p.hx;
+    a.ihy = 1.0 / c->p.hy;
+    a.ihz = 1.0 / c->p.hz;
     a.epn2 = c->p.epn2;
     a.kkx = c->p.kkx;
     a.kky = c->p.kky;

@@ -74,6 +74,8 @@ struct etd_ctx {
   int split_persistent;    /* 1: persistent workgroups looping over tiles (default) */
   int split_kernel;        /* 0: field chunks staged through LDS, 8 waves x 32 columns; 1: field chunks
                               prefetched into registers; 2: LDS-staged, 4 waves x 64 columns */
+  int nl_batch;            /* AC fields whose loads the nonlinear kernel issues together (4, 8, 12, 16);
+                              0: by storage type and field count */
   int num_cu;              /* compute units of the device */
   /* mixed precision (ETD_PREC_MIXED): the same buffers in FP32, and V / V^-1
    * split into split_terms BF16 terms in MFMA fragment order */

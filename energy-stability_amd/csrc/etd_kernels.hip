@@ -745,58 +745,65 @@ __device__ __forceinline__ int felas_slot(const nl_args &a, int p, int lag) {
 }
 
 // ac_calc_FU (calc.c:8-68) -> ac_calc_F1 (anisotropic.c:9-161): all AC fields
-// of one point per thread (sum u^2 couples them).
-template <typename T>
-__global__ void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out, const T *__restrict__ sub,
-                            const T *__restrict__ E, nl_args a, int with_felas) {
+// of one point per thread (sum u^2 couples them).  Fields go in batches of FB
+// whose loads are issued before any is used (past the last field a batch
+// re-reads it and ignores it); FB >= nac puts every field's loads of a pass in
+// flight at once, which the FP32 fields need to keep HBM busy (a wave's load
+// moves half the bytes of an FP64 one).  Sums in field order for any FB.
+template <typename T, int FB>
+__global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out,
+                                                   const T *__restrict__ sub, const T *__restrict__ E, nl_args a,
+                                                   int with_felas) {
   const int n = a.n, g = a.g;
-  const size_t n3 = (size_t)n * n * n;
+  const unsigned n3 = (unsigned)n * n * n;  // n <= 272: every offset below fits 32 bits
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n3) return;
   int i, j, k;
   idx3(l, n, i, j, k);
-  // fields in batches of FB whose loads are issued before any is used (past
-  // the last field a batch re-reads it and ignores it); sums in field order
-  constexpr int FB = 4;
+  const unsigned lo = (unsigned)l;
+  // explicit outside coupling (the reference's calc_mu values): a cell lies on
+  // at most one box face per axis, and its image offset is the same for every
+  // field; -1 when the cell is on no face of that axis
+  const int wk = wrap_e(k, n, g), wj = wrap_e(j, n, g), wi = wrap_e(i, n, g);
+  const int ox = i == 0 ? (int)IDX(n, wk, wj, wrap_e(-1, n, g)) : i == n - 1 ? (int)IDX(n, wk, wj, wrap_e(n, n, g)) : -1;
+  const int oy = j == 0 ? (int)IDX(n, wk, wrap_e(-1, n, g), wi) : j == n - 1 ? (int)IDX(n, wk, wrap_e(n, n, g), wi) : -1;
+  const int oz = k == 0 ? (int)IDX(n, wrap_e(-1, n, g), wj, wi) : k == n - 1 ? (int)IDX(n, wrap_e(n, n, g), wj, wi) : -1;
   double s = 0.0;
   for (int m0 = 0; m0 < a.nac; m0 += FB) {
     T um[FB];
 #pragma unroll
-    for (int e = 0; e < FB; e++) um[e] = u[n3 * (m0 + e < a.nac ? m0 + e : a.nac - 1) + l];
+    for (int e = 0; e < FB; e++) um[e] = u[n3 * (unsigned)(m0 + e < a.nac ? m0 + e : a.nac - 1) + lo];
 #pragma unroll
     for (int e = 0; e < FB; e++)
       if (m0 + e < a.nac) s = s + (double)um[e] * (double)um[e];
   }
   const felas_cell fc = felas_locate(a, k, j, i);
-  const size_t N3 = (size_t)a.N * a.N * a.N;
+  const unsigned N3 = (unsigned)a.N * a.N * a.N, foff = (unsigned)fc.off;
+  const double ce = a.LE * a.epn2;
   for (int p0 = 0; p0 < a.nac; p0 += FB) {
     T upv[FB], fel[FB], sbv[FB];
 #pragma unroll
     for (int e = 0; e < FB; e++) {
-      const int p = p0 + e < a.nac ? p0 + e : a.nac - 1;
-      upv[e] = u[n3 * p + l];
-      if (with_felas) fel[e] = E[(size_t)felas_slot(a, p, fc.lag) * N3 + fc.off];
-      if (sub) sbv[e] = sub[n3 * p + l];
+      const unsigned p = (unsigned)(p0 + e < a.nac ? p0 + e : a.nac - 1);
+      upv[e] = u[n3 * p + lo];
+      if (with_felas) fel[e] = E[(unsigned)felas_slot(a, (int)p, fc.lag) * N3 + foff];
+      if (sub) sbv[e] = sub[n3 * p + lo];
     }
 #pragma unroll
     for (int e = 0; e < FB; e++) {
       const int p = p0 + e;
       if (p >= a.nac) break;
-      const T *up_f = u + n3 * p;
+      const T *up_f = u + n3 * (unsigned)p;
       const double up = upv[e];
       const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
       double f = a.LE * a.KE * up;
       f -= a.LE * fp;
-      const double ce = a.LE * a.epn2;
-      if (i == 0) f += ce * Pimg(up_f, n, g, k, j, -1) / a.hx / a.hx;
-      if (i == n - 1) f += ce * Pimg(up_f, n, g, k, j, n) / a.hx / a.hx;
-      if (j == 0) f += ce * Pimg(up_f, n, g, k, -1, i) / a.hy / a.hy;
-      if (j == n - 1) f += ce * Pimg(up_f, n, g, k, n, i) / a.hy / a.hy;
-      if (k == 0) f += ce * Pimg(up_f, n, g, -1, j, i) / a.hz / a.hz;
-      if (k == n - 1) f += ce * Pimg(up_f, n, g, n, j, i) / a.hz / a.hz;
+      if (ox >= 0) f += ce * (double)up_f[ox] / a.hx / a.hx;
+      if (oy >= 0) f += ce * (double)up_f[oy] / a.hy / a.hy;
+      if (oz >= 0) f += ce * (double)up_f[oz] / a.hz / a.hz;
       if (with_felas) f -= a.LE * a.es * (double)fel[e];
       if (sub) f -= sbv[e];
-      out[n3 * p + l] = (T)f;
+      out[n3 * (unsigned)p + lo] = (T)f;
     }
   }
 }
@@ -972,7 +979,20 @@ void launch_nonlinear(etd_ctx *c, T *out, const T *sub) {
   if (c->nac > 0) {
     const int aniso = c->p.anisotropic;
     // with F2 the felas and ETD2 subtraction move to the divergence pass
-    k_nonlin_ac<T><<<nb, 256, 0, c->stream>>>(F.u, out, aniso ? nullptr : sub, F.E, a, felas && !aniso);
+    const T *sb = aniso ? nullptr : sub;
+    const int fl = felas && !aniso;
+    // FP32: all fields' loads at once; FP64 loads move twice the bytes, and
+    // batches of 4 keep 8 waves per SIMD
+    int fb = c->nl_batch;
+    if (!fb) fb = sizeof(T) == 8 ? 4 : (c->nac <= 4 ? 4 : c->nac <= 8 ? 8 : c->nac <= 12 ? 12 : 16);
+    if (fb == 4)
+      k_nonlin_ac<T, 4><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+    else if (fb == 8)
+      k_nonlin_ac<T, 8><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+    else if (fb == 12)
+      k_nonlin_ac<T, 12><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+    else
+      k_nonlin_ac<T, 16><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
     if (aniso)
       for (int p = 0; p < c->nac; p++) {
         k_f2_grad<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * p, F.gtmp, c->lam + 9 * p, a);
