@@ -72,8 +72,11 @@ struct etd_ctx {
   int fft_variant;         /* 1: two-stage register FFT for N % 16 == 0 (default), 0: radix passes in LDS */
   int split_waves;         /* mixed mode products: 8 or 4 waves per workgroup, 0 by size */
   int split_persistent;    /* 1: persistent workgroups looping over tiles (default) */
-  int split_kernel;        /* 0: field chunks staged through LDS, 8 waves x 32 columns; 1: field chunks
-                              prefetched into registers; 2: LDS-staged, 4 waves x 64 columns */
+  int split_kernel;        /* BF16 split-3: 0: field chunks staged through LDS, 8 waves x 32 columns;
+                              1: field chunks prefetched into registers; 2: LDS-staged, 4 waves x 64
+                              columns; 3: two workgroups per CU; 4: wave grid.  FP16 split-2 (n <= 256):
+                              0: three-slot field ring, whole-line copies; 5: three-slot ring, 32-byte
+                              copies; 7: two-slot ring */
   int nl_batch;            /* AC fields whose loads the nonlinear kernel issues together (4, 8, 12, 16);
                               0: by storage type and field count */
   int num_cu;              /* compute units of the device */

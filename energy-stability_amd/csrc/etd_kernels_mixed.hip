@@ -753,6 +753,194 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_p(const uint16_t *__res
   }
 }
 
+// FP16 split-2 with field chunks prefetched two chunks ahead: the V / V^-1
+// operand (L2-resident) keeps a two-slot LDS ring, the field operand (from
+// HBM) a three-slot ring, so while chunk g is multiplied the copies of field
+// chunks g + 1 and g + 2 are in flight (2 x 32 KB per CU instead of one).
+// Arithmetic, operand layout and results are those of k_modeprod_f16_p.
+// Chunk g of a workgroup is K chunk g % nkc of its tile
+// blockIdx.x + (g / nkc) gridDim.x.  Per chunk, after the previous chunk's
+// reads: wait until the copies of chunk g are complete (vmcnt counts this
+// wave's copies, which complete in issue order; only field chunk g + 1's XPT
+// copies may remain), barrier, then issue V chunk g + 1 and field chunk g + 2
+// into the slots that chunk g - 1 used.  After a tile's stores, and where no
+// field chunk g + 1 was issued, the wait drains everything.
+// XL = 0: a field chunk sits in LDS as [k/8][col][8 k] and each copy
+// instruction takes 32 bytes from each of 32 columns.  XL = 1: it sits as
+// [col][k/4 ^ (col & 7)] in 16-byte pieces, so each copy instruction takes
+// 8 whole 128-byte lines (8 columns x 32 k) and the XOR keeps the 16 lanes of
+// an operand read on distinct banks as in XL = 0.
+template <int MT>
+struct f16r_cfg {
+  static constexpr int THREADS = 512, WAVES = 8, COLS = 256;
+  static constexpr int APIECES = 2 * MT * 64;             // 16-byte pieces of one V chunk (2 terms)
+  static constexpr int AFULL = APIECES / THREADS;         // rounds every thread copies
+  static constexpr int AREM = APIECES - AFULL * THREADS;  // pieces of the last round
+  static constexpr int ASZ = APIECES * 16;
+  static constexpr int XSZ = 32 * COLS * 4;
+  static constexpr int XPT = XSZ / 16 / THREADS;
+  static constexpr int LDS = 2 * ASZ + 3 * XSZ;
+  static constexpr bool FITS = LDS <= 160 * 1024;  // MT <= 16 (n <= 256)
+  static_assert(AREM % 64 == 0, "the last round of V pieces is whole waves");
+  static_assert(XPT == 4, "the partial wait below counts 4 copies");
+};
+
+__device__ __forceinline__ void wait_copies4_barrier() { asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" : : : "memory"); }
+
+template <int MT, int XL>
+__global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__restrict__ A2, const float *__restrict__ in,
+                                                            size_t in_fs, float *__restrict__ out, size_t out_fs, int n,
+                                                            int rotate, int accumulate, int a_chunk, int nfields,
+                                                            float a_inv) {
+  using CF = f16r_cfg<MT>;
+  static_assert(CF::FITS, "LDS budget");
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long n2 = (long)n * n;
+  const int nkc = (n + 31) / 32;
+  const int ncb = (int)((n2 + CF::COLS - 1) / CF::COLS);
+  const int ntiles = ncb * nfields;
+  if ((int)blockIdx.x >= ntiles) return;  // uniform
+  const int my_tiles = (ntiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int total = my_tiles * nkc;
+  const i32x4m rsA = mk_rsrc(A2, (uint32_t)((size_t)nkc * a_chunk));
+  const uint32_t lds0 = (uint32_t)(uintptr_t)smem;
+
+  auto issue_a = [&](int gg) {
+    const int kc = gg % nkc;
+    const uint32_t sbase = lds0 + (gg & 1) * CF::ASZ;
+#pragma unroll
+    for (int e = 0; e < CF::AFULL; e++)
+      copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
+    if (CF::AREM > 0 && wave < CF::AREM / 64)  // wave-uniform
+      copy16_to_lds(rsA, sbase + (CF::AFULL * CF::THREADS + wave * 64) * 16,
+                    (int)((size_t)kc * a_chunk + ((size_t)CF::AFULL * CF::THREADS + threadIdx.x) * 16));
+  };
+  auto issue_x = [&](int gg) {
+    const int tl = (int)blockIdx.x + (gg / nkc) * (int)gridDim.x;
+    const int kc = gg % nkc;
+    const int f = tl / ncb;
+    const long wgc = (long)(tl % ncb) * CF::COLS;
+    i32x4m rsX = mk_rsrc(in + in_fs * f, (uint32_t)((size_t)n * n2 * 4));
+    rsX.x = __builtin_amdgcn_readfirstlane(rsX.x);
+    rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
+    rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
+    const uint32_t sbase = lds0 + 2 * CF::ASZ + (gg % 3) * CF::XSZ;
+#pragma unroll
+    for (int e = 0; e < CF::XPT; e++) {
+      long col;
+      int k;
+      if constexpr (XL) {
+        const int c = 64 * e + 8 * wave + (lane >> 3);  // piece index e 512 + 64 wave + lane = 8 c + slot
+        col = wgc + c;
+        k = 32 * kc + 4 * ((lane & 7) ^ (c & 7));
+      } else {
+        col = wgc + 32 * wave + (lane >> 1);
+        k = 32 * kc + 8 * e + 4 * (lane & 1);
+      }
+      const bool ok = col < n2 && k < n;
+      const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
+      copy16_to_lds(rsX, sbase + (e * CF::THREADS + wave * 64) * 16, voff);
+    }
+  };
+
+  const int q = lane >> 4, r16 = lane & 15;
+  issue_x(0);
+  issue_a(0);
+  if (total > 1) issue_x(1);
+  f32x4 acc[MT][2];
+  int xs = 0;  // g % 3
+#pragma unroll 1
+  for (int g = 0; g < total; g++, xs = xs == 2 ? 0 : xs + 1) {
+    const int kc = g % nkc;
+    const int tile = (int)blockIdx.x + (g / nkc) * (int)gridDim.x;
+    if (kc == 0) {
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
+    if (g + 1 < total && (kc != 0 || g == 0))  // uniform
+      wait_copies4_barrier();
+    else
+      wait_all_copies_barrier();
+    if (g + 1 < total) issue_a(g + 1);
+    if (g + 2 < total) issue_x(g + 2);
+    const f16x8 *sa = reinterpret_cast<const f16x8 *>(smem + (g & 1) * CF::ASZ);
+    const float *sx = reinterpret_cast<const float *>(smem + 2 * CF::ASZ + xs * CF::XSZ);
+    f16x8 b[2][2];
+    float back[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      float4 lo, hi;
+      if constexpr (XL) {
+        const int c = 32 * wave + 16 * u + r16;
+        const float4 *cp = reinterpret_cast<const float4 *>(sx) + 8 * c;
+        lo = cp[(2 * q) ^ (c & 7)];
+        hi = cp[(2 * q + 1) ^ (c & 7)];
+      } else {
+        const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+        lo = *reinterpret_cast<const float4 *>(bp);
+        hi = *reinterpret_cast<const float4 *>(bp + 4);
+      }
+      const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      float m = 0.f;
+#pragma unroll
+      for (int e = 0; e < 8; e++) m = fmaxf(m, fabsf(x[e]));
+      m = fmaxf(m, __shfl_xor(m, 16, 64));
+      m = fmaxf(m, __shfl_xor(m, 32, 64));
+      const int ex = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+      const float sc = __builtin_ldexpf(1.0f, 14 - ex);
+      back[u] = __builtin_ldexpf(a_inv, ex - 14);
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float xsv = x[e] * sc;
+        const _Float16 h0 = (_Float16)xsv;
+        b[u][0][e] = h0;
+        b[u][1][e] = (_Float16)(xsv - (float)h0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MT; t += 2) {
+      f16x8 a[2][2];
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+        if (t + d < MT)
+#pragma unroll
+          for (int tm = 0; tm < 2; tm++) a[d][tm] = sa[(tm * MT + t + d) * 64 + lane];
+      f32x4 dch[2][2];
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][1], b[u][0], (f32x4){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][1], dch[d][u], 0, 0, 0);
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][0], dch[d][u], 0, 0, 0);
+#pragma unroll
+      for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+          if (t + d < MT)
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[t + d][u][i] = fmaf(dch[d][u][i], back[u], acc[t + d][u][i]);
+    }
+    if (kc == nkc - 1) {
+      const int f = tile / ncb;
+      const long wgcol = (long)(tile % ncb) * CF::COLS;
+      store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+    }
+  }
+}
+
 // Register-prefetch variant (persistent): the field operand goes straight
 // from HBM into registers with compiler-visible buffer loads issued two K
 // chunks ahead, so HBM latency is covered by two chunks of MFMA work; only
@@ -1009,6 +1197,27 @@ static void f16_launch(etd_ctx *c, const uint16_t *A2, const float *in, size_t i
   const long ncb = ((long)c->n * c->n + CF::COLS - 1) / CF::COLS;
   const long ntiles = ncb * nfields;
   const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;
+  if constexpr (f16r_cfg<MT>::FITS) {
+    if (c->split_kernel != 7) {  // 7: the two-slot kernel below
+      using CR = f16r_cfg<MT>;
+      static_assert(CR::COLS == CF::COLS && CR::THREADS == CF::THREADS, "same tiling");
+      static bool attr_r = false;
+      if (!attr_r) {
+        (void)hipFuncSetAttribute((const void *)k_modeprod_f16_r3<MT, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  CR::LDS);
+        (void)hipFuncSetAttribute((const void *)k_modeprod_f16_r3<MT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  CR::LDS);
+        attr_r = true;
+      }
+      if (c->split_kernel != 5)
+        k_modeprod_f16_r3<MT, 1><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
+            A2, in, in_fs, out, out_fs, c->n, rotate, accumulate, (int)chunk_bytes<MT, 2>(), nfields, a_inv);
+      else
+        k_modeprod_f16_r3<MT, 0><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
+            A2, in, in_fs, out, out_fs, c->n, rotate, accumulate, (int)chunk_bytes<MT, 2>(), nfields, a_inv);
+      return;
+    }
+  }
   k_modeprod_f16_p<MT><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
       A2, in, in_fs, out, out_fs, c->n, rotate, accumulate, (int)chunk_bytes<MT, 2>(), nfields, a_inv);
 }

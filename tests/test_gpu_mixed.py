@@ -203,3 +203,39 @@ def test_f16_mixed_step_matches_oracle(case):
     worst = max(np.abs(g.get_field(v) - o.get_field(v)).max() for v in range(prm["nac"] + prm["nch"]))
     say(f"  max |d| = {worst:.3e}")
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("kernel", ["0", "5"])
+@pytest.mark.parametrize("n,nac,nch", [(20, 2, 1), (36, 2, 1), (100, 4, 1), (132, 6, 1)])
+def test_f16_three_slot_pipeline_equals_default(n, nac, nch, kernel, monkeypatch):
+    """The FP16 split-2 kernels whose field chunks are prefetched two chunks
+    ahead (the default, with whole-line field copies into an XOR-swizzled LDS
+    layout; ETD_SPLIT_KERNEL=5, 32-byte copies) compute exactly the products
+    of the two-slot kernel (ETD_SPLIT_KERNEL=7): bit-identical fields after
+    three elastic steps.  The cases cover one K
+    chunk per tile (n = 20), the partial last column block, and workgroups that
+    run several tiles in a row (n = 132: 14 fields x 69 column blocks over 256
+    workgroups), where the copy pipeline crosses tile boundaries."""
+    say(f"f16 three-slot pipeline kernel={kernel} n={n} nac={nac} nch={nch}")
+    prm = small_params(n=n, nac=nac, nch=nch, elastic=1, dt=0.02)
+    fields = noise_fields(prm, amp=0.3)
+    Vm, Vim, D = O.eigbasis(n, 3)
+
+    def run(kernel):
+        if kernel is None:
+            monkeypatch.delenv("ETD_SPLIT_KERNEL", raising=False)
+        else:
+            monkeypatch.setenv("ETD_SPLIT_KERNEL", kernel)
+        g = pyetd.Etd(pyetd.params_from_dict(dict(prm, precision=2)), Vm, Vim, D)
+        for v, f in enumerate(fields):
+            g.set_field(v, f)
+        g.step(3)
+        out = [g.get_field(v) for v in range(nac + nch)]
+        g.close()
+        return out
+
+    a = run("7")
+    b = run(kernel)
+    for v, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), f"field {v}: max |d| {np.abs(x - y).max():.3e}"
+    monkeypatch.delenv("ETD_SPLIT_KERNEL", raising=False)
