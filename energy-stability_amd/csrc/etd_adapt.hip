@@ -482,11 +482,11 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   if (!c->ad_S0) {
     ACHK(hipMalloc(&c->ad_S0, sizeof(T) * cnt));
     ACHK(hipMalloc(&c->ad_SB, sizeof(T) * cnt));
-    ACHK(hipMalloc(&c->ad_spec, sizeof(T) * cnt));
+    ACHK(hipMalloc(&c->ad_spec, sizeof(T) * 2 * cnt));  // the phi-scaled pair (dt, dt / 2)
     ACHK(hipMalloc(&c->ad_part, sizeof(double) * RBLOCKS * RCOLS));
     ACHK(hipHostMalloc(&c->ad_hpart, sizeof(double) * RBLOCKS * RCOLS, hipHostMallocDefault));
   }
-  T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec;
+  T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec, *SP2 = SP + cnt;
   const double dt_fixed = c->p.dt;
   // energy-increase slack: the rounding level of the storage type times the
   // energy's scale, for the scheme energy the larger absolute term sum of
@@ -558,17 +558,14 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     if (!c->p.etd2) {
       if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
       stage_forward<T>(c);  // N(u_n) and the forward transforms, shared
-      c->p.dt = dt;
-      launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
+      // the full step and the first half step from one phi pass
+      launch_phiscale_pair<T>(c, F.spec, F.spec + cnt, SP, dt, SP2, 0.5 * dt);
       modeprod3<T>(c, 0, SP, SB, V, 0);
+      modeprod3<T>(c, 0, SP2, F.u, V, 0);
       c->p.dt = 0.5 * dt;
-      launch_phiscale<T>(c, F.spec, F.spec + cnt, SP, 0, V, 0);
-      modeprod3<T>(c, 0, SP, F.u, V, 0);
       if constexpr (sizeof(T) == 4) {
-        ch_center(c, SB, nullptr, +1);
-        ch_center(c, F.u, nullptr, +1);
-        ch_mean_restore(c, SB);
-        ch_mean_restore(c, F.u);
+        ch_mean_restore_centered(c, SB);
+        ch_mean_restore_centered(c, F.u);
       }
       if ((rc = step_once_t<T>(c))) break;
     } else {

@@ -485,7 +485,10 @@ void stage_forward(etd_ctx *c) {
   launch_nonlinear<T>(c, F.N1, nullptr);
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, F.N1, -1);
   modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
-  if constexpr (sizeof(T) == 4) ch_center(c, F.u, F.N1, +1);
+  // u stays centred: every caller overwrites it with the backward transform;
+  // the ETD2 stage needs N1 itself (N(a) - N1)
+  if constexpr (sizeof(T) == 4)
+    if (c->p.etd2) ch_center(c, nullptr, F.N1, +1);
 }
 
 template <typename T>
@@ -497,16 +500,18 @@ int step_once_t(etd_ctx *c) {
   stage_forward<T>(c);
   launch_phiscale<T>(c, F.spec, F.spec + n3 * V, F.spec, 0, V, 0);
   modeprod3<T>(c, 0, F.spec, F.u, V, 0);
-  if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
   if (c->p.etd2) {
+    if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
     launch_ghost_fill<T>(c, F.u, V, n3);
     if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
     launch_nonlinear<T>(c, F.N2, F.N1);  // N(a) - N(u_n)
     modeprod3<T>(c, 1, F.N2, F.spec, V, 0);
     launch_phiscale<T>(c, F.spec, (const T *)nullptr, F.spec, 0, V, 1);
     modeprod3<T>(c, 0, F.spec, F.u, V, 1);  // u = a + S phi2 S^-1 dN
+    if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
+  } else {
+    if constexpr (sizeof(T) == 4) ch_mean_restore_centered(c, F.u);  // adds the mean back
   }
-  if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
   HCHK(c, hipGetLastError());
   return ETD_OK;
 }

@@ -151,6 +151,8 @@ void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride);
 template <typename T>
 void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2);
 template <typename T>
+void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2);
+template <typename T>
 void launch_nonlinear(etd_ctx *c, T *out, const T *sub);
 template <typename T>
 void launch_elastic(etd_ctx *c);
@@ -169,12 +171,18 @@ int launch_diag(etd_ctx *c, double *stats);
 #define CH_BLOCKS 256
 void ch_mean_mark(etd_ctx *c, const float *u);
 void ch_mean_restore(etd_ctx *c, float *u);
+/* ch_center(c, u, NULL, +1) followed by ch_mean_restore(c, u) in one pass:
+ * ghost cells get the mean added, interior cells the shift that gives the
+ * target sum (computed from the centred field). */
+void ch_mean_restore_centered(etd_ctx *c, float *u);
 /* The mixed path transforms the CH fields' deviation from their conserved
  * mean: a uniform field is a fixed point of the scheme, so the step evolves
  * u - c_mean with N(u) - N(c_mean) and adds c_mean back.  The split-BF16
  * products then round the small deviation instead of the large mean, whose
  * rounding would otherwise leak into the barely damped long-wave CH modes
- * every step.  sign -1 centres (u and/or N, either may be null), +1 undoes. */
+ * every step.  sign -1 centres (u and/or N, either may be null), +1 undoes.
+ * The step leaves u centred after the forward transform (the backward
+ * transform overwrites it) and undoes it in ch_mean_restore_centered. */
 void ch_center(etd_ctx *c, float *u, float *N, int sign);
 
 /* step pieces (etd_api.hip) */

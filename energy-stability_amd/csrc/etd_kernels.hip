@@ -596,6 +596,7 @@ void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_f
 struct phi_args {
   int n, nac;
   double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
+  double dt2;  // with a second output: its step
 };
 
 // phi1, phi2 and 1 - phi1 H' of one point for one kind of field (AC or CH);
@@ -621,9 +622,13 @@ __device__ __forceinline__ phi_pt phi_at(double H, double dt) {
 // One thread per spectral point, looping over the fields: every AC field
 // shares H (and so phi), as do the CH fields, so exp and the divisions run
 // once per point and kind instead of once per field.
+// With outp2 (first order only) the same spectra also go to outp2 scaled for
+// step a.dt2: one read of u-hat and N-hat for two step sizes, each output
+// computed exactly as a launch of its own would.
 template <typename T>
 __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int first_var, int nvars,
-                           const double *DDX, const double *DDY, const double *DDZ, phi_args a, int order2) {
+                           const double *DDX, const double *DDY, const double *DDZ, phi_args a, int order2,
+                           T *outp2) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -631,14 +636,16 @@ __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int
   int i, j, k;
   idx3(l, n, i, j, k);
   const int last = first_var + nvars;
-  phi_pt pac, pch;
+  phi_pt pac, pch, pac2, pch2;
   if (first_var < a.nac) {
     const double tmp = a.kkz * DDZ[k] + a.kky * DDY[j] + a.kkx * DDX[i];
     pac = phi_at(-a.LE * (tmp * a.epn2 - a.KE), a.dt);
+    if (outp2) pac2 = phi_at(-a.LE * (tmp * a.epn2 - a.KE), a.dt2);
   }
   if (last > a.nac) {
     const double tmp = DDZ[k] + DDY[j] + DDX[i];
     pch = phi_at(a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp), a.dt);
+    if (outp2) pch2 = phi_at(a.LCI * (tmp * tmp * a.epn2 - a.KCI * tmp), a.dt2);
   }
   // fields in batches of FB whose loads are issued before any is used
   constexpr int FB = 4;
@@ -660,6 +667,10 @@ __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int
         o[l] = (T)(ph.phi2 * (double)uv[e]);
       else
         o[l] = (T)(ph.lin * (double)uv[e] + ph.phi1 * (double)nv[e]);
+      if (outp2) {
+        const phi_pt &p2 = v < a.nac ? pac2 : pch2;
+        outp2[fs * (v - first_var) + l] = (T)(p2.lin * (double)uv[e] + p2.phi1 * (double)nv[e]);
+      }
     }
   }
 }
@@ -680,9 +691,33 @@ void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first
   a.kkx = c->p.kkx;
   a.kky = c->p.kky;
   a.kkz = c->p.kkz;
+  a.dt2 = 0.0;
   etd_prof_begin(c, PC_PHI);
   k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, first_var, nvars,
-                                                                      c->DDX, c->DDY, c->DDZ, a, order2);
+                                                                      c->DDX, c->DDY, c->DDZ, a, order2, nullptr);
+  etd_prof_end(c, PC_PHI);
+}
+
+// first order, all variables: out = phi-update at step dt, out2 at step dt2,
+// from one read of uhat and nhat (the step-doubling pair)
+template <typename T>
+void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2) {
+  phi_args a;
+  a.n = c->n;
+  a.nac = c->nac;
+  a.dt = dt;
+  a.epn2 = c->p.epn2;
+  a.LE = c->p.LE;
+  a.KE = c->p.KE;
+  a.LCI = c->p.LCI;
+  a.KCI = c->p.KCI;
+  a.kkx = c->p.kkx;
+  a.kky = c->p.kky;
+  a.kkz = c->p.kkz;
+  a.dt2 = dt2;
+  etd_prof_begin(c, PC_PHI);
+  k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, 0, c->V, c->DDX,
+                                                                      c->DDY, c->DDZ, a, 0, out2);
   etd_prof_end(c, PC_PHI);
 }
 
@@ -1953,6 +1988,33 @@ __global__ __launch_bounds__(256) void k_ch_shift(float *__restrict__ u, size_t 
   }
 }
 
+// The undo of the centring and the mean restoration of CH field q =
+// blockIdx.y in one pass over the extended box: ghost cells get the mean m
+// added, interior cells (target_q - a_q) / N^3 with a_q the interior sum of the
+// centred field (the partials of k_ch_sum, reduced in the same order by every
+// block).
+__global__ __launch_bounds__(256) void k_ch_shift_center(float *__restrict__ u, size_t fs, int n, int g,
+                                                         const double *__restrict__ target,
+                                                         const double *__restrict__ part, double invN3) {
+  float *f = u + fs * blockIdx.y;
+  const int N = n - 2 * g;
+  const int q = blockIdx.y;
+  const double a = block_sum256(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
+  const double shift = (target[q] - a) / ((double)N * N * N);
+  const float m = (float)(target[q] * invN3);
+  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < n * n; r += gridDim.x * 4) {
+    const int z = r / n, y = r - z * n;
+    const bool row_in = z >= g && z < n - g && y >= g && y < n - g;
+    for (int x = threadIdx.x & 63; x < n; x += 64) {
+      const size_t l = IDX(n, z, y, x);
+      if (row_in && x >= g && x < n - g)
+        f[l] = (float)((double)f[l] + shift);
+      else
+        f[l] = f[l] + m;
+    }
+  }
+}
+
 // f = target[q] / N^3 on the whole extended box
 __global__ void k_ch_fill(float *__restrict__ f, size_t n3, const double *__restrict__ target, int q, double invN3) {
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2004,6 +2066,15 @@ void ch_center(etd_ctx *c, float *u, float *N, int sign) {
       u ? u + off : nullptr, N ? N + off : nullptr, c->n3, c->ch_sum, c->ch_const, invN3, sign);
 }
 
+void ch_mean_restore_centered(etd_ctx *c, float *u) {
+  if (!c->ch_sum) return;
+  double *part = c->ch_sum + c->nch;
+  const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_shift_center<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g,
+                                                                    c->ch_sum, part, invN3);
+}
+
 void ch_mean_restore(etd_ctx *c, float *u) {
   if (!c->ch_sum) return;
   double *part = c->ch_sum + c->nch;
@@ -2015,6 +2086,7 @@ void ch_mean_restore(etd_ctx *c, float *u) {
 #define ETD_INST(T)                                                                   \
   template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
   template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \
+  template void launch_phiscale_pair<T>(etd_ctx *, const T *, const T *, T *, double, T *, double); \
   template void launch_nonlinear<T>(etd_ctx *, T *, const T *);                      \
   template void launch_elastic<T>(etd_ctx *);                                        \
   template void fft3d<T>(etd_ctx *, T *, int, int);                                  \
