@@ -1530,6 +1530,9 @@ __device__ __forceinline__ void b_wrow(const double *sp, const double *nv, const
 // geometry at -n is an exact sign flip of the one at n, squared away), so
 // acc_p(-k) = conj acc_p(k) bit for bit and Y_r(-k) = (a_x + b_y, b_x - a_y)
 // with a = acc_2r, b = acc_2r+1.
+// Arithmetic in the storage precision: FP64 for FP64 fields, FP32 for the
+// mixed path's FP32 spectra (half the registers: 4 waves per SIMD instead of
+// 2 at NAC = 12); the geometry Omega(n) is always formed in FP64.
 template <int NAC, bool MIRROR, typename T2, bool WRITE_NEG = false>
 __device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__restrict__ Y, int N,
                                                int npair, size_t m, const etd_dev_consts *__restrict__ K) {
@@ -1538,50 +1541,74 @@ __device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__r
   idx3(m, N, x, y, z);
   const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
   const size_t mm = ((size_t)zm * N + ym) * N + xm;
-  double2 F[NAC];
+  using CT = typename std::conditional<sizeof(T2) == 8, float, double>::type;
+  struct C2 {
+    CT x, y;
+  };
+  C2 F[NAC];
 #pragma unroll
   for (int r = 0; r < (NAC + 1) / 2; r++) {
     const T2 za = Z[N3 * r + m], zb = Z[N3 * r + mm];
-    const double2 a = make_double2(za.x, za.y), b = make_double2(zb.x, zb.y);
+    const C2 a = {(CT)za.x, (CT)za.y}, b = {(CT)zb.x, (CT)zb.y};
     // F_2r = (Z(m) + conj Z(-m)) / 2, F_2r+1 = (Z(m) - conj Z(-m)) / 2i
-    F[2 * r] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-    if (2 * r + 1 < NAC) F[2 * r + 1] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+    F[2 * r] = C2{(CT)0.5 * (a.x + b.x), (CT)0.5 * (a.y - b.y)};
+    if (2 * r + 1 < NAC) F[2 * r + 1] = C2{(CT)0.5 * (a.y + b.y), (CT)-0.5 * (a.x - b.x)};
   }
-  double nv[3], om[9];
-  b_geometry(signed_wave(x, N), signed_wave(y, N), signed_wave(z, N), K, nv, om);
-  double bv[NAC][3], cv[NAC][3];
+  CT nv[3], om[9];
+  {
+    double nvd[3], omd[9];
+    b_geometry(signed_wave(x, N), signed_wave(y, N), signed_wave(z, N), K, nvd, omd);
+#pragma unroll
+    for (int k = 0; k < 3; k++) nv[k] = (CT)nvd[k];
+#pragma unroll
+    for (int k = 0; k < 9; k++) om[k] = (CT)omd[k];
+  }
+  // b_q = sigma_q n for every variant; c_p = b_p Omega only for the row p
+  // being summed (3 live values instead of 3 NAC: fewer VGPRs, more waves)
+  CT bv[NAC][3];
 #pragma unroll
   for (int q = 0; q < NAC; q++) {
     const double *sg = K->sig + 9 * q;
 #pragma unroll
-    for (int k = 0; k < 3; k++) bv[q][k] = sg[k * 3 + 0] * nv[0] + sg[k * 3 + 1] * nv[1] + sg[k * 3 + 2] * nv[2];
-#pragma unroll
-    for (int k = 0; k < 3; k++) cv[q][k] = bv[q][0] * om[0 * 3 + k] + bv[q][1] * om[1 * 3 + k] + bv[q][2] * om[2 * 3 + k];
+    for (int k = 0; k < 3; k++)
+      bv[q][k] = (CT)sg[k * 3 + 0] * nv[0] + (CT)sg[k * 3 + 1] * nv[1] + (CT)sg[k * 3 + 2] * nv[2];
   }
-  double bm[MIRROR ? NAC : 1][3], cm[MIRROR ? NAC : 1][3];
+  CT bm[MIRROR ? NAC : 1][3], cm[MIRROR ? NAC : 1][3];
   if constexpr (MIRROR) {
-    double nv2[3], om2[9];
-    b_geometry(signed_wave(xm, N), signed_wave(ym, N), signed_wave(zm, N), K, nv2, om2);
+    CT nv2[3], om2[9];
+    {
+      double nvd[3], omd[9];
+      b_geometry(signed_wave(xm, N), signed_wave(ym, N), signed_wave(zm, N), K, nvd, omd);
+#pragma unroll
+      for (int k = 0; k < 3; k++) nv2[k] = (CT)nvd[k];
+#pragma unroll
+      for (int k = 0; k < 9; k++) om2[k] = (CT)omd[k];
+    }
 #pragma unroll
     for (int q = 0; q < NAC; q++) {
       const double *sg = K->sig + 9 * q;
 #pragma unroll
-      for (int k = 0; k < 3; k++) bm[q][k] = sg[k * 3 + 0] * nv2[0] + sg[k * 3 + 1] * nv2[1] + sg[k * 3 + 2] * nv2[2];
+      for (int k = 0; k < 3; k++)
+        bm[q][k] = (CT)sg[k * 3 + 0] * nv2[0] + (CT)sg[k * 3 + 1] * nv2[1] + (CT)sg[k * 3 + 2] * nv2[2];
 #pragma unroll
       for (int k = 0; k < 3; k++)
         cm[q][k] = bm[q][0] * om2[0 * 3 + k] + bm[q][1] * om2[1 * 3 + k] + bm[q][2] * om2[2 * 3 + k];
     }
   }
-  double2 acc[NAC];
+  C2 acc[NAC];
 #pragma unroll
-  for (int p = 0; p < NAC; p++) acc[p] = make_double2(0.0, 0.0);
+  for (int p = 0; p < NAC; p++) acc[p] = C2{(CT)0, (CT)0};
 #pragma unroll
-  for (int p = 0; p < NAC; p++)
+  for (int p = 0; p < NAC; p++) {
+    CT cvp[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) cvp[k] = bv[p][0] * om[0 * 3 + k] + bv[p][1] * om[1 * 3 + k] + bv[p][2] * om[2 * 3 + k];
 #pragma unroll
     for (int q = p; q < NAC; q++) {
-      const double t0 = K->t0[p * ETD_MAX_VAR + q];
-      double B = t0 - (cv[p][0] * bv[q][0] + cv[p][1] * bv[q][1] + cv[p][2] * bv[q][2]);
-      if constexpr (MIRROR) B = 0.5 * (B + (t0 - (cm[p][0] * bm[q][0] + cm[p][1] * bm[q][1] + cm[p][2] * bm[q][2])));
+      const CT t0 = (CT)K->t0[p * ETD_MAX_VAR + q];
+      CT B = t0 - (cvp[0] * bv[q][0] + cvp[1] * bv[q][1] + cvp[2] * bv[q][2]);
+      if constexpr (MIRROR)
+        B = (CT)0.5 * (B + (t0 - (cm[p][0] * bm[q][0] + cm[p][1] * bm[q][1] + cm[p][2] * bm[q][2])));
       acc[p].x += F[q].x * B;
       acc[p].y += F[q].y * B;
       if (q != p) {
@@ -1589,6 +1616,7 @@ __device__ __forceinline__ void contract_point(const T2 *__restrict__ Z, T2 *__r
         acc[q].y += F[p].y * B;
       }
     }
+  }
 #pragma unroll
   for (int r = 0; r < (NAC + 1) / 2; r++) {
     T2 y;
