@@ -1774,6 +1774,7 @@ void launch_felas_materialize(etd_ctx *c, T *felas) {
 struct diag_args {
   int n, g, nac, nch;
   double hx, hy, hz, epn2, a2, a3, a4;
+  double ihx, ihy, ihz;  // 1 / h
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1835,6 +1836,16 @@ __device__ __forceinline__ void cell_grad(const T *u, int n, int k, int j, int i
   uz = ((f4 - f0) + (f5 - f1) + (f6 - f2) + (f7 - f3)) / 4 / hz;
 }
 
+// the same cell quantities from the 8 corner values already loaded; / 8 and
+// / 4 are exact as products, 1 / h is one rounding away from / h
+__device__ __forceinline__ void cell_grad_v(const double (&f)[8], double ihx, double ihy, double ihz, double &mean,
+                                            double &ux, double &uy, double &uz) {
+  mean = (f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7]) * 0.125;
+  ux = ((f[1] - f[0]) + (f[3] - f[2]) + (f[5] - f[4]) + (f[7] - f[6])) * 0.25 * ihx;
+  uy = ((f[2] - f[0]) + (f[3] - f[1]) + (f[6] - f[4]) + (f[7] - f[5])) * 0.25 * ihy;
+  uz = ((f[4] - f[0]) + (f[5] - f[1]) + (f[6] - f[2]) + (f[7] - f[3])) * 0.25 * ihz;
+}
+
 // grid (planes = n - 2g - 1, 1 + nch + couple, ECHUNK): y = 0 AC energy,
 // 1..nch CH energy of field nac + y - 1, nch + 1 couple gradient terms; each
 // plane is split into ECHUNK row blocks
@@ -1850,10 +1861,32 @@ __global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, co
   for (int t = r0 * m + threadIdx.x; t < r1 * m; t += blockDim.x) {
     const int j = g + t / m, i = g + t % m;
     if (kind == 0) {
+      // the 8 corners of 4 fields loaded before any is used; sums in field order
       double s2 = 0.0, s3 = 0.0;
-      for (int mm = 0; mm < a.nac; mm++) {
+      const size_t c0 = IDX(n, k, j, i), dy = (size_t)n, dz = (size_t)n * n;
+      for (int m0 = 0; m0 < a.nac; m0 += 4) {
+        T fv[4][8];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const T *f = u + n3 * (m0 + q < a.nac ? m0 + q : a.nac - 1) + c0;
+          fv[q][0] = f[0];
+          fv[q][1] = f[1];
+          fv[q][2] = f[dy];
+          fv[q][3] = f[dy + 1];
+          fv[q][4] = f[dz];
+          fv[q][5] = f[dz + 1];
+          fv[q][6] = f[dz + dy];
+          fv[q][7] = f[dz + dy + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+        const int mm = m0 + q;
+        if (mm >= a.nac) break;
+        double fd[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) fd[r] = (double)fv[q][r];
         double um, ux, uy, uz;
-        cell_grad(u + n3 * mm, n, k, j, i, a.hx, a.hy, a.hz, um, ux, uy, uz);
+        cell_grad_v(fd, a.ihx, a.ihy, a.ihz, um, ux, uy, uz);
         const double *L = lam + 9 * mm;
         const double ffx = ux * L[0] + uy * L[3] + uz * L[6];
         const double ffy = ux * L[1] + uy * L[4] + uz * L[7];
@@ -1861,6 +1894,7 @@ __global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, co
         e += a.epn2 * (ffx * ux + ffy * uy + ffz * uz) / 2;
         s2 += um * um;
         s3 += um * um * um;
+        }
       }
       e += (a.a2 / 2.0) * s2 - (a.a3 / 3.0) * s3 + (a.a4 / 4.0) * s2 * s2;
     } else if (kind <= a.nch) {
@@ -1903,6 +1937,9 @@ int launch_diag(etd_ctx *c, double *stats) {
   a.a2 = c->p.a2;
   a.a3 = c->p.a3;
   a.a4 = c->p.a4;
+  a.ihx = 1.0 / c->p.hx;
+  a.ihy = 1.0 / c->p.hy;
+  a.ihz = 1.0 / c->p.hz;
   const int n = c->n, g = c->g, V = c->V;
   const int ms = n - 2 * g, me = n - 2 * g - 1;
   const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);

@@ -181,9 +181,12 @@ def test_beta_omega_64cube_elastic_etd2_matches_oracle():
     assert worst < STEP_TOL
 
 
-def test_diag_matches_oracle():
-    say("diag")
-    prm = small_params(n=14, nac=3, nch=1, landau=OMEGA_LANDAU, aniso=1)
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.9, 1.0, 1.2)])
+def test_diag_matches_oracle(h):
+    """check_soln_new statistics and energy; with non-unit spacing the GPU
+    multiplies by 1 / h where the reference divides by h (one rounding)."""
+    say(f"diag h={h}")
+    prm = small_params(n=14, nac=5, nch=1, landau=OMEGA_LANDAU, aniso=1, h=h)
     o, g = pair(prm, noise_fields(prm, amp=0.5))
     st_o, cp_o = o.diag()
     st_g, cp_g = g.diag()
