@@ -21,6 +21,17 @@ __device__ __forceinline__ int wrap_e(int e, int n, int g) {
 }
 __device__ __forceinline__ size_t IDX(int n, int k, int j, int i) { return ((size_t)k * n + j) * n + i; }
 
+// Workgroups go to the 8 XCDs in turn, each XCD with its own L2.  For a
+// stencil whose neighbours lie in nearby workgroups, renumber a grid of a
+// multiple of 8 workgroups so that each XCD runs one contiguous range:
+// workgroup b works on block (b % 8) (G / 8) + b / 8, and a neighbouring
+// block's rows are then in the same L2.
+__device__ __forceinline__ size_t xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  return (G & 7u) ? b : (size_t)(b & 7u) * (G >> 3) + (b >> 3);
+}
+__host__ __forceinline__ unsigned round8(size_t blocks) { return (unsigned)((blocks + 7) & ~(size_t)7); }
+
 // l = (k n + j) n + i, for l < 2^32: each quotient from a double-precision
 // reciprocal (within one of the true quotient: the relative error is about
 // 2^-52) and one correction step, instead of 64-bit integer division.
@@ -849,7 +860,7 @@ template <typename T>
 __global__ void k_f2_grad(const T *__restrict__ u, T *__restrict__ G, const double *__restrict__ lam, nl_args a) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;  // neighbours +-n, +-n^2 on this XCD
   if (l >= n3) return;
   int i, j, k;
   idx3(l, n, i, j, k);
@@ -874,7 +885,7 @@ __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *
                          int with_felas) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;  // neighbours +-n, +-n^2 on this XCD
   if (l >= n3) return;
   int i, j, k;
   idx3(l, n, i, j, k);
@@ -940,7 +951,7 @@ __global__ void k_ch_lap(const T *__restrict__ cfield, const T *__restrict__ t, 
                          const T *__restrict__ sub, nl_args a) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;  // neighbours +-n, +-n^2 on this XCD
   if (l >= n3) return;
   int i, j, k;
   idx3(l, n, i, j, k);
@@ -1030,15 +1041,16 @@ void launch_nonlinear(etd_ctx *c, T *out, const T *sub) {
       k_nonlin_ac<T, 16><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
     if (aniso)
       for (int p = 0; p < c->nac; p++) {
-        k_f2_grad<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * p, F.gtmp, c->lam + 9 * p, a);
-        k_f2_div<T><<<nb, 256, 0, c->stream>>>(F.gtmp, out + n3 * p, F.E, a, p, felas);
+        k_f2_grad<T><<<round8(nb), 256, 0, c->stream>>>(F.u + n3 * p, F.gtmp, c->lam + 9 * p, a);
+        k_f2_div<T><<<round8(nb), 256, 0, c->stream>>>(F.gtmp, out + n3 * p, F.E, a, p, felas);
         if (sub) launch_sub(c, out + n3 * p, sub + n3 * p, n3);  // after the F2 / felas terms
       }
   }
   for (int q = 0; q < c->nch; q++) {
     const int v = c->nac + q;
     k_ch_t<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, a);
-    k_ch_lap<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr, a);
+    k_ch_lap<T><<<round8(nb), 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr,
+                                                   a);
   }
   etd_prof_end(c, PC_NONLIN);
 }
@@ -1846,17 +1858,26 @@ __device__ __forceinline__ void cell_grad_v(const double (&f)[8], double ihx, do
   uz = ((f[4] - f[0]) + (f[5] - f[1]) + (f[6] - f[2]) + (f[7] - f[3])) * 0.25 * ihz;
 }
 
-// grid (planes = n - 2g - 1, 1 + nch + couple, ECHUNK): y = 0 AC energy,
-// 1..nch CH energy of field nac + y - 1, nch + 1 couple gradient terms; each
-// plane is split into ECHUNK row blocks
+// work items (plane = n - 2g - 1 of them, kind, row block), kind 0 AC
+// energy, 1..nch CH energy of field nac + kind - 1, nch + 1 couple gradient
+// terms; each plane split into ECHUNK row blocks.  A cell reads planes k and
+// k + 1, so neighbouring planes share half their reads: the 1-D grid is
+// mapped so that each XCD (workgroups go to the 8 XCDs in turn) runs a
+// contiguous range of work items, planes fastest, and the shared plane is
+// read once into that XCD's L2 instead of once per XCD.
 #define ECHUNK 8
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, const double *__restrict__ lam,
                                                      double *__restrict__ part, diag_args a) {
   const int n = a.n, g = a.g, m = n - 2 * g - 1;
-  const int k = g + blockIdx.x, kind = blockIdx.y;
+  const int per = (int)gridDim.x / 8;  // gridDim.x = m kinds ECHUNK, a multiple of 8
+  const int w = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+  const int plane = w % m, rest = w / m;
+  const int kinds = (int)gridDim.x / (m * ECHUNK);
+  const int kind = rest % kinds, chunk = rest / kinds;
+  const int k = g + plane;
   const size_t n3 = (size_t)n * n * n;
-  const int r0 = (int)(((long)m * blockIdx.z) / ECHUNK), r1 = (int)(((long)m * (blockIdx.z + 1)) / ECHUNK);
+  const int r0 = (int)(((long)m * chunk) / ECHUNK), r1 = (int)(((long)m * (chunk + 1)) / ECHUNK);
   double e = 0.0;
   for (int t = r0 * m + threadIdx.x; t < r1 * m; t += blockDim.x) {
     const int j = g + t / m, i = g + t % m;
@@ -1919,7 +1940,7 @@ __global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, co
   e = wave_sum(e);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
   __syncthreads();
-  if (threadIdx.x == 0) part[((size_t)kind * m + blockIdx.x) * ECHUNK + blockIdx.z] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  if (threadIdx.x == 0) part[((size_t)kind * m + plane) * ECHUNK + chunk] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 template <typename T>
@@ -1946,7 +1967,8 @@ int launch_diag(etd_ctx *c, double *stats) {
   double *pst = c->part, *pen = c->part + (size_t)3 * ms * V;
   etd_prof_begin(c, PC_ENERGY);
   k_diag_stats<T><<<dim3(ms, V), 256, 0, c->stream>>>(U, pst, a);
-  k_diag_energy<T><<<dim3(me, kinds, ECHUNK), 256, 0, c->stream>>>(U, c->lam, pen, a);
+  static_assert(ECHUNK % 8 == 0, "the XCD mapping needs a multiple of 8 work items");
+  k_diag_energy<T><<<(unsigned)(me * kinds * ECHUNK), 256, 0, c->stream>>>(U, c->lam, pen, a);
   etd_prof_end(c, PC_ENERGY);
   const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds * ECHUNK;
   if (hipMemcpyAsync(c->hpart, c->part, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -2118,7 +2140,7 @@ void ch_mean_mark(etd_ctx *c, const float *u) {
   for (int q = 0; q < c->nch; q++) {
     k_ch_fill<<<nb, 256, 0, c->stream>>>(uni, c->n3, c->ch_sum, q, invN3);
     k_ch_t<float><<<nb, 256, 0, c->stream>>>(uni, t, a);
-    k_ch_lap<float><<<nb, 256, 0, c->stream>>>(uni, t, c->ch_const + c->n3 * q, nullptr, a);
+    k_ch_lap<float><<<round8(nb), 256, 0, c->stream>>>(uni, t, c->ch_const + c->n3 * q, nullptr, a);
   }
   c->ch_stale = 0;
 }
