@@ -31,7 +31,7 @@ def instances(count, seed):
         prm = W.scaled(W.beta_omega_64(seed=seed + k), N + 4)
         prm["landau"] = (float(rng.uniform(0.1, 0.4)), 3.0, 3.0)
         prm["eps0"] = W.eigenstrains(4, float(rng.uniform(0.01, 0.04)), seed + k)
-        prm["precision"] = 1
+        prm["precision"] = 2  # mixed: FP16 split-2 mode products
         prm["seed"] = seed + k
         out.append(prm)
     return out

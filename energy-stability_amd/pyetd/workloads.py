@@ -85,7 +85,7 @@ def beta_alpha_192(seed=2024):
 
 def beta_omega_256_coarsening(seed=2024):
     prm = beta_omega_64(seed)
-    prm.update(name="beta_omega_256_coarsening", n=260, precision=1, init="nuclei", nuclei_per_cell=16.0 ** -3,
+    prm.update(name="beta_omega_256_coarsening", n=260, precision=2, init="nuclei", nuclei_per_cell=16.0 ** -3,
                nuclei_radius=(3.0, 6.0), steps=20000)
     return prm
 
