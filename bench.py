@@ -82,7 +82,7 @@ def main():
                     help="run one part (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: libetd runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 

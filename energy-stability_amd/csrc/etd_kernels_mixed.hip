@@ -774,14 +774,14 @@ template <int MT>
 struct f16r_cfg {
   static constexpr int THREADS = 512, WAVES = 8, COLS = 256;
   static constexpr int APIECES = 2 * MT * 64;             // 16-byte pieces of one V chunk (2 terms)
-  static constexpr int AFULL = APIECES / THREADS;         // rounds every thread copies
-  static constexpr int AREM = APIECES - AFULL * THREADS;  // pieces of the last round
+  static constexpr int AFULL = APIECES / THREADS;         // passes every thread copies
+  static constexpr int AREM = APIECES - AFULL * THREADS;  // pieces of the last pass
   static constexpr int ASZ = APIECES * 16;
   static constexpr int XSZ = 32 * COLS * 4;
   static constexpr int XPT = XSZ / 16 / THREADS;
   static constexpr int LDS = 2 * ASZ + 3 * XSZ;
   static constexpr bool FITS = LDS <= 160 * 1024;  // MT <= 16 (n <= 256)
-  static_assert(AREM % 64 == 0, "the last round of V pieces is whole waves");
+  static_assert(AREM % 64 == 0, "the last pass of V pieces is whole waves");
   static_assert(XPT == 4, "the partial wait below counts 4 copies");
 };
 

@@ -7,7 +7,7 @@ state is dt_best = dt_k sqrt(tol / err_k) (ETD1's local error grows as dt^2),
 clipped to [dt_min, dt_max]: what a perfect controller would have proposed.
 The target is log10(dt_best / dt_plain), dt_plain being the plain
 controller's proposal for the same step (feature 15), so the network learns
-a correction to the plain rule, which the driver bounds to a decade.  The network is trained on the GPU by libetd's own kernels
+a correction to the plain rule, which the adaptive stepper bounds to a decade.  The network is trained on the GPU by libetd's own kernels
 (etd_mlp_train) on standardised features and targets."""
 import numpy as np
 

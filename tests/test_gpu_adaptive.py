@@ -1,6 +1,6 @@
 """GPU tests of adaptive ETD and the learned dt controller, through the C ABI.
 
-- The driver with every attempt accepted and no extrapolation is plain
+- The adaptive stepper with every attempt accepted and no extrapolation is plain
   stepping at dt/2, bit for bit.
 - Its accept / reject decisions, error estimates, energies and states match
   the CPU restatement on the FP64 oracle (oracle/adaptive.py).
