@@ -186,7 +186,7 @@ extern "C" int etd_autotune(const etd_params *p, const double *V, const double *
       for (int f = 0; f < 2 && rc == ETD_OK; f++) {
         etd_tuning t = {mp_cands[a], c->mixed ? sw_cands[b] : 0, fft_cands[f], 0.0};
         if (c->mixed && t.split_waves == 8 && (c->n + 15) / 16 > 14) continue;
-        if (t.fft_variant == 1 && (c->N % 16 || !(c->p.elastic == 1 && c->nac > 0))) continue;
+        if (t.fft_variant == 1 && (!fft16_supported(c->N) || !(c->p.elastic == 1 && c->nac > 0))) continue;
         if ((rc = etd_set_tuning(c, &t))) break;
         if ((rc = etd_time_steps(c, 1, 0, ms, nullptr))) break;  // warm: first launches
         if ((rc = etd_time_steps(c, steps, 0, ms, nullptr))) break;

@@ -162,6 +162,8 @@ template <typename T>
 void launch_felas_materialize(etd_ctx *c, T *felas);
 template <typename T>
 int launch_diag(etd_ctx *c, double *stats);
+/* the two-stage register FFT (k_fft16) handles this N */
+bool fft16_supported(int N);
 /* CH mean conservation of the mixed path: ch_mean_mark takes the interior
  * sums of the CH fields of the state u (V fields, stride n3) as the targets
  * if the fields were set since the last mark; ch_mean_restore shifts the CH

@@ -1416,10 +1416,21 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   }
 }
 
+// N = 16 R2 with R2 one of the instantiated register DFT lengths below
+bool fft16_supported(int N) {
+  if (N % 16) return false;
+  switch (N / 16) {
+    case 2: case 3: case 4: case 5: case 6: case 8: case 9: case 10: case 12: case 15: case 16:
+      return true;
+    default:
+      return false;
+  }
+}
+
 template <int SGN, typename T2, int IO = 0>
 static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis, fft_io io = fft_io{}) {
   const int N = c->N;
-  if (N % 16) return false;
+  if (!fft16_supported(N)) return false;
   dim3 grid((unsigned)((size_t)N * N / 16), npair);
   const double2 *tw = (const double2 *)c->tw;
   switch (N / 16) {
@@ -1722,7 +1733,9 @@ void launch_elastic(etd_ctx *c) {
   const int slot0 = (int)(Lbase % c->ring);
   // two-stage FFT: the pack is the forward x pass's input and the unpack the
   // inverse z pass's output
-  const bool fused = c->fft_variant == 1 && N % 16 == 0;
+  // (the fused passes exist only where the two-stage kernel does: N = 16 R2
+  // with an instantiated R2; elsewhere pack, radix passes and unpack)
+  const bool fused = c->fft_variant == 1 && fft16_supported(N);
   fft_io io{F.u, F.E, c->n, c->g, c->nac, c->ring, slot0};
   if (fused) {
     etd_prof_begin(c, PC_FFT);

@@ -103,6 +103,7 @@ def test_mixed_nonlinear_and_elastic_match_oracle():
 
 MIXED_CASES = [
     dict(n=16, nac=3, nch=1, etd2=1, elastic=1, aniso=0, steps=5),
+    dict(n=20, nac=3, nch=1, etd2=0, elastic=1, aniso=0, steps=4),  # N = 16: no two-stage FFT instance
     dict(n=36, nac=4, nch=1, etd2=0, elastic=1, aniso=1, steps=8),
 ]
 

@@ -144,6 +144,9 @@ CASES = [
     dict(n=12, nac=2, nch=0, etd2=0, elastic=0, aniso=0, steps=6),
     dict(n=12, nac=2, nch=0, etd2=1, elastic=0, aniso=1, steps=6),
     dict(n=16, nac=3, nch=1, etd2=1, elastic=1, aniso=0, steps=5),
+    # N = 16: divisible by 16 but without a two-stage FFT instance (R2 = 1):
+    # elasticity must take the pack / radix / unpack path
+    dict(n=20, nac=3, nch=1, etd2=1, elastic=1, aniso=0, steps=3),
     dict(n=14, nac=4, nch=0, etd2=0, elastic=1, aniso=1, steps=5),
     dict(n=36, nac=4, nch=1, etd2=1, elastic=1, aniso=0, steps=8),
 ]
