@@ -40,7 +40,7 @@ def pair(prm, fields, precision=1):
     return o, g
 
 
-@pytest.mark.parametrize("n", [12, 36, 68, 100])
+@pytest.mark.parametrize("n", [12, 36, 68, 100, 260])
 @pytest.mark.parametrize("rotate", [True, False])
 def test_split3_modeprod_matches_oracle(n, rotate):
     say(f"split-3 modeprod n={n} rotate={rotate}")
@@ -160,7 +160,7 @@ def test_mixed_path_conserves_the_ch_mean(etd2):
     g.close()
 
 
-@pytest.mark.parametrize("n", [12, 36, 68, 100])
+@pytest.mark.parametrize("n", [12, 36, 68, 100, 260])
 @pytest.mark.parametrize("rotate", [True, False])
 def test_f16_split2_modeprod_matches_oracle(n, rotate):
     """FP16 split-2 mode products (ETD_PREC_MIXED_F16): power-of-two scaled V

@@ -84,9 +84,10 @@ def test_nonlinear_ch_matches_oracle():
     assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("n", [10, 12, 20, 36, 52])
+@pytest.mark.parametrize("n", [10, 12, 20, 36, 52, 100, 196])
 def test_fft3d_matches_numpy(n):
-    """Kernel (d): own mixed-radix FFT (N = 6, 8, 16, 32, 48) vs numpy."""
+    """Kernel (d): own FFT (N = 6, 8, 16 on radix passes; 32, 48, 96, 192 on
+    the two-stage register kernel, R2 = 2, 3, 6, 12) vs numpy."""
     N = n - 4
     say(f"fft3d N={N}")
     g = pyetd.Etd.from_dict(small_params(n=n, nac=2, elastic=1))
@@ -99,7 +100,7 @@ def test_fft3d_matches_numpy(n):
     assert np.abs(b / N ** 3 - x).max() < 1e-13
 
 
-@pytest.mark.parametrize("nac,n", [(2, 12), (3, 16), (5, 22)])
+@pytest.mark.parametrize("nac,n", [(2, 12), (3, 16), (5, 22), (3, 20), (4, 36), (5, 52)])
 def test_elastic_matches_oracle_over_two_evaluations(nac, n):
     """felas on every extended cell, including the ghost cells that inherit
     earlier variants through the shared transfer buffer."""
