@@ -39,10 +39,20 @@ def gpu(prm, fields):
     return g
 
 
-def test_all_accepted_without_extrapolation_is_half_steps():
-    say("adaptive == half steps")
+@pytest.mark.parametrize("precision,etd2,elastic", [(0, 0, 1), (2, 0, 1), (0, 1, 0), (0, 1, 1)])
+def test_all_accepted_without_extrapolation_is_half_steps(precision, etd2, elastic):
+    """With every attempt accepted and no extrapolation the adaptive stepper is
+    plain stepping at dt / 2: bit for bit for ETD1, in FP64 and in the mixed
+    path (whose full and first half step share one phi pass and whose CH mean
+    restoration runs per state).  For ETD2 the attempt's full step makes two
+    extra elastic evaluations, and the ghost cells of felas read the ring's
+    older slots (the reference's shared transfer buffer), so the half steps
+    see a different history in those cells: without elasticity ETD2 is bit
+    for bit too, with it the difference stays at the ghost-cell level."""
+    say(f"adaptive == half steps precision={precision} etd2={etd2} elastic={elastic}")
     dt = 2.0 ** -6
-    prm = small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, dt=dt)
+    prm = small_params(n=16, nac=3, nch=1, elastic=elastic, aniso=1, dt=dt, etd2=etd2)
+    prm["precision"] = precision
     f = noise_fields(prm, amp=0.3)
     a = gpu(prm, f)
     b = gpu(dict(prm, dt=dt / 2), f)
@@ -52,7 +62,12 @@ def test_all_accepted_without_extrapolation_is_half_steps():
     assert a.time == 3 * dt and len(log) == 3
     b.step(6)
     for v in range(4):
-        assert np.array_equal(a.get_field(v), b.get_field(v))
+        if etd2 and elastic:
+            d = np.abs(a.get_field(v) - b.get_field(v)).max()
+            say(f"  field {v}: max |d| = {d:.2e}")
+            assert d < 1e-6
+        else:
+            assert np.array_equal(a.get_field(v), b.get_field(v))
 
 
 @pytest.mark.parametrize("elastic", [0, 1])
