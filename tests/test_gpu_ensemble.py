@@ -24,7 +24,8 @@ def say(msg):
 MEMBERS = [
     dict(n=36, nac=4, nch=0, etd2=0, elastic=1, aniso=0, precision=0),   # N = 32, FP64
     dict(n=20, nac=2, nch=1, etd2=1, elastic=0, aniso=1, precision=0),   # ETD2 + CH
-    dict(n=52, nac=4, nch=0, etd2=0, elastic=1, aniso=0, precision=1),   # N = 48, mixed
+    dict(n=52, nac=4, nch=0, etd2=0, elastic=1, aniso=0, precision=1),   # N = 48, mixed BF16
+    dict(n=68, nac=4, nch=1, etd2=0, elastic=1, aniso=0, precision=2),   # N = 64, mixed FP16 + CH mean
 ]
 
 
@@ -39,7 +40,7 @@ def build(spec, seed):
 
 
 def test_ensemble_replays_equal_solo_stepping():
-    say("ensemble of 3 uneven members, 2 replays of 3 steps")
+    say("ensemble of 4 uneven members, 2 replays of 3 steps")
     ens_m = [build(s, 10 + i) for i, s in enumerate(MEMBERS)]
     solo = [build(s, 10 + i) for i, s in enumerate(MEMBERS)]
     K = 3  # nac = 4, ETD1: the felas ring (6 slots) repeats every 3 steps
