@@ -155,7 +155,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   if (spv && atoi(spv) == 0) c->split_persistent = 0;
   c->split_kernel = 0;
   const char *skv = getenv("ETD_SPLIT_KERNEL");
-  if (skv && atoi(skv) >= 1 && atoi(skv) <= 7) c->split_kernel = atoi(skv);
+  if (skv && atoi(skv) >= 1 && atoi(skv) <= 9) c->split_kernel = atoi(skv);
   c->nl_batch = 0;
   const char *nlb = getenv("ETD_NL_BATCH");
   if (nlb && (atoi(nlb) == 4 || atoi(nlb) == 8 || atoi(nlb) == 12 || atoi(nlb) == 16)) c->nl_batch = atoi(nlb);

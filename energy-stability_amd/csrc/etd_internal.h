@@ -74,7 +74,8 @@ struct etd_ctx {
   int split_persistent;    /* 1: persistent workgroups looping over tiles (default) */
   int split_kernel;        /* BF16 split-3: 0: field chunks staged through LDS, 8 waves x 32 columns;
                               1: field chunks prefetched into registers; 2: LDS-staged, 4 waves x 64
-                              columns; 3: two workgroups per CU; 4: wave grid.  FP16 split-2 (n <= 256):
+                              columns; 3: two workgroups per CU; 4: wave grid; 9: the default with
+                              32-byte field copies (default: whole 128-byte runs).  FP16 split-2 (n <= 256):
                               0: three-slot field ring, whole-line copies; 5: three-slot ring, 32-byte
                               copies; 7: two-slot ring */
   int nl_batch;            /* AC fields whose loads the nonlinear kernel issues together (4, 8, 12, 16);

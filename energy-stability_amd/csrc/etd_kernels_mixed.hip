@@ -257,7 +257,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split(const 
 // (field, 32 WAVES columns); the double-buffered chunk pipeline runs across
 // tile boundaries, so the next tile's first chunk loads during the current
 // tile's last chunk and no workgroup start-up is exposed.
-template <int MT, int NT, int WAVES, int UT = 2>
+template <int MT, int NT, int WAVES, int UT = 2, int XL = 1>
 __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     const uint16_t *__restrict__ A3, const float *__restrict__ in, size_t in_fs, float *__restrict__ out,
     size_t out_fs, int n, int rotate, int accumulate, int a_chunk, int nfields) {
@@ -286,16 +286,30 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
     for (int e = 0; e < CF::APT; e++)
       copy16_to_lds(rsA, sbase + (e * CF::THREADS + wave * 64) * 16,
                     (int)((size_t)kc * a_chunk + ((size_t)e * CF::THREADS + threadIdx.x) * 16));
-    // X piece e: k group q = e % 4 of 32-column block cb; lane L brings
-    // column 32 cb + L / 2, k half L % 2, to [q][col][8 k]
+    // XL = 0: X piece e is k group q = e % 4 of 32-column block cb; lane L
+    // brings column 32 cb + L / 2, k half L % 2, to [q][col][8 k] (32 bytes
+    // from each of 32 columns per instruction).  XL = 1: 16-byte piece p of
+    // the chunk holds k quad (p % 8) ^ (c % 8) of column c = p / 8, so each
+    // instruction brings 8 whole 128-byte column runs.
 #pragma unroll
     for (int e = 0; e < CF::XPT; e++) {
-      const int qe = e & 3, cb = wave * (UT / 2) + (e >> 2);
-      const long col = wgc + 32 * cb + (lane >> 1);
-      const int k = 32 * kc + 8 * qe + 4 * (lane & 1);
+      long col;
+      int k;
+      uint32_t ldsoff;
+      if constexpr (XL) {
+        const int c = (e * CF::THREADS + wave * 64 + lane) >> 3;
+        col = wgc + c;
+        k = 32 * kc + 4 * ((lane & 7) ^ (c & 7));
+        ldsoff = (uint32_t)(e * CF::THREADS + wave * 64) * 16;
+      } else {
+        const int qe = e & 3, cb = wave * (UT / 2) + (e >> 2);
+        col = wgc + 32 * cb + (lane >> 1);
+        k = 32 * kc + 8 * qe + 4 * (lane & 1);
+        ldsoff = (uint32_t)(qe * CF::COLS + 32 * cb) * 32;
+      }
       const bool ok = col < n2 && k < n;
       const int voff = ok ? (int)((k + (long)n * col) * 4) : (int)0x80000000;
-      copy16_to_lds(rsX, sbase + CF::ASZ + (qe * CF::COLS + 32 * cb) * 32, voff);
+      copy16_to_lds(rsX, sbase + CF::ASZ + ldsoff, voff);
     }
   };
 
@@ -328,9 +342,17 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 4) void k_modeprod_split_p(
       // column tile u's split; tiles after the first are split between the
       // MFMAs of row tile 0, so that their VALU work overlaps MFMA work
       auto split_tile = [&](int u) {
-        const float *bp = sx + ((size_t)q * CF::COLS + 16 * UT * wave + 16 * u + r16) * 8;
-        const float4 lo = *reinterpret_cast<const float4 *>(bp);
-        const float4 hi = *reinterpret_cast<const float4 *>(bp + 4);
+        float4 lo, hi;
+        if constexpr (XL) {
+          const int c = 16 * UT * wave + 16 * u + r16;
+          const float4 *cp = reinterpret_cast<const float4 *>(sx) + 8 * c;
+          lo = cp[(2 * q) ^ (c & 7)];
+          hi = cp[(2 * q + 1) ^ (c & 7)];
+        } else {
+          const float *bp = sx + ((size_t)q * CF::COLS + 16 * UT * wave + 16 * u + r16) * 8;
+          lo = *reinterpret_cast<const float4 *>(bp);
+          hi = *reinterpret_cast<const float4 *>(bp + 4);
+        }
         const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -1237,6 +1259,8 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
                                 CF::LDS);
       (void)hipFuncSetAttribute((const void *)k_modeprod_split_p<MT, NT, W>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
+      (void)hipFuncSetAttribute((const void *)k_modeprod_split_p<MT, NT, W, 2, 0>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS);
       attr_set = true;
     }
     const long ncb = (n2 + CF::COLS - 1) / CF::COLS;
@@ -1260,8 +1284,12 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
     if (c->split_persistent) {
       const long ntiles = ncb * nfields;
       const long nwg = ntiles < c->num_cu ? ntiles : c->num_cu;  // one workgroup per CU (LDS-bound)
-      k_modeprod_split_p<MT, NT, W><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
-          A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+      if (c->split_kernel == 9)  // 32-byte field copies, [k/8][col][8 k] layout
+        k_modeprod_split_p<MT, NT, W, 2, 0><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+            A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
+      else
+        k_modeprod_split_p<MT, NT, W><<<(unsigned)nwg, CF::THREADS, CF::LDS, c->stream>>>(
+            A3, in, in_fs, out, out_fs, n, rotate, accumulate, (int)chunk_bytes<MT, NT>(), nfields);
     } else {
       dim3 grid((unsigned)ncb, nfields);
       k_modeprod_split<MT, NT, W><<<grid, CF::THREADS, CF::LDS, c->stream>>>(A3, in, in_fs, out, out_fs, n, rotate,

@@ -240,3 +240,32 @@ def test_f16_three_slot_pipeline_equals_default(n, nac, nch, kernel, monkeypatch
     for v, (x, y) in enumerate(zip(a, b)):
         assert np.array_equal(x, y), f"field {v}: max |d| {np.abs(x - y).max():.3e}"
     monkeypatch.delenv("ETD_SPLIT_KERNEL", raising=False)
+
+
+@pytest.mark.parametrize("n,nac,nch", [(20, 2, 1), (36, 2, 1), (132, 6, 1), (260, 2, 1)])
+def test_bf16_whole_line_copies_equal_32byte_copies(n, nac, nch, monkeypatch):
+    """The BF16 split-3 kernel's default field copies (whole 128-byte column
+    runs into an XOR-swizzled LDS layout) give exactly the products of its
+    32-byte copies into [k/8][col][8 k] (ETD_SPLIT_KERNEL=9): bit-identical
+    fields after three elastic steps, with 8 waves (n <= 224) and with the
+    4-wave tiling of larger n (260)."""
+    say(f"bf16 whole-line copies n={n}")
+    prm = small_params(n=n, nac=nac, nch=nch, elastic=1, dt=0.02)
+    fields = noise_fields(prm, amp=0.3)
+    Vm, Vim, D = O.eigbasis(n, 3)
+
+    def run(kernel):
+        monkeypatch.setenv("ETD_SPLIT_KERNEL", kernel)
+        g = pyetd.Etd(pyetd.params_from_dict(dict(prm, precision=1)), Vm, Vim, D)
+        for v, f in enumerate(fields):
+            g.set_field(v, f)
+        g.step(3)
+        out = [g.get_field(v) for v in range(nac + nch)]
+        g.close()
+        return out
+
+    a = run("9")
+    b = run("0")
+    for v, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), f"field {v}: max |d| {np.abs(x - y).max():.3e}"
+    monkeypatch.delenv("ETD_SPLIT_KERNEL", raising=False)
