@@ -286,22 +286,27 @@ __global__ __launch_bounds__(256) void k_accept_stats(const T *__restrict__ u, c
                                                         : ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
 }
 
-// max |u - b| per block into part[block * RCOLS] and, with extrapolate,
-// u = 2 u - b (local extrapolation of the step-doubling pair) in the same pass
+// max that keeps a NaN once it has seen one (fmax drops NaN operands, which
+// would hide a NaN cell from the error estimate)
+__host__ __device__ __forceinline__ double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// max |u - b| per block into part[block * RCOLS] (NaN if any cell is NaN) and,
+// with extrapolate, u = 2 u - b (local extrapolation of the step-doubling
+// pair) in the same pass
 template <typename T>
 __global__ __launch_bounds__(256) void k_maxdiff_extrap(T *__restrict__ u, const T *__restrict__ b, size_t cnt,
                                                         int extrapolate, double *__restrict__ part) {
   double m = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x) {
     const double x = u[i], y = b[i];
-    m = fmax(m, fabs(x - y));
+    m = nanmax(m, fabs(x - y));
     if (extrapolate) u[i] = (T)(2.0 * x - y);
   }
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_down(m, o, 64));
   __shared__ double sh[4];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
 }
 
 
@@ -506,6 +511,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   int rc = energy_of<T>(c, ekind, with_el, &E, &Eabs, F.u, &rf, &hf);
   double Escale = fabs(E);
   if (rc) return rc;
+  if (!isfinite(E)) {
+    snprintf(c->err, sizeof(c->err), "adaptive: the energy of the initial state is not finite (%g)", E);
+    return ETD_ERR_NONFINITE;
+  }
   if (!hf && (rc = features_of<T>(c, F.u, F.u, &rf))) return rc;
   double hist[4] = {o->dt0, o->dt0, o->dt0, o->dt0};
   double err_last = 0.1 * o->tol, dEdt = 0.0, plain_next = o->dt0;
@@ -582,16 +591,30 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     double err = 0.0;
     k_maxdiff_extrap<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt, o->extrapolate, c->ad_part);
     if ((rc = fetch_part(c, 1))) break;
-    for (int i = 0; i < RBLOCKS; i++) err = fmax(err, c->ad_hpart[i * RCOLS]);
+    for (int i = 0; i < RBLOCKS; i++) err = nanmax(err, c->ad_hpart[i * RCOLS]);
     double Enew, Eabs_new;
     run_feat nf;
     int nhf = 0;
     if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new, S0, &nf, &nhf))) break;
     const double scale = ekind == 0 ? fmax(Escale, fabs(E)) : fmax(Eabs, Eabs_new);
-    const double fac = err > 0.0 ? o->safety * sqrt(o->tol / err) : o->grow;
-    const int err_ok = err <= o->tol && isfinite(err);
-    const int energy_ok = !o->energy_test || Enew <= E + slack * scale;
-    const int forced = dt <= o->dt_min * (1.0 + 1e-12) && isfinite(err);
+    // a non-finite candidate (error estimate or energy) is never accepted:
+    // it is rejected with the largest shrink, and at dt_min the run stops
+    // with an error, the fields restored to the last accepted state
+    const int finite = isfinite(err) && isfinite(Enew);
+    const double fac = !finite ? o->shrink : (err > 0.0 ? o->safety * sqrt(o->tol / err) : o->grow);
+    const int err_ok = err <= o->tol && finite;
+    const int energy_ok = finite && (!o->energy_test || Enew <= E + slack * scale);
+    const int at_min = dt <= o->dt_min * (1.0 + 1e-12);
+    if (at_min && !finite) {
+      ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
+      c->elastic_fresh = 0;
+      snprintf(c->err, sizeof(c->err),
+               "adaptive: the candidate at dt_min = %g is not finite (err %g, energy %g) at t = %g", dt, err, Enew,
+               c->t);
+      rc = ETD_ERR_NONFINITE;
+      break;
+    }
+    const int forced = at_min && finite;
     if ((err_ok && energy_ok) || forced) {
       if (!nhf && (rc = features_of<T>(c, F.u, S0, &nf))) break;
       c->t += dt;

@@ -36,6 +36,7 @@ extern "C" const char *etd_error_string(int code) {
     case ETD_ERR_UNSUPPORTED: return "unsupported configuration";
     case ETD_ERR_DEVICE: return "HIP device error";
     case ETD_ERR_NOMEM: return "out of memory";
+    case ETD_ERR_NONFINITE: return "non-finite state";
     default: return "unknown error";
   }
 }
@@ -153,9 +154,9 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   c->split_persistent = 1;
   const char *spv = getenv("ETD_SPLIT_PERSIST");
   if (spv && atoi(spv) == 0) c->split_persistent = 0;
-  c->split_kernel = 0;
+  c->split_kernel = 10;
   const char *skv = getenv("ETD_SPLIT_KERNEL");
-  if (skv && atoi(skv) >= 1 && atoi(skv) <= 9) c->split_kernel = atoi(skv);
+  if (skv && atoi(skv) >= 0 && atoi(skv) <= 10) c->split_kernel = atoi(skv);
   c->nl_batch = 0;
   const char *nlb = getenv("ETD_NL_BATCH");
   if (nlb && (atoi(nlb) == 4 || atoi(nlb) == 8 || atoi(nlb) == 12 || atoi(nlb) == 16)) c->nl_batch = atoi(nlb);
@@ -167,6 +168,8 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   const char *sv = getenv("ETD_SPLIT");
   if (sv && atoi(sv) == 2) c->split_terms = 2;
   if (c->split_f16) c->split_terms = 2;
+  c->split_zs = c->split_f16 && c->split_kernel == 10 && f16_zs_supported(p->nx);
+  if (c->split_kernel == 10 && !c->split_zs) c->split_kernel = 0;  // BF16, or n > 256: the default of those
   if (!c->mixed) {
     HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
     c->N1 = c->u + n3 * V;
@@ -462,9 +465,11 @@ void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumul
   } else {
     const uint16_t *A = forward ? c->Vi3 : c->V3;
     const float ai = forward ? c->Vi3_inv : c->V3_inv;
-    launch_modeprod_split(c, A, in, n3, F.w1, n3, nf, 1, 0, ai);
-    launch_modeprod_split(c, A, F.w1, n3, F.w2, n3, nf, 1, 0, ai);
-    launch_modeprod_split(c, A, F.w2, n3, out, n3, nf, 1, accumulate, ai);
+    // contracting the slowest axis rotates the other way: z, y, x, back to xyz
+    const int rot = c->split_zs ? 2 : 1;
+    launch_modeprod_split(c, A, in, n3, F.w1, n3, nf, rot, 0, ai);
+    launch_modeprod_split(c, A, F.w1, n3, F.w2, n3, nf, rot, 0, ai);
+    launch_modeprod_split(c, A, F.w2, n3, out, n3, nf, rot, accumulate, ai);
   }
 }
 
@@ -644,8 +649,11 @@ extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
 }
 
 extern "C" int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *in, double *out, int rotate) {
-  if (!c || !A || !in || !out) return ETD_ERR_ARG;
+  if (!c || !A || !in || !out || rotate < 0 || rotate > 3) return ETD_ERR_ARG;
   if (n != c->n) return seterr(c, ETD_ERR_ARG, "modeprod: n must equal the context edge %d", c->n);
+  if (rotate >= 2 && !(c->split_f16 && f16_zs_supported(n)))
+    return seterr(c, ETD_ERR_UNSUPPORTED, "modeprod: the slowest-axis contraction (rotate %d) is the FP16 split-2 "
+                  "kernel's, for n <= 256", rotate);
   if (c->mixed) {
     const size_t sb = split_A_bytes(n, c->split_terms);
     uint16_t *h3 = (uint16_t *)malloc(sb);

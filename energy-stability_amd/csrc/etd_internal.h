@@ -77,7 +77,9 @@ struct etd_ctx {
                               columns; 3: two workgroups per CU; 4: wave grid; 9: the default with
                               32-byte field copies (default: whole 128-byte runs).  FP16 split-2 (n <= 256):
                               0: three-slot field ring, whole-line copies; 5: three-slot ring, 32-byte
-                              copies; 7: two-slot ring */
+                              copies; 7: two-slot ring; 10 (the default where f16_zs_supported): the
+                              three-slot ring contracting the slowest axis (whole 1 KB rows per copy) */
+  int split_zs;            /* the step's FP16 mode products contract the slowest axis (split_kernel 10) */
   int nl_batch;            /* AC fields whose loads the nonlinear kernel issues together (4, 8, 12, 16);
                               0: by storage type and field count */
   int num_cu;              /* compute units of the device */
@@ -211,5 +213,10 @@ void arrange_split_A(int n, const double *Acm, uint16_t *A3host, int terms);
 /* FP16 split-2 arrangement of A / s (s a power of two bringing max |A| to
  * [2^13, 2^14)); returns 1 / s. */
 float arrange_split_A_f16(int n, const double *Acm, uint16_t *A2host);
+/* The FP16 split-2 three-slot kernel can contract the slowest axis at this n
+ * (launch_modeprod_split with rotate = 2: rotated, out[m + n col]; 3: not
+ * rotated, out[col + n^2 m]; col = i + n j, contracted index k of
+ * in[col + n^2 k]) */
+bool f16_zs_supported(int n);
 
 #endif

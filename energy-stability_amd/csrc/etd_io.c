@@ -95,7 +95,7 @@ int etd_parse_ini_alpha(const char *path, etd_params *p) {
   if (!path || !p) return ETD_ERR_ARG;
   FILE *f = fopen(path, "r");
   if (!f) return ETD_ERR_IO;
-  int ok = 1, chk[3], nout, approx;
+  int ok = 1;
   ok &= fscanf(f, "%d,%d,%d", &p->nx, &p->ny, &p->nz) == 3;
   ok &= fscanf(f, "%d,%d,%d", &p->px, &p->py, &p->pz) == 3;
   ok &= fscanf(f, "%d", &p->nghost) == 1;
@@ -110,9 +110,9 @@ int etd_parse_ini_alpha(const char *path, etd_params *p) {
   ok &= fscanf(f, "%lf,%lf,%lf", &p->xmin, &p->ymin, &p->zmin) == 3;
   ok &= fscanf(f, "%lf,%lf,%lf", &p->xmax, &p->ymax, &p->zmax) == 3;
   ok &= fscanf(f, "%lf,%lf,%lf", &p->kkx, &p->kky, &p->kkz) == 3;
-  ok &= fscanf(f, "%d,%d,%d", &chk[0], &chk[1], &chk[2]) == 3;
-  ok &= fscanf(f, "%d", &nout) == 1;
-  ok &= fscanf(f, "%d", &approx) == 1;
+  ok &= fscanf(f, "%d,%d,%d", &p->checkpoint, &p->nchk, &p->chk) == 3;
+  ok &= fscanf(f, "%d", &p->nout) == 1;
+  ok &= fscanf(f, "%d", &p->approx) == 1;
   fclose(f);
   if (!ok) return ETD_ERR_IO;
   p->epn2 = 0.048;

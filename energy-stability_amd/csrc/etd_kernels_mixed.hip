@@ -22,6 +22,7 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x4m __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
 
 // ---- global -> LDS copies (buffer_load ... lds), as in the FP64 kernel
 __device__ __forceinline__ i32x4m mk_rsrc(const void *base, uint32_t bytes) {
@@ -149,6 +150,35 @@ __device__ __forceinline__ void store_rows_(const f32x4 (&acc)[MTH][UT], float *
         o += s_row;
       }
       o += 12u * s_row;
+    }
+  }
+}
+
+// Epilogue with out[m + n col]: a lane's 4 rows 16 t + 4 q + [0, 4) of one
+// column are 16 contiguous bytes, one 128-bit store (or load-add-store).  n is
+// a multiple of 4, so the 4 rows are inside or outside the matrix together.
+template <int MT, int UT>
+__device__ __forceinline__ void store_tile_m4(const f32x4 (&acc)[MT][UT], float *dst, int n, long col0, int q,
+                                              int accumulate) {
+  const long n2 = (long)n * n;
+  const __amdgpu_buffer_rsrc_t rD =
+      __builtin_amdgcn_make_buffer_rsrc((void *)dst, 0, (int)((size_t)n * n2 * 4), 0x00020000);
+  const uint32_t OOB = 0x80000000u;
+#pragma unroll
+  for (int u = 0; u < UT; u++) {
+    const long col = col0 + 16 * u;
+    const bool cok = col < n2;
+    const uint32_t o0 = (uint32_t)col * (uint32_t)n * 4u + (uint32_t)(16 * q);
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      const bool rok = 16 * t + 4 * q < n;
+      const int off = (int)((cok && rok) ? o0 + 64u * (uint32_t)t : OOB);
+      f32x4 v = acc[t][u];
+      if (accumulate) {
+        const f32x4 o = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rD, off, 0, 0));
+        v = v + o;
+      }
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4v, v), rD, off, 0, 0);
     }
   }
 }
@@ -792,6 +822,19 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_p(const uint16_t *__res
 // [col][k/4 ^ (col & 7)] in 16-byte pieces, so each copy instruction takes
 // 8 whole 128-byte lines (8 columns x 32 k) and the XOR keeps the 16 lanes of
 // an operand read on distinct banks as in XL = 0.
+// XL = 2 contracts the SLOWEST axis instead: X(k, col) = in[col + n^2 k]
+// (col = i + n j), written rotated as out[m + n col] (or in place of k,
+// out[col + n^2 m], without rotation).  Three rotated products still run
+// x, y, z -> x', y', z' (z first) and end in the same spectral layout.  The
+// 32 rows of a field chunk are 32 runs of 256 consecutive floats (1 KB), so
+// every copy instruction reads whole 128-byte lines and no line is shared
+// between K chunks: the contract-fastest kernels' 784-byte columns start
+// mid-line for 7 of 8 columns, and the half line that the next chunk needs
+// is fetched again when it has left the cache.  A row sits in LDS as 64
+// 16-byte pieces, piece L holding the row's piece L ^ 4 (r / 8) for row r of
+// the chunk: a lane reads the 8 rows 8 q .. 8 q + 7 of its column, and the
+// XOR puts the four q groups of an operand read on four distinct groups of
+// 16 banks.
 template <int MT>
 struct f16r_cfg {
   static constexpr int THREADS = 512, WAVES = 8, COLS = 256;
@@ -849,6 +892,21 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
     rsX.y = __builtin_amdgcn_readfirstlane(rsX.y);
     rsX.z = __builtin_amdgcn_readfirstlane(rsX.z);
     const uint32_t sbase = lds0 + 2 * CF::ASZ + (gg % 3) * CF::XSZ;
+    if constexpr (XL == 2) {
+      // row r = 8 e + wave of the chunk: one 1 KB run, lane L -> LDS piece L
+      // holding the run's piece L ^ 4 e (n^2 is a multiple of 4: a piece is
+      // inside or outside the matrix as a whole)
+#pragma unroll
+      for (int e = 0; e < CF::XPT; e++) {
+        const int r = 8 * e + wave;
+        const int k = 32 * kc + r;
+        const long col = wgc + 4 * (lane ^ (4 * e));
+        const bool ok = col < n2 && k < n;
+        const int voff = ok ? (int)((col + n2 * k) * 4) : (int)0x80000000;
+        copy16_to_lds(rsX, sbase + r * (CF::COLS * 4), voff);
+      }
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < CF::XPT; e++) {
       long col;
@@ -895,18 +953,26 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
     float back[2];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
-      float4 lo, hi;
-      if constexpr (XL) {
+      float x[8];
+      if constexpr (XL == 2) {
         const int c = 32 * wave + 16 * u + r16;
-        const float4 *cp = reinterpret_cast<const float4 *>(sx) + 8 * c;
-        lo = cp[(2 * q) ^ (c & 7)];
-        hi = cp[(2 * q + 1) ^ (c & 7)];
+        const float *rp = sx + 8 * q * CF::COLS + 4 * ((c >> 2) ^ (4 * q)) + (c & 3);
+#pragma unroll
+        for (int e = 0; e < 8; e++) x[e] = rp[e * CF::COLS];
       } else {
-        const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
-        lo = *reinterpret_cast<const float4 *>(bp);
-        hi = *reinterpret_cast<const float4 *>(bp + 4);
+        float4 lo, hi;
+        if constexpr (XL) {
+          const int c = 32 * wave + 16 * u + r16;
+          const float4 *cp = reinterpret_cast<const float4 *>(sx) + 8 * c;
+          lo = cp[(2 * q) ^ (c & 7)];
+          hi = cp[(2 * q + 1) ^ (c & 7)];
+        } else {
+          const float *bp = sx + ((size_t)q * CF::COLS + 32 * wave + 16 * u + r16) * 8;
+          lo = *reinterpret_cast<const float4 *>(bp);
+          hi = *reinterpret_cast<const float4 *>(bp + 4);
+        }
+        x[0] = lo.x, x[1] = lo.y, x[2] = lo.z, x[3] = lo.w, x[4] = hi.x, x[5] = hi.y, x[6] = hi.z, x[7] = hi.w;
       }
-      const float x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
       float m = 0.f;
 #pragma unroll
       for (int e = 0; e < 8; e++) m = fmaxf(m, fabsf(x[e]));
@@ -958,7 +1024,14 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
     if (kc == nkc - 1) {
       const int f = tile / ncb;
       const long wgcol = (long)(tile % ncb) * CF::COLS;
-      store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+      if constexpr (XL == 2) {
+        if (rotate)
+          store_tile_m4<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, accumulate);
+        else  // k replaced by m in place: the strides of the other kernels' rotated store
+          store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, 1, accumulate);
+      } else {
+        store_tile<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, rotate, accumulate);
+      }
     }
   }
 }
@@ -1229,9 +1302,15 @@ static void f16_launch(etd_ctx *c, const uint16_t *A2, const float *in, size_t i
                                   CR::LDS);
         (void)hipFuncSetAttribute((const void *)k_modeprod_f16_r3<MT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   CR::LDS);
+        (void)hipFuncSetAttribute((const void *)k_modeprod_f16_r3<MT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  CR::LDS);
         attr_r = true;
       }
-      if (c->split_kernel != 5)
+      if (rotate >= 2)  // contraction over the slowest axis: 2 rotated, 3 not
+        k_modeprod_f16_r3<MT, 2><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
+            A2, in, in_fs, out, out_fs, c->n, rotate == 2 ? 1 : 0, accumulate, (int)chunk_bytes<MT, 2>(), nfields,
+            a_inv);
+      else if (c->split_kernel != 5)
         k_modeprod_f16_r3<MT, 1><<<(unsigned)nwg, CR::THREADS, CR::LDS, c->stream>>>(
             A2, in, in_fs, out, out_fs, c->n, rotate, accumulate, (int)chunk_bytes<MT, 2>(), nfields, a_inv);
       else
@@ -1360,6 +1439,8 @@ static void mps_launch(etd_ctx *c, const uint16_t *A3, const float *in, size_t i
     go(std::integral_constant<int, 4>{});
   }
 }
+
+bool f16_zs_supported(int n) { return f16r_cfg<16>::FITS && (n + 15) / 16 <= 16; }
 
 void launch_modeprod_split(etd_ctx *c, const uint16_t *A3, const float *in, size_t in_fs, float *out, size_t out_fs,
                            int nfields, int rotate, int accumulate, float a_inv) {

@@ -23,7 +23,14 @@ _fp = C.POINTER(C.c_float)
 
 
 class EtdError(RuntimeError):
-    pass
+    """A negative return code of libetd: .code is the code (ETD_ERR_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+ERR_ARG, ERR_IO, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NOMEM, ERR_NONFINITE = -1, -2, -3, -4, -5, -6
 
 
 class EtdParams(C.Structure):
@@ -51,6 +58,7 @@ class EtdParams(C.Structure):
         ("eps0", C.c_double * (MAX_VAR * 9)),
         ("lam", C.c_double * (MAX_VAR * 9)),
         ("precision", C.c_int),
+        ("checkpoint", C.c_int), ("nchk", C.c_int), ("chk", C.c_int), ("nout", C.c_int), ("approx", C.c_int),
     ]
 
 
@@ -131,6 +139,10 @@ def lib():
             "etd_k_phiscale": (C.c_int, [ctxp, C.c_int, _dp, _dp]),
             "etd_time_steps": (C.c_int, [ctxp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]),
             "etd_calib_copy": (C.c_int, [ctxp, C.c_size_t]),
+            "etd_write_checkpoint": (C.c_int, [ctxp, C.c_int, C.c_int]),
+            "etd_read_checkpoint": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_int)]),
+            "etd_write_field2b": (C.c_int, [ctxp, C.c_int]),
+            "etd_detector_feed": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_ubyte), _fp]),
             "etd_adapt_defaults": (None, [C.POINTER(AdaptOpts)]),
             "etd_run_adaptive": (C.c_int, [ctxp, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), _dp, C.c_int,
                                            C.POINTER(C.c_int)]),
@@ -293,7 +305,7 @@ def _check(rc, ctx=None, what=""):
             detail = lib().etd_last_error(ctx)
             if detail:
                 msg += ": " + detail.decode()
-        raise EtdError(f"{what}: {msg}")
+        raise EtdError(f"{what}: {msg}", rc)
 
 
 def default_params():
@@ -473,13 +485,17 @@ class Etd:
         _check(lib().etd_k_nonlinear(self.ctx, var, _ptr(out)), self.ctx, "nonlinear")
         return out.reshape(self.n, self.n, self.n)
 
-    def modeprod(self, A, u, rotate=True):
+    def modeprod(self, A, u, rotate=True, slowest=False):
+        """One mode product by A (n x n): over the fastest axis (PUX,
+        out[(b + n c) + n^2 m] rotated, out[m + n (b + n c)] not), or with
+        slowest=True over the slowest axis (FP16 split-2 contexts:
+        out[m + n col] rotated, out[col + n^2 m] not, col = a + n b)."""
         n = self.n
         Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T).reshape(-1)
         inp = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
         out = np.empty(n ** 3)
-        _check(lib().etd_k_modeprod(self.ctx, n, _ptr(Acm), _ptr(inp), _ptr(out), 1 if rotate else 0),
-               self.ctx, "modeprod")
+        code = (2 if rotate else 3) if slowest else (1 if rotate else 0)
+        _check(lib().etd_k_modeprod(self.ctx, n, _ptr(Acm), _ptr(inp), _ptr(out), code), self.ctx, "modeprod")
         return out
 
     def phiscale(self, var, uhat, nhat):
@@ -487,6 +503,27 @@ class Etd:
         nh = np.ascontiguousarray(np.asarray(nhat, dtype=np.float64).reshape(-1))
         _check(lib().etd_k_phiscale(self.ctx, var, _ptr(u), _ptr(nh)), self.ctx, "phiscale")
         return u
+
+    # the beta->alpha output and restart formats
+    def write_checkpoint(self, irun, chk):
+        _check(lib().etd_write_checkpoint(self.ctx, irun, chk), self.ctx, "write_checkpoint")
+
+    def read_checkpoint(self, chk):
+        irun = C.c_int(0)
+        _check(lib().etd_read_checkpoint(self.ctx, chk, C.byref(irun)), self.ctx, "read_checkpoint")
+        return irun.value
+
+    def write_field2b(self, irun):
+        _check(lib().etd_write_field2b(self.ctx, irun), self.ctx, "write_field2b")
+
+    def detector_feed(self, var, want_binary=True):
+        """(bits (N/2)^3 / 8 uint8, binary (N/2)^3 float32 or None) of AC field var."""
+        d = (self.n - 2 * self.p.nghost) // 2
+        bits = np.zeros(d ** 3 // 8, np.uint8)
+        binary = np.zeros(d ** 3, np.float32) if want_binary else None
+        _check(lib().etd_detector_feed(self.ctx, var, bits.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                       _fptr(binary) if want_binary else None), self.ctx, "detector_feed")
+        return bits, binary
 
     def calib_copy(self, nbytes):
         """One plain device copy of nbytes (HBM counter calibration)."""
@@ -518,8 +555,13 @@ class Etd:
         st = AdaptStats()
         log = np.zeros((max(log_cap, 1), LOG_COLS))
         rows = C.c_int(0)
-        _check(lib().etd_run_adaptive(self.ctx, C.byref(o), C.byref(st), _ptr(log) if log_cap else None, log_cap,
-                                      C.byref(rows)), self.ctx, "run_adaptive")
+        rc = lib().etd_run_adaptive(self.ctx, C.byref(o), C.byref(st), _ptr(log) if log_cap else None, log_cap,
+                                    C.byref(rows))
+        try:
+            _check(rc, self.ctx, "run_adaptive")
+        except EtdError as e:  # the run's statistics up to the failure
+            e.stats, e.log = st.as_dict(), log[: rows.value].copy()
+            raise
         return st.as_dict(), log[: rows.value].copy()
 
     def time_steps(self, nsteps, with_diag=True):

@@ -14,9 +14,11 @@ def ac_energy(stats, nac):
     return float(stats[4 * 0 + 3]) if nac > 0 else 0.0
 
 
-def compare(prm_ref, prm_test, steps, every=100, fields=None, log=print):
+def compare(prm_ref, prm_test, steps, every=100, fields=None, log=print, keep_ref=False):
     """Runs both configurations for `steps` steps of their own dt (the two dt
-    must be equal) and returns a list of dicts {step, t, max_deta, rel_energy}."""
+    must be equal) and returns a list of dicts {step, t, max_deta, max_dc,
+    rel_energy, energy_ref, energy_test}; with keep_ref, (rows, the reference's
+    final fields, its final check_soln_new statistics)."""
     assert abs(prm_ref["dt"] - prm_test["dt"]) == 0.0
     a = Etd.from_dict(prm_ref)
     b = Etd.from_dict(prm_test)
@@ -48,6 +50,8 @@ def compare(prm_ref, prm_test, steps, every=100, fields=None, log=print):
         rows.append(row)
         if log:
             log(row)
+    ref = [a.get_field(v) for v in range(V)] if keep_ref else None
+    st_ref = a.diag()[0] if keep_ref else None
     a.close()
     b.close()
-    return rows
+    return (rows, ref, st_ref) if keep_ref else rows

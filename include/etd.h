@@ -35,7 +35,8 @@ enum {
   ETD_ERR_IO = -2,       /* missing / malformed input file */
   ETD_ERR_UNSUPPORTED = -3,
   ETD_ERR_DEVICE = -4,   /* HIP runtime failure */
-  ETD_ERR_NOMEM = -5
+  ETD_ERR_NOMEM = -5,
+  ETD_ERR_NONFINITE = -6 /* a NaN / infinite state or candidate that no step size cures */
 };
 
 /* precision of the step: FP64 throughout; or FP32 fields with the mode
@@ -76,6 +77,12 @@ typedef struct etd_params {
   double eps0[ETD_MAX_VAR * 9];
   double lam[ETD_MAX_VAR * 9];  /* Lambda = diag(kk) - lambda_in */
   int precision;           /* ETD_PREC_FP64 or ETD_PREC_MIXED */
+  /* the beta->alpha ini's output and restart schedule
+   * (source_beta_to_alpha/src/ScLETD.cpp:29-31): checkpoint 1 starts fresh,
+   * 2 reads checkpoint buffer chk; a checkpoint every nchk steps (alternating
+   * buffers), uint16 field dumps every nout steps; approx (the reference's
+   * truncated-elasticity mode count) is read and not used */
+  int checkpoint, nchk, chk, nout, approx;
 } etd_params;
 
 typedef struct etd_ctx etd_ctx;
@@ -173,7 +180,12 @@ int etd_k_fft3d(etd_ctx *c, double *data, int dir);
  * current device state (after etd_k_transfer / etd_k_elastic). */
 int etd_k_nonlinear(etd_ctx *c, int var, double *host_out);
 /* PUX/PUY/PUZ (calc.c:631-684): out = A x_1 in, optionally rotated
- * (flat (a,b,c) -> (b,c,a)); A column-major n x n; n^3 in/out. */
+ * (flat (a,b,c) -> (b,c,a)); A column-major n x n; n^3 in/out.
+ * rotate 0 / 1: contraction over the fastest axis a, as the reference's
+ * products (out[m + n (b + n c)] / out[(b + n c) + n^2 m]); 2 / 3 (FP16
+ * split-2 contexts, n <= 256): over the slowest axis c, rotated
+ * (out[m + n (a + n b)], (a,b,c) -> (c',a,b)) / not (out[(a + n b) + n^2 m]),
+ * the kernel the step uses there. */
 int etd_k_modeprod(etd_ctx *c, int n, const double *A, const double *in, double *out, int rotate);
 /* ac_updateU_new / ch_updateU_new (calc.c:489-535) on spectral data in
  * the rotated (x'y'z', x' fastest) layout. */
@@ -204,7 +216,11 @@ int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launc
  * times the energy's scale: the larger sum of the absolute values of its
  * terms over the two states (scheme energy), or the largest |E| of the run
  * (check_soln_new energy).  Rejected
- * attempts restore u_n.  The plain controller proposes
+ * attempts restore u_n.  A candidate whose error estimate or energy is NaN or
+ * infinite is rejected with the factor shrink; at dt_min the run stops with
+ * ETD_ERR_NONFINITE and u_n restored (a step at dt_min is accepted only when
+ * finite), and a non-finite initial energy is ETD_ERR_NONFINITE before any
+ * step.  The plain controller proposes
  * dt * clamp(safety (tol/err)^(1/2), shrink, grow); the learned controller
  * proposes learned_safety * dt_plain * 10^y, where y = MLP(features) over
  * ETD_NFEAT features of the run (below) predicts log10 of the error model's
@@ -325,6 +341,30 @@ int etd_autotune(const etd_params *p, const double *V, const double *Vi, const d
                  etd_tuning *best);
 int etd_set_tuning(etd_ctx *c, const etd_tuning *t);
 int etd_get_tuning(const etd_ctx *c, etd_tuning *t);
+
+/* ---- the beta->alpha driver's output and restart formats ----
+ * Files go under work_dir / data_dir plus the reference's per-rank
+ * subdirectory "00/" (one subdomain: cart ids 0,0,0), which must exist. */
+
+/* write_chk (source_beta_to_alpha/src/vars.c:967-1011):
+ * <work_dir>00/eta<v>_chk<chk>_000000.dat = int irun, then the n^3 doubles of
+ * AC field v (ghosts included, x fastest); CH fields as c<q>_chk<chk>_000000.dat
+ * (the reference's disabled CH branch, vars.c:990-1010).  chk is 0 or 1. */
+int etd_write_checkpoint(etd_ctx *c, int irun, int chk);
+/* read_chk (vars.c:1013-1065): the AC files must exist, CH files are read
+ * when present; *irun from the files. */
+int etd_read_checkpoint(etd_ctx *c, int chk, int *irun);
+/* Dtype2B_E + write_field2B (source_beta_to_alpha/src/anisotropic_hip.c:3920-3931,
+ * 4290-4299; vars.c:1068-1094): each AC field as
+ * <data_dir>00/eta<v>_<irun:06d>_000000.dat, n^3 uint16 (eta + 0.1) * 30000
+ * (truncated, saturated to [0, 2^32 - 1], low 16 bits kept), converted on the GPU. */
+int etd_write_field2b(etd_ctx *c, int irun);
+/* The detector feed (anisotropic_hip.c:3897-3918 skip_copyin_new_2,
+ * 4264-4278 binarisation, 4400-4427 float2bin): the interior of AC field var
+ * averaged over 2x2x2 blocks ((N/2)^3 cells, x fastest), each cell 1 if its
+ * mean exceeds 0.1; bits[t] packs cells 8t .. 8t+7 (bit e = cell 8t + e);
+ * binary (NULL to skip) gets the (N/2)^3 cells as 1.0f / 0.0f.  N/2 even. */
+int etd_detector_feed(etd_ctx *c, int var, unsigned char *bits, float *binary);
 
 /* Measurement support, no counterpart in the reference: one plain copy
  * kernel that reads and writes exactly `bytes` (a multiple of 16, at most the

@@ -9,10 +9,23 @@ is 2 u_half - u_full.  Accept when err <= tol and the check_soln_new energy
 (AC + CH) has not increased (slack 1e-12 relative); the next proposal is
 dt * clamp(safety sqrt(tol / err), shrink, grow); a rejection for err
 shrinks by clamp(safety sqrt(tol / err), shrink, 1), for energy by 1/2.
-The last step is clipped to t_end; a step at dt_min is always accepted."""
+The last step is clipped to t_end; a step at dt_min is accepted when its
+error estimate and energy are finite.  A non-finite candidate (NaN or
+infinite err or energy) is rejected with the factor shrink; at dt_min the run
+raises NonFinite with the fields restored to the last accepted state, and a
+non-finite initial energy raises NonFinite before any step."""
 import math
 
 import numpy as np
+
+
+class NonFinite(ArithmeticError):
+    pass
+
+
+def nanmax(values):
+    """max that returns NaN when any value is NaN (numpy's max does too)."""
+    return float(np.max(values))
 
 
 def energy(o):
@@ -39,6 +52,8 @@ def run(o, t_end, dt0, dt_min, dt_max, tol, safety=0.9, grow=2.0, shrink=0.2, ex
     clamp = lambda x, lo, hi: min(max(x, lo), hi)
     t = 0.0
     E = energy(o)
+    if not math.isfinite(E):
+        raise NonFinite(f"the energy of the initial state is not finite ({E})")
     dt = dt0
     plain_next = dt0
     rej_since = 0
@@ -55,14 +70,19 @@ def run(o, t_end, dt0, dt_min, dt_max, tol, safety=0.9, grow=2.0, shrink=0.2, ex
         o.set_dt(0.5 * dt)
         o.step()
         half = fields(o)
-        err = float(np.abs(half - full).max())
+        err = nanmax(np.abs(half - full))
         cand = 2.0 * half - full if extrapolate else half
         set_fields(o, cand)
         Enew = energy(o)
-        fac = safety * math.sqrt(tol / err) if err > 0.0 else grow
-        err_ok = err <= tol and math.isfinite(err)
-        en_ok = (not energy_test) or Enew <= E + 1e-12 * abs(E)
-        forced = dt <= dt_min * (1.0 + 1e-12) and math.isfinite(err)
+        finite = math.isfinite(err) and math.isfinite(Enew)
+        fac = shrink if not finite else (safety * math.sqrt(tol / err) if err > 0.0 else grow)
+        err_ok = err <= tol and finite
+        en_ok = finite and ((not energy_test) or Enew <= E + 1e-12 * abs(E))
+        at_min = dt <= dt_min * (1.0 + 1e-12)
+        if at_min and not finite:
+            set_fields(o, u0)
+            raise NonFinite(f"the candidate at dt_min = {dt} is not finite (err {err}, energy {Enew}) at t = {t}")
+        forced = at_min and finite
         if (err_ok and en_ok) or forced:
             t += dt
             accepted += 1

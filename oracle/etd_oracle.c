@@ -22,9 +22,16 @@
  *   orc_step (ETD1)  pinned by dense expm(dt*L) at n = 6 with N == 0.
  *   orc_step (ETD2)  pinned by order-2 convergence.
  *   orc_diag         pinned by hand-computed 2^3 / 3^3 grids.
- *   CH free energy   the reference's CH_f is broken (formula.c:40-56); the
- *                    double well (c^2-1)^2/4 of field.c:575 / trans.c:993 is
- *                    used, so CH parity is against this oracle only.
+ *   CH (ch_nonlinear, h_ch)
+ *                    the reference's CH_f is broken (formula.c:40-56); the
+ *                    double well W = (c^2-1)^2/4 of field.c:575 / trans.c:993
+ *                    is used (CH_f = W').  Pinned by: t(c) up to a constant,
+ *                    pointwise, read off the Laplacian of a one-cell bump;
+ *                    a Fourier-mode closed form of the nonlinear term away
+ *                    from the box edges; one ETD1 step against dense
+ *                    expm of the truncated CH operator plus the periodic CH
+ *                    right-hand side.  Planted mistakes (orc_set_mutation)
+ *                    fail them.
  */
 #include "etd_oracle.h"
 
@@ -56,6 +63,16 @@ int orc_threads(void) {
   return omp_get_max_threads();
 #else
   return 1;
+#endif
+}
+
+/* thread count of the following parallel regions (CPU baseline timing at 1
+ * core and at all cores) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
 #endif
 }
 
@@ -307,7 +324,13 @@ static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
  *   out -= mu_out/h^2 on each face, mu_out = LCI*(epn2*Lap c - CH_f(c)) at
  *          the outside cell, evaluated on the periodic image.
  * CH_f(c) = c (c^2 - 1), the derivative of (c^2-1)^2/4 (field.c:575). */
-static inline double ch_f(double cc) { return cc * (cc * cc - 1.0); }
+/* Planted mistakes for the tests that show the CH checks can fail
+ * (orc_set_mutation): 1 flips the sign of CH_f, 2 uses lam in place of
+ * lam^2 in the CH linear operator.  0 (the default) is the restatement. */
+static int orc_mutation = 0;
+void orc_set_mutation(int m) { orc_mutation = m; }
+
+static inline double ch_f(double cc) { return orc_mutation == 1 ? -cc * (cc * cc - 1.0) : cc * (cc * cc - 1.0); }
 
 static double ch_mu_out(const orc_ctx *c, const double *u, int z, int y, int x) {
   const orc_params *p = &c->p;
@@ -744,6 +767,7 @@ static double h_ac(const orc_ctx *c, int i, int j, int k) {
 static double h_ch(const orc_ctx *c, int i, int j, int k) {
   const orc_params *p = &c->p;
   double tmp = c->DDZ[k] + c->DDY[j] + c->DDX[i];
+  if (orc_mutation == 2) return p->LCI * (-tmp * p->epn2 - p->KCI * tmp);
   return p->LCI * (tmp * tmp * p->epn2 - p->KCI * tmp);
 }
 

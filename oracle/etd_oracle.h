@@ -60,6 +60,10 @@ void orc_destroy(orc_ctx *c);
 double *orc_field(orc_ctx *c, int v);     /* state of field v (AC first, then CH), n^3 */
 double *orc_felas(orc_ctx *c, int p);     /* elastic driving force of AC field p, n^3 */
 int orc_threads(void);
+void orc_set_threads(int n);
+/* test hook: plant a known mistake (1: sign of CH_f, 2: lam for lam^2 in the
+ * CH operator) so a test can show its check fails; 0 restores the oracle */
+void orc_set_mutation(int m);
 
 /* individual stages (for unit parity) */
 void orc_ghost_fill(int n, int g, double *u);

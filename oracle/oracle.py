@@ -61,6 +61,8 @@ def lib():
         L.orc_bpq.restype = C.c_double
         L.orc_bpq.argtypes = [C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_mutation.argtypes = [C.c_int]
         _LIB = L
     return _LIB
 
@@ -225,3 +227,13 @@ def ghost_fill(u, g=2):
 
 def threads():
     return lib().orc_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def set_mutation(m):
+    """Plant a known mistake in the oracle (tests of the tests): 1 flips the
+    sign of CH_f, 2 uses lambda for lambda^2 in the CH operator; 0 undoes."""
+    lib().orc_set_mutation(int(m))

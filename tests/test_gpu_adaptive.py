@@ -235,3 +235,81 @@ def test_learned_controller_keeps_guarantees_and_reports_gain():
     assert st_l["learned_proposals"] > 0
     g.close()
     m.close()
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_nan_state_is_an_error_and_is_left_untouched(precision):
+    """A NaN cell in the state (set from the host: a data value, not a device
+    fault) makes the run stop with ETD_ERR_NONFINITE before any step; nothing
+    is accepted and the fields are what was set."""
+    say(f"adaptive: NaN state precision={precision}")
+    prm = dict(small_params(n=16, nac=3, nch=1, elastic=1, dt=0.01), precision=precision)
+    f = noise_fields(prm, amp=0.05)
+    f[1] = f[1].copy()
+    f[1][7, 8, 9] = np.nan
+    g = gpu(prm, f)
+    with pytest.raises(pyetd.EtdError) as ei:
+        g.run_adaptive(log_cap=8, t_end=0.1, dt0=0.01, dt_min=1e-4, tol=1e-4)
+    assert ei.value.code == pyetd.ERR_NONFINITE and "not finite" in str(ei.value)
+    assert ei.value.stats["accepted"] == 0 and g.time == 0.0
+    for v in range(4):
+        a, b = g.get_field(v), f[v].astype(np.float32).astype(np.float64) if precision else f[v]
+        assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    g.close()
+
+
+def test_overflowing_candidates_are_rejected_down_to_dt_min_then_an_error():
+    """One cell inflated to 1e75: the state's energy is finite (1e300), every
+    candidate overflows (eta ~ -dt a4 eta^3 ~ 1e222, eta^4 = inf), so each is
+    rejected with the factor shrink until dt_min, where the run stops with
+    ETD_ERR_NONFINITE (before the fix a step at dt_min was accepted whenever
+    the error estimate was finite) and the state is restored; the CPU
+    restatement makes the same decisions."""
+    say("adaptive: overflowing candidates")
+    prm = small_params(n=16, nac=3, nch=1, elastic=0, dt=0.01)
+    f = noise_fields(prm, amp=0.05)
+    f[0] = f[0].copy()
+    f[0][8, 8, 8] = 1e75
+    kw = dict(t_end=0.1, dt0=0.01, dt_min=1e-4, dt_max=0.1, tol=1e-4)
+    o = O.Oracle(prm)
+    for v, x in enumerate(f):
+        o.set_field(v, x)
+    with pytest.raises(AD.NonFinite):
+        AD.run(o, **kw)
+    g = gpu(prm, f)
+    with pytest.raises(pyetd.EtdError) as ei:
+        g.run_adaptive(log_cap=8, energy_kind=0, energy_elastic=0, **kw)
+    st = ei.value.stats
+    say(f"  {st}: {ei.value}")
+    assert ei.value.code == pyetd.ERR_NONFINITE and "dt_min" in str(ei.value)
+    # dt 1e-2 -> 2e-3 -> 4e-4 -> 1e-4 (clamped): four attempts, all rejected
+    assert st["accepted"] == 0 and st["rejected_err"] + st["rejected_energy"] == 3 and st["step_evals"] == 12
+    for v in range(4):
+        assert np.array_equal(g.get_field(v), f[v])
+    g.close()
+
+
+def test_inflated_field_is_cured_by_smaller_steps():
+    """Fields inflated far past the Landau minima (|eta| up to 2.5): the first
+    attempts fail the error test, smaller steps are accepted, every accepted
+    state is finite and within tolerance, and the energy falls."""
+    say("adaptive: inflated field")
+    prm = dict(small_params(n=20, nac=3, nch=1, elastic=1, dt=0.05, KE=3.2, landau=(0.2, 3.0, 3.0)),
+               eps0=pyetd.workloads.eigenstrains(3, 0.03, 1))
+    # periodic ghost images: the ghost fill copies edge and corner ghosts from
+    # the pre-fill values of other ghost cells (trans.c:154-251), so random
+    # ghosts would make one fill (full step) and two fills (half steps)
+    # disagree there by O(1) at any dt
+    gg, n = prm["g"], prm["n"]
+    idx = (np.arange(n) - gg) % (n - 2 * gg)
+    f = [x[gg:-gg, gg:-gg, gg:-gg][np.ix_(idx, idx, idx)] for x in noise_fields(prm, amp=2.5, ch_amp=0.4)]
+    g = gpu(prm, f)
+    st, log = g.run_adaptive(log_cap=10000, t_end=0.5, dt0=0.05, dt_min=1e-6, dt_max=0.5, tol=1e-4)
+    say(f"  {st}")
+    assert st["rejected_err"] > 0 and abs(st["t"] - 0.5) < 1e-12
+    assert (log[:, 2] <= 1e-4).all()
+    E = log[:, 3]
+    # accepted when E_new <= E + 1e-12 x (sum of the energy's absolute terms)
+    assert np.isfinite(E).all() and (np.diff(E) <= 1e-10 * np.abs(E).max()).all() and E[-1] < E[0]
+    assert all(np.isfinite(g.get_field(v)).all() for v in range(4))
+    g.close()

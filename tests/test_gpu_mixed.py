@@ -178,6 +178,16 @@ def test_f16_split2_modeprod_matches_oracle(n, rotate):
         err = np.abs(got - ref).max() / np.abs(ref).max()
         say(f"  rel err {err:.2e}")
         assert err < 2e-6
+        if n <= 256:
+            # the kernel the step uses at n <= 256: contraction over the slowest
+            # axis, i.e. the oracle's fastest-axis product of the field with its
+            # slowest axis moved to the front
+            got = g.modeprod(A, u, rotate, slowest=True)
+            ut = np.ascontiguousarray(u.reshape(n, n, n).transpose(1, 2, 0)).reshape(-1)
+            ref = O.modeprod(A, ut, not rotate)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            say(f"  slowest-axis rel err {err:.2e}")
+            assert err < 2e-6
     # columns (lines along x) scaled over 12 decades: each column's own error
     # stays relative to that column
     scale = 10.0 ** rng.uniform(-12, 0, size=(n, n, 1))

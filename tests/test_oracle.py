@@ -467,3 +467,184 @@ def test_seeded_nuclei_grow_and_energy_falls():
     assert vol > 2.0 * vol0
     assert all(b < a for a, b in zip(e, e[1:]))
     assert max(float(o.get_field(v).max()) for v in range(4)) > 0.85
+
+
+def test_adaptive_restatement_refuses_non_finite_states():
+    """A NaN state raises before any step; a state whose candidates overflow
+    (one cell at 1e75: finite energy, infinite candidate energy) is rejected
+    with the factor shrink down to dt_min and then raises, with the fields
+    restored."""
+    import adaptive as AD
+    prm = small_params(n=12, nac=2, nch=1, dt=0.01)
+    f = noise_fields(prm, amp=0.05)
+    o = O.Oracle(prm)
+    g = [x.copy() for x in f]
+    g[0][5, 6, 7] = np.nan
+    for v, x in enumerate(g):
+        o.set_field(v, x)
+    with pytest.raises(AD.NonFinite, match="initial state"):
+        AD.run(o, t_end=0.1, dt0=0.01, dt_min=1e-4, dt_max=0.1, tol=1e-4)
+    g = [x.copy() for x in f]
+    g[0][6, 6, 6] = 1e75
+    for v, x in enumerate(g):
+        o.set_field(v, x)
+    with pytest.raises(AD.NonFinite, match="dt_min"):
+        AD.run(o, t_end=0.1, dt0=0.01, dt_min=1e-4, dt_max=0.1, tol=1e-4)
+    for v in range(3):
+        assert np.array_equal(o.get_field(v), g[v])
+
+
+# ---------------------------------------------------------------- CH pinning
+# The reference's CH_f is broken (formula.c:40-56); the oracle uses the
+# double well W(c) = (c^2 - 1)^2 / 4, CH_f = W'(c) = c^3 - c.  Three checks
+# fixed by the mathematics, each failing for a planted sign flip of CH_f or
+# for lambda in place of lambda^2 in the CH operator (oracle mutations).
+CH_PRM = dict(LCI=0.7, KCI=1.3, epn2=0.6, h=(1.0, 0.9, 1.2))
+
+
+def W_prime(c):
+    """dW/dc of W = (c^2 - 1)^2 / 4 by a central difference of W itself."""
+    W = lambda x: (x * x - 1.0) ** 2 / 4.0
+    e = 1e-5
+    return (W(c + e) - W(c - e)) / (2 * e)
+
+
+def ch_bump_error():
+    """t(c) = -LCI KCI c + LCI W'(c) pointwise, up to a constant: a uniform
+    field c0 plus delta at one interior cell p has out(p + x) = (t(c0 + delta)
+    - t(c0)) / hx^2 (the Laplacian of a one-cell bump), away from the box
+    edges where the outside-value terms enter."""
+    worst = 0.0
+    for c0 in (-0.8, 0.1, 0.65):
+        for delta in (-0.3, 0.05, 0.4):
+            prm = dict(small_params(n=12, nac=1, nch=1), **CH_PRM)
+            o = O.Oracle(prm)
+            f = np.full((12, 12, 12), c0)
+            f[6, 6, 6] += delta
+            o.set_field(0, np.zeros((12, 12, 12)))
+            o.set_field(1, f)
+            out = o.nonlinear(1)
+            hx = prm["h"][0]
+            t = lambda c: -prm["LCI"] * prm["KCI"] * c + prm["LCI"] * W_prime(c)
+            got = out[6, 6, 7] * hx * hx
+            worst = max(worst, abs(got - (t(c0 + delta) - t(c0))) / max(1e-3, abs(t(c0 + delta) - t(c0))))
+    return worst
+
+
+def ch_fourier_error():
+    """c = c0 + A cos(k.x): W'(c) = c^3 - c has cos(m k.x) components
+    m = 1: 3 c0^2 A + 3 A^3 / 4 - A, m = 2: 3 c0 A^2 / 2, m = 3: A^3 / 4; the
+    7-point Laplacian multiplies cos(m k.x) by -sum_a 4 sin^2(m k_a h_a / 2) / h_a^2.
+    Interior cells two away from the box edges."""
+    n = 16
+    prm = dict(small_params(n=n, nac=1, nch=1), **CH_PRM)
+    hx, hy, hz = prm["h"]
+    kv = np.array([2 * np.pi * 2 / (n * hx), 2 * np.pi * 1 / (n * hy), 2 * np.pi * 3 / (n * hz)])
+    z, y, x = np.meshgrid(np.arange(n) * hz, np.arange(n) * hy, np.arange(n) * hx, indexing="ij")
+    th = kv[0] * x + kv[1] * y + kv[2] * z
+    c0, A = 0.15, 0.4
+    o = O.Oracle(prm)
+    o.set_field(0, np.zeros((n, n, n)))
+    o.set_field(1, c0 + A * np.cos(th))
+    out = o.nonlinear(1)
+    lam = lambda m: -sum(4.0 * np.sin(m * kv[a] * h / 2) ** 2 / h ** 2 for a, h in enumerate(prm["h"]))
+    coef = {1: 3 * c0 ** 2 * A + 0.75 * A ** 3 - A, 2: 1.5 * c0 * A ** 2, 3: 0.25 * A ** 3}
+    L, K = prm["LCI"], prm["KCI"]
+    expect = L * (lam(1) * (coef[1] - K * A) * np.cos(th) + lam(2) * coef[2] * np.cos(2 * th) +
+                  lam(3) * coef[3] * np.cos(3 * th))
+    inner = slice(2, n - 2)
+    return np.abs(out[inner, inner, inner] - expect[inner, inner, inner]).max() / np.abs(expect).max()
+
+
+def ch_expm_error():
+    """One ETD1 step of the CH field is exact for du/dt = L u + N(u0) with N
+    frozen at u0: u1 = e^{dt L} u0 + L^-1 (e^{dt L} - I) N(u0), where
+    L = -LCI (epn2 Lt^2 - KCI Lt), Lt the truncated (Dirichlet) Kronecker
+    Laplacian of the extended box, and N(u0) = R(u0) - L u0 with R the
+    periodic CH right-hand side LCI Lap(W'(c) - epn2 Lap c) at the periodic
+    image of every extended cell (truncated operator + outside terms = the
+    periodic operator)."""
+    n, g = 8, 2
+    N = n - 2 * g
+    prm = dict(small_params(n=n, nac=1, nch=1, dt=0.01), **CH_PRM)
+    hx, hy, hz = prm["h"]
+    rng = np.random.default_rng(21)
+    P = 0.2 + 0.3 * rng.standard_normal((N, N, N))  # [z][y][x]
+    idx = (np.arange(n) - g) % N
+    u0 = P[np.ix_(idx, idx, idx)]
+    o = O.Oracle(prm)
+    o.set_field(0, np.zeros((n, n, n)))
+    o.set_field(1, u0)
+    o.step(1)
+    T = np.diag(-2.0 * np.ones(n)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    I = np.eye(n)
+    Lt = np.kron(I, np.kron(I, T)) / hx ** 2 + np.kron(I, np.kron(T, I)) / hy ** 2 + np.kron(T, np.kron(I, I)) / hz ** 2
+    Lop = -prm["LCI"] * (prm["epn2"] * Lt @ Lt - prm["KCI"] * Lt)
+
+    def lap(a):
+        return ((np.roll(a, 1, 2) + np.roll(a, -1, 2) - 2 * a) / hx ** 2 + (np.roll(a, 1, 1) + np.roll(a, -1, 1) - 2 * a) / hy ** 2
+                + (np.roll(a, 1, 0) + np.roll(a, -1, 0) - 2 * a) / hz ** 2)
+
+    R = prm["LCI"] * lap(W_prime(P) - prm["epn2"] * lap(P))
+    Nu = R[np.ix_(idx, idx, idx)].reshape(-1) - Lop @ u0.reshape(-1)
+    E = expm(prm["dt"] * Lop)
+    u1 = E @ u0.reshape(-1) + np.linalg.solve(Lop, (E - np.eye(n ** 3)) @ Nu)
+    return np.abs(o.get_field(1).reshape(-1) - u1).max() / np.abs(u1).max()
+
+
+@pytest.mark.parametrize("check", [ch_bump_error, ch_fourier_error, ch_expm_error])
+def test_ch_oracle_pinned(check):
+    O.set_mutation(0)
+    err = check()
+    # W' by a central difference carries ~1e-11; dense expm ~1e-13
+    assert err < 1e-8, err
+
+
+@pytest.mark.parametrize("mutation,checks", [(1, [ch_bump_error, ch_fourier_error, ch_expm_error]),
+                                             (2, [ch_expm_error])])
+def test_ch_checks_fail_for_planted_mistakes(mutation, checks):
+    """Each check catches the mistakes it is meant to catch: a sign-flipped
+    CH_f (all three) and lambda in place of lambda^2 in the CH linear
+    operator (the expm step; the nonlinear term does not contain it)."""
+    O.set_mutation(mutation)
+    try:
+        errs = [chk() for chk in checks]
+    finally:
+        O.set_mutation(0)
+    assert all(e > 1e-3 for e in errs), errs
+
+
+# ---------------------------------------------------------------- beta -> alpha output formats
+def test_field2b_known_answers():
+    import formats as F
+    u = np.array([-0.1, 0.0, 0.9, 2.0, 2.5, -1.0, np.nan, 0.25])
+    got = F.field2b(u)
+    # (u + 0.1) * 30000 truncated: 0, 3000, 30000, 63000; 2.5 -> 78000 wraps to
+    # 78000 - 65536; -1 and NaN saturate to 0; 0.25 -> 10500 (0.35 * 30000 is
+    # 10499.999..., truncated)
+    assert list(got) == [0, 3000, 30000, 63000, 78000 - 65536, 0, 0, int((0.25 + 0.1) * 30000.0)]
+    assert got.dtype == np.uint16
+
+
+def test_downsample_and_detector_pinned():
+    import formats as F
+    rng = np.random.default_rng(8)
+    n, g = 20, 2
+    u = rng.uniform(-0.2, 0.4, size=(n, n, n))
+    m = F.downsample(u, g)
+    inner = u[g:-g, g:-g, g:-g]
+    ref = inner.reshape(8, 2, 8, 2, 8, 2).mean(axis=(1, 3, 5))
+    assert np.abs(m - ref).max() < 1e-15
+    bits, binary = F.detector(u, g)
+    assert np.abs(ref.reshape(-1) - 0.1).min() > 1e-12  # no ties between the two sum orders
+    assert np.array_equal(binary, (ref.reshape(-1) > 0.1).astype(np.float32))
+    assert np.array_equal(bits, np.packbits(binary.astype(bool), bitorder="little"))
+    # single cells: the first cell of a byte is bit 0, the eighth bit 7; the
+    # threshold is strict
+    u = np.zeros((n, n, n))
+    u[g:g + 2, g:g + 2, g:g + 2] = 0.3              # cell 0
+    u[g:g + 2, g:g + 2, g + 14:g + 16] = 0.3        # cell 7 (x = 7)
+    u[g, g + 2, g] = 0.8                            # cell 8: sum 0.8, mean 0.8 / 8 == 0.1 exactly -> off
+    u[g, g + 2, g + 2] = np.nextafter(0.8, 1.0)     # cell 9: one ulp above -> on
+    bits, binary = F.detector(u, g)
+    assert bits[0] == 0b10000001 and bits[1] == 0b00000010 and binary[8] == 0.0 and binary[9] == 1.0
