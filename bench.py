@@ -159,7 +159,8 @@ def main():
                     help="mixed-precision mode products: 1 BF16 split-3, 2 FP16 split-2")
     ap.add_argument("--learned-safety", type=float, default=None,
                     help="margin of the learned proposal below the error model's largest step (library default)")
-    ap.add_argument("--repeats", type=int, default=3, help="runs per controller in the comparisons")
+    ap.add_argument("--repeats", type=int, default=3, help="runs per controller in the headline comparisons")
+    ap.add_argument("--nuc-repeats", type=int, default=1, help="runs per controller on the nucleated state")
     ap.add_argument("--quick", action="store_true", help="skip the accuracy runs and the nucleated line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-full", action="store_true",
@@ -264,7 +265,7 @@ def main():
                                      dt_largest=float(log[:, 1].max()),
                                      options={k: getattr(pyetd.adapt_opts(**kw), k)
                                               for k in ("tol", "safety", "learned_safety", "grow", "shrink",
-                                                        "extrapolate", "energy_test", "energy_elastic",
+                                                        "extrapolate", "energy_test", "energy_elastic", "learned_online",
                                                         "energy_kind")})})
         if mlp is not None:
             cmp_ = controller_comparison(pyetd, q, init, kw, t_end, args.repeats)
@@ -308,7 +309,7 @@ def main():
             nt = dt * nsteps
             nkw = dict(kw, t_end=nt)
             nw, nst, nlog, nfields = horizon_runs(pyetd, dict(nq, precision=args.precision), ninit, nkw,
-                                                  args.repeats, mlp, log_cap=100000)
+                                                  args.nuc_repeats, mlp, log_cap=100000)
             nrows, nref, nsa = AC.compare(nq, dict(nq, precision=args.precision), nsteps, every=nsteps,
                                           fields=ninit, log=None, keep_ref=True)
             nmed = statistics.median(nw)
@@ -323,7 +324,8 @@ def main():
             if fp is not None:
                 nuc["speedup_vs_plain_fp64"] = nuc["sim_time_per_s"] / fp["sim_time_per_s"]
             if mlp is not None:
-                ncmp = controller_comparison(pyetd, dict(nq, precision=args.precision), ninit, nkw, nt, args.repeats)
+                ncmp = controller_comparison(pyetd, dict(nq, precision=args.precision), ninit, nkw, nt,
+                                             args.nuc_repeats)
                 nuc.update(ncmp)
                 for name in ncmp:
                     nuc[f"learned_gain_vs_{name}"] = ncmp[name]["wall_s_median"] / nmed

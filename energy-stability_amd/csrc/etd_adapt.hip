@@ -462,6 +462,7 @@ extern "C" void etd_adapt_defaults(etd_adapt_opts *o) {
   o->tol = 1e-4;
   o->safety = 0.9;
   o->learned_safety = 0.97;
+  o->learned_online = 1;
   o->grow = 2.0;
   o->shrink = 0.2;
   o->controller = ETD_CTRL_PLAIN;
@@ -519,6 +520,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   double hist[4] = {o->dt0, o->dt0, o->dt0, o->dt0};
   double err_last = 0.1 * o->tol, dEdt = 0.0, plain_next = o->dt0;
   int rej_since = 0, fallback = 0;
+  // online correction of the network (decades): the largest recent
+  // over-prediction y_pred - y_obs, decaying by 0.8 per learned step, is
+  // taken off the next prediction (learned_online)
+  double lbias = 0.0;
   double dt = o->dt0;
   int attempts = 0;
   rc = ETD_OK;
@@ -538,6 +543,9 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     feat[13] = lg(rf.emax, 1e-30);
     feat[14] = log10(o->tol);
     feat[15] = log10(plain_next);
+    int used_learned = 0;
+    double y_pred = 0.0;
+    const double plain_at_proposal = plain_next;
     if (rej_since == 0 && st->accepted + attempts > 0) {
       dt = plain_next;
       if (learned && !fallback) {
@@ -551,8 +559,11 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
           // y predicts log10(dt_best / dt_plain), dt_best the error model's
           // largest step; its accuracy allows a smaller margin than the
           // plain rule's
-          dt = clampd(o->learned_safety * plain_next * pow(10.0, y), fmin(plain_next, hist[0]), 10.0 * plain_next);
+          const double yc = o->learned_online ? y - lbias : y;
+          dt = clampd(o->learned_safety * plain_next * pow(10.0, yc), fmin(plain_next, hist[0]), 10.0 * plain_next);
           st->learned_proposals++;
+          used_learned = 1;
+          y_pred = y;
         } else {
           st->learned_fallbacks++;
         }
@@ -615,6 +626,12 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       break;
     }
     const int forced = at_min && finite;
+    if (used_learned && finite && err > 0.0) {
+      // what the network should have predicted for this attempt: the error
+      // model's largest step at the observed err, over the plain proposal
+      const double y_obs = log10(dt * sqrt(o->tol / err)) - log10(plain_at_proposal);
+      lbias = fmax(0.8 * lbias, y_pred - y_obs);
+    }
     if ((err_ok && energy_ok) || forced) {
       if (!nhf && (rc = features_of<T>(c, F.u, S0, &nf))) break;
       c->t += dt;

@@ -2050,16 +2050,37 @@ __device__ __forceinline__ double block_sum256(double v) {
   return ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-__global__ __launch_bounds__(256) void k_ch_sum(const float *__restrict__ u, size_t fs, int n, int g,
-                                                double *__restrict__ part) {
+// block sum over CH_WAVES waves of 64 (the row kernels below)
+#define CH_WAVES 16
+__device__ __forceinline__ double block_sum_w(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  __shared__ double sh[CH_WAVES];
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < CH_WAVES; w++) s += sh[w];
+  return s;
+}
+
+// Interior rows (z, y) one per wave, CH_WAVES waves per block; each lane
+// loads its (up to 5) cells of a row before adding them, so a wave has the
+// whole row in flight (N <= 268 < 5 x 64).
+__global__ __launch_bounds__(64 * CH_WAVES) void k_ch_sum(const float *__restrict__ u, size_t fs, int n, int g,
+                                                          double *__restrict__ part) {
   const float *f = u + fs * blockIdx.y;
-  const int N = n - 2 * g;
+  const int N = n - 2 * g, lane = threadIdx.x & 63;
   double acc = 0.0;
-  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < N * N; r += gridDim.x * 4) {
+  for (int r = blockIdx.x * CH_WAVES + (threadIdx.x >> 6); r < N * N; r += gridDim.x * CH_WAVES) {
     const int z = r / N, y = r - z * N;
-    for (int x = threadIdx.x & 63; x < N; x += 64) acc += (double)f[IDX(n, z + g, y + g, x + g)];
+    const float *row = f + IDX(n, z + g, y + g, g);
+    float v[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) v[e] = lane + 64 * e < N ? row[lane + 64 * e] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 5; e++) acc += (double)v[e];
   }
-  const double t = block_sum256(acc);
+  const double t = block_sum_w(acc);
   if (threadIdx.x == 0) part[blockIdx.y * CH_BLOCKS + blockIdx.x] = t;
 }
 
@@ -2071,20 +2092,23 @@ __global__ __launch_bounds__(256) void k_ch_target(const double *__restrict__ pa
 
 // u_q += (target_q - sum_q) / N^3 on the interior of CH field q = blockIdx.y;
 // every block reduces the CH_BLOCKS partials of sum_q in the same order
-__global__ __launch_bounds__(256) void k_ch_shift(float *__restrict__ u, size_t fs, int n, int g,
-                                                  const double *__restrict__ target,
-                                                  const double *__restrict__ part) {
+__global__ __launch_bounds__(64 * CH_WAVES) void k_ch_shift(float *__restrict__ u, size_t fs, int n, int g,
+                                                            const double *__restrict__ target,
+                                                            const double *__restrict__ part) {
   float *f = u + fs * blockIdx.y;
-  const int N = n - 2 * g;
+  const int N = n - 2 * g, lane = threadIdx.x & 63;
   const int q = blockIdx.y;
-  const double a = block_sum256(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
+  const double a = block_sum_w(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
   const double shift = (target[q] - a) / ((double)N * N * N);
-  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < N * N; r += gridDim.x * 4) {
+  for (int r = blockIdx.x * CH_WAVES + (threadIdx.x >> 6); r < N * N; r += gridDim.x * CH_WAVES) {
     const int z = r / N, y = r - z * N;
-    for (int x = threadIdx.x & 63; x < N; x += 64) {
-      const size_t l = IDX(n, z + g, y + g, x + g);
-      f[l] = (float)((double)f[l] + shift);
-    }
+    float *row = f + IDX(n, z + g, y + g, g);
+    float v[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) v[e] = lane + 64 * e < N ? row[lane + 64 * e] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 5; e++)
+      if (lane + 64 * e < N) row[lane + 64 * e] = (float)((double)v[e] + shift);
   }
 }
 
@@ -2093,24 +2117,26 @@ __global__ __launch_bounds__(256) void k_ch_shift(float *__restrict__ u, size_t 
 // added, interior cells (target_q - a_q) / N^3 with a_q the interior sum of the
 // centred field (the partials of k_ch_sum, reduced in the same order by every
 // block).
-__global__ __launch_bounds__(256) void k_ch_shift_center(float *__restrict__ u, size_t fs, int n, int g,
-                                                         const double *__restrict__ target,
-                                                         const double *__restrict__ part, double invN3) {
+__global__ __launch_bounds__(64 * CH_WAVES) void k_ch_shift_center(float *__restrict__ u, size_t fs, int n, int g,
+                                                                   const double *__restrict__ target,
+                                                                   const double *__restrict__ part, double invN3) {
   float *f = u + fs * blockIdx.y;
-  const int N = n - 2 * g;
+  const int N = n - 2 * g, lane = threadIdx.x & 63;
   const int q = blockIdx.y;
-  const double a = block_sum256(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
+  const double a = block_sum_w(threadIdx.x < CH_BLOCKS ? part[q * CH_BLOCKS + threadIdx.x] : 0.0);
   const double shift = (target[q] - a) / ((double)N * N * N);
   const float m = (float)(target[q] * invN3);
-  for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < n * n; r += gridDim.x * 4) {
+  for (int r = blockIdx.x * CH_WAVES + (threadIdx.x >> 6); r < n * n; r += gridDim.x * CH_WAVES) {
     const int z = r / n, y = r - z * n;
     const bool row_in = z >= g && z < n - g && y >= g && y < n - g;
-    for (int x = threadIdx.x & 63; x < n; x += 64) {
-      const size_t l = IDX(n, z, y, x);
-      if (row_in && x >= g && x < n - g)
-        f[l] = (float)((double)f[l] + shift);
-      else
-        f[l] = f[l] + m;
+    float *row = f + IDX(n, z, y, 0);
+    float v[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) v[e] = lane + 64 * e < n ? row[lane + 64 * e] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      const int x = lane + 64 * e;
+      if (x < n) row[x] = (row_in && x >= g && x < n - g) ? (float)((double)v[e] + shift) : v[e] + m;
     }
   }
 }
@@ -2142,7 +2168,7 @@ __global__ void k_ch_center(float *__restrict__ u, float *__restrict__ N, size_t
 void ch_mean_mark(etd_ctx *c, const float *u) {
   if (!c->ch_sum || !c->ch_stale) return;
   double *part = c->ch_sum + c->nch;
-  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 64 * CH_WAVES, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
   k_ch_target<<<c->nch, 256, 0, c->stream>>>(part, c->ch_sum);
   // N(c_mean) of each CH field, through the same kernels as the step, on a
   // uniform field in the mode-product scratch
@@ -2170,17 +2196,17 @@ void ch_mean_restore_centered(etd_ctx *c, float *u) {
   if (!c->ch_sum) return;
   double *part = c->ch_sum + c->nch;
   const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
-  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
-  k_ch_shift_center<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g,
-                                                                    c->ch_sum, part, invN3);
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 64 * CH_WAVES, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_shift_center<<<dim3(CH_BLOCKS, c->nch), 64 * CH_WAVES, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n,
+                                                                              c->g, c->ch_sum, part, invN3);
 }
 
 void ch_mean_restore(etd_ctx *c, float *u) {
   if (!c->ch_sum) return;
   double *part = c->ch_sum + c->nch;
-  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
-  k_ch_shift<<<dim3(CH_BLOCKS, c->nch), 256, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, c->ch_sum,
-                                                             part);
+  k_ch_sum<<<dim3(CH_BLOCKS, c->nch), 64 * CH_WAVES, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g, part);
+  k_ch_shift<<<dim3(CH_BLOCKS, c->nch), 64 * CH_WAVES, 0, c->stream>>>(u + c->n3 * c->nac, c->n3, c->n, c->g,
+                                                                       c->ch_sum, part);
 }
 
 #define ETD_INST(T)                                                                   \

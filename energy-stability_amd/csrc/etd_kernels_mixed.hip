@@ -930,6 +930,15 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
   issue_a(0);
   if (total > 1) issue_x(1);
   f32x4 acc[MT][2];
+  // XL = 2: one scale per column and tile instead of one per chunk: the
+  // column's values are scaled by 2^(12 - cexp[u]), cexp the exponent of the
+  // largest magnitude seen so far in the column (-1000 while all are zero),
+  // so the three MFMAs of every chunk accumulate straight into acc.  A chunk
+  // whose largest value exceeds 2^(cexp + 3) (its scaled values would leave
+  // FP16's range) raises cexp and rescales the column's acc first, in a
+  // branch taken only when a lane of the wave needs it.  The tile's acc is
+  // multiplied by 2^(cexp - 12) / s once, in the epilogue.
+  int cexp[2] = {-1000, -1000};
   int xs = 0;  // g % 3
 #pragma unroll 1
   for (int g = 0; g < total; g++, xs = xs == 2 ? 0 : xs + 1) {
@@ -940,6 +949,7 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
       for (int t = 0; t < MT; t++)
 #pragma unroll
         for (int u = 0; u < 2; u++) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      cexp[0] = cexp[1] = -1000;
     }
     if (g + 1 < total && (kc != 0 || g == 0))  // uniform
       wait_copies4_barrier();
@@ -978,9 +988,22 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
       for (int e = 0; e < 8; e++) m = fmaxf(m, fabsf(x[e]));
       m = fmaxf(m, __shfl_xor(m, 16, 64));
       m = fmaxf(m, __shfl_xor(m, 32, 64));
-      const int ex = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
-      const float sc = __builtin_ldexpf(1.0f, 14 - ex);
-      back[u] = __builtin_ldexpf(a_inv, ex - 14);
+      float sc;
+      if constexpr (XL == 2) {
+        const int ex = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : -1000;
+        const bool grow = ex > cexp[u] + 3;
+        if (__any(grow)) {  // wave-uniform; lanes that keep their scale multiply by 1
+          const float f = grow ? __builtin_ldexpf(1.0f, cexp[u] - ex) : 1.0f;
+#pragma unroll
+          for (int t = 0; t < MT; t++) acc[t][u] = acc[t][u] * f;
+        }
+        if (grow) cexp[u] = ex;
+        sc = __builtin_ldexpf(1.0f, 12 - cexp[u]);
+      } else {
+        const int ex = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+        sc = __builtin_ldexpf(1.0f, 14 - ex);
+        back[u] = __builtin_ldexpf(a_inv, ex - 14);
+      }
 #pragma unroll
       for (int e = 0; e < 8; e++) {
         const float xsv = x[e] * sc;
@@ -989,42 +1012,61 @@ __global__ __launch_bounds__(512, 2) void k_modeprod_f16_r3(const uint16_t *__re
         b[u][1][e] = (_Float16)(xsv - (float)h0);
       }
     }
+    if constexpr (XL == 2) {
 #pragma unroll
-    for (int t = 0; t < MT; t += 2) {
-      f16x8 a[2][2];
+      for (int t = 0; t < MT; t++) {
+        f16x8 a0 = sa[t * 64 + lane], a1 = sa[(MT + t) * 64 + lane];
 #pragma unroll
-      for (int d = 0; d < 2; d++)
-        if (t + d < MT)
+        for (int u = 0; u < 2; u++) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b[u][0], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b[u][1], acc[t][u], 0, 0, 0);
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b[u][0], acc[t][u], 0, 0, 0);
+        }
+      }
+    } else {
 #pragma unroll
-          for (int tm = 0; tm < 2; tm++) a[d][tm] = sa[(tm * MT + t + d) * 64 + lane];
-      f32x4 dch[2][2];
+      for (int t = 0; t < MT; t += 2) {
+        f16x8 a[2][2];
 #pragma unroll
-      for (int d = 0; d < 2; d++)
-#pragma unroll
-        for (int u = 0; u < 2; u++)
-          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][1], b[u][0], (f32x4){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-      for (int d = 0; d < 2; d++)
-#pragma unroll
-        for (int u = 0; u < 2; u++)
-          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][1], dch[d][u], 0, 0, 0);
-#pragma unroll
-      for (int d = 0; d < 2; d++)
-#pragma unroll
-        for (int u = 0; u < 2; u++)
-          if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][0], dch[d][u], 0, 0, 0);
-#pragma unroll
-      for (int d = 0; d < 2; d++)
-#pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int d = 0; d < 2; d++)
           if (t + d < MT)
 #pragma unroll
-            for (int i = 0; i < 4; i++) acc[t + d][u][i] = fmaf(dch[d][u][i], back[u], acc[t + d][u][i]);
+            for (int tm = 0; tm < 2; tm++) a[d][tm] = sa[(tm * MT + t + d) * 64 + lane];
+        f32x4 dch[2][2];
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][1], b[u][0], (f32x4){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][1], dch[d][u], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT) dch[d][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[d][0], b[u][0], dch[d][u], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (t + d < MT)
+#pragma unroll
+              for (int i = 0; i < 4; i++) acc[t + d][u][i] = fmaf(dch[d][u][i], back[u], acc[t + d][u][i]);
+      }
     }
     if (kc == nkc - 1) {
       const int f = tile / ncb;
       const long wgcol = (long)(tile % ncb) * CF::COLS;
       if constexpr (XL == 2) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const float bk = __builtin_ldexpf(a_inv, cexp[u] - 12);
+#pragma unroll
+          for (int t = 0; t < MT; t++) acc[t][u] = acc[t][u] * bk;
+        }
         if (rotate)
           store_tile_m4<MT, 2>(acc, out + out_fs * f, n, wgcol + 32 * wave + r16, q, accumulate);
         else  // k replaced by m in place: the strides of the other kernels' rotated store

@@ -74,7 +74,8 @@ class AdaptOpts(C.Structure):
     _fields_ = [("t_end", C.c_double), ("dt0", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
                 ("tol", C.c_double), ("safety", C.c_double), ("grow", C.c_double), ("shrink", C.c_double),
                 ("learned_safety", C.c_double), ("controller", C.c_int), ("extrapolate", C.c_int), ("energy_test", C.c_int),
-                ("energy_elastic", C.c_int), ("energy_kind", C.c_int), ("max_attempts", C.c_int)]
+                ("energy_elastic", C.c_int), ("energy_kind", C.c_int), ("max_attempts", C.c_int),
+                ("learned_online", C.c_int)]
 
 
 class AdaptStats(C.Structure):

@@ -1,8 +1,11 @@
 """The learned-controller configuration: a 256^3 beta -> omega coarsening run
 (4 variants, mixed precision, 4096 seeded nuclei) with adaptive ETD, step
-doubling and the energy test at tol 1e-4, once with the plain controller and
-once with the learned one, for the same budget of step attempts.  Reports the
-rejection rate, the accepted steps and the total simulated time of each.
+doubling and the energy test at tol 1e-4, for the same budget of step
+attempts with each controller: the plain rule (safety 0.9, growth 2), the
+plain rule retuned to the learned controller's margin and growth bound
+(safety = learned_safety, growth 20), the learned controller, and the learned
+controller without its online correction.  Reports the rejection rate, the
+accepted steps and the total simulated time of each.
 
     python -m pyetd.coarsening_c5 [--attempts 20000] [--weights controller/mlp.npz]
 """
@@ -21,7 +24,7 @@ from pyetd import controller as CT  # noqa: E402
 from pyetd import workloads as W  # noqa: E402
 
 
-def run(prm, attempts, controller, mlp, tol, chunk=1000):
+def run(prm, attempts, controller, mlp, tol, chunk=1000, **extra):
     """Adaptive run for a budget of step attempts, in chunks of `chunk`
     attempts (each chunk continues from the last accepted dt; progress goes to
     stderr)."""
@@ -39,7 +42,7 @@ def run(prm, attempts, controller, mlp, tol, chunk=1000):
     while done < attempts:
         k = min(chunk, attempts - done)
         st, log = g.run_adaptive(log_cap=k, t_end=1e9, tol=tol, dt0=dt0, dt_min=1e-5, dt_max=1.0,
-                                 controller=controller, max_attempts=k)
+                                 controller=controller, max_attempts=k, **extra)
         for key in tot:
             tot[key] += st[key]
         logs.append(log)
@@ -72,10 +75,17 @@ def main():
     mlp = CT.load(args.weights)
     out = {"config": f"beta -> omega coarsening from seeded nuclei, {args.n - 4}^3 x 4 variants, mixed precision, "
                       f"tol {args.tol:g}, {args.attempts} step attempts per controller"}
-    out["plain"] = run(prm, args.attempts, "plain", None, args.tol)
-    print(json.dumps({"plain": out["plain"]}), file=sys.stderr, flush=True)
-    out["learned"] = run(prm, args.attempts, "learned", mlp, args.tol)
-    out["learned_over_plain_sim_time"] = out["learned"]["sim_time"] / out["plain"]["sim_time"]
+    lsafety = pyetd.adapt_opts().learned_safety
+    runs = [("plain", "plain", None, {}),
+            ("tuned_plain", "plain", None, dict(safety=lsafety, grow=20.0)),
+            ("learned", "learned", mlp, {}),
+            ("learned_no_online_correction", "learned", mlp, dict(learned_online=0))]
+    for name, ctrl, net, extra in runs:
+        out[name] = run(prm, args.attempts, ctrl, net, args.tol, **extra)
+        out[name]["options"] = extra
+        print(json.dumps({name: out[name]}), file=sys.stderr, flush=True)
+    for name in ("plain", "tuned_plain", "learned_no_online_correction"):
+        out[f"learned_over_{name}_sim_time"] = out["learned"]["sim_time"] / out[name]["sim_time"]
     mlp.close()
     print(json.dumps(out), flush=True)
 

@@ -2,13 +2,14 @@
 
     python -m pyetd.train_controller --out energy-stability_amd/controller/mlp.npz [--pairs 100000]
 
-Data: plain adaptive runs (mixed precision) of seeded synthetic workloads at
-64^3 - 128^3: beta -> omega with 4 variants, a2 in [0.1, 0.4], eigenstrain
-scale in [0.01, 0.04], either seeded nuclei (0.5 - 2 per 16^3 cells, radius
-3 - 6: growth, impingement and coarsening, t_end 10 - 100) or noise of
-amplitude 0.01 (decay to the beta state, t_end 5 - 20); and beta -> alpha
-(12 variants + 1 CH, t_end 5 - 20); tolerances 1e-5 - 3e-4; each run stops at t_end or after 1000 - 3000 step
-attempts, so that the pairs come from many runs.  Every accepted step gives one (features, target) pair
+Data: plain adaptive runs (mixed precision, FP16 split-2) of seeded synthetic
+workloads at 64^3 - 128^3: beta -> omega with 4 variants, a2 in [0.1, 0.4],
+eigenstrain scale in [0.01, 0.04], and beta -> alpha with 12 variants + 1 CH,
+each either from seeded nuclei (0.5 - 2 per 16^3 cells, radius 3 - 6: growth,
+impingement and coarsening, t_end 10 - 100) or from noise of amplitude 0.01
+(decay to the beta state, t_end 5 - 20); tolerances 1e-5 - 3e-4; each run
+stops at t_end or after 1000 - 3000 step attempts, so that the pairs come
+from many runs.  Every accepted step gives one (features, target) pair
 (pyetd.controller.targets_from_log).  Training uses libetd's own kernels
 (etd_mlp_train: forward, backward, Adam); 5% of the runs are held out and
 their loss is reported."""
@@ -34,20 +35,21 @@ def run_configs(seed):
     while True:
         N = int(rng.choice([64, 96, 128]))
         r = rng.random()
-        family = "alpha" if r < 0.3 else ("decay" if r < 0.5 else "nuclei")
+        family = ("alpha-decay" if r < 0.15 else "alpha-nuclei" if r < 0.35 else "decay" if r < 0.5
+                  else "nuclei")
         t_end = float(rng.choice([5.0, 10.0, 20.0]))
-        if family != "alpha":
+        if not family.startswith("alpha"):
             prm = W.scaled(W.beta_omega_64(seed=1000 + k), N + 4)
             a2 = float(rng.uniform(0.1, 0.4))
             prm["landau"] = (a2, 3.0, 3.0)
             prm["eps0"] = W.eigenstrains(4, float(rng.uniform(0.01, 0.04)), 1000 + k)
-            if family == "nuclei":
-                prm.update(init="nuclei", nuclei_per_cell=float(rng.uniform(0.5, 2.0)) * 16.0 ** -3,
-                           nuclei_radius=(3.0, 6.0))
-                t_end = float(rng.choice([10.0, 30.0, 100.0]))
         else:
             prm = W.scaled(W.beta_alpha_192(seed=1000 + k), N + 4)
-        prm["precision"] = 1
+        if family.endswith("nuclei"):
+            prm.update(init="nuclei", nuclei_per_cell=float(rng.uniform(0.5, 2.0)) * 16.0 ** -3,
+                       nuclei_radius=(3.0, 6.0))
+            t_end = float(rng.choice([10.0, 30.0, 100.0]))
+        prm["precision"] = 2
         prm["seed"] = 1000 + k
         opts = dict(t_end=t_end, tol=float(rng.choice([1e-5, 3e-5, 1e-4, 3e-4])),
                     dt0=float(rng.choice([0.002, 0.005, 0.01])), dt_min=1e-5, dt_max=1.0,

@@ -245,6 +245,9 @@ typedef struct etd_adapt_opts {
   int energy_elastic;  /* 1: the tested energy adds the elastic energy ES/2 h^3 sum eta_p felas_p */
   int energy_kind;     /* 1: the scheme's discrete free energy (default); 0: check_soln_new's */
   int max_attempts;
+  int learned_online;  /* 1 (default): the learned proposal's prediction is lowered by the largest recent
+                          over-prediction (decaying by 0.8 per learned step), measured after each attempt
+                          against the error model's largest step at the observed error; 0: the network alone */
 } etd_adapt_opts;
 
 typedef struct etd_adapt_stats {
