@@ -8,6 +8,8 @@
 #include <string.h>
 
 #include <stdint.h>
+#include <stdio.h>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -2023,6 +2025,132 @@ int launch_diag(etd_ctx *c, double *stats) {
   }
   stats[4 * V] = couple;
   return ETD_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free diagnostics
+// check_soln_new into a caller's device buffer with no allocation, no host
+// synchronisation and no memset: the per-plane partial sums go to a
+// file-scope device array (one copy per GPU, kept for the process's
+// lifetime) and k_diag_finish adds them up in the order launch_diag uses on
+// the host, so the statistics are bit for bit those of etd_diag.
+#define DIAG_SCRATCH_DOUBLES 65536  // 512 KiB
+#define DIAG_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[DIAG_SCRATCH_DOUBLES];
+static double *scratch_dev_ptr[DIAG_SCRATCH_DEVICES];
+static std::mutex scratch_mu;
+
+// the device address of `scratch` on device dev (looked up once per device,
+// with that device current; the host-side name is not a device address)
+static int scratch_address(int dev, double **out) {
+  if (dev < 0 || dev >= DIAG_SCRATCH_DEVICES) return ETD_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev_ptr[dev]) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return ETD_ERR_DEVICE;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return ETD_ERR_DEVICE;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr) return ETD_ERR_DEVICE;
+    scratch_dev_ptr[dev] = (double *)ptr;
+  }
+  *out = scratch_dev_ptr[dev];
+  return ETD_OK;
+}
+
+// One thread per result, each a sequential sum in launch_diag's host order:
+// threads 0..V-1 max / min / vol of field t, thread V the AC energy, threads
+// V+1 .. V+nch the CH energies, thread V+nch+1 the couple energy.
+__global__ __launch_bounds__(256) void k_diag_finish(const double *__restrict__ part, int ms, int me, int V, int nac,
+                                                     int nch, double h3, double *__restrict__ stats) {
+#pragma clang fp contract(off)  // the host sums without fused multiply-adds
+  const int t = threadIdx.x;
+  const double *hs = part, *he = part + (size_t)3 * ms * V;
+  const long mc = (long)me * ECHUNK;
+  if (t < V) {
+    double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
+    for (int k = 0; k < ms; k++) {
+      const double *o = hs + ((size_t)t * ms + k) * 3;
+      if (o[0] > mx) mx = o[0];
+      if (o[1] < mn) mn = o[1];
+      vol += o[2];
+    }
+    stats[4 * t + 0] = mx;
+    stats[4 * t + 1] = mn;
+    stats[4 * t + 2] = vol * h3;  // the energy column [4 t + 3] belongs to the energy threads below
+  } else if (t == V) {
+    if (nac > 0) {
+      double e = 0.0;
+      for (long k = 0; k < mc; k++) e += he[k];
+      e = e * h3;
+      for (int v = 0; v < nac; v++) stats[4 * v + 3] = e;
+    }
+  } else if (t <= V + nch) {
+    const int q = t - V - 1;
+    double e = 0.0;
+    for (long k = 0; k < mc; k++) e += he[(size_t)(1 + q) * mc + k];
+    stats[4 * (nac + q) + 3] = e * h3;
+  } else if (t == V + nch + 1) {
+    double couple = 0.0;
+    if (nac > 0 && nch > 0) {
+      double last = 0.0;  // the energy of the last CH field, as launch_diag carries it over
+      for (long k = 0; k < mc; k++) last += he[(size_t)nch * mc + k];
+      couple = last * h3;
+      for (long k = 0; k < mc; k++) couple += he[(size_t)(1 + nch) * mc + k];
+      couple = couple * h3;
+    }
+    stats[4 * V] = couple;
+  }
+}
+
+template <typename T>
+static int diag_noalloc_t(etd_ctx *c, double *stats_dev, double *sc) {
+  const T *U = fields<T>(c).u;
+  diag_args a;
+  a.n = c->n;
+  a.g = c->g;
+  a.nac = c->nac;
+  a.nch = c->nch;
+  a.hx = c->p.hx;
+  a.hy = c->p.hy;
+  a.hz = c->p.hz;
+  a.epn2 = c->p.epn2;
+  a.a2 = c->p.a2;
+  a.a3 = c->p.a3;
+  a.a4 = c->p.a4;
+  a.ihx = 1.0 / c->p.hx;
+  a.ihy = 1.0 / c->p.hy;
+  a.ihz = 1.0 / c->p.hz;
+  const int n = c->n, g = c->g, V = c->V;
+  const int ms = n - 2 * g, me = n - 2 * g - 1;
+  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
+  k_diag_stats<T><<<dim3(ms, V), 256, 0, c->stream>>>(U, sc, a);
+  k_diag_energy<T><<<(unsigned)(me * kinds * ECHUNK), 256, 0, c->stream>>>(U, c->lam, sc + (size_t)3 * ms * V, a);
+  k_diag_finish<<<1, 256, 0, c->stream>>>(sc, ms, me, V, c->nac, c->nch, c->p.hx * c->p.hy * c->p.hz, stats_dev);
+  return hipGetLastError() == hipSuccess ? ETD_OK : ETD_ERR_DEVICE;
+}
+
+extern "C" int etd_diag_noalloc(etd_ctx *c, double *stats_dev) {
+  if (!c || !stats_dev) return ETD_ERR_ARG;
+  const int ms = c->n - 2 * c->g, me = ms - 1;
+  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
+  const size_t need = (size_t)3 * ms * c->V + (size_t)me * kinds * ECHUNK;
+  if (need > DIAG_SCRATCH_DOUBLES || c->V + c->nch + 2 > 256) {
+    snprintf(c->err, sizeof(c->err),
+             "etd_diag_noalloc: this context needs %zu doubles of partial sums (3 N V + (N - 1)(1 + nch + "
+             "couple) 8); the static buffer holds %d",
+             need, DIAG_SCRATCH_DOUBLES);
+    return ETD_ERR_UNSUPPORTED;
+  }
+  double *sc = nullptr;
+  int rc = scratch_address(c->device, &sc);
+  if (rc) {
+    snprintf(c->err, sizeof(c->err), "etd_diag_noalloc: no address for the static scratch on device %d", c->device);
+    return rc;
+  }
+  rc = c->mixed ? diag_noalloc_t<float>(c, stats_dev, sc) : diag_noalloc_t<double>(c, stats_dev, sc);
+  if (rc) snprintf(c->err, sizeof(c->err), "etd_diag_noalloc: %s", hipGetErrorString(hipGetLastError()));
+  return rc;
 }
 
 // ------------------------------------------------------------------ counter calibration

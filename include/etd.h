@@ -155,6 +155,20 @@ int etd_step(etd_ctx *c, int nsteps);
  * stats[4*(nac+nch)] = couple energy (0 unless nac > 0 and nch > 0). */
 int etd_diag(etd_ctx *c, double *stats);
 
+/* The allocation-free variant of etd_diag; it has no counterpart in the
+ * reference.  It is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops.  The same statistics, bit for bit, go to
+ * stats_dev, a caller's device buffer of 4 (nac + nch) + 1 doubles, from
+ * kernels queued on the context's stream: no host synchronisation, no
+ * allocation, no memset.  The partial sums live in a static 512 KiB device
+ * array (one per GPU, held until the process ends), which must hold
+ * 3 N (nac + nch) + (N - 1)(1 + nch + [nac > 0 and nch > 0]) 8 doubles
+ * (65536; e.g. 12,072 for 192^3 x (12 + 1) fields); larger contexts get
+ * ETD_ERR_UNSUPPORTED.  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (other streams, other threads) race.  The
+ * result is ready when the context's stream has reached it. */
+int etd_diag_noalloc(etd_ctx *c, double *stats_dev);
+
 /* Appends the CSV rows of check_soln_new (phi_ac_<v>.csv, phi_ch_<v>.csv,
  * phi_couple.csv, format "%d,%+1.15lf,...") under work_dir; iter == 0 also
  * writes the headers. */
