@@ -285,8 +285,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     free(k);
   }
   // energy partials
-  const int ms = n - 2 * c->g, me = n - 2 * c->g - 1;
-  c->npart = 3 * ms * V + me * (2 + c->nch) * 8;  // 8 row blocks per energy plane
+  c->npart = (int)etd_diag_partials(c);
   HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
   if (c->mixed && c->nch > 0) {
     HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));

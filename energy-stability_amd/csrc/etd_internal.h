@@ -165,6 +165,8 @@ template <typename T>
 void launch_felas_materialize(etd_ctx *c, T *felas);
 template <typename T>
 int launch_diag(etd_ctx *c, double *stats);
+/* doubles of partial sums one check_soln_new evaluation writes (the size of c->part) */
+size_t etd_diag_partials(const etd_ctx *c);
 /* the two-stage register FFT (k_fft16) handles this N */
 bool fft16_supported(int N);
 /* CH mean conservation of the mixed path: ch_mean_mark takes the interior

@@ -1796,8 +1796,10 @@ void launch_felas_materialize(etd_ctx *c, T *felas) {
 }
 
 // ------------------------------------------------------------------ energy
-// check_soln_new (field.c:407-678): per-plane partial sums in FP64 (wave
-// shuffles, then LDS across the 4 waves), summed in plane order on the host.
+// check_soln_new (field.c:407-678) in one kernel that reads every field
+// once: the statistics (max, min, sum of u + 1 over [g, n - g)^3) and the
+// energy sums (cells [g, n - g - 1)^3) as FP64 partial sums per block,
+// added up in a fixed order by diag_reduce (host) or k_diag_finish (device).
 struct diag_args {
   int n, g, nac, nch;
   double hx, hy, hz, epn2, a2, a3, a4;
@@ -1817,150 +1819,250 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// grid (planes = n - 2g, V): max, min, sum(u + 1) over the plane interior
-template <typename T>
-__global__ __launch_bounds__(256) void k_diag_stats(const T *__restrict__ u, double *__restrict__ part, diag_args a) {
-  const int n = a.n, g = a.g, m = n - 2 * g;
-  const int k = g + blockIdx.x, v = blockIdx.y;
-  const T *f = u + (size_t)n * n * n * v;
-  double mx = -1.0e30, mn = 1.0e30, s = 0.0;
-  for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-    const int j = g + t / m, i = g + t % m;
-    const double x = f[IDX(n, k, j, i)];
-    mx = fmax(mx, x);
-    mn = fmin(mn, x);
-    s += x + 1.0;
-  }
-  __shared__ double sh[3][4];
-  mx = wave_max(mx);
-  mn = wave_min(mn);
-  s = wave_sum(s);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][w] = mx;
-    sh[1][w] = mn;
-    sh[2][w] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double* o = part + ((size_t)v * m + blockIdx.x) * 3;
-    o[0] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
-    o[1] = fmin(fmin(sh[1][0], sh[1][1]), fmin(sh[1][2], sh[1][3]));
-    o[2] = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void cell_grad(const T *u, int n, int k, int j, int i, double hx, double hy, double hz,
-                                          double &mean, double &ux, double &uy, double &uz) {
-  double f0 = u[IDX(n, k, j, i)], f1 = u[IDX(n, k, j, i + 1)], f2 = u[IDX(n, k, j + 1, i)],
-         f3 = u[IDX(n, k, j + 1, i + 1)];
-  double f4 = u[IDX(n, k + 1, j, i)], f5 = u[IDX(n, k + 1, j, i + 1)], f6 = u[IDX(n, k + 1, j + 1, i)],
-         f7 = u[IDX(n, k + 1, j + 1, i + 1)];
-  mean = (f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7) / 8;
-  ux = ((f1 - f0) + (f3 - f2) + (f5 - f4) + (f7 - f6)) / 4 / hx;
-  uy = ((f2 - f0) + (f3 - f1) + (f6 - f4) + (f7 - f5)) / 4 / hy;
-  uz = ((f4 - f0) + (f5 - f1) + (f6 - f2) + (f7 - f3)) / 4 / hz;
-}
-
-// the same cell quantities from the 8 corner values already loaded; / 8 and
-// / 4 are exact as products, 1 / h is one rounding away from / h
-__device__ __forceinline__ void cell_grad_v(const double (&f)[8], double ihx, double ihy, double ihz, double &mean,
-                                            double &ux, double &uy, double &uz) {
-  mean = (f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7]) * 0.125;
-  ux = ((f[1] - f[0]) + (f[3] - f[2]) + (f[5] - f[4]) + (f[7] - f[6])) * 0.25 * ihx;
-  uy = ((f[2] - f[0]) + (f[3] - f[1]) + (f[6] - f[4]) + (f[7] - f[5])) * 0.25 * ihy;
-  uz = ((f[4] - f[0]) + (f[5] - f[1]) + (f[6] - f[2]) + (f[7] - f[3])) * 0.25 * ihz;
-}
-
-// work items (plane = n - 2g - 1 of them, kind, row block), kind 0 AC
-// energy, 1..nch CH energy of field nac + kind - 1, nch + 1 couple gradient
-// terms; each plane split into ECHUNK row blocks.  A cell reads planes k and
-// k + 1, so neighbouring planes share half their reads: the 1-D grid is
-// mapped so that each XCD (workgroups go to the 8 XCDs in turn) runs a
-// contiguous range of work items, planes fastest, and the shared plane is
-// read once into that XCD's L2 instead of once per XCD.
+// One block per (row j, chunk of planes), threads along i, 63 cells per wave
+// (lane 63 only supplies the i + 1 corners of lane 62).  The work of a block
+// is a list of tasks (field, sub-range of E = 8 planes): the AC
+// fields sub-range by sub-range (all AC fields of a sub-range in turn), then
+// each CH field's sub-ranges.  A task loads rows j and j + 1 of the E + 1
+// planes it needs into registers; the loads of task t + 1 are issued before
+// task t is computed (two register buffers, used in turn).  Every row of
+// every field is read from HBM once per chunk, plus one shared plane per
+// sub-range; the i + 1 corners come by lane shifts.
+//
+// Per plane p the thread forms, from its four corners a = (j, i),
+// a' = (j, i + 1), b = (j + 1, i), b' = (j + 1, i + 1):
+//   Dx = (a' - a) + (b' - b),  Dy = (b - a) + (b' - a'),  P = (a + a') + (b + b')
+// and a cell between planes p and p + 1 has the corner sums S = P_p + P_{p+1},
+// SX = Dx_p + Dx_{p+1}, SY = Dy_p + Dy_{p+1}, SZ = P_{p+1} - P_p: mean S / 8,
+// gradient (SX / 4hx, SY / 4hy, SZ / 4hz).  These are the reference's
+// eight-corner sums (field.c:466-486) regrouped; the gradient energy
+// epn2/2 grad.Lambda grad is the quadratic form of (SX, SY, SZ) with the
+// symmetric matrix Q = epn2/2 D (Lambda + Lambda^T)/2 D, D = diag(1/4h), and
+// the Landau sums take S^2 = 64 mean^2 and S^3 = 512 mean^3 (exact scalings).
+// About 26 FP64 operations per AC cell and field instead of about 40: the
+// kernel is bound by FP64 issue, not bytes.  The regrouping changes each
+// cell's terms by a few roundings.  The per-cell arithmetic runs on every
+// lane; the lanes that hold no cell or point (lane 63, lanes past the row)
+// are masked once per task, not per cell.
+//
+// Energy: kind 0 the AC gradient energy plus the Landau terms of the cell
+// means (S2, S3 of each plane wait in registers while the AC fields of the
+// sub-range pass), kind 1 + q the CH field's gradient + double-well energy,
+// kind 1 + nch the couple term (gradient energy of the AC fields q < nch and
+// of the CH fields, from the same loads).  Statistics: a point is counted by
+// the block whose row j is its row (the last row, n - g - 1, by the last row
+// block as its row j + 1), the chunk holding its plane (plane n - g - 1 by
+// the last chunk) and one lane; sum(u + 1) is kept as sum(u) plus the count.
+// Each task's statistics are wave-reduced into a per-wave slot in LDS, the
+// slots combined at the end.
+//
+// Partials: b = row ECHUNK + chunk of nb = m ECHUNK; statistics at
+// part[(v nb + b) 3 + {max, min, sum}], energies at part[3 V nb + kind nb + b].
+// The 1-D grid is numbered so that each XCD (workgroups go to the 8 XCDs in
+// turn) runs a contiguous range of rows: row j + 1, read by blocks j and
+// j + 1, stays in one L2.
 #define ECHUNK 8
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_energy(const T *__restrict__ u, const double *__restrict__ lam,
-                                                     double *__restrict__ part, diag_args a) {
-  const int n = a.n, g = a.g, m = n - 2 * g - 1;
-  const int per = (int)gridDim.x / 8;  // gridDim.x = m kinds ECHUNK, a multiple of 8
+__global__ __launch_bounds__(320) void k_diag(const T *__restrict__ u, const double *__restrict__ lam,
+                                              double *__restrict__ part, diag_args a) {
+  constexpr int E = 8;  // planes per sub-range: 152-192 VGPRs, 2-3 waves per SIMD (12 and 6 measured slower)
+  extern __shared__ double wst[];             // [wave][field][max, min, sum]
+  const int n = a.n, g = a.g, m = n - 2 * g - 1, V = a.nac + a.nch, nac = a.nac, nch = a.nch;
+  const int per = (int)gridDim.x / 8;  // gridDim.x = m ECHUNK, a multiple of 8
   const int w = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
-  const int plane = w % m, rest = w / m;
-  const int kinds = (int)gridDim.x / (m * ECHUNK);
-  const int kind = rest % kinds, chunk = rest / kinds;
-  const int k = g + plane;
-  const size_t n3 = (size_t)n * n * n;
-  const int r0 = (int)(((long)m * chunk) / ECHUNK), r1 = (int)(((long)m * (chunk + 1)) / ECHUNK);
-  double e = 0.0;
-  for (int t = r0 * m + threadIdx.x; t < r1 * m; t += blockDim.x) {
-    const int j = g + t / m, i = g + t % m;
-    if (kind == 0) {
-      // the 8 corners of 4 fields loaded before any is used; sums in field order
-      double s2 = 0.0, s3 = 0.0;
-      const size_t c0 = IDX(n, k, j, i), dy = (size_t)n, dz = (size_t)n * n;
-      for (int m0 = 0; m0 < a.nac; m0 += 4) {
-        T fv[4][8];
+  const int row = w % m, chunk = w / m;
+  const int nb = m * ECHUNK, bi = row * ECHUNK + chunk;
+  const int j = g + row;
+  const int kb = g + (int)(((long)m * chunk) / ECHUNK), ke = g + (int)(((long)m * (chunk + 1)) / ECHUNK);
+  const int nsub = (ke - kb + E - 1) / E;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+  const int i = g + 63 * wv + lane;
+  const bool cell = lane < 63 && i < g + m;
+  const bool st_lane = i < n - g && (lane < 63 || wv == nw - 1);  // each i of [g, n - g) once
+  const bool st_row1 = row == m - 1;                              // also count row j + 1 = n - g - 1
+  const bool last_chunk = chunk == ECHUNK - 1;
+  const size_t n2 = (size_t)n * n, n3 = n2 * n;
+  const size_t r0 = (size_t)j * n + (i < n ? i : 0);
+  const double he = 0.5 * a.epn2, dx = 0.25 * a.ihx, dy = 0.25 * a.ihy, dz = 0.25 * a.ihz;
+  const double wx = he * dx * dx, wy = he * dy * dy, wz = he * dz * dz;  // |grad|^2 weights of SX^2, SY^2, SZ^2
+  for (int t = threadIdx.x; t < nw * V; t += blockDim.x) {
+    wst[3 * t + 0] = -1.0e30;
+    wst[3 * t + 1] = 1.0e30;
+    wst[3 * t + 2] = 0.0;
+  }
+  __syncthreads();
+  __shared__ double sh[5];  // up to 5 waves: m <= 315 (contexts have n <= 272)
+  auto block_sum_out = [&](double v, int kind) {
+    v = wave_sum(cell ? v : 0.0);
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int q = 0; q < nw; q++) s += sh[q];
+      part[(size_t)3 * V * nb + (size_t)kind * nb + bi] = s;
+    }
+  };
+  // task (v, s): field v, sub-range s (planes kb + E s ..); AC fields first, all of a sub-range in turn, then
+  // the CH fields one by one
+  struct tk {
+    int v, s;
+  };
+  auto next = [&](tk x) {
+    if (x.v < nac) {
+      if (++x.v == nac) {
+        x.v = 0;
+        if (++x.s == nsub) x.v = nac, x.s = 0;
+      }
+    } else if (++x.s == nsub) {
+      x.s = 0;
+      ++x.v;
+    }
+    return x;
+  };
+  const int nt = V * nsub;
+  // every lane loads (lanes past the box read column 0: their values feed no counted cell or point)
+  auto load = [&](tk x, T (&x0)[E + 1], T (&x1)[E + 1]) {
+    const int k0 = kb + E * x.s, last = ke - k0;
+    const T *f0 = u + n3 * x.v + (size_t)k0 * n2 + r0, *f1 = f0 + n;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const T *f = u + n3 * (m0 + q < a.nac ? m0 + q : a.nac - 1) + c0;
-          fv[q][0] = f[0];
-          fv[q][1] = f[1];
-          fv[q][2] = f[dy];
-          fv[q][3] = f[dy + 1];
-          fv[q][4] = f[dz];
-          fv[q][5] = f[dz + 1];
-          fv[q][6] = f[dz + dy];
-          fv[q][7] = f[dz + dy + 1];
+    for (int p = 0; p <= E; p++) {
+      const size_t o = (size_t)(p < last ? p : last) * n2;
+      x0[p] = f0[o];
+      x1[p] = f1[o];
+    }
+  };
+  const bool couple = nac > 0 && nch > 0;
+  double e = 0.0, ec = 0.0, sq = 0.0;
+  double s2[E], s3[E];
+#pragma unroll
+  for (int p = 0; p < E; p++) s2[p] = s3[p] = 0.0;
+  auto process = [&](tk x, const T (&x0)[E + 1], const T (&x1)[E + 1]) {
+    const int v = x.v, k0 = kb + E * x.s;
+    const int np = ke - k0 < E ? ke - k0 : E;
+    const bool ac = v < nac, cp = couple && (v >= nac || v < nch);
+    // the planes whose points this block counts: [k0, k0 + np), and plane ke in the last chunk's last sub-range
+    const int nst = np + ((last_chunk && k0 + np == ke) ? 1 : 0);
+    double Qxx = 0, Qyy = 0, Qzz = 0, Qxy = 0, Qxz = 0, Qyz = 0;
+    if (ac) {
+      const double *L = lam + 9 * v;
+      Qxx = he * dx * dx * L[0];
+      Qyy = he * dy * dy * L[4];
+      Qzz = he * dz * dz * L[8];
+      Qxy = he * dx * dy * (L[1] + L[3]);  // off-diagonal pairs: the form counts each once
+      Qxz = he * dx * dz * (L[2] + L[6]);
+      Qyz = he * dy * dz * (L[5] + L[7]);
+    }
+    T mx = (T)-1.0e30, mn = (T)1.0e30;
+    double sm = 0.0;
+    double Dx0 = 0, Dy0 = 0, P0 = 0;
+#pragma unroll
+    for (int p = 0; p <= E; p++) {
+      const T a1 = __shfl_down(x0[p], 1, 64), b1 = __shfl_down(x1[p], 1, 64);  // shifted in the field's type
+      if (p < nst) {
+        mx = fmax(mx, x0[p]);
+        mn = fmin(mn, x0[p]);
+        sm += (double)x0[p];
+        if (st_row1) {
+          mx = fmax(mx, x1[p]);
+          mn = fmin(mn, x1[p]);
+          sm += (double)x1[p];
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-        const int mm = m0 + q;
-        if (mm >= a.nac) break;
-        double fd[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++) fd[r] = (double)fv[q][r];
-        double um, ux, uy, uz;
-        cell_grad_v(fd, a.ihx, a.ihy, a.ihz, um, ux, uy, uz);
-        const double *L = lam + 9 * mm;
-        const double ffx = ux * L[0] + uy * L[3] + uz * L[6];
-        const double ffy = ux * L[1] + uy * L[4] + uz * L[7];
-        const double ffz = ux * L[2] + uy * L[5] + uz * L[8];
-        e += a.epn2 * (ffx * ux + ffy * uy + ffz * uz) / 2;
-        s2 += um * um;
-        s3 += um * um * um;
+      }
+      const double A = x0[p], B = x1[p], A1 = a1, B1 = b1;
+      const double Dx1 = (A1 - A) + (B1 - B), Dy1 = (B - A) + (B1 - A1), P1 = (A + A1) + (B + B1);
+      if (p > 0 && p - 1 < np) {  // the cell between planes p - 1 and p (uniform condition)
+        const int c = p - 1;
+        const double S = P0 + P1, SX = Dx0 + Dx1, SY = Dy0 + Dy1, SZ = P1 - P0;
+        if (ac) {
+          const double qa = Qxx * SX + Qxy * SY + Qxz * SZ, qb = Qyy * SY + Qyz * SZ, qc = Qzz * SZ;
+          e = fma(SX, qa, e);
+          e = fma(SY, qb, e);
+          e = fma(SZ, qc, e);
+          const double S2 = S * S;
+          s2[c] += S2;
+          s3[c] = fma(S2, S, s3[c]);
+        }
+        if (cp || !ac) {
+          const double g2 = fma(wx * SX, SX, fma(wy * SY, SY, wz * SZ * SZ));
+          if (cp) ec += g2;
+          if (!ac) {
+            const double w2 = fma(S * S, 1.0 / 64.0, -1.0);
+            sq += g2 + w2 * w2 * 0.25;
+          }
         }
       }
-      e += (a.a2 / 2.0) * s2 - (a.a3 / 3.0) * s3 + (a.a4 / 4.0) * s2 * s2;
-    } else if (kind <= a.nch) {
-      double cm, ux, uy, uz;
-      cell_grad(u + n3 * (a.nac + kind - 1), n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
-      e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
-      e += ((cm * cm - 1) * (cm * cm - 1) / 4);
-    } else {
-      for (int q = 0; q < a.nch && q < a.nac; q++) {
-        double cm, ux, uy, uz;
-        cell_grad(u + n3 * q, n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
-        e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
+      Dx0 = Dx1, Dy0 = Dy1, P0 = P1;
+    }
+    // this task's statistics: lanes that count no point give the identity
+    const double cnt = (double)(nst * (st_row1 ? 2 : 1));
+    double dmx = st_lane ? (double)mx : -1.0e30, dmn = st_lane ? (double)mn : 1.0e30;
+    sm = st_lane ? sm + cnt : 0.0;
+    dmx = wave_max(dmx);
+    dmn = wave_min(dmn);
+    sm = wave_sum(sm);
+    if (lane == 0) {
+      double *o = wst + 3 * (wv * V + v);
+      o[0] = fmax(o[0], dmx);
+      o[1] = fmin(o[1], dmn);
+      o[2] += sm;
+    }
+    if (ac && v == nac - 1) {  // the sub-range's Landau terms, after its last AC field
+#pragma unroll
+      for (int p = 0; p < E; p++) {
+        if (p < np)
+          e += (a.a2 / 128.0) * s2[p] - (a.a3 / 1536.0) * s3[p] + (a.a4 / 16384.0) * s2[p] * s2[p];
+        s2[p] = s3[p] = 0.0;
       }
-      for (int q = 0; q < a.nch; q++) {
-        double cm, ux, uy, uz;
-        cell_grad(u + n3 * (a.nac + q), n, k, j, i, a.hx, a.hy, a.hz, cm, ux, uy, uz);
-        e += a.epn2 * (ux * ux + uy * uy + uz * uz) / 2;
-      }
+      if (x.s == nsub - 1) block_sum_out(e, 0);
+    }
+    if (!ac && x.s == nsub - 1) {  // the CH field's last sub-range
+      block_sum_out(sq, 1 + v - nac);
+      sq = 0.0;
+    }
+  };
+  T xa0[E + 1], xa1[E + 1], xb0[E + 1], xb1[E + 1];
+  tk ta = {nac > 0 ? 0 : nac, 0};
+  if (nt > 0) load(ta, xa0, xa1);
+  for (int t = 0; t < nt; t += 2) {
+    const tk tb = next(ta);
+    if (t + 1 < nt) load(tb, xb0, xb1);
+    process(ta, xa0, xa1);
+    if (t + 1 < nt) {
+      ta = next(tb);
+      if (t + 2 < nt) load(ta, xa0, xa1);
+      process(tb, xb0, xb1);
     }
   }
-  __shared__ double sh[4];
-  e = wave_sum(e);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
+  if (nsub == 0) {  // an empty chunk (fewer than ECHUNK planes in all): zero partials
+    for (int q = 0; q <= nch; q++) block_sum_out(0.0, q);
+  } else if (nac == 0) {
+    block_sum_out(0.0, 0);  // no AC fields: kind 0 is read by nobody, kept defined
+  }
+  if (couple) block_sum_out(ec, 1 + nch);
   __syncthreads();
-  if (threadIdx.x == 0) part[((size_t)kind * m + plane) * ECHUNK + chunk] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  for (int v = threadIdx.x; v < V; v += blockDim.x) {
+    double mx = -1.0e30, mn = 1.0e30, sm = 0.0;
+    for (int q = 0; q < nw; q++) {
+      const double *o = wst + 3 * (q * V + v);
+      mx = fmax(mx, o[0]);
+      mn = fmin(mn, o[1]);
+      sm += o[2];
+    }
+    double *o = part + ((size_t)v * nb + bi) * 3;
+    o[0] = mx;
+    o[1] = mn;
+    o[2] = sm;
+  }
 }
 
-template <typename T>
-int launch_diag(etd_ctx *c, double *stats) {
-  const T *U = fields<T>(c).u;
+static int diag_kinds(const etd_ctx *c) { return 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0); }
+static int diag_blocks(const etd_ctx *c) { return (c->n - 2 * c->g - 1) * ECHUNK; }
+// doubles of partial sums one diagnostics call writes
+static size_t diag_partials(const etd_ctx *c) { return ((size_t)3 * c->V + diag_kinds(c)) * diag_blocks(c); }
+size_t etd_diag_partials(const etd_ctx *c) { return diag_partials(c); }
+
+static diag_args make_diag_args(const etd_ctx *c) {
   diag_args a;
   a.n = c->n;
   a.g = c->g;
@@ -1976,60 +2078,83 @@ int launch_diag(etd_ctx *c, double *stats) {
   a.ihx = 1.0 / c->p.hx;
   a.ihy = 1.0 / c->p.hy;
   a.ihz = 1.0 / c->p.hz;
-  const int n = c->n, g = c->g, V = c->V;
-  const int ms = n - 2 * g, me = n - 2 * g - 1;
-  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
-  double *pst = c->part, *pen = c->part + (size_t)3 * ms * V;
-  etd_prof_begin(c, PC_ENERGY);
-  k_diag_stats<T><<<dim3(ms, V), 256, 0, c->stream>>>(U, pst, a);
+  return a;
+}
+
+template <typename T>
+static void launch_diag_kernel(etd_ctx *c, double *part) {
   static_assert(ECHUNK % 8 == 0, "the XCD mapping needs a multiple of 8 work items");
-  k_diag_energy<T><<<(unsigned)(me * kinds * ECHUNK), 256, 0, c->stream>>>(U, c->lam, pen, a);
-  etd_prof_end(c, PC_ENERGY);
-  const size_t cnt = (size_t)3 * ms * V + (size_t)me * kinds * ECHUNK;
-  if (hipMemcpyAsync(c->hpart, c->part, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-    return ETD_ERR_DEVICE;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return ETD_ERR_DEVICE;
-  const double h3 = c->p.hx * c->p.hy * c->p.hz;
-  const double *hs = c->hpart, *he = c->hpart + (size_t)3 * ms * V;
-  for (int v = 0; v < V; v++) {
+  const int me = c->n - 2 * c->g - 1, nw = (me + 62) / 63;
+  k_diag<T><<<(unsigned)(me * ECHUNK), 64 * nw, sizeof(double) * 3 * nw * c->V, c->stream>>>(fields<T>(c).u, c->lam,
+                                                                                           part, make_diag_args(c));
+}
+
+// Reduction of the partials, one result per thread index t (0 .. V + nch + 1),
+// each a sequential sum in block order; shared by the host (diag_reduce) and
+// k_diag_finish so the two give the same bits.  t < V: max, min, vol of field
+// t; t == V: the AC energy (reported for every AC field); V < t <= V + nch:
+// CH field t - V - 1; t == V + nch + 1: the couple energy, which continues
+// from the last CH field's energy as check_soln_new's accumulator does.
+__host__ __device__ static inline void diag_reduce_one(const double *part, int nb, int V, int nac, int nch, double h3,
+                                                       int t, double *stats) {
+#pragma clang fp contract(off)
+  const double *he = part + (size_t)3 * V * nb;
+  if (t < V) {
     double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
-    for (int k = 0; k < ms; k++) {
-      const double *o = hs + ((size_t)v * ms + k) * 3;
+    for (int b = 0; b < nb; b++) {
+      const double *o = part + ((size_t)t * nb + b) * 3;
       if (o[0] > mx) mx = o[0];
       if (o[1] < mn) mn = o[1];
       vol += o[2];
     }
-    stats[4 * v + 0] = mx;
-    stats[4 * v + 1] = mn;
-    stats[4 * v + 2] = vol * h3;
-    stats[4 * v + 3] = 0.0;
-  }
-  if (c->nac > 0) {
+    stats[4 * t + 0] = mx;
+    stats[4 * t + 1] = mn;
+    stats[4 * t + 2] = vol * h3;  // the energy column [4 t + 3] belongs to the energy indices below
+  } else if (t == V) {
+    if (nac > 0) {
+      double e = 0.0;
+      for (int b = 0; b < nb; b++) e += he[b];
+      e = e * h3;
+      for (int v = 0; v < nac; v++) stats[4 * v + 3] = e;
+    }
+  } else if (t <= V + nch) {
+    const int q = t - V - 1;
     double e = 0.0;
-    for (int k = 0; k < me * ECHUNK; k++) e += he[k];
-    e = e * h3;
-    for (int v = 0; v < c->nac; v++) stats[4 * v + 3] = e;
+    for (int b = 0; b < nb; b++) e += he[(size_t)(1 + q) * nb + b];
+    stats[4 * (nac + q) + 3] = e * h3;
+  } else if (t == V + nch + 1) {
+    double couple = 0.0;
+    if (nac > 0 && nch > 0) {
+      double last = 0.0;
+      for (int b = 0; b < nb; b++) last += he[(size_t)nch * nb + b];
+      couple = last * h3;
+      for (int b = 0; b < nb; b++) couple += he[(size_t)(1 + nch) * nb + b];
+      couple = couple * h3;
+    }
+    stats[4 * V] = couple;
   }
-  double engtmp = 0.0;
-  for (int q = 0; q < c->nch; q++) {
-    engtmp = 0.0;
-    for (int k = 0; k < me * ECHUNK; k++) engtmp += he[(size_t)(1 + q) * me * ECHUNK + k];
-    engtmp = engtmp * h3;
-    stats[4 * (c->nac + q) + 3] = engtmp;
-  }
-  double couple = 0.0;
-  if (c->nac > 0 && c->nch > 0) {
-    couple = engtmp;
-    for (int k = 0; k < me * ECHUNK; k++) couple += he[(size_t)(1 + c->nch) * me * ECHUNK + k];
-    couple = couple * h3;
-  }
-  stats[4 * V] = couple;
+}
+
+template <typename T>
+int launch_diag(etd_ctx *c, double *stats) {
+  etd_prof_begin(c, PC_ENERGY);
+  launch_diag_kernel<T>(c, c->part);
+  etd_prof_end(c, PC_ENERGY);
+  if (hipGetLastError() != hipSuccess) return ETD_ERR_DEVICE;
+  if (hipMemcpyAsync(c->hpart, c->part, diag_partials(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+      hipSuccess)
+    return ETD_ERR_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return ETD_ERR_DEVICE;
+  const double h3 = c->p.hx * c->p.hy * c->p.hz;
+  for (int v = 0; v < c->V; v++) stats[4 * v + 3] = 0.0;
+  for (int t = 0; t <= c->V + c->nch + 1; t++)
+    diag_reduce_one(c->hpart, diag_blocks(c), c->V, c->nac, c->nch, h3, t, stats);
   return ETD_OK;
 }
 
 // ------------------------------------------------------------------ allocation-free diagnostics
 // check_soln_new into a caller's device buffer with no allocation, no host
-// synchronisation and no memset: the per-plane partial sums go to a
+// synchronisation and no memset: the per-block partial sums go to a
 // file-scope device array (one copy per GPU, kept for the process's
 // lifetime) and k_diag_finish adds them up in the order launch_diag uses on
 // the host, so the statistics are bit for bit those of etd_diag.
@@ -2058,87 +2183,20 @@ static int scratch_address(int dev, double **out) {
   return ETD_OK;
 }
 
-// One thread per result, each a sequential sum in launch_diag's host order:
-// threads 0..V-1 max / min / vol of field t, thread V the AC energy, threads
-// V+1 .. V+nch the CH energies, thread V+nch+1 the couple energy.
-__global__ __launch_bounds__(256) void k_diag_finish(const double *__restrict__ part, int ms, int me, int V, int nac,
-                                                     int nch, double h3, double *__restrict__ stats) {
-#pragma clang fp contract(off)  // the host sums without fused multiply-adds
-  const int t = threadIdx.x;
-  const double *hs = part, *he = part + (size_t)3 * ms * V;
-  const long mc = (long)me * ECHUNK;
-  if (t < V) {
-    double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
-    for (int k = 0; k < ms; k++) {
-      const double *o = hs + ((size_t)t * ms + k) * 3;
-      if (o[0] > mx) mx = o[0];
-      if (o[1] < mn) mn = o[1];
-      vol += o[2];
-    }
-    stats[4 * t + 0] = mx;
-    stats[4 * t + 1] = mn;
-    stats[4 * t + 2] = vol * h3;  // the energy column [4 t + 3] belongs to the energy threads below
-  } else if (t == V) {
-    if (nac > 0) {
-      double e = 0.0;
-      for (long k = 0; k < mc; k++) e += he[k];
-      e = e * h3;
-      for (int v = 0; v < nac; v++) stats[4 * v + 3] = e;
-    }
-  } else if (t <= V + nch) {
-    const int q = t - V - 1;
-    double e = 0.0;
-    for (long k = 0; k < mc; k++) e += he[(size_t)(1 + q) * mc + k];
-    stats[4 * (nac + q) + 3] = e * h3;
-  } else if (t == V + nch + 1) {
-    double couple = 0.0;
-    if (nac > 0 && nch > 0) {
-      double last = 0.0;  // the energy of the last CH field, as launch_diag carries it over
-      for (long k = 0; k < mc; k++) last += he[(size_t)nch * mc + k];
-      couple = last * h3;
-      for (long k = 0; k < mc; k++) couple += he[(size_t)(1 + nch) * mc + k];
-      couple = couple * h3;
-    }
-    stats[4 * V] = couple;
-  }
-}
-
-template <typename T>
-static int diag_noalloc_t(etd_ctx *c, double *stats_dev, double *sc) {
-  const T *U = fields<T>(c).u;
-  diag_args a;
-  a.n = c->n;
-  a.g = c->g;
-  a.nac = c->nac;
-  a.nch = c->nch;
-  a.hx = c->p.hx;
-  a.hy = c->p.hy;
-  a.hz = c->p.hz;
-  a.epn2 = c->p.epn2;
-  a.a2 = c->p.a2;
-  a.a3 = c->p.a3;
-  a.a4 = c->p.a4;
-  a.ihx = 1.0 / c->p.hx;
-  a.ihy = 1.0 / c->p.hy;
-  a.ihz = 1.0 / c->p.hz;
-  const int n = c->n, g = c->g, V = c->V;
-  const int ms = n - 2 * g, me = n - 2 * g - 1;
-  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
-  k_diag_stats<T><<<dim3(ms, V), 256, 0, c->stream>>>(U, sc, a);
-  k_diag_energy<T><<<(unsigned)(me * kinds * ECHUNK), 256, 0, c->stream>>>(U, c->lam, sc + (size_t)3 * ms * V, a);
-  k_diag_finish<<<1, 256, 0, c->stream>>>(sc, ms, me, V, c->nac, c->nch, c->p.hx * c->p.hy * c->p.hz, stats_dev);
-  return hipGetLastError() == hipSuccess ? ETD_OK : ETD_ERR_DEVICE;
+// One thread per result index of diag_reduce_one (the host's reduction, so
+// the statistics are bit for bit those of etd_diag).
+__global__ __launch_bounds__(256) void k_diag_finish(const double *__restrict__ part, int nb, int V, int nac, int nch,
+                                                     double h3, double *__restrict__ stats) {
+  if ((int)threadIdx.x <= V + nch + 1) diag_reduce_one(part, nb, V, nac, nch, h3, (int)threadIdx.x, stats);
 }
 
 extern "C" int etd_diag_noalloc(etd_ctx *c, double *stats_dev) {
   if (!c || !stats_dev) return ETD_ERR_ARG;
-  const int ms = c->n - 2 * c->g, me = ms - 1;
-  const int kinds = 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0);
-  const size_t need = (size_t)3 * ms * c->V + (size_t)me * kinds * ECHUNK;
+  const size_t need = diag_partials(c);
   if (need > DIAG_SCRATCH_DOUBLES || c->V + c->nch + 2 > 256) {
     snprintf(c->err, sizeof(c->err),
-             "etd_diag_noalloc: this context needs %zu doubles of partial sums (3 N V + (N - 1)(1 + nch + "
-             "couple) 8); the static buffer holds %d",
+             "etd_diag_noalloc: this context needs %zu doubles of partial sums ((3 V + 1 + nch + couple) (N - 1) 8); "
+             "the static buffer holds %d",
              need, DIAG_SCRATCH_DOUBLES);
     return ETD_ERR_UNSUPPORTED;
   }
@@ -2148,9 +2206,18 @@ extern "C" int etd_diag_noalloc(etd_ctx *c, double *stats_dev) {
     snprintf(c->err, sizeof(c->err), "etd_diag_noalloc: no address for the static scratch on device %d", c->device);
     return rc;
   }
-  rc = c->mixed ? diag_noalloc_t<float>(c, stats_dev, sc) : diag_noalloc_t<double>(c, stats_dev, sc);
-  if (rc) snprintf(c->err, sizeof(c->err), "etd_diag_noalloc: %s", hipGetErrorString(hipGetLastError()));
-  return rc;
+  if (c->mixed)
+    launch_diag_kernel<float>(c, sc);
+  else
+    launch_diag_kernel<double>(c, sc);
+  k_diag_finish<<<1, 256, 0, c->stream>>>(sc, diag_blocks(c), c->V, c->nac, c->nch, c->p.hx * c->p.hy * c->p.hz,
+                                          stats_dev);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(c->err, sizeof(c->err), "etd_diag_noalloc: %s", hipGetErrorString(e));
+    return ETD_ERR_DEVICE;
+  }
+  return ETD_OK;
 }
 
 // ------------------------------------------------------------------ counter calibration

@@ -162,8 +162,8 @@ int etd_diag(etd_ctx *c, double *stats);
  * kernels queued on the context's stream: no host synchronisation, no
  * allocation, no memset.  The partial sums live in a static 512 KiB device
  * array (one per GPU, held until the process ends), which must hold
- * 3 N (nac + nch) + (N - 1)(1 + nch + [nac > 0 and nch > 0]) 8 doubles
- * (65536; e.g. 12,072 for 192^3 x (12 + 1) fields); larger contexts get
+ * (3 (nac + nch) + 1 + nch + [nac > 0 and nch > 0]) (N - 1) 8 doubles
+ * (65536; e.g. 64,176 for 192^3 x (12 + 1) fields); larger contexts get
  * ETD_ERR_UNSUPPORTED.  Calls on one GPU share that array: calls that
  * overlap in time on one GPU (other streams, other threads) race.  The
  * result is ready when the context's stream has reached it. */

@@ -199,6 +199,32 @@ def test_diag_matches_oracle(h):
     assert abs(cp_o - cp_g) < 1e-12 * abs(cp_o)
 
 
+@pytest.mark.parametrize("n", [68, 100, 196, 260])
+@pytest.mark.parametrize("precision", [0, 2])
+def test_diag_matches_oracle_wide(n, precision):
+    """The energy kernel at the sizes where a row spans 2 to 5 waves and a
+    chunk of planes spans one to two register sub-ranges (rows of n - 5
+    cells, chunks of (n - 5) / 8 planes); AC, CH and couple sums."""
+    say(f"diag n={n} precision={precision}")
+    prm = dict(small_params(n=n, nac=3, nch=1, landau=OMEGA_LANDAU, aniso=1, h=(0.9, 1.0, 1.2)),
+               precision=precision)
+    f = noise_fields(prm, amp=0.5)
+    if precision:  # FP32 state: inputs exactly representable in FP32, sums in FP64
+        f = [x.astype(np.float32).astype(np.float64) for x in f]
+    o = O.Oracle(dict(prm, precision=0))
+    Vm, Vim, D = O.eigbasis(n, 3)
+    g = pyetd.Etd(pyetd.params_from_dict(prm), Vm, Vim, D)
+    for v, x in enumerate(f):
+        o.set_field(v, x)
+        g.set_field(v, x)
+    st_o, cp_o = o.diag()
+    st_g, cp_g = g.diag()
+    g.close()
+    assert np.array_equal(st_o[:, :2], st_g[:, :2])
+    assert np.abs(st_o[:, 2:] - st_g[:, 2:]).max() < 1e-12 * np.abs(st_o[:, 2:]).max()
+    assert abs(cp_o - cp_g) < 1e-12 * abs(cp_o)
+
+
 def test_init_noise_matches_splitmix_reference():
     say("init noise")
     prm = small_params(n=10, nac=1)

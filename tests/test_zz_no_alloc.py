@@ -82,7 +82,7 @@ def test_diag_noalloc_equals_diag_on_two_calls_the_second_smaller():
 
 
 def test_diag_noalloc_refuses_contexts_beyond_its_buffer():
-    # 3 N V + (N - 1)(2 + nch) 8 = 3 96 64 + 95 65 8 = 67832 > 65536 doubles
+    # (3 V + 1 + nch + 1)(N - 1) 8 = (192 + 65) 95 8 = 195320 > 65536 doubles
     prm = dict(small_params(n=100, nac=1, nch=63), precision=2)
     g = pyetd.Etd.from_dict(prm)
     L = pyetd.lib()
@@ -90,5 +90,5 @@ def test_diag_noalloc_refuses_contexts_beyond_its_buffer():
     L.etd_diag_noalloc.argtypes = [C.c_void_p, C.c_void_p]
     rc = L.etd_diag_noalloc(g.ctx, C.c_void_p(16))  # refused before any use of the pointer
     assert rc == pyetd.ERR_UNSUPPORTED
-    assert "67832 doubles" in L.etd_last_error(g.ctx).decode()
+    assert "195320 doubles" in L.etd_last_error(g.ctx).decode()
     g.close()
