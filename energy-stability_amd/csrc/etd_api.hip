@@ -486,8 +486,8 @@ void stage_forward(etd_ctx *c) {
   // these fields (same ghost fill, same kernels): reuse it
   if (c->p.elastic == 1 && c->nac > 0 && !c->elastic_fresh) launch_elastic<T>(c);
   c->elastic_fresh = 0;
-  launch_nonlinear<T>(c, F.N1, nullptr);
-  if constexpr (sizeof(T) == 4) ch_center(c, F.u, F.N1, -1);
+  launch_nonlinear<T>(c, F.N1, nullptr, 1);  // mixed: CH terms centred, N1 - N(c_mean)
+  if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
   modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
   // u stays centred: every caller overwrites it with the backward transform;
   // the ETD2 stage needs N1 itself (N(a) - N1)

@@ -156,7 +156,7 @@ void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first
 template <typename T>
 void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2);
 template <typename T>
-void launch_nonlinear(etd_ctx *c, T *out, const T *sub);
+void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre = 0);
 template <typename T>
 void launch_elastic(etd_ctx *c);
 template <typename T>

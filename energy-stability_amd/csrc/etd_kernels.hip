@@ -740,6 +740,7 @@ struct nl_args {
   long long Lbase;  // linear index of AC field 0 of the latest elastic evaluation
   int slot0;        // Lbase mod ring
   double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
+  double ihx2, ihy2, ihz2;  // 1 / h^2, formed once on the host
 };
 
 template <typename T>
@@ -910,70 +911,133 @@ __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *
 
 __device__ __forceinline__ double ch_f(double cc) { return cc * (cc * cc - 1.0); }
 
-// ch_calc_FU_ConstantMobility (calc.c:70-473), pass 1:
-// t = -LCI KCI c + LCI CH_f(c), minus LCI epn2 c_out / h^2 on the faces.
+// ch_calc_FU_ConstantMobility (calc.c:70-473) in one pass per CH field:
+//   t   = -LCI KCI c + LCI CH_f(c), minus LCI epn2 c_out / h^2 on the faces
+//   out = Lap_T t (out-of-box neighbours 0) - mu_out / h^2 on the faces
+// The reference stores t and then takes its Laplacian; here each thread forms
+// t at its 7 stencil points from c in FP64 (a few dozen FP64 operations a
+// point, cheap next to a pass of t through HBM), so t is never rounded to
+// the storage type.  1 / h^2 is applied as a product (one rounding away from
+// the reference's two divisions).
+struct ch_h {
+  double x, y, z;  // 1 / h^2
+};
 template <typename T>
-__global__ void k_ch_t(const T *__restrict__ cfield, T *__restrict__ t, nl_args a) {
+__device__ __forceinline__ double ch_t_at(const T *c, const nl_args &a, const ch_h &ih, int k, int j, int i) {
   const int n = a.n, g = a.g;
-  const size_t n3 = (size_t)n * n * n;
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= n3) return;
-  int i, j, k;
-  idx3(l, n, i, j, k);
-  const double cc = cfield[l];
+  const double cc = c[IDX(n, k, j, i)];
   double f = -a.LCI * a.KCI * cc;
   f += a.LCI * ch_f(cc);
   const double ce = a.LCI * a.epn2;
-  if (i == 0) f -= ce * Pimg(cfield, n, g, k, j, -1) / a.hx / a.hx;
-  if (i == n - 1) f -= ce * Pimg(cfield, n, g, k, j, n) / a.hx / a.hx;
-  if (j == 0) f -= ce * Pimg(cfield, n, g, k, -1, i) / a.hy / a.hy;
-  if (j == n - 1) f -= ce * Pimg(cfield, n, g, k, n, i) / a.hy / a.hy;
-  if (k == 0) f -= ce * Pimg(cfield, n, g, -1, j, i) / a.hz / a.hz;
-  if (k == n - 1) f -= ce * Pimg(cfield, n, g, n, j, i) / a.hz / a.hz;
-  t[l] = (T)f;
+  if (i == 0) f -= ce * Pimg(c, n, g, k, j, -1) * ih.x;
+  if (i == n - 1) f -= ce * Pimg(c, n, g, k, j, n) * ih.x;
+  if (j == 0) f -= ce * Pimg(c, n, g, k, -1, i) * ih.y;
+  if (j == n - 1) f -= ce * Pimg(c, n, g, k, n, i) * ih.y;
+  if (k == 0) f -= ce * Pimg(c, n, g, -1, j, i) * ih.z;
+  if (k == n - 1) f -= ce * Pimg(c, n, g, n, j, i) * ih.z;
+  return f;
 }
 
 // chemical potential LCI (epn2 Lap c - CH_f(c)) at an outside cell, from the
 // periodic image (ch_mu, trans.c:1433-1937)
 template <typename T>
-__device__ double ch_mu_out(const T *c, const nl_args &a, int z, int y, int x) {
+__device__ double ch_mu_out(const T *c, const nl_args &a, const ch_h &ih, int z, int y, int x) {
   const int n = a.n, g = a.g;
   const double v0 = Pimg(c, n, g, z, y, x);
-  double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) / a.hx / a.hx;
-  mu += (Pimg(c, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(c, n, g, z, y + 1, x)) / a.hy / a.hy;
-  mu += (Pimg(c, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(c, n, g, z + 1, y, x)) / a.hz / a.hz;
+  double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) * ih.x;
+  mu += (Pimg(c, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(c, n, g, z, y + 1, x)) * ih.y;
+  mu += (Pimg(c, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(c, n, g, z + 1, y, x)) * ih.z;
   mu = a.LCI * a.epn2 * mu;
   mu -= a.LCI * ch_f(v0);
   return mu;
 }
 
-// pass 2: out = Lap_T t (out-of-box neighbours 0) - mu_out / h^2 on faces
+// out = N_c - sub (ETD2 difference, when sub != NULL) - nc (the mixed path's
+// N(c_mean), when nc != NULL: the centred nonlinear term).
+//
+// Points whose whole stencil is at least one cell from the box faces (the
+// inner (n - 4)^3) need t only pointwise; the others (the shell, two cells
+// thick) need the face terms, the outside values and mu_out.  The grid keeps
+// them apart so every wave runs one of the two paths: blocks [0, nbi8) take
+// the inner points (x fastest, blocks spread over the XCDs in contiguous
+// ranges so the +-n^2 neighbours share an L2), the rest the shell points in
+// the order of shell_point.
+__device__ __forceinline__ void shell_point(long s, int n, int &i, int &j, int &k) {
+  const long n2 = (long)n * n, ring = 8L * n - 16;  // points of a middle plane's shell
+  if (s < 4 * n2) {                                   // planes 0, 1, n - 2, n - 1
+    const int p = (int)(s / n2);
+    k = p < 2 ? p : n - 4 + p;
+    const int r = (int)(s - p * n2);
+    j = r / n;
+    i = r - j * n;
+    return;
+  }
+  s -= 4 * n2;
+  const int p = (int)(s / ring);
+  k = 2 + p;
+  int r = (int)(s - p * ring);
+  if (r < 4 * n) {  // rows 0, 1, n - 2, n - 1
+    const int q = r / n;
+    j = q < 2 ? q : n - 4 + q;
+    i = r - q * n;
+    return;
+  }
+  r -= 4 * n;
+  j = 2 + r / 4;
+  const int e = r & 3;
+  i = e < 2 ? e : n - 4 + e;
+}
+__host__ __device__ __forceinline__ long shell_count(int n) {
+  return 4L * n * n + (long)(n - 4) * (8L * n - 16);
+}
+
 template <typename T>
-__global__ void k_ch_lap(const T *__restrict__ cfield, const T *__restrict__ t, T *__restrict__ out,
-                         const T *__restrict__ sub, nl_args a) {
+__global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__restrict__ out, const T *__restrict__ sub,
+                                               const float *__restrict__ nc, nl_args a, unsigned nbi8) {
   const int n = a.n;
-  const size_t n3 = (size_t)n * n * n;
-  const size_t l = xcd_block() * blockDim.x + threadIdx.x;  // neighbours +-n, +-n^2 on this XCD
-  if (l >= n3) return;
+  const ch_h ih = {a.ihx2, a.ihy2, a.ihz2};
+  const unsigned b = blockIdx.x;
+  if (b < nbi8) {
+    const int NI = n - 4;
+    const size_t bb = (size_t)(b & 7u) * (nbi8 >> 3) + (b >> 3);
+    const size_t li = bb * blockDim.x + threadIdx.x;
+    if (li >= (size_t)NI * NI * NI) return;
+    int i, j, k;
+    idx3(li, NI, i, j, k);
+    const size_t dz = (size_t)n * n, l = IDX(n, k + 2, j + 2, i + 2);
+    auto tp = [&](double cc) { return -a.LCI * a.KCI * cc + a.LCI * ch_f(cc); };
+    const double tc = tp(c[l]);
+    double r = (tp(c[l - 1]) + tp(c[l + 1]) - 2.0 * tc) * ih.x;
+    r += (tp(c[l - n]) + tp(c[l + n]) - 2.0 * tc) * ih.y;
+    r += (tp(c[l - dz]) + tp(c[l + dz]) - 2.0 * tc) * ih.z;
+    if (sub) r -= sub[l];
+    if (nc) r -= nc[l];
+    out[l] = (T)r;
+    return;
+  }
+  const long s = (long)(b - nbi8) * blockDim.x + threadIdx.x;
+  if (s >= shell_count(n)) return;
   int i, j, k;
-  idx3(l, n, i, j, k);
-  const double tc = t[l];
-  const double f_left = i > 0 ? t[l - 1] : 0.0;
-  const double f_right = i < n - 1 ? t[l + 1] : 0.0;
-  const double f_top = j > 0 ? t[l - n] : 0.0;
-  const double f_bottom = j < n - 1 ? t[l + n] : 0.0;
-  const double f_front = k > 0 ? t[l - (size_t)n * n] : 0.0;
-  const double f_back = k < n - 1 ? t[l + (size_t)n * n] : 0.0;
-  double r = (f_left + f_right - 2.0 * tc) / a.hx / a.hx;
-  r += (f_top + f_bottom - 2.0 * tc) / a.hy / a.hy;
-  r += (f_front + f_back - 2.0 * tc) / a.hz / a.hz;
-  if (i == 0) r -= ch_mu_out(cfield, a, k, j, -1) / a.hx / a.hx;
-  if (i == n - 1) r -= ch_mu_out(cfield, a, k, j, n) / a.hx / a.hx;
-  if (j == 0) r -= ch_mu_out(cfield, a, k, -1, i) / a.hy / a.hy;
-  if (j == n - 1) r -= ch_mu_out(cfield, a, k, n, i) / a.hy / a.hy;
-  if (k == 0) r -= ch_mu_out(cfield, a, -1, j, i) / a.hz / a.hz;
-  if (k == n - 1) r -= ch_mu_out(cfield, a, n, j, i) / a.hz / a.hz;
+  shell_point(s, n, i, j, k);
+  const size_t l = IDX(n, k, j, i);
+  const double tc = ch_t_at(c, a, ih, k, j, i);
+  const double f_left = i > 0 ? ch_t_at(c, a, ih, k, j, i - 1) : 0.0;
+  const double f_right = i < n - 1 ? ch_t_at(c, a, ih, k, j, i + 1) : 0.0;
+  const double f_top = j > 0 ? ch_t_at(c, a, ih, k, j - 1, i) : 0.0;
+  const double f_bottom = j < n - 1 ? ch_t_at(c, a, ih, k, j + 1, i) : 0.0;
+  const double f_front = k > 0 ? ch_t_at(c, a, ih, k - 1, j, i) : 0.0;
+  const double f_back = k < n - 1 ? ch_t_at(c, a, ih, k + 1, j, i) : 0.0;
+  double r = (f_left + f_right - 2.0 * tc) * ih.x;
+  r += (f_top + f_bottom - 2.0 * tc) * ih.y;
+  r += (f_front + f_back - 2.0 * tc) * ih.z;
+  if (i == 0) r -= ch_mu_out(c, a, ih, k, j, -1) * ih.x;
+  if (i == n - 1) r -= ch_mu_out(c, a, ih, k, j, n) * ih.x;
+  if (j == 0) r -= ch_mu_out(c, a, ih, k, -1, i) * ih.y;
+  if (j == n - 1) r -= ch_mu_out(c, a, ih, k, n, i) * ih.y;
+  if (k == 0) r -= ch_mu_out(c, a, ih, -1, j, i) * ih.z;
+  if (k == n - 1) r -= ch_mu_out(c, a, ih, n, j, i) * ih.z;
   if (sub) r -= sub[l];
+  if (nc) r -= nc[l];
   out[l] = (T)r;
 }
 
@@ -1011,13 +1075,25 @@ static nl_args make_nl(etd_ctx *c) {
   a.LCI = c->p.LCI;
   a.KCI = c->p.KCI;
   a.es = c->p.elastic_scale;
+  a.ihx2 = 1.0 / (a.hx * a.hx);
+  a.ihy2 = 1.0 / (a.hy * a.hy);
+  a.ihz2 = 1.0 / (a.hz * a.hz);
   return a;
 }
 
-// N for every variable into out (V x n3); when sub != NULL the result is
-// N - sub (the ETD2 difference prepare_U1_new, calc.c:537-551).
 template <typename T>
-void launch_nonlinear(etd_ctx *c, T *out, const T *sub) {
+static void launch_ch_nl(etd_ctx *c, const T *cf, T *out, const T *sub, const float *nc, const nl_args &a) {
+  const long NI = c->n - 4;
+  const unsigned nbi8 = round8((size_t)((NI * NI * NI + 255) / 256));
+  const unsigned nbs = (unsigned)((shell_count(c->n) + 255) / 256);
+  k_ch_nl<T><<<nbi8 + nbs, 256, 0, c->stream>>>(cf, out, sub, nc, a, nbi8);
+}
+
+// N for every variable into out (V x n3); when sub != NULL the result is
+// N - sub (the ETD2 difference prepare_U1_new, calc.c:537-551).  ch_centre
+// (mixed path): the CH outputs are N - N(c_mean), the centred term.
+template <typename T>
+void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre) {
   nl_args a = make_nl(c);
   const etd_fields<T> F = fields<T>(c);
   const size_t n3 = c->n3;
@@ -1050,9 +1126,8 @@ void launch_nonlinear(etd_ctx *c, T *out, const T *sub) {
   }
   for (int q = 0; q < c->nch; q++) {
     const int v = c->nac + q;
-    k_ch_t<T><<<nb, 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, a);
-    k_ch_lap<T><<<round8(nb), 256, 0, c->stream>>>(F.u + n3 * v, F.gtmp, out + n3 * v, sub ? sub + n3 * v : nullptr,
-                                                   a);
+    const float *nc = ch_centre && sizeof(T) == 4 && c->ch_const ? c->ch_const + n3 * q : nullptr;
+    launch_ch_nl<T>(c, F.u + n3 * v, out + n3 * v, sub ? sub + n3 * v : nullptr, nc, a);
   }
   etd_prof_end(c, PC_NONLIN);
 }
@@ -2370,11 +2445,10 @@ void ch_mean_mark(etd_ctx *c, const float *u) {
   const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
   nl_args a = make_nl(c);
   const unsigned nb = (unsigned)((c->n3 + 255) / 256);
-  float *uni = c->f32.w1, *t = c->f32.gtmp;
+  float *uni = c->f32.w1;
   for (int q = 0; q < c->nch; q++) {
     k_ch_fill<<<nb, 256, 0, c->stream>>>(uni, c->n3, c->ch_sum, q, invN3);
-    k_ch_t<float><<<nb, 256, 0, c->stream>>>(uni, t, a);
-    k_ch_lap<float><<<round8(nb), 256, 0, c->stream>>>(uni, t, c->ch_const + c->n3 * q, nullptr, a);
+    launch_ch_nl<float>(c, uni, c->ch_const + c->n3 * q, nullptr, nullptr, a);
   }
   c->ch_stale = 0;
 }
@@ -2408,7 +2482,7 @@ void ch_mean_restore(etd_ctx *c, float *u) {
   template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
   template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \
   template void launch_phiscale_pair<T>(etd_ctx *, const T *, const T *, T *, double, T *, double); \
-  template void launch_nonlinear<T>(etd_ctx *, T *, const T *);                      \
+  template void launch_nonlinear<T>(etd_ctx *, T *, const T *, int);                 \
   template void launch_elastic<T>(etd_ctx *);                                        \
   template void fft3d<T>(etd_ctx *, T *, int, int);                                  \
   template void launch_felas_materialize<T>(etd_ctx *, T *);                         \
