@@ -227,7 +227,8 @@ def main():
                  "traffic": measured_traffic("k_fft16_f32_192cube_hbm_counters.md"), "launch_ms": fft_launch_ms,
                  "ms_per_step": ms[7] / K}
         r_mp = {"kernel": ("k_modeprod_split_p (split-BF16 x3 MFMA, FP32 accumulate)" if args.precision == 1 else
-                           "k_modeprod_f16_r3<13,1> (split-FP16 x2 MFMA, three-slot field ring, FP32 accumulate)"),
+                           "k_modeprod_f16_r3<13,2> (split-FP16 x2 MFMA, contraction over the slowest axis, "
+                           "FP32 accumulate)"),
                 "bound": "hbm", "achieved": mp_bytes / (mp_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": mp_bytes / (mp_launch_ms / 1e3) / PEAK_HBM,
                 "traffic": measured_traffic("k_modeprod_split_192cube_hbm_counters.md" if args.precision == 1 else
