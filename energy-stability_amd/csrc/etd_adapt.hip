@@ -692,6 +692,13 @@ extern "C" int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_s
     snprintf(c->err, sizeof(c->err), "learned controller requested without etd_set_controller");
     return ETD_ERR_ARG;
   }
+  if (c->p.free_energy != 0 || c->mask || c->p.variable_mobility) {
+    // the acceptance test needs the free energy the step descends; it is the Landau / double-well one
+    snprintf(c->err, sizeof(c->err),
+             "adaptive stepping accepts on the Landau / double-well energy: not with the CALPHAD driving force, "
+             "a grain mask or variable mobility");
+    return ETD_ERR_UNSUPPORTED;
+  }
   return c->mixed ? run_adaptive<float>(c, o, st, log, log_cap, log_rows)
                   : run_adaptive<double>(c, o, st, log, log_cap, log_rows);
 }

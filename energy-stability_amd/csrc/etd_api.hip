@@ -87,7 +87,17 @@ static int validate(etd_ctx *c, const etd_params *p) {
   if (p->px != 1 || p->py != 1 || p->pz != 1)
     return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz = %d,%d,%d: one subdomain per GPU (use 1,1,1)", p->px, p->py, p->pz);
   if (!p->periodic) return seterr(c, ETD_ERR_UNSUPPORTED, "periodic = 0 is not supported");
-  if (p->variable_mobility) return seterr(c, ETD_ERR_UNSUPPORTED, "VariableMobility = 1 is not supported");
+  if (p->variable_mobility != 0 && p->variable_mobility != 1)
+    return seterr(c, ETD_ERR_ARG, "VariableMobility = %d is not 0 or 1", p->variable_mobility);
+  if (p->variable_mobility && (p->nch != 3 || p->nac < 1))
+    return seterr(c, ETD_ERR_UNSUPPORTED,
+                  "VariableMobility = 1 needs nch = 3 (Cu, Mn, Ni, mobility.c:860-922) and an AC field 0 "
+                  "(nac = %d, nch = %d)",
+                  p->nac, p->nch);
+  if (p->anisotropic < 0 || p->anisotropic > 2)
+    return seterr(c, ETD_ERR_ARG, "ANISOTROPIC = %d is not 0, 1 or 2", p->anisotropic);
+  if (p->free_energy != 0 && p->free_energy != 1)
+    return seterr(c, ETD_ERR_ARG, "free_energy = %d is not 0 (Landau) or 1 (CALPHAD)", p->free_energy);
   if (p->elastic != 0 && p->elastic != 1) return seterr(c, ETD_ERR_UNSUPPORTED, "ELASTIC = %d is not supported", p->elastic);
   if (p->nac < 0 || p->nch < 0 || p->nac + p->nch < 1) return seterr(c, ETD_ERR_ARG, "nac = 0 and nch = 0");
   if (p->nac + p->nch > ETD_MAX_VAR) return seterr(c, ETD_ERR_UNSUPPORTED, "too many fields");
@@ -287,7 +297,13 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   // energy partials
   c->npart = (int)etd_diag_partials(c);
   HCHK(c, hipMalloc(&c->part, sizeof(double) * c->npart));
-  if (c->mixed && c->nch > 0) {
+  if (p->free_energy == 1) etd_calphad(&c->gbcc, &c->ghcp);
+  if (p->variable_mobility) {
+    etd_vm_constants(c->vm_X, c->vm_DA, c->vm_DG);
+    HCHK(c, hipMalloc(&c->vm_part, sizeof(double) * (3 * 256 + 3)));
+  }
+  // the mixed path's CH centring relies on the constant-mobility term's conservation
+  if (c->mixed && c->nch > 0 && !p->variable_mobility) {
     HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
     HCHK(c, hipMalloc(&c->ch_const, sizeof(float) * c->nch * c->n3));
   }
@@ -328,6 +344,8 @@ extern "C" void etd_close(etd_ctx *c) {
   (void)hipFree(c->tw);
   (void)hipFree(c->dconst);
   (void)hipFree(c->part);
+  (void)hipFree(c->mask);
+  (void)hipFree(c->vm_part);
   if (c->hpart) (void)hipHostFree(c->hpart);
   (void)hipFree(c->f32.u);
   (void)hipFree(c->f32.N2);
@@ -428,6 +446,27 @@ extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
   HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
+  return ETD_OK;
+}
+
+extern "C" int etd_set_mask(etd_ctx *c, const int *mask) {
+  if (!c) return ETD_ERR_ARG;
+  if (!mask) {
+    if (c->mask) HCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->mask);
+    c->mask = nullptr;
+    return ETD_OK;
+  }
+  unsigned char *h = (unsigned char *)malloc(c->n3);
+  if (!h) return ETD_ERR_NOMEM;
+  for (size_t l = 0; l < c->n3; l++) h[l] = mask[l] != 0;
+  hipError_t e = hipSuccess;
+  if (!c->mask) e = hipMalloc(&c->mask, c->n3);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->mask, h, c->n3, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  free(h);
+  if (e != hipSuccess) return seterr(c, ETD_ERR_DEVICE, "set_mask: %s", hipGetErrorString(e));
+  c->elastic_fresh = 0;
   return ETD_OK;
 }
 

@@ -105,6 +105,12 @@ struct etd_ctx {
   void *ad_S0, *ad_SB, *ad_spec;
   double *ad_part, *ad_hpart;
   struct etd_mlp *mlp;
+  /* beta->alpha grain mask (n^3 bytes, 1 where the grain is, or NULL) */
+  unsigned char *mask;
+  /* variable-mobility CH: per-block maxima of the three mobilities, then C_q */
+  double *vm_part;
+  double vm_X[3], vm_DA[3], vm_DG[3];  /* x (1 - x), D_A, D_G of Cu, Mn, Ni */
+  double gbcc, ghcp;       /* CALPHAD Gibbs energies (free_energy 1) */
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
@@ -128,6 +134,9 @@ extern "C" {
 void etd_host_noise(int n, int g, unsigned long long seed, double mean, double amp, double *u);
 int etd_csv_rows(const char *work_dir, int nac, int nch, int iter, const double *stats);
 int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u);
+/* CALPHAD Gibbs energies (free_energy 1); variable-mobility x(1-x), D_A, D_G of Cu, Mn, Ni */
+void etd_calphad(double *gbcc, double *ghcp);
+void etd_vm_constants(double X[3], double DA[3], double DG[3]);
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,66 @@ void etd_default_params(etd_params *p) {
   p->precision = ETD_PREC_FP64;
 }
 
+/* CALPHAD Gibbs energies of bcc and hcp Ti-Al-V at the beta->alpha driver's
+ * fixed compositions (anisotropic_hip.c:2029-2048): lattice stabilities and
+ * Redlich-Kister parameters of init_vars (vars.c:146-190) at T = 1093 K,
+ * R = 8.31451 (ScLETD.h:24-26). */
+void etd_calphad(double *gbcc, double *ghcp) {
+  const double T = 1093.0, R = 8.31451, lT = log(T);
+  const double ser_al = -11277.683 + 188.661987 * T - 31.748192 * T * lT - 1.234e28 * pow(T, -9) + 137529.83;
+  const double ser_ti = 908.837 + 67.048538 * T - 14.9466 * T * lT - 0.0081465 * T * T + 2.02715e-7 * T * T * T -
+                        1477660.0 / T + 58166.677;
+  const double ser_v = -7967.84 + 143.291 * T - 25.9 * T * lT + 6.25e-5 * T * T - 6.8e-7 * T * T * T + 77277.217;
+  const double g_bcc[3] = {/* Al, V, Ti */ 10083.0 - 4.812 * T + ser_al, ser_v,
+                           5758.6 + 38.38987 * T - 7.4305 * T * lT + 0.0093636 * T * T - 1.04805e-6 * T * T * T -
+                               525093.0 / T + ser_ti};
+  const double g_hcp[3] = {5481.0 - 1.799 * T + ser_al, 4000.0 + 2.4 * T + ser_v, ser_ti};
+  /* x = (Al, V, Ti); binary terms x_a x_b sum_k L_k (x_a - x_b)^k for (Al,V), (Ti,V), (Al,Ti) */
+  const double xb[3] = {0.102, 0.036, 1.0 - 0.102 - 0.036}, xh[3] = {0.1057, 0.0223, 1.0 - 0.1057 - 0.0223};
+  const double Lb[3][3] = {{-98000.0 + 32.0 * T, 3500.0 - 5.0 * T, 0.0},
+                           {10500.0 - 1.5 * T, 2000.0, 1000.0},
+                           {-128500.0 + 39.0 * T, 6000.0, 21200.0}};
+  const double Lh[3][3] = {{-98000.0 + 32.0 * T, 3500.0 - 5.0 * T, 0.0},
+                           {30250.0 - 10.0 * T, 0.0, 0.0},
+                           {-133750.0 + 39.0 * T, 250.0, 17250.0}};
+  const int pa[3] = {0, 2, 0}, pb[3] = {1, 1, 2};
+  for (int ph = 0; ph < 2; ph++) {
+    const double *x = ph ? xh : xb, *G = ph ? g_hcp : g_bcc;
+    const double(*L)[3] = ph ? Lh : Lb;
+    double s = x[2] * G[2] + x[0] * G[0] + x[1] * G[1];
+    s += R * T * (x[2] * log(x[2]) + x[0] * log(x[0]) + x[1] * log(x[1]));
+    for (int q = 0; q < 3; q++) {
+      const double d = x[pa[q]] - x[pb[q]];
+      s += x[pa[q]] * x[pb[q]] * (L[q][0] + L[q][1] * d + L[q][2] * d * d);
+    }
+    /* ternary: bcc x_Al x_V x_Ti (-8000); hcp x_Al x_V x_Ti (x_Al L0 + x_Ti L1 + x_V L2) */
+    if (ph == 0)
+      s += x[0] * x[1] * x[2] * -8000.0;
+    else
+      s += x[0] * x[1] * x[2] * (x[0] * -60000.0 + x[2] * -30000.0 + x[1] * 60000.0);
+    if (ph)
+      *ghcp = s;
+    else
+      *gbcc = s;
+  }
+}
+
+/* Mobility constants of update_M_C (mobility.c:864-906) for Cu, Mn, Ni:
+ * x (1 - x), and D_A, D_G of D = D0 exp(-Q / RT) relative to D_Cu in the A
+ * phase, T = 823 K. */
+void etd_vm_constants(double X[3], double DA[3], double DG[3]) {
+  const double RT = 8.314472 * 823.0;
+  const double x0[3] = {0.15, 0.01, 0.01};
+  const double dcu = 4.7e-5 * exp(-2.44e5 / RT);
+  DA[0] = 1.0;
+  DG[0] = (4.3e-5 * exp(-2.80e5 / RT)) / dcu;
+  DA[1] = (1.49e-4 * exp(-2.63e5 / RT)) / dcu;
+  DG[1] = (2.78e-5 * exp(-2.64e5 / RT)) / dcu;
+  DA[2] = (1.4e-4 * exp(-2.56e5 / RT)) / dcu;
+  DG[2] = (1.08e-5 * exp(-2.73e5 / RT)) / dcu;
+  for (int q = 0; q < 3; q++) X[q] = x0[q] * (1.0 - x0[q]);
+}
+
 static void set_lambda_iso(etd_params *p) {
   for (int v = 0; v < ETD_MAX_VAR; v++) {
     double *L = p->lam + 9 * v;

@@ -34,6 +34,20 @@ __device__ __forceinline__ size_t xcd_block() {
 }
 __host__ __forceinline__ unsigned round8(size_t blocks) { return (unsigned)((blocks + 7) & ~(size_t)7); }
 
+// wave-wide (64 lanes) reductions; lane 0 holds the result
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
+  return v;
+}
+
 // l = (k n + j) n + i, for l < 2^32: each quotient from a double-precision
 // reciprocal (within one of the true quotient: the relative error is about
 // 2^-52) and one correction step, instead of 64-bit integer division.
@@ -741,6 +755,9 @@ struct nl_args {
   int slot0;        // Lbase mod ring
   double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
   double ihx2, ihy2, ihz2;  // 1 / h^2, formed once on the host
+  int fe;                   // free energy: 0 Landau polynomial, 1 CALPHAD (alpha)
+  double gbcc, ghcp;        // CALPHAD Gibbs energies (fe == 1)
+  const unsigned char *mask;  // grain mask or NULL
 };
 
 template <typename T>
@@ -799,10 +816,14 @@ __device__ __forceinline__ int felas_slot(const nl_args &a, int p, int lag) {
 // re-reads it and ignores it); FB >= nac puts every field's loads of a pass in
 // flight at once, which the FP32 fields need to keep HBM busy (a wave's load
 // moves half the bytes of an FP64 one).  Sums in field order for any FB.
+#define CAL_WMEGA 100.0    // vars.c:144
+#define CAL_GNORMAL 50000.0  // ScLETD.h:27
+#define CAL_ALPHA 0.02       // ScLETD.h:29
+// mask_here: the grain-mask term is applied here (no gradient pass follows)
 template <typename T, int FB>
 __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *__restrict__ out,
                                                    const T *__restrict__ sub, const T *__restrict__ E, nl_args a,
-                                                   int with_felas) {
+                                                   int with_felas, int mask_here) {
   const int n = a.n, g = a.g;
   const unsigned n3 = (unsigned)n * n * n;  // n <= 272: every offset below fits 32 bits
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -844,13 +865,22 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
       if (p >= a.nac) break;
       const T *up_f = u + n3 * (unsigned)p;
       const double up = upv[e];
-      const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
       double f = a.LE * a.KE * up;
-      f -= a.LE * fp;
+      if (a.fe == 1) {
+        // CALPHAD driving force (anisotropic_hip.c:2075-2140); sum_{m != p} u_m^2 = s - u_p^2
+        const double dp = 30.0 * up * up * (1 - up) * (1 - up);
+        const double dq = 4.0 * CAL_WMEGA * up * (1 - up) * (2 - up);
+        const double dfeta = -(a.gbcc * dp) + (a.ghcp * dp) + dq;
+        f -= a.LE * (dfeta / CAL_GNORMAL + 4.0 * CAL_ALPHA * (up * (s - up * up)));
+      } else {
+        const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
+        f -= a.LE * fp;
+      }
       if (ox >= 0) f += ce * (double)up_f[ox] / a.hx / a.hx;
       if (oy >= 0) f += ce * (double)up_f[oy] / a.hy / a.hy;
       if (oz >= 0) f += ce * (double)up_f[oz] / a.hz / a.hz;
       if (with_felas) f -= a.LE * a.es * (double)fel[e];
+      if (mask_here && a.mask && a.mask[lo] == 0) f -= 0.5 * up;
       if (sub) f -= sbv[e];
       out[n3 * (unsigned)p + lo] = (T)f;
     }
@@ -885,7 +915,7 @@ __global__ void k_f2_grad(const T *__restrict__ u, T *__restrict__ G, const doub
 // reference's order f1 - epn2 LE F2 - LE ES felas)
 template <typename T>
 __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *__restrict__ E, nl_args a, int p,
-                         int with_felas) {
+                         int with_felas, const T *__restrict__ u) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = xcd_block() * blockDim.x + threadIdx.x;  // neighbours +-n, +-n^2 on this XCD
@@ -904,8 +934,62 @@ __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *
   tmp += (a_top - a_bottom) / 2.0 / a.hy;
   tmp += (a_front - a_back) / 2.0 / a.hz;
   double f = out[l];
+  if (a.fe == 1 && with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);  // the alpha driver's order
   f -= a.epn2 * a.LE * tmp;
-  if (with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+  if (a.fe != 1 && with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+  if (a.mask && a.mask[l] == 0) f -= 0.5 * (double)u[l];
+  out[l] = (T)f;
+}
+
+// The beta->alpha driver's gradient term (anisotropic_calc_dev_2,
+// anisotropic_hip.c:528-647): f2 = sum Lambda_ij d_i d_j u with three-point
+// second differences and four-corner cross differences on every extended
+// cell, neighbours outside the box read from the periodic image (the
+// enlarged slabs); then, in that driver's order (ac_calc_F2_dev, 2190-2211
+// for fe == 1; omega's order otherwise), the elastic term, the grain mask and
+// the ETD2 difference.
+template <typename T>
+__global__ __launch_bounds__(256) void k_f2_full(const T *__restrict__ u, T *__restrict__ out,
+                                                 const T *__restrict__ sub, const T *__restrict__ E,
+                                                 const double *__restrict__ lam, nl_args a, int p, int with_felas) {
+  const int n = a.n, g = a.g;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  int i, j, k;
+  idx3(l, n, i, j, k);
+  const bool inner = i > 0 && i < n - 1 && j > 0 && j < n - 1 && k > 0 && k < n - 1;
+  // cells inside the box read the field (its edge and corner ghosts are the ghost fill's, not pure
+  // images); cells outside it the periodic image, as the enlarged slabs hold
+  auto U = [&](int dz, int dy, int dx) -> double {
+    if (inner) return u[l + ((long)dz * n + dy) * n + dx];
+    const int z = k + dz, y = j + dy, x = i + dx;
+    if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
+    return Pimg(u, n, g, z, y, x);
+  };
+  const double m = U(0, 0, 0);
+  const double fxx = (U(0, 0, 1) - 2 * m + U(0, 0, -1)) * a.ihx2;
+  const double fyy = (U(0, 1, 0) - 2 * m + U(0, -1, 0)) * a.ihy2;
+  const double fzz = (U(1, 0, 0) - 2 * m + U(-1, 0, 0)) * a.ihz2;
+  const double fxy = ((U(0, 1, 1) - U(0, 1, -1)) - (U(0, -1, 1) - U(0, -1, -1))) / (4.0 * a.hx * a.hy);
+  const double fxz = ((U(1, 0, 1) - U(1, 0, -1)) - (U(-1, 0, 1) - U(-1, 0, -1))) / (4.0 * a.hx * a.hz);
+  const double fyz = ((U(1, 1, 0) - U(1, -1, 0)) - (U(-1, 1, 0) - U(-1, -1, 0))) / (4.0 * a.hy * a.hz);
+  double t = 0.0;
+  t += lam[0] * fxx;
+  t += lam[1] * fxy;
+  t += lam[2] * fxz;
+  t += lam[3] * fxy;
+  t += lam[4] * fyy;
+  t += lam[5] * fyz;
+  t += lam[6] * fxz;
+  t += lam[7] * fyz;
+  t += lam[8] * fzz;
+  double f = out[l];
+  if (a.fe == 1 && with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+  f -= a.epn2 * a.LE * t;
+  if (a.fe != 1 && with_felas) f -= a.LE * a.es * felas_at(E, a, p, k, j, i);
+  if (a.mask && a.mask[l] == 0) f -= 0.5 * (double)u[l];
+  if (sub) f -= (double)sub[l];
   out[l] = (T)f;
 }
 
@@ -1078,7 +1162,156 @@ static nl_args make_nl(etd_ctx *c) {
   a.ihx2 = 1.0 / (a.hx * a.hx);
   a.ihy2 = 1.0 / (a.hy * a.hy);
   a.ihz2 = 1.0 / (a.hz * a.hz);
+  a.fe = c->p.free_energy;
+  a.gbcc = c->gbcc;
+  a.ghcp = c->ghcp;
+  a.mask = c->mask;
   return a;
+}
+
+// ------------------------------------------------------------------ variable-mobility CH
+// ch_calc_FU_VariableMobility (mobility.c:838-858) with update_M_C
+// (860-922), one periodic box, CH fields q = 0, 1, 2 (Cu, Mn, Ni):
+//   M_q = x_q (1 - x_q) (D_qA (1 - eta_0) + D_qG eta_0), C_q = max_box M_q
+//   f0  = -LCI KCI c + LCI CH_f(c)
+//   ft1 = -f0 + KCI c,            + epn2 c_out / h^2 on the faces
+//   ft2 = epn2 Lap_T c - f0,      + epn2 c_out / h^2 on the faces
+//   N   = LCI (-C Lap_T ft1 - C mu_out / h^2 (faces) - div((M - C) grad ft2))
+// Lap_T and the divergence read 0 outside the box.  Three passes: the box
+// maxima of M (order-free, so exact), ft1 and ft2 into scratch, and N; M at
+// the stencil points is recomputed from eta_0 instead of being stored.
+#define VM_BLOCKS 256
+struct vm_args {
+  double X[3], DA[3], DG[3];
+};
+__device__ __forceinline__ double vm_M(const vm_args &m, int q, double eta) {
+  return m.X[q] * (m.DA[q] * (1.0 - eta) + m.DG[q] * eta);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_vm_max(const T *__restrict__ eta, size_t n3, vm_args m,
+                                                double *__restrict__ part) {
+  double mx[3] = {-1.0e30, -1.0e30, -1.0e30};
+  for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < n3; l += (size_t)gridDim.x * blockDim.x) {
+    const double e = eta[l];
+#pragma unroll
+    for (int q = 0; q < 3; q++) mx[q] = fmax(mx[q], vm_M(m, q, e));
+  }
+  __shared__ double sh[3][4];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double v = wave_max(mx[q]);
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    part[q * VM_BLOCKS + blockIdx.x] = fmax(fmax(sh[q][0], sh[q][1]), fmax(sh[q][2], sh[q][3]));
+  }
+}
+// C_q = the maximum of the block maxima, into part[3 VM_BLOCKS + q]
+__global__ __launch_bounds__(256) void k_vm_cmax(double *__restrict__ part) {
+  double v = -1.0e30;
+  for (int b = threadIdx.x; b < VM_BLOCKS; b += blockDim.x) v = fmax(v, part[blockIdx.x * VM_BLOCKS + b]);
+  v = wave_max(v);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[3 * VM_BLOCKS + blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+template <typename T>
+__device__ __forceinline__ double lap_zero_at(const T *a, int n, size_t l, int k, int j, int i, const nl_args &q) {
+  const size_t dz = (size_t)n * n;
+  const double ac = a[l];
+  const double al = i > 0 ? (double)a[l - 1] : 0.0, ar = i < n - 1 ? (double)a[l + 1] : 0.0;
+  const double at = j > 0 ? (double)a[l - n] : 0.0, ab = j < n - 1 ? (double)a[l + n] : 0.0;
+  const double af = k > 0 ? (double)a[l - dz] : 0.0, ak = k < n - 1 ? (double)a[l + dz] : 0.0;
+  double t = 0.0;
+  t += (al + ar - 2.0 * ac) * q.ihx2;
+  t += (at + ab - 2.0 * ac) * q.ihy2;
+  t += (af + ak - 2.0 * ac) * q.ihz2;
+  return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_vm_ft(const T *__restrict__ c, T *__restrict__ ft1, T *__restrict__ ft2,
+                                               nl_args a) {
+  const int n = a.n, g = a.g;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  int i, j, k;
+  idx3(l, n, i, j, k);
+  const double cc = c[l];
+  const double f0 = -a.LCI * a.KCI * cc + a.LCI * ch_f(cc);
+  double t1 = -f0 + a.KCI * cc;
+  double t2 = a.epn2 * lap_zero_at(c, n, l, k, j, i, a) - f0;
+  double fc = 0.0;  // epn2 c_out / h^2 on the faces, left, right, top, bottom, front, back
+  if (i == 0) fc += a.epn2 * Pimg(c, n, g, k, j, -1) * a.ihx2;
+  if (i == n - 1) fc += a.epn2 * Pimg(c, n, g, k, j, n) * a.ihx2;
+  if (j == 0) fc += a.epn2 * Pimg(c, n, g, k, -1, i) * a.ihy2;
+  if (j == n - 1) fc += a.epn2 * Pimg(c, n, g, k, n, i) * a.ihy2;
+  if (k == 0) fc += a.epn2 * Pimg(c, n, g, -1, j, i) * a.ihz2;
+  if (k == n - 1) fc += a.epn2 * Pimg(c, n, g, n, j, i) * a.ihz2;
+  ft1[l] = (T)(t1 + fc);
+  ft2[l] = (T)(t2 + fc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_vm_out(const T *__restrict__ c, const T *__restrict__ eta,
+                                                const T *__restrict__ ft1, const T *__restrict__ ft2,
+                                                T *__restrict__ out, const T *__restrict__ sub,
+                                                const double *__restrict__ cmax, vm_args m, int q, nl_args a) {
+  const int n = a.n;
+  const size_t n3 = (size_t)n * n * n;
+  const size_t l = xcd_block() * blockDim.x + threadIdx.x;
+  if (l >= n3) return;
+  int i, j, k;
+  idx3(l, n, i, j, k);
+  const double C = cmax[q];
+  const ch_h ih = {a.ihx2, a.ihy2, a.ihz2};
+  double f1 = -C * lap_zero_at(ft1, n, l, k, j, i, a);
+  if (i == 0) f1 -= C * ch_mu_out(c, a, ih, k, j, -1) * ih.x;
+  if (i == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, j, n) * ih.x;
+  if (j == 0) f1 -= C * ch_mu_out(c, a, ih, k, -1, i) * ih.y;
+  if (j == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, n, i) * ih.y;
+  if (k == 0) f1 -= C * ch_mu_out(c, a, ih, -1, j, i) * ih.z;
+  if (k == n - 1) f1 -= C * ch_mu_out(c, a, ih, n, j, i) * ih.z;
+  // div((M - C) grad ft2), mobility.c:435-722: (m_nb + m_mid)(a_nb - a_mid) / 2h^2, 0 outside the box
+  const size_t dz = (size_t)n * n;
+  const double am = ft2[l], mm = vm_M(m, q, (double)eta[l]) - C;
+  auto term = [&](bool in, size_t o, double ih2) {
+    const double an = in ? (double)ft2[o] : 0.0, mn = in ? vm_M(m, q, (double)eta[o]) - C : 0.0;
+    return (mn + mm) * (an - am) * 0.5 * ih2;
+  };
+  double d = 0.0;
+  d += term(i > 0, l - 1, a.ihx2);
+  d += term(i < n - 1, l + 1, a.ihx2);
+  d += term(j > 0, l - n, a.ihy2);
+  d += term(j < n - 1, l + n, a.ihy2);
+  d += term(k > 0, l - dz, a.ihz2);
+  d += term(k < n - 1, l + dz, a.ihz2);
+  double r = a.LCI * (f1 + -d);
+  if (sub) r -= (double)sub[l];
+  out[l] = (T)r;
+}
+
+template <typename T>
+static void launch_vm(etd_ctx *c, const T *u, T *out, const T *sub, T *scratch, const nl_args &a) {
+  vm_args m;
+  for (int q = 0; q < 3; q++) m.X[q] = c->vm_X[q], m.DA[q] = c->vm_DA[q], m.DG[q] = c->vm_DG[q];
+  const size_t n3 = c->n3;
+  const unsigned nb = round8((n3 + 255) / 256);
+  k_vm_max<T><<<VM_BLOCKS, 256, 0, c->stream>>>(u, n3, m, c->vm_part);  // eta_0 = AC field 0
+  k_vm_cmax<<<3, 256, 0, c->stream>>>(c->vm_part);
+  for (int q = 0; q < 3; q++) {
+    const T *cf = u + n3 * (c->nac + q);
+    k_vm_ft<T><<<nb, 256, 0, c->stream>>>(cf, scratch, scratch + n3, a);
+    k_vm_out<T><<<nb, 256, 0, c->stream>>>(cf, u, scratch, scratch + n3, out + n3 * (c->nac + q),
+                                           sub ? sub + n3 * (c->nac + q) : nullptr, c->vm_part + 3 * VM_BLOCKS, m,
+                                           q, a);
+  }
 }
 
 template <typename T>
@@ -1109,26 +1342,34 @@ void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre) {
     // batches of 4 keep 8 waves per SIMD
     int fb = c->nl_batch;
     if (!fb) fb = sizeof(T) == 8 ? 4 : (c->nac <= 4 ? 4 : c->nac <= 8 ? 8 : c->nac <= 12 ? 12 : 16);
+    const int mh = aniso ? 0 : 1;  // the grain mask goes with the last pass over the field
     if (fb == 4)
-      k_nonlin_ac<T, 4><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+      k_nonlin_ac<T, 4><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl, mh);
     else if (fb == 8)
-      k_nonlin_ac<T, 8><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+      k_nonlin_ac<T, 8><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl, mh);
     else if (fb == 12)
-      k_nonlin_ac<T, 12><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
+      k_nonlin_ac<T, 12><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl, mh);
     else
-      k_nonlin_ac<T, 16><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl);
-    if (aniso)
+      k_nonlin_ac<T, 16><<<nb, 256, 0, c->stream>>>(F.u, out, sb, F.E, a, fl, mh);
+    if (aniso == 1)
       for (int p = 0; p < c->nac; p++) {
         k_f2_grad<T><<<round8(nb), 256, 0, c->stream>>>(F.u + n3 * p, F.gtmp, c->lam + 9 * p, a);
-        k_f2_div<T><<<round8(nb), 256, 0, c->stream>>>(F.gtmp, out + n3 * p, F.E, a, p, felas);
+        k_f2_div<T><<<round8(nb), 256, 0, c->stream>>>(F.gtmp, out + n3 * p, F.E, a, p, felas, F.u + n3 * p);
         if (sub) launch_sub(c, out + n3 * p, sub + n3 * p, n3);  // after the F2 / felas terms
       }
+    if (aniso == 2)
+      for (int p = 0; p < c->nac; p++)
+        k_f2_full<T><<<round8(nb), 256, 0, c->stream>>>(F.u + n3 * p, out + n3 * p, sub ? sub + n3 * p : nullptr,
+                                                        F.E, c->lam + 9 * p, a, p, felas);
   }
-  for (int q = 0; q < c->nch; q++) {
-    const int v = c->nac + q;
-    const float *nc = ch_centre && sizeof(T) == 4 && c->ch_const ? c->ch_const + n3 * q : nullptr;
-    launch_ch_nl<T>(c, F.u + n3 * v, out + n3 * v, sub ? sub + n3 * v : nullptr, nc, a);
-  }
+  if (c->p.variable_mobility)
+    launch_vm<T>(c, F.u, out, sub, F.gtmp, a);
+  else
+    for (int q = 0; q < c->nch; q++) {
+      const int v = c->nac + q;
+      const float *nc = ch_centre && sizeof(T) == 4 && c->ch_const ? c->ch_const + n3 * q : nullptr;
+      launch_ch_nl<T>(c, F.u + n3 * v, out + n3 * v, sub ? sub + n3 * v : nullptr, nc, a);
+    }
   etd_prof_end(c, PC_NONLIN);
 }
 
@@ -1881,18 +2122,6 @@ struct diag_args {
   double ihx, ihy, ihz;  // 1 / h
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
-  return v;
-}
 
 // One block per (row j, chunk of planes), threads along i, 63 cells per wave
 // (lane 63 only supplies the i + 1 corners of lane 62).  The work of a block

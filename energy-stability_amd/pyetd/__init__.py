@@ -59,6 +59,7 @@ class EtdParams(C.Structure):
         ("lam", C.c_double * (MAX_VAR * 9)),
         ("precision", C.c_int),
         ("checkpoint", C.c_int), ("nchk", C.c_int), ("chk", C.c_int), ("nout", C.c_int), ("approx", C.c_int),
+        ("free_energy", C.c_int),
     ]
 
 
@@ -126,6 +127,7 @@ def lib():
             "etd_error_string": (C.c_char_p, [C.c_int]),
             "etd_get_params": (C.c_int, [ctxp, P]),
             "etd_set_field": (C.c_int, [ctxp, C.c_int, _dp]),
+            "etd_set_mask": (C.c_int, [ctxp, C.POINTER(C.c_int)]),
             "etd_get_field": (C.c_int, [ctxp, C.c_int, _dp]),
             "etd_init_noise": (C.c_int, [ctxp, C.c_int, C.c_ulonglong, C.c_double, C.c_double]),
             "etd_step": (C.c_int, [ctxp, C.c_int]),
@@ -382,6 +384,9 @@ def params_from_dict(prm):
         for i, x in enumerate(lam):
             p.lam[i] = x
     p.precision = prm.get("precision", 0)
+    p.free_energy = prm.get("free_energy", 0)
+    p.variable_mobility = prm.get("varmob", 0)
+    p.periodic = prm.get("periodic", 1)
     return p
 
 
@@ -440,6 +445,16 @@ class Etd:
         self.close()
 
     # fields
+    def set_mask(self, mask):
+        """Grain mask (n^3 ints, 0 outside the grain) or None."""
+        L = lib()
+        if mask is None:
+            _check(L.etd_set_mask(self.ctx, None), self.ctx, "set_mask")
+            return
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32).reshape(-1))
+        assert m.size == self.n ** 3
+        _check(L.etd_set_mask(self.ctx, m.ctypes.data_as(C.POINTER(C.c_int))), self.ctx, "set_mask")
+
     def set_field(self, var, data):
         a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
         if a.size != self.n ** 3:

@@ -64,9 +64,12 @@ typedef struct etd_params {
   double xmin, ymin, zmin, xmax, ymax, zmax;
   int etd2;
   int elastic;             /* 0 none, 1 AC (2 CH / 3 both: unsupported) */
-  int anisotropic;
+  int anisotropic;         /* 0 none; 1 F2 = div(Lambda grad u), omega (anisotropic.c:164-627);
+                              2 the compact Lambda_ij d_i d_j stencil of the beta->alpha driver
+                              (anisotropic_hip.c:528-647) */
   double kkx, kky, kkz;
-  int variable_mobility;   /* must be 0 */
+  int variable_mobility;   /* 0; 1 the variable-mobility CH of mobility.c:838-922 (nch == 3:
+                              Cu, Mn, Ni mobilities from AC field 0) */
   /* derived / hard-coded in the reference */
   double hx, hy, hz;       /* (max-min)/gnx, vars.c:153-155 */
   double LE, KE;           /* formula.c:9-18: 1, 42 */
@@ -83,6 +86,13 @@ typedef struct etd_params {
    * buffers), uint16 field dumps every nout steps; approx (the reference's
    * truncated-elasticity mode count) is read and not used */
   int checkpoint, nchk, chk, nout, approx;
+  /* AC driving force: 0 the Landau polynomial (omega, formula.c:30-38); 1 the
+   * CALPHAD Ti-Al-V driving force of the beta->alpha driver
+   * (anisotropic_hip.c:2002-2188, parameters vars.c:144-299): bcc / hcp Gibbs
+   * energies at its fixed compositions, f = LE KE u - LE (dG/gnormal + 4 ALPHA
+   * u sum_{m != p} u_m^2), with that driver's order of the elastic, gradient
+   * and grain-mask terms (ac_calc_F2_dev, 2190-2211) */
+  int free_energy;
 } etd_params;
 
 typedef struct etd_ctx etd_ctx;
@@ -139,6 +149,11 @@ int etd_get_params(const etd_ctx *c, etd_params *p);
 /* Host <-> device copies of one field (n^3 doubles, ghosts included). */
 int etd_set_field(etd_ctx *c, int var, const double *host);
 int etd_get_field(etd_ctx *c, int var, double *host);
+
+/* The grain mask of the beta->alpha driver (read_ori, field.c:111-129;
+ * ac_calc_F2_dev, anisotropic_hip.c:2206-2210): n^3 ints, x fastest; every AC
+ * nonlinear term gets -0.5 u where mask == 0.  NULL removes it. */
+int etd_set_mask(etd_ctx *c, const int *mask);
 
 /* Seeded initial state: interior U[mean-amp, mean+amp) from splitmix64
  * ((x >> 11) * 2^-53), ghosts as periodic images.  Replaces init_field

@@ -56,6 +56,7 @@ struct orc_ctx {
   double *fbuf;          /* persistent elastic transfer buffer (elastic.c:228) */
   double *phi1, *phi2;   /* (nac? 1:0 + nch? 1:0) tables, zxy layout */
   double *t0, *t1, *t2, *t3;  /* n3 scratch */
+  int *mask;             /* copy of p.mask (or NULL) */
 };
 
 int orc_threads(void) {
@@ -268,17 +269,25 @@ static void ac_f2(const orc_ctx *c, int pv, const double *u, double *gx_, double
       }
 }
 
+static double calphad_f(const orc_ctx *c, size_t l, int pv, double gbcc, double ghcp);
+static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2);
 static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
   const orc_params *p = &c->p;
   const int n = p->n, g = p->g, nac = p->nac;
   const double LE = p->LE, KE = p->KE, epn2 = p->epn2;
   const double hx = p->hx, hy = p->hy, hz = p->hz;
   const double *u = c->field + c->n3 * pv;
+  double gbcc = 0.0, ghcp = 0.0;
+  if (p->free_energy == 1) orc_calphad(&gbcc, &ghcp);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
         size_t l = IDX(n, k, j, i);
+        if (p->free_energy == 1) {
+          out[l] = calphad_f(c, l, pv, gbcc, ghcp);
+          continue;
+        }
         double s = 0.0;
         for (int m = 0; m < nac; m++) {
           double um = c->field[c->n3 * m + l];
@@ -303,15 +312,19 @@ static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
     for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] += LE * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] += LE * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
-  if (p->aniso) {
-    ac_f2(c, pv, u, c->t0, c->t1, c->t2, c->t3);
-  }
+  if (p->aniso == 1) ac_f2(c, pv, u, c->t0, c->t1, c->t2, c->t3);
+  if (p->aniso == 2) ac_f2_full(c, pv, u, c->t3);
   const double *felas = p->elastic == 1 ? c->felas + c->n3 * pv : NULL;
+  /* omega's order: F2 then felas (calc.c:20-50); alpha's (ac_calc_F2_dev,
+   * anisotropic_hip.c:2190-2211): felas, F2, then the grain mask */
+  const int alpha_order = p->free_energy == 1;
 #pragma omp parallel for schedule(static)
   for (size_t l = 0; l < c->n3; l++) {
     double f = out[l];
+    if (alpha_order && felas) f -= LE * p->elastic_scale * felas[l];
     if (p->aniso) f -= epn2 * LE * c->t3[l];
-    if (felas) f -= LE * p->elastic_scale * felas[l];
+    if (!alpha_order && felas) f -= LE * p->elastic_scale * felas[l];
+    if (p->mask && p->mask[l] == 0) f -= 0.5 * u[l];
     out[l] = f;
   }
 }
@@ -400,8 +413,278 @@ static void ch_nonlinear(orc_ctx *c, int v, double *out) {
     for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] -= ch_mu_out(c, u, n, j, i) / hz / hz;
 }
 
+/* ------------------------------------------------------------------ */
+/* beta->alpha physics (source_beta_to_alpha/src).                      */
+
+/* CALPHAD Gibbs energies of the bcc and hcp phases (J/mol) at the fixed
+ * compositions of ac_calc_F1_dev (anisotropic_hip.c:2029-2048): x_Al, x_V =
+ * 0.102, 0.036 (bcc) and 0.1057, 0.0223 (hcp), x_Ti the rest; lattice
+ * stabilities and Redlich-Kister parameters of init_vars (vars.c:146-190) at
+ * T_a = 1093 K, R_a = 8.31451 (ScLETD.h:24-26).  Pinned by the values the
+ * reference notes beside its parameters (GHSERAL 8.9664e4, GHSERTI 7.2379e3,
+ * GHSERV 2.7046e4, GBCCAL 9.4487e4, GBCCTI 7.4701e3, GHCPAL 9.3179e4,
+ * GHCPV 3.3669e4, BL13_0 -85873, BL12_0 -63024, BL12_1 -1965, BL32_0 8860.5,
+ * HL13_0 -91123, HL32_0 19320) and by the ideal-mixing term recomputed from
+ * the compositions. */
+#define CAL_T 1093.0
+#define CAL_R 8.31451
+/* t: GHSERAL, GHSERTI, GHSERV, GBCCAL, GBCCTI, GHCPAL, GHCPV (vars.c:146-160),
+ * then BL13_0, BL12_0, BL12_1, BL32_0, HL13_0, HL32_0 at T_a */
+static void calphad_terms(double *t) {
+  const double T = CAL_T, lT = log(T);
+  t[0] = -11277.683 + 188.661987 * T - 31.748192 * T * lT - 1.234e28 * pow(T, -9) + 137529.83;
+  t[1] = 908.837 + 67.048538 * T - 14.9466 * T * lT - 0.0081465 * T * T + 2.02715e-7 * T * T * T - 1477660.0 / T +
+         58166.677;
+  t[2] = -7967.84 + 143.291 * T - 25.9 * T * lT + 6.25e-5 * T * T - 6.8e-7 * T * T * T + 77277.217;
+  t[3] = 10083.0 - 4.812 * T + t[0];
+  t[4] = 5758.6 + 38.38987 * T - 7.4305 * T * lT + 0.0093636 * T * T - 1.04805e-6 * T * T * T - 525093.0 / T + t[1];
+  t[5] = 5481.0 - 1.799 * T + t[0];
+  t[6] = 4000.0 + 2.4 * T + t[2];
+  t[7] = -128500.0 + 39.0 * T;
+  t[8] = -98000.0 + 32.0 * T;
+  t[9] = 3500.0 - 5.0 * T;
+  t[10] = 10500.0 - 1.5 * T;
+  t[11] = -133750.0 + 39.0 * T;
+  t[12] = 30250.0 - 10.0 * T;
+}
+void orc_calphad_terms(double *t) { calphad_terms(t); }
+
+void orc_calphad(double *gbcc, double *ghcp) {
+  double t[13];
+  calphad_terms(t);
+  const double T = CAL_T;
+  const double ser_ti = t[1], ser_v = t[2], bcc_al = t[3], bcc_ti = t[4], hcp_al = t[5], hcp_v = t[6];
+  /* interaction parameters: 1 = Al, 2 = V, 3 = Ti */
+  const double b13_0 = -128500.0 + 39.0 * T, b13_1 = 6000.0, b13_2 = 21200.0, b132_0 = -8000.0;
+  const double b12_0 = -98000.0 + 32.0 * T, b12_1 = 3500.0 - 5.0 * T;
+  const double b32_0 = 10500.0 - 1.5 * T, b32_1 = 2000.0, b32_2 = 1000.0;
+  const double h13_0 = -133750.0 + 39.0 * T, h13_1 = 250.0, h13_2 = 17250.0;
+  const double h132_0 = -60000.0, h132_1 = -30000.0, h132_2 = 60000.0;
+  const double h12_0 = -98000.0 + 32.0 * T, h12_1 = 3500.0 - 5.0 * T, h32_0 = 30250.0 - 10.0 * T;
+  {
+    const double al = 0.102, v = 0.036, ti = 1.0 - al - v;
+    *gbcc = ti * bcc_ti + al * bcc_al + v * ser_v + CAL_R * T * (ti * log(ti) + al * log(al) + v * log(v)) +
+            al * v * (b12_0 + b12_1 * (al - v)) + ti * v * (b32_0 + b32_1 * (ti - v) + b32_2 * (ti - v) * (ti - v)) +
+            al * ti * (b13_0 + b13_1 * (al - ti) + b13_2 * (al - ti) * (al - ti)) + al * v * ti * b132_0;
+  }
+  {
+    const double al = 0.1057, v = 0.0223, ti = 1.0 - al - v;
+    *ghcp = ti * ser_ti + al * hcp_al + v * hcp_v + CAL_R * T * (ti * log(ti) + al * log(al) + v * log(v)) +
+            al * v * (h12_0 + h12_1 * (al - v)) + ti * v * h32_0 +
+            al * ti * (h13_0 + h13_1 * (al - ti) + h13_2 * (al - ti) * (al - ti)) +
+            al * v * ti * (al * h132_0 + ti * h132_1 + v * h132_2);
+  }
+}
+
+/* alpha's driving force (ac_calc_F1_dev, anisotropic_hip.c:2075-2140) for
+ * variant p, generalised from 6 variants to nac:
+ *   dp   = 30 u^2 (1 - u)^2                 (derivative of u^3 (10 - 15u + 6u^2))
+ *   dq   = 4 wmega u (1 - u)(2 - u),        wmega = 100 (vars.c:144)
+ *   f    = LE KE u - LE (( -(Gbcc dp) + (Ghcp dp) + dq) / gnormal
+ *                        + 4 ALPHA sum_{m != p} u u_m u_m)
+ * gnormal = 50000, ALPHA = 0.02 (ScLETD.h:27-29); the sum in field order.
+ * Pinned: dp and dq as derivatives of their polynomials (finite differences
+ * of independently written energies), the coupling as the derivative of
+ * ALPHA sum_{m<m'} u_m^2 u_m'^2 * 4/2 ... (see tests/test_oracle.py). */
+#define CAL_WMEGA 100.0
+#define CAL_GNORMAL 50000.0
+#define CAL_ALPHA 0.02
+static double calphad_f(const orc_ctx *c, size_t l, int pv, double gbcc, double ghcp) {
+  const orc_params *p = &c->p;
+  const double up = c->field[c->n3 * pv + l];
+  double q = 0.0;
+  for (int m = 0; m < p->nac; m++) {
+    if (m == pv) continue;
+    const double um = c->field[c->n3 * m + l];
+    q += up * um * um;
+  }
+  const double dp = 30.0 * up * up * (1 - up) * (1 - up);
+  const double dq = 4.0 * CAL_WMEGA * up * (1 - up) * (2 - up);
+  const double dfeta = -(gbcc * dp) + (ghcp * dp) + dq;
+  double f = p->LE * p->KE * up;
+  f -= p->LE * (dfeta / CAL_GNORMAL + 4.0 * CAL_ALPHA * q);
+  return f;
+}
+
+/* alpha's full gradient term (anisotropic_calc_dev_2, anisotropic_hip.c:
+ * 528-647): f2 = sum_ij Lambda_ij d_i d_j u with three-point second
+ * differences and four-corner cross differences,
+ *   fxx = (u[i+1] - 2u + u[i-1]) / hx^2
+ *   fxy = ((u[i+1,j+1] - u[i-1,j+1]) - (u[i+1,j-1] - u[i-1,j-1])) / (2hx 2hy)
+ * on every extended cell; the neighbours outside the box come from the
+ * enlarged slabs, i.e. the periodic image (one box, periodic).  Pinned: exact
+ * on quadratic fields away from the box faces (any Lambda, cross terms
+ * included), and the periodic image against a shifted field. */
+static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int x);
+static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
+  const int n = c->p.n, g = c->p.g;
+  const double hx = c->p.hx, hy = c->p.hy, hz = c->p.hz;
+  const double *L = c->lam + 9 * pv;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+#define U(dz, dy, dx) pimg_or_in(u, n, g, k + (dz), j + (dy), i + (dx))
+        const double m = U(0, 0, 0);
+        const double fxx = (U(0, 0, 1) - 2 * m + U(0, 0, -1)) / hx / hx;
+        const double fyy = (U(0, 1, 0) - 2 * m + U(0, -1, 0)) / hy / hy;
+        const double fzz = (U(1, 0, 0) - 2 * m + U(-1, 0, 0)) / hz / hz;
+        const double fxy = ((U(0, 1, 1) - U(0, 1, -1)) - (U(0, -1, 1) - U(0, -1, -1))) / 2.0 / hx / 2.0 / hy;
+        const double fxz = ((U(1, 0, 1) - U(1, 0, -1)) - (U(-1, 0, 1) - U(-1, 0, -1))) / 2.0 / hx / 2.0 / hz;
+        const double fyz = ((U(1, 1, 0) - U(1, -1, 0)) - (U(-1, 1, 0) - U(-1, -1, 0))) / 2.0 / hy / 2.0 / hz;
+#undef U
+        double t = 0.0;
+        t += L[0] * fxx;
+        t += L[1] * fxy;
+        t += L[2] * fxz;
+        t += L[3] * fxy;
+        t += L[4] * fyy;
+        t += L[5] * fyz;
+        t += L[6] * fxz;
+        t += L[7] * fyz;
+        t += L[8] * fzz;
+        f2[IDX(n, k, j, i)] = t;
+      }
+}
+/* a cell inside the box reads the field, one outside its periodic image */
+static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int x) {
+  if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
+  return Pimg(u, n, g, z, y, x);
+}
+
+/* ------------------------------------------------------------------ */
+/* Variable-mobility CH: ch_calc_FU_VariableMobility (mobility.c:838-858)
+ * with update_M_C (860-922) and the pieces of 12-836, one periodic box:
+ *   M_q   = x_q (1 - x_q)(D_qA (1 - eta_0) + D_qG eta_0) on every cell,
+ *           (x, D) of Cu, Mn, Ni for q = 0, 1, 2 (mobility.c:864-906),
+ *           C_q = max over the box of M_q
+ *   f0    = -LCI KCI c + LCI CH_f(c)           (SWITCH_CH_FIELD1)
+ *   ft    = -f0 + KCI c, + epn2 c_out / h^2 on the faces
+ *   f1    = -C Lap_T ft - C mu_out / h^2 on the faces
+ *   ft'   = epn2 Lap_T c - f0, + epn2 c_out / h^2 on the faces
+ *   f2    = -div((M - C) grad ft')             (ch_divergence_M_gradient)
+ *   N     = LCI (f1 + f2)
+ * Lap_T and div read 0 outside the box (mobility.c:12-244, 435-722, the
+ * periodic branch).  mu_out as in the constant-mobility term.  Pinned: at
+ * eta = 0 (M = C) N is C times the constant-mobility term for LCI = 1, and
+ * the divergence is checked on its own against a closed form. */
+static void vm_mobility(const orc_ctx *c, int q, double *M, double *Cmax) {
+  const double gconst = 8.314472, tempr = 823.0, RT = gconst * tempr;
+  const double x0[3] = {0.15, 0.01, 0.01};
+  const double DCuA0 = 4.7e-5 * exp(-2.44e5 / RT);
+  double DA, DG;
+  if (q == 0) {
+    DA = 1.0;
+    DG = (4.3e-5 * exp(-2.80e5 / RT)) / DCuA0;
+  } else if (q == 1) {
+    DA = (1.49e-4 * exp(-2.63e5 / RT)) / DCuA0;
+    DG = (2.78e-5 * exp(-2.64e5 / RT)) / DCuA0;
+  } else {
+    DA = (1.4e-4 * exp(-2.56e5 / RT)) / DCuA0;
+    DG = (1.08e-5 * exp(-2.73e5 / RT)) / DCuA0;
+  }
+  const double *eta = c->field;
+  double cm = -1.0e30;
+  for (size_t l = 0; l < c->n3; l++) {
+    M[l] = x0[q] * (1.0 - x0[q]) * (DA * (1.0 - eta[l]) + DG * eta[l]);
+    cm = cm > M[l] ? cm : M[l];
+  }
+  *Cmax = cm;
+}
+
+static void lap_zero(int n, double hx, double hy, double hz, const double *a, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        const double ac = a[IDX(n, k, j, i)];
+        const double al = i > 0 ? a[IDX(n, k, j, i - 1)] : 0.0, ar = i < n - 1 ? a[IDX(n, k, j, i + 1)] : 0.0;
+        const double at = j > 0 ? a[IDX(n, k, j - 1, i)] : 0.0, ab = j < n - 1 ? a[IDX(n, k, j + 1, i)] : 0.0;
+        const double af = k > 0 ? a[IDX(n, k - 1, j, i)] : 0.0, ak = k < n - 1 ? a[IDX(n, k + 1, j, i)] : 0.0;
+        out[IDX(n, k, j, i)] = (al + ar - 2 * ac) / hx / hx + (at + ab - 2 * ac) / hy / hy + (af + ak - 2 * ac) / hz / hz;
+      }
+}
+
+void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        const size_t l = IDX(n, k, j, i);
+        const int hl = i > 0, hr = i < n - 1, ht = j > 0, hb = j < n - 1, hf = k > 0, hk = k < n - 1;
+        const double am = a[l], mm = M[l];
+        const double al = hl ? a[l - 1] : 0.0, ml = hl ? M[l - 1] : 0.0;
+        const double ar = hr ? a[l + 1] : 0.0, mr = hr ? M[l + 1] : 0.0;
+        const double at = ht ? a[l - n] : 0.0, mt = ht ? M[l - n] : 0.0;
+        const double ab = hb ? a[l + n] : 0.0, mb = hb ? M[l + n] : 0.0;
+        const size_t nn = (size_t)n * n;
+        const double af = hf ? a[l - nn] : 0.0, mf = hf ? M[l - nn] : 0.0;
+        const double ak = hk ? a[l + nn] : 0.0, mk = hk ? M[l + nn] : 0.0;
+        double t = 0.0;
+        t += (ml + mm) * (al - am) / 2.0 / hx / hx;
+        t += (mr + mm) * (ar - am) / 2.0 / hx / hx;
+        t += (mt + mm) * (at - am) / 2.0 / hy / hy;
+        t += (mb + mm) * (ab - am) / 2.0 / hy / hy;
+        t += (mf + mm) * (af - am) / 2.0 / hz / hz;
+        t += (mk + mm) * (ak - am) / 2.0 / hz / hz;
+        out[l] = t;
+      }
+}
+
+static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
+  const orc_params *p = &c->p;
+  const int n = p->n, g = p->g, q = v - p->nac;
+  const double LCI = p->LCI, KCI = p->KCI, epn2 = p->epn2, hx = p->hx, hy = p->hy, hz = p->hz;
+  const double *u = c->field + c->n3 * v;
+  double *M = c->t0, *f0 = c->t1, *ft = c->t2, *w = c->t3;
+  double *f1 = malloc(sizeof(double) * c->n3);
+  double Cq;
+  vm_mobility(c, q, M, &Cq);
+  for (size_t l = 0; l < c->n3; l++) f0[l] = -LCI * KCI * u[l] + LCI * ch_f(u[l]);
+  /* f1 = -C Lap_T(-f0 + KCI c + faces) - C mu_out / h^2 */
+  for (size_t l = 0; l < c->n3; l++) ft[l] = -f0[l] + KCI * u[l];
+#define FACES(arr, coef)                                                                                 \
+  for (int k = 0; k < n; k++)                                                                            \
+    for (int j = 0; j < n; j++) arr[IDX(n, k, j, 0)] += coef * Pimg(u, n, g, k, j, -1) / hx / hx;       \
+  for (int k = 0; k < n; k++)                                                                            \
+    for (int j = 0; j < n; j++) arr[IDX(n, k, j, n - 1)] += coef * Pimg(u, n, g, k, j, n) / hx / hx;    \
+  for (int k = 0; k < n; k++)                                                                            \
+    for (int i = 0; i < n; i++) arr[IDX(n, k, 0, i)] += coef * Pimg(u, n, g, k, -1, i) / hy / hy;       \
+  for (int k = 0; k < n; k++)                                                                            \
+    for (int i = 0; i < n; i++) arr[IDX(n, k, n - 1, i)] += coef * Pimg(u, n, g, k, n, i) / hy / hy;    \
+  for (int j = 0; j < n; j++)                                                                            \
+    for (int i = 0; i < n; i++) arr[IDX(n, 0, j, i)] += coef * Pimg(u, n, g, -1, j, i) / hz / hz;       \
+  for (int j = 0; j < n; j++)                                                                            \
+    for (int i = 0; i < n; i++) arr[IDX(n, n - 1, j, i)] += coef * Pimg(u, n, g, n, j, i) / hz / hz;
+  FACES(ft, epn2)
+  lap_zero(n, hx, hy, hz, ft, f1);
+  for (size_t l = 0; l < c->n3; l++) f1[l] = -Cq * f1[l];
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) f1[IDX(n, k, j, 0)] -= Cq * ch_mu_out(c, u, k, j, -1) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) f1[IDX(n, k, j, n - 1)] -= Cq * ch_mu_out(c, u, k, j, n) / hx / hx;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) f1[IDX(n, k, 0, i)] -= Cq * ch_mu_out(c, u, k, -1, i) / hy / hy;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) f1[IDX(n, k, n - 1, i)] -= Cq * ch_mu_out(c, u, k, n, i) / hy / hy;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) f1[IDX(n, 0, j, i)] -= Cq * ch_mu_out(c, u, -1, j, i) / hz / hz;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) f1[IDX(n, n - 1, j, i)] -= Cq * ch_mu_out(c, u, n, j, i) / hz / hz;
+  /* f2 = -div((M - C) grad(epn2 Lap_T c - f0 + faces)) */
+  lap_zero(n, hx, hy, hz, u, ft);
+  for (size_t l = 0; l < c->n3; l++) ft[l] = epn2 * ft[l] - f0[l];
+  FACES(ft, epn2)
+#undef FACES
+  for (size_t l = 0; l < c->n3; l++) M[l] = M[l] - Cq;
+  orc_div_m_grad(n, hx, hy, hz, M, ft, w);
+  for (size_t l = 0; l < c->n3; l++) out[l] = LCI * (f1[l] + -w[l]);
+  free(f1);
+}
+
 void orc_nonlinear(orc_ctx *c, int v, double *out) {
   if (v < c->p.nac) ac_nonlinear(c, v, out);
+  else if (c->p.varmob) ch_nonlinear_vm(c, v, out);
   else ch_nonlinear(c, v, out);
 }
 
@@ -835,6 +1118,15 @@ orc_ctx *orc_create(const orc_params *p) {
   c->p.V = c->Vm;
   c->p.Vi = c->Vim;
   c->p.D = c->D;
+  if (p->mask) {
+    c->mask = malloc(sizeof(int) * (size_t)n * n * n);
+    memcpy(c->mask, p->mask, sizeof(int) * (size_t)n * n * n);
+  }
+  c->p.mask = c->mask;
+  if (p->varmob && (p->nch != 3 || p->nac < 1)) {
+    free(c->Vm); free(c->Vim); free(c->D); free(c->eps0); free(c->lam); free(c->mask); free(c);
+    return NULL;
+  }
   c->DDX = malloc(sizeof(double) * n);
   c->DDY = malloc(sizeof(double) * n);
   c->DDZ = malloc(sizeof(double) * n);
@@ -864,6 +1156,7 @@ void orc_destroy(orc_ctx *c) {
   free(c->DDX); free(c->DDY); free(c->DDZ);
   free(c->field); free(c->N1); free(c->N2); free(c->felas); free(c->fbuf);
   free(c->phi1); free(c->phi2); free(c->t0); free(c->t1); free(c->t2); free(c->t3);
+  free(c->mask);
   free(c);
 }
 

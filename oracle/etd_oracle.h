@@ -17,6 +17,11 @@
  *   source_beta_to_omega/src/elastic.c:349-762
  *   source_beta_to_omega/src/fft3d.c:98-138    (fftMPI forward / scaled backward)
  *   source_beta_to_omega/src/field.c:407-678   (diagnostics / energy)
+ *   source_beta_to_omega/src/mobility.c:1-944  (variable-mobility CH)
+ *   source_beta_to_alpha/src/anisotropic_hip.c:528-647, 2002-2211
+ *                                               (full Lambda_ij d_i d_j stencil,
+ *                                                CALPHAD driving force, grain mask)
+ *   source_beta_to_alpha/src/vars.c:144-299    (CALPHAD parameters)
  * Each function in etd_oracle.c cites the lines it follows, and states what
  * pins it (closed forms, invariants, golden vectors) or "parity unpinned".
  *
@@ -37,7 +42,8 @@ typedef struct orc_params {
   int nac, nch;     /* Allen-Cahn and Cahn-Hilliard field counts */
   int etd2;         /* 0: ETD1, 1: ETD2 corrector */
   int elastic;      /* 0: none, 1: on AC fields */
-  int aniso;        /* 1: anisotropic gradient term F2 */
+  int aniso;        /* 1: gradient term F2 = div(Lambda grad u) (omega); 2: the
+                       compact Lambda_ij d_i d_j stencil of alpha */
   double dt, epn2;
   double hx, hy, hz;
   double kk[3];     /* kkx, kky, kkz */
@@ -51,6 +57,10 @@ typedef struct orc_params {
   const double *V;    /* n x n column-major eigenvectors of the 1D operator */
   const double *Vi;   /* n x n column-major inverse */
   const double *D;    /* n eigenvalues (unscaled, for h = 1) */
+  /* beta->alpha and variable-mobility physics (appended; 0 / NULL = omega) */
+  int free_energy;    /* 0: Landau polynomial (omega), 1: CALPHAD Ti-Al-V (alpha) */
+  const int *mask;    /* NULL, or n^3 grain mask: N -= 0.5 u where mask == 0 (alpha) */
+  int varmob;         /* 1: variable-mobility CH (omega mobility.c; nch == 3, nac >= 1) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -78,6 +88,15 @@ void orc_step(orc_ctx *c);
  * context's fields: u_n + dt/2; dt left at dt/2) */
 void orc_set_dt(orc_ctx *c, double dt);
 void orc_step_pair(orc_ctx *c, double dt, double *ufull);
+/* the CALPHAD bcc / hcp Gibbs energies at alpha's fixed compositions */
+void orc_calphad(double *gbcc, double *ghcp);
+/* its lattice stabilities and interaction parameters at T_a (13 values:
+ * GHSERAL, GHSERTI, GHSERV, GBCCAL, GBCCTI, GHCPAL, GHCPV, BL13_0, BL12_0,
+ * BL12_1, BL32_0, HL13_0, HL32_0) */
+void orc_calphad_terms(double *t);
+/* the variable-mobility divergence div((M) grad a) of mobility.c:435-722 on
+ * one field (out-of-box neighbours 0), for unit checks */
+void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out);
 /* stats: per field v (nac+nch) 4 doubles max,min,vol,eng; then couple eng */
 void orc_diag(orc_ctx *c, double *stats);
 

@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
         ("elastic_scale", C.c_double),
         ("C", C.c_double * 36),
         ("eps0", _dp), ("lam", _dp), ("V", _dp), ("Vi", _dp), ("D", _dp),
+        ("free_energy", C.c_int), ("mask", C.POINTER(C.c_int)), ("varmob", C.c_int),
     ]
 
 
@@ -63,6 +64,9 @@ def lib():
         L.orc_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_calphad.argtypes = [_dp, _dp]
+        L.orc_calphad_terms.argtypes = [_dp]
+        L.orc_div_m_grad.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -90,6 +94,29 @@ def fft3d(x, direction):
     buf = a.view(np.float64).reshape(-1)
     lib().orc_fft3d(N, _ptr(buf), int(direction))
     return a
+
+
+def calphad():
+    """(G_bcc, G_hcp) at the alpha driver's fixed compositions."""
+    a, b = C.c_double(), C.c_double()
+    lib().orc_calphad(C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def calphad_terms():
+    t = np.zeros(13)
+    lib().orc_calphad_terms(_ptr(t))
+    return t
+
+
+def div_m_grad(M, a, h=(1.0, 1.0, 1.0)):
+    """div(M grad a) as the variable-mobility CH forms it (n^3 arrays)."""
+    n = M.shape[0]
+    Mf = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+    af = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    out = np.zeros(n ** 3)
+    lib().orc_div_m_grad(n, h[0], h[1], h[2], _ptr(Mf), _ptr(af), _ptr(out))
+    return out.reshape(n, n, n)
 
 
 def modeprod(A, u, rotate=True):
@@ -159,6 +186,12 @@ class Oracle:
         p.V = _ptr(self._V)
         p.Vi = _ptr(self._Vi)
         p.D = _ptr(self._D)
+        p.free_energy = prm.get("free_energy", 0)
+        p.varmob = prm.get("varmob", 0)
+        mask = prm.get("mask")
+        if mask is not None:
+            self._mask = np.ascontiguousarray(np.asarray(mask, dtype=np.int32).reshape(-1))
+            p.mask = self._mask.ctypes.data_as(C.POINTER(C.c_int))
         self._p = p
         self.ctx = lib().orc_create(C.byref(p))
         if not self.ctx:
