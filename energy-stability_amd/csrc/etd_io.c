@@ -175,7 +175,15 @@ int etd_parse_ini_alpha(const char *path, etd_params *p) {
   ok &= fscanf(f, "%d", &p->approx) == 1;
   fclose(f);
   if (!ok) return ETD_ERR_IO;
+  /* the beta->alpha driver's physics: ac[n].LE = 2, KE = 3, epn2 = 0.048
+   * (vars.c:114-139), the CALPHAD driving force (anisotropic_hip.c:2002-2188),
+   * ETD1; its ANISOTROPIC = 1 is the compact Lambda_ij d_i d_j stencil
+   * (anisotropic_calc_dev_2, launched at anisotropic_hip.c:4090) */
   p->epn2 = 0.048;
+  p->LE = 2.0;
+  p->KE = 3.0;
+  p->free_energy = 1;
+  if (p->anisotropic == 1) p->anisotropic = 2;
   p->etd2 = 0;
   derive(p);
   set_lambda_iso(p);

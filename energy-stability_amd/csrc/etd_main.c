@@ -115,7 +115,7 @@ static int run_alpha(const char *suffix, const char *dir) {
       return 1;
     }
   }
-  if (p.anisotropic == 1) {
+  if (p.anisotropic) {
     snprintf(fn, sizeof fn, "%s/anisotropic_input.txt", dir);
     if (etd_parse_anisotropic(fn, &p)) {
       fprintf(stderr, "open anisotropic_input.txt Fail!");
@@ -134,6 +134,22 @@ static int run_alpha(const char *suffix, const char *dir) {
     fprintf(stderr, "etd_create: %s%s%s\n", etd_error_string(rc), c ? ": " : "", c ? etd_last_error(c) : "");
     if (c) etd_close(c);
     return 1;
+  }
+  /* the grain mask: n^3 native ints, x fastest, as read_ori reads its f file (field.c:111-129) */
+  const char *mpath = getenv("ETD_ALPHA_MASK");
+  if (mpath) {
+    const size_t n3 = (size_t)p.nx * p.ny * p.nz;
+    int *m = (int *)malloc(sizeof(int) * n3);
+    FILE *mf = fopen(mpath, "rb");
+    const int ok = m && mf && fread(m, sizeof(int), n3, mf) == n3;
+    if (mf) fclose(mf);
+    if (!ok || etd_set_mask(c, m)) {
+      fprintf(stderr, "fopen error %s!\n", mpath);
+      free(m);
+      etd_close(c);
+      return 1;
+    }
+    free(m);
   }
   printf("restart\t\t%d\n", p.restart);
   printf("nx,ny,nz\t%d\t%d\t%d\n", p.nx, p.ny, p.nz);

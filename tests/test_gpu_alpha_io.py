@@ -156,3 +156,47 @@ def test_alpha_driver_schedule_and_restart(tmp_path):
         ia, ua = read_chk(os.path.join(a, "out", "00", f"eta{v}_chk0_000000.dat"))  # run a: step 300 -> buffer 0
         ib, ub = read_chk(os.path.join(b, "out", "00", f"eta{v}_chk0_000000.dat"))
         assert ia == 6 and ib == 6 and np.array_equal(ua, ub), v
+
+
+def test_alpha_driver_runs_the_alpha_physics(tmp_path):
+    """etd_run --alpha with the driver's own physics: CALPHAD driving force,
+    its compact gradient stencil (ANISOTROPIC = 1), LE = 2, KE = 3,
+    epn2 = 0.048, and a grain mask file (ETD_ALPHA_MASK); the step-100
+    checkpoint against the oracle stepping the same seeded state."""
+    n = 20
+    d = str(tmp_path)
+    ini = (f" {n}, {n}, {n}\n 1, 1, 1\n 2\n ./out/\n ./data/\n 0.01, 1.0\n 0\n 1\n 2, 0\n 0\n 1\n"
+           f" 0, 0, 0\n {n - 4}, {n - 4}, {n - 4}\n 1.0, 1.0, 1.0\n 1, 100, 0\n 1000\n 16\n")
+    open(os.path.join(d, "input7.ini"), "w").write(ini)
+    rng = np.random.default_rng(11)
+    lam_in = rng.uniform(-0.2, 0.2, size=(2, 3, 3))
+    open(os.path.join(d, "anisotropic_input.txt"), "w").write(" ".join(repr(float(x)) for x in lam_in.reshape(-1)) + "\n")
+    mask = (rng.uniform(size=(n, n, n)) > 0.3).astype(np.int32)
+    mask.tofile(os.path.join(d, "grains.bin"))
+    for sub in ("out/00", "data/00", f"downsample_output_{n}_111"):
+        os.makedirs(os.path.join(d, sub), exist_ok=True)
+    env = dict(os.environ, ETD_SEED="5", ETD_ALPHA_MASK=os.path.join(d, "grains.bin"))
+    say("alpha driver, alpha physics: 100 steps")
+    r = subprocess.run([DRIVER, "--alpha", "7", "."], cwd=d, env=env, capture_output=True, text=True, timeout=300)
+    say(r.stdout[-200:])
+    assert r.returncode == 0, r.stderr
+    # the same initial state: etd_init_noise(seed + 7919 v), AC mean 0 amplitude 0.01
+    prm = dict(small_params(n=n, nac=2), free_energy=1, LE=2.0, KE=3.0, epn2=0.048, dt=0.01,
+               lam=np.eye(3)[None, :, :] - lam_in, mask=mask)
+    prm["aniso"] = 2
+    g = pyetd.Etd.from_dict(prm)
+    f0 = []
+    for v in range(2):
+        g.init_noise(v, 5 + 7919 * v, 0.0, 0.01)
+        f0.append(g.get_field(v))
+    g.close()
+    o = O.Oracle(prm)
+    for v in range(2):
+        o.set_field(v, f0[v])
+    o.step(100)
+    for v in range(2):
+        raw = open(os.path.join(d, "out", "00", f"eta{v}_chk0_000000.dat"), "rb").read()
+        got = np.frombuffer(raw[4:], np.float64).reshape(n, n, n)
+        err = float(np.abs(got - o.get_field(v)).max())
+        say(f"  field {v}: max |d| vs oracle after 100 steps = {err:.2e}")
+        assert err < 1e-9

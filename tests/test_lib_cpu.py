@@ -53,9 +53,12 @@ def test_parse_alpha_ini_fixture():
     assert rc == 0
     assert (p.nx, p.px, p.nghost) == (260, 4, 2)
     assert (p.dt, p.t_total) == (2.0, 10000.0)
-    assert (p.nac, p.nch, p.elastic, p.anisotropic) == (6, 2, 1, 1)
+    # the driver's ANISOTROPIC = 1 is its compact stencil (anisotropic = 2 here)
+    assert (p.nac, p.nch, p.elastic, p.anisotropic) == (6, 2, 1, 2)
     assert p.hx == 1.0  # 1024 / (260*4 - 16)
     assert p.epn2 == 0.048 and p.etd2 == 0
+    # its physics: CALPHAD driving force, LE = 2, KE = 3 (vars.c:114-127)
+    assert (p.free_energy, p.LE, p.KE) == (1, 2.0, 3.0)
 
 
 def test_parse_elastic_and_anisotropic_fixtures():
