@@ -86,7 +86,7 @@ static int validate(etd_ctx *c, const etd_params *p) {
     return seterr(c, ETD_ERR_UNSUPPORTED, "nx, ny, nz must be equal (the axis rotation of calc.c:585-629 needs a cube)");
   if (p->px != 1 || p->py != 1 || p->pz != 1)
     return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz = %d,%d,%d: one subdomain per GPU (use 1,1,1)", p->px, p->py, p->pz);
-  if (!p->periodic) return seterr(c, ETD_ERR_UNSUPPORTED, "periodic = 0 is not supported");
+  if (p->periodic != 0 && p->periodic != 1) return seterr(c, ETD_ERR_ARG, "periodic = %d is not 0 or 1", p->periodic);
   if (p->variable_mobility != 0 && p->variable_mobility != 1)
     return seterr(c, ETD_ERR_ARG, "VariableMobility = %d is not 0 or 1", p->variable_mobility);
   if (p->variable_mobility && (p->nch != 3 || p->nac < 1))
@@ -140,7 +140,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     memcpy(c->hV, Vh, sizeof(double) * n * n);
     memcpy(c->hVi, Vih, sizeof(double) * n * n);
     memcpy(c->hD, Dh, sizeof(double) * n);
-  } else if ((rc = etd_eigbasis(n, 3, c->hV, c->hVi, c->hD)) != ETD_OK) {
+  } else if ((rc = etd_eigbasis(n, p->periodic ? 3 : 0, c->hV, c->hVi, c->hD)) != ETD_OK) {
     return seterr(c, rc, "eigenbasis");
   }
   c->mp_variant = 1;  // direct-to-LDS pipelined 4x4x4 kernel, two workgroups per CU
@@ -303,7 +303,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     HCHK(c, hipMalloc(&c->vm_part, sizeof(double) * (3 * 256 + 3)));
   }
   // the mixed path's CH centring relies on the constant-mobility term's conservation
-  if (c->mixed && c->nch > 0 && !p->variable_mobility) {
+  if (c->mixed && c->nch > 0 && !p->variable_mobility && p->periodic) {
     HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
     HCHK(c, hipMalloc(&c->ch_const, sizeof(float) * c->nch * c->n3));
   }
@@ -408,7 +408,8 @@ extern "C" int etd_open(const char *dir, const char *suffix, etd_ctx **out) {
   const int n = p.nx;
   double *V = (double *)malloc(sizeof(double) * n * n), *Vi = (double *)malloc(sizeof(double) * n * n),
          *D = (double *)malloc(sizeof(double) * n);
-  int have = etd_read_matrices(dir, n, 3, V, Vi, D) == ETD_OK;
+  // boundary id (read_matrices, vars.c:488-582): 3 with neighbours on both sides, 0 with physical sides
+  int have = etd_read_matrices(dir, n, p.periodic ? 3 : 0, V, Vi, D) == ETD_OK;
   rc = etd_create(&p, have ? V : nullptr, have ? Vi : nullptr, have ? D : nullptr, out);
   free(V);
   free(Vi);

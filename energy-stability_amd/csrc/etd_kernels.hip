@@ -120,6 +120,7 @@ __global__ void k_ghost_x(T *base, size_t fs, int n, int g) {
 
 template <typename T>
 void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
+  if (!c->p.periodic) return;  // physical sides: no neighbour, unpack writes nothing (trans.c:154-251)
   const int n = c->n, g = c->g;
   const size_t n2 = (size_t)n * n;
   size_t tz = 2 * g * n2, ty = (size_t)(n - 2 * g) * 2 * g * n, tx = (size_t)(n - 2 * g) * (n - 2 * g) * 2 * g;
@@ -755,6 +756,7 @@ struct nl_args {
   int slot0;        // Lbase mod ring
   double LE, KE, a2, a3, a4, epn2, hx, hy, hz, LCI, KCI, es;
   double ihx2, ihy2, ihz2;  // 1 / h^2, formed once on the host
+  int phys;                 // physical boundaries on every side (periodic = 0, one box)
   int fe;                   // free energy: 0 Landau polynomial, 1 CALPHAD (alpha)
   double gbcc, ghcp;        // CALPHAD Gibbs energies (fe == 1)
   const unsigned char *mask;  // grain mask or NULL
@@ -774,6 +776,8 @@ __device__ __forceinline__ double Pimg(const T *u, int n, int g, int z, int y, i
 // fully wrapped position (variants before 0 come from the previous call).
 template <typename T>
 __device__ __forceinline__ double felas_at(const T *E, const nl_args &a, int p, int k, int j, int i) {
+  if (a.phys && (k < a.g || k >= a.n - a.g || j < a.g || j >= a.n - a.g || i < a.g || i >= a.n - a.g))
+    return 0.0;  // physical sides: the force buffer is never exchanged, 0 outside the interior
   const int n = a.n, g = a.g, N = a.N;
   const int gk = (k < g || k >= n - g), gj = (j < g || j >= n - g), gi = (i < g || i >= n - g);
   int lag = gk + gj + gi - 1;
@@ -835,9 +839,12 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
   // at most one box face per axis, and its image offset is the same for every
   // field; -1 when the cell is on no face of that axis
   const int wk = wrap_e(k, n, g), wj = wrap_e(j, n, g), wi = wrap_e(i, n, g);
-  const int ox = i == 0 ? (int)IDX(n, wk, wj, wrap_e(-1, n, g)) : i == n - 1 ? (int)IDX(n, wk, wj, wrap_e(n, n, g)) : -1;
-  const int oy = j == 0 ? (int)IDX(n, wk, wrap_e(-1, n, g), wi) : j == n - 1 ? (int)IDX(n, wk, wrap_e(n, n, g), wi) : -1;
-  const int oz = k == 0 ? (int)IDX(n, wrap_e(-1, n, g), wj, wi) : k == n - 1 ? (int)IDX(n, wrap_e(n, n, g), wj, wi) : -1;
+  int ox = i == 0 ? (int)IDX(n, wk, wj, wrap_e(-1, n, g)) : i == n - 1 ? (int)IDX(n, wk, wj, wrap_e(n, n, g)) : -1;
+  int oy = j == 0 ? (int)IDX(n, wk, wrap_e(-1, n, g), wi) : j == n - 1 ? (int)IDX(n, wk, wrap_e(n, n, g), wi) : -1;
+  int oz = k == 0 ? (int)IDX(n, wrap_e(-1, n, g), wj, wi) : k == n - 1 ? (int)IDX(n, wrap_e(n, n, g), wj, wi) : -1;
+  if (a.phys) ox = oy = oz = -1;  // no outside values on physical sides
+  // physical sides: felas is 0 outside the interior
+  const bool fel_zero = a.phys && (k < g || k >= n - g || j < g || j >= n - g || i < g || i >= n - g);
   double s = 0.0;
   for (int m0 = 0; m0 < a.nac; m0 += FB) {
     T um[FB];
@@ -856,7 +863,7 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
     for (int e = 0; e < FB; e++) {
       const unsigned p = (unsigned)(p0 + e < a.nac ? p0 + e : a.nac - 1);
       upv[e] = u[n3 * p + lo];
-      if (with_felas) fel[e] = E[(unsigned)felas_slot(a, (int)p, fc.lag) * N3 + foff];
+      if (with_felas) fel[e] = fel_zero ? (T)0 : E[(unsigned)felas_slot(a, (int)p, fc.lag) * N3 + foff];
       if (sub) sbv[e] = sub[n3 * p + lo];
     }
 #pragma unroll
@@ -897,11 +904,12 @@ __global__ void k_f2_grad(const T *__restrict__ u, T *__restrict__ G, const doub
   if (l >= n3) return;
   int i, j, k;
   idx3(l, n, i, j, k);
-  const double a_left = i > 0 ? u[l - 1] : 0.0;
+  // low neighbour of cell 0: 0 beside a neighbour, the mirror cell 1 on a physical side; high: 0
+  const double a_left = i > 0 ? u[l - 1] : (a.phys ? (double)u[l + 1] : 0.0);
   const double a_right = i < n - 1 ? u[l + 1] : 0.0;
-  const double a_top = j > 0 ? u[l - n] : 0.0;
+  const double a_top = j > 0 ? u[l - n] : (a.phys ? (double)u[l + n] : 0.0);
   const double a_bottom = j < n - 1 ? u[l + n] : 0.0;
-  const double a_front = k > 0 ? u[l - (size_t)n * n] : 0.0;
+  const double a_front = k > 0 ? u[l - (size_t)n * n] : (a.phys ? (double)u[l + (size_t)n * n] : 0.0);
   const double a_back = k < n - 1 ? u[l + (size_t)n * n] : 0.0;
   const double gx = (a_right - a_left) / (2.0 * a.hx);
   const double gy = (a_top - a_bottom) / (2.0 * a.hy);
@@ -923,11 +931,11 @@ __global__ void k_f2_div(const T *__restrict__ G, T *__restrict__ out, const T *
   int i, j, k;
   idx3(l, n, i, j, k);
   const T *gx = G, *gy = G + n3, *gz = G + 2 * n3;
-  const double a_left = i > 0 ? gx[l - 1] : 0.0;
+  const double a_left = i > 0 ? gx[l - 1] : (a.phys ? (double)gx[l + 1] : 0.0);
   const double a_right = i < n - 1 ? gx[l + 1] : 0.0;
-  const double a_top = j > 0 ? gy[l - n] : 0.0;
+  const double a_top = j > 0 ? gy[l - n] : (a.phys ? (double)gy[l + n] : 0.0);
   const double a_bottom = j < n - 1 ? gy[l + n] : 0.0;
-  const double a_front = k > 0 ? gz[l - (size_t)n * n] : 0.0;
+  const double a_front = k > 0 ? gz[l - (size_t)n * n] : (a.phys ? (double)gz[l + (size_t)n * n] : 0.0);
   const double a_back = k < n - 1 ? gz[l + (size_t)n * n] : 0.0;
   double tmp = 0.0;
   tmp += (a_right - a_left) / 2.0 / a.hx;
@@ -963,7 +971,12 @@ __global__ __launch_bounds__(256) void k_f2_full(const T *__restrict__ u, T *__r
   // images); cells outside it the periodic image, as the enlarged slabs hold
   auto U = [&](int dz, int dy, int dx) -> double {
     if (inner) return u[l + ((long)dz * n + dy) * n + dx];
-    const int z = k + dz, y = j + dy, x = i + dx;
+    int z = k + dz, y = j + dy, x = i + dx;
+    if (a.phys) {  // physical sides mirror: n_left = i + 1 at i == 0, n_right = i - 1 at i == n - 1
+      z = z < 0 ? -z : (z >= n ? 2 * (n - 1) - z : z);
+      y = y < 0 ? -y : (y >= n ? 2 * (n - 1) - y : y);
+      x = x < 0 ? -x : (x >= n ? 2 * (n - 1) - x : x);
+    }
     if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
     return Pimg(u, n, g, z, y, x);
   };
@@ -1013,6 +1026,7 @@ __device__ __forceinline__ double ch_t_at(const T *c, const nl_args &a, const ch
   double f = -a.LCI * a.KCI * cc;
   f += a.LCI * ch_f(cc);
   const double ce = a.LCI * a.epn2;
+  if (a.phys) return f;  // no outside values on physical sides
   if (i == 0) f -= ce * Pimg(c, n, g, k, j, -1) * ih.x;
   if (i == n - 1) f -= ce * Pimg(c, n, g, k, j, n) * ih.x;
   if (j == 0) f -= ce * Pimg(c, n, g, k, -1, i) * ih.y;
@@ -1105,15 +1119,22 @@ __global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__res
   shell_point(s, n, i, j, k);
   const size_t l = IDX(n, k, j, i);
   const double tc = ch_t_at(c, a, ih, k, j, i);
-  const double f_left = i > 0 ? ch_t_at(c, a, ih, k, j, i - 1) : 0.0;
   const double f_right = i < n - 1 ? ch_t_at(c, a, ih, k, j, i + 1) : 0.0;
-  const double f_top = j > 0 ? ch_t_at(c, a, ih, k, j - 1, i) : 0.0;
   const double f_bottom = j < n - 1 ? ch_t_at(c, a, ih, k, j + 1, i) : 0.0;
-  const double f_front = k > 0 ? ch_t_at(c, a, ih, k - 1, j, i) : 0.0;
   const double f_back = k < n - 1 ? ch_t_at(c, a, ih, k + 1, j, i) : 0.0;
+  // low neighbour of cell 0: 0 beside a neighbour, the mirror (cell 1) on a physical side
+  const double f_left = i > 0 ? ch_t_at(c, a, ih, k, j, i - 1) : (a.phys ? f_right : 0.0);
+  const double f_top = j > 0 ? ch_t_at(c, a, ih, k, j - 1, i) : (a.phys ? f_bottom : 0.0);
+  const double f_front = k > 0 ? ch_t_at(c, a, ih, k - 1, j, i) : (a.phys ? f_back : 0.0);
   double r = (f_left + f_right - 2.0 * tc) * ih.x;
   r += (f_top + f_bottom - 2.0 * tc) * ih.y;
   r += (f_front + f_back - 2.0 * tc) * ih.z;
+  if (a.phys) {
+    if (sub) r -= sub[l];
+    if (nc) r -= nc[l];
+    out[l] = (T)r;
+    return;
+  }
   if (i == 0) r -= ch_mu_out(c, a, ih, k, j, -1) * ih.x;
   if (i == n - 1) r -= ch_mu_out(c, a, ih, k, j, n) * ih.x;
   if (j == 0) r -= ch_mu_out(c, a, ih, k, -1, i) * ih.y;
@@ -1162,6 +1183,7 @@ static nl_args make_nl(etd_ctx *c) {
   a.ihx2 = 1.0 / (a.hx * a.hx);
   a.ihy2 = 1.0 / (a.hy * a.hy);
   a.ihz2 = 1.0 / (a.hz * a.hz);
+  a.phys = !c->p.periodic;
   a.fe = c->p.free_energy;
   a.gbcc = c->gbcc;
   a.ghcp = c->ghcp;
@@ -1224,9 +1246,11 @@ template <typename T>
 __device__ __forceinline__ double lap_zero_at(const T *a, int n, size_t l, int k, int j, int i, const nl_args &q) {
   const size_t dz = (size_t)n * n;
   const double ac = a[l];
-  const double al = i > 0 ? (double)a[l - 1] : 0.0, ar = i < n - 1 ? (double)a[l + 1] : 0.0;
-  const double at = j > 0 ? (double)a[l - n] : 0.0, ab = j < n - 1 ? (double)a[l + n] : 0.0;
-  const double af = k > 0 ? (double)a[l - dz] : 0.0, ak = k < n - 1 ? (double)a[l + dz] : 0.0;
+  const double ar = i < n - 1 ? (double)a[l + 1] : 0.0, ab = j < n - 1 ? (double)a[l + n] : 0.0;
+  const double ak = k < n - 1 ? (double)a[l + dz] : 0.0;
+  const double al = i > 0 ? (double)a[l - 1] : (q.phys ? ar : 0.0);
+  const double at = j > 0 ? (double)a[l - n] : (q.phys ? ab : 0.0);
+  const double af = k > 0 ? (double)a[l - dz] : (q.phys ? ak : 0.0);
   double t = 0.0;
   t += (al + ar - 2.0 * ac) * q.ihx2;
   t += (at + ab - 2.0 * ac) * q.ihy2;
@@ -1248,6 +1272,11 @@ __global__ __launch_bounds__(256) void k_vm_ft(const T *__restrict__ c, T *__res
   double t1 = -f0 + a.KCI * cc;
   double t2 = a.epn2 * lap_zero_at(c, n, l, k, j, i, a) - f0;
   double fc = 0.0;  // epn2 c_out / h^2 on the faces, left, right, top, bottom, front, back
+  if (a.phys) {
+    ft1[l] = (T)t1;
+    ft2[l] = (T)t2;
+    return;
+  }
   if (i == 0) fc += a.epn2 * Pimg(c, n, g, k, j, -1) * a.ihx2;
   if (i == n - 1) fc += a.epn2 * Pimg(c, n, g, k, j, n) * a.ihx2;
   if (j == 0) fc += a.epn2 * Pimg(c, n, g, k, -1, i) * a.ihy2;
@@ -1272,12 +1301,14 @@ __global__ __launch_bounds__(256) void k_vm_out(const T *__restrict__ c, const T
   const double C = cmax[q];
   const ch_h ih = {a.ihx2, a.ihy2, a.ihz2};
   double f1 = -C * lap_zero_at(ft1, n, l, k, j, i, a);
-  if (i == 0) f1 -= C * ch_mu_out(c, a, ih, k, j, -1) * ih.x;
-  if (i == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, j, n) * ih.x;
-  if (j == 0) f1 -= C * ch_mu_out(c, a, ih, k, -1, i) * ih.y;
-  if (j == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, n, i) * ih.y;
-  if (k == 0) f1 -= C * ch_mu_out(c, a, ih, -1, j, i) * ih.z;
-  if (k == n - 1) f1 -= C * ch_mu_out(c, a, ih, n, j, i) * ih.z;
+  if (!a.phys) {
+    if (i == 0) f1 -= C * ch_mu_out(c, a, ih, k, j, -1) * ih.x;
+    if (i == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, j, n) * ih.x;
+    if (j == 0) f1 -= C * ch_mu_out(c, a, ih, k, -1, i) * ih.y;
+    if (j == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, n, i) * ih.y;
+    if (k == 0) f1 -= C * ch_mu_out(c, a, ih, -1, j, i) * ih.z;
+    if (k == n - 1) f1 -= C * ch_mu_out(c, a, ih, n, j, i) * ih.z;
+  }
   // div((M - C) grad ft2), mobility.c:435-722: (m_nb + m_mid)(a_nb - a_mid) / 2h^2, 0 outside the box
   const size_t dz = (size_t)n * n;
   const double am = ft2[l], mm = vm_M(m, q, (double)eta[l]) - C;
@@ -1285,12 +1316,13 @@ __global__ __launch_bounds__(256) void k_vm_out(const T *__restrict__ c, const T
     const double an = in ? (double)ft2[o] : 0.0, mn = in ? vm_M(m, q, (double)eta[o]) - C : 0.0;
     return (mn + mm) * (an - am) * 0.5 * ih2;
   };
+  // the low neighbour of cell 0 on a physical side is the mirror cell 1
   double d = 0.0;
-  d += term(i > 0, l - 1, a.ihx2);
+  d += term(i > 0 || a.phys, i > 0 ? l - 1 : l + 1, a.ihx2);
   d += term(i < n - 1, l + 1, a.ihx2);
-  d += term(j > 0, l - n, a.ihy2);
+  d += term(j > 0 || a.phys, j > 0 ? l - n : l + n, a.ihy2);
   d += term(j < n - 1, l + n, a.ihy2);
-  d += term(k > 0, l - dz, a.ihz2);
+  d += term(k > 0 || a.phys, k > 0 ? l - dz : l + dz, a.ihz2);
   d += term(k < n - 1, l + dz, a.ihz2);
   double r = a.LCI * (f1 + -d);
   if (sub) r -= (double)sub[l];
@@ -2118,6 +2150,7 @@ void launch_felas_materialize(etd_ctx *c, T *felas) {
 // added up in a fixed order by diag_reduce (host) or k_diag_finish (device).
 struct diag_args {
   int n, g, nac, nch;
+  int lo;  // first cell of the sums: g, or 0 when the low side is physical (ix2 = 0, vars.c:58-157)
   double hx, hy, hz, epn2, a2, a3, a4;
   double ihx, ihy, ihz;  // 1 / h
 };
@@ -2171,18 +2204,18 @@ __global__ __launch_bounds__(320) void k_diag(const T *__restrict__ u, const dou
                                               double *__restrict__ part, diag_args a) {
   constexpr int E = 8;  // planes per sub-range: 152-192 VGPRs, 2-3 waves per SIMD (12 and 6 measured slower)
   extern __shared__ double wst[];             // [wave][field][max, min, sum]
-  const int n = a.n, g = a.g, m = n - 2 * g - 1, V = a.nac + a.nch, nac = a.nac, nch = a.nch;
+  const int n = a.n, g = a.g, lo = a.lo, m = n - g - 1 - lo, V = a.nac + a.nch, nac = a.nac, nch = a.nch;
   const int per = (int)gridDim.x / 8;  // gridDim.x = m ECHUNK, a multiple of 8
   const int w = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
   const int row = w % m, chunk = w / m;
   const int nb = m * ECHUNK, bi = row * ECHUNK + chunk;
-  const int j = g + row;
-  const int kb = g + (int)(((long)m * chunk) / ECHUNK), ke = g + (int)(((long)m * (chunk + 1)) / ECHUNK);
+  const int j = lo + row;
+  const int kb = lo + (int)(((long)m * chunk) / ECHUNK), ke = lo + (int)(((long)m * (chunk + 1)) / ECHUNK);
   const int nsub = (ke - kb + E - 1) / E;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
-  const int i = g + 63 * wv + lane;
-  const bool cell = lane < 63 && i < g + m;
-  const bool st_lane = i < n - g && (lane < 63 || wv == nw - 1);  // each i of [g, n - g) once
+  const int i = lo + 63 * wv + lane;
+  const bool cell = lane < 63 && i < lo + m;
+  const bool st_lane = i < n - g && (lane < 63 || wv == nw - 1);  // each i of [lo, n - g) once
   const bool st_row1 = row == m - 1;                              // also count row j + 1 = n - g - 1
   const bool last_chunk = chunk == ECHUNK - 1;
   const size_t n2 = (size_t)n * n, n3 = n2 * n;
@@ -2361,7 +2394,8 @@ __global__ __launch_bounds__(320) void k_diag(const T *__restrict__ u, const dou
 }
 
 static int diag_kinds(const etd_ctx *c) { return 1 + c->nch + ((c->nac > 0 && c->nch > 0) ? 1 : 0); }
-static int diag_blocks(const etd_ctx *c) { return (c->n - 2 * c->g - 1) * ECHUNK; }
+static int diag_lo(const etd_ctx *c) { return c->p.periodic ? c->g : 0; }
+static int diag_blocks(const etd_ctx *c) { return (c->n - c->g - 1 - diag_lo(c)) * ECHUNK; }
 // doubles of partial sums one diagnostics call writes
 static size_t diag_partials(const etd_ctx *c) { return ((size_t)3 * c->V + diag_kinds(c)) * diag_blocks(c); }
 size_t etd_diag_partials(const etd_ctx *c) { return diag_partials(c); }
@@ -2372,6 +2406,7 @@ static diag_args make_diag_args(const etd_ctx *c) {
   a.g = c->g;
   a.nac = c->nac;
   a.nch = c->nch;
+  a.lo = diag_lo(c);
   a.hx = c->p.hx;
   a.hy = c->p.hy;
   a.hz = c->p.hz;
@@ -2388,7 +2423,7 @@ static diag_args make_diag_args(const etd_ctx *c) {
 template <typename T>
 static void launch_diag_kernel(etd_ctx *c, double *part) {
   static_assert(ECHUNK % 8 == 0, "the XCD mapping needs a multiple of 8 work items");
-  const int me = c->n - 2 * c->g - 1, nw = (me + 62) / 63;
+  const int me = c->n - c->g - 1 - diag_lo(c), nw = (me + 62) / 63;
   k_diag<T><<<(unsigned)(me * ECHUNK), 64 * nw, sizeof(double) * 3 * nw * c->V, c->stream>>>(fields<T>(c).u, c->lam,
                                                                                            part, make_diag_args(c));
 }
@@ -2499,8 +2534,8 @@ extern "C" int etd_diag_noalloc(etd_ctx *c, double *stats_dev) {
   const size_t need = diag_partials(c);
   if (need > DIAG_SCRATCH_DOUBLES || c->V + c->nch + 2 > 256) {
     snprintf(c->err, sizeof(c->err),
-             "etd_diag_noalloc: this context needs %zu doubles of partial sums ((3 V + 1 + nch + couple) (N - 1) 8); "
-             "the static buffer holds %d",
+             "etd_diag_noalloc: this context needs %zu doubles of partial sums ((3 V + 1 + nch + couple) x cells "
+             "per axis x 8); the static buffer holds %d",
              need, DIAG_SCRATCH_DOUBLES);
     return ETD_ERR_UNSUPPORTED;
   }

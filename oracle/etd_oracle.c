@@ -204,6 +204,7 @@ void orc_ghost_fill(int n, int g, double *u) {
 }
 
 void orc_transfer(orc_ctx *c) {
+  if (c->p.physical) return;  /* no neighbour: unpack writes nothing (trans.c:154-251, left < 0) */
   for (int v = 0; v < c->V; v++) orc_ghost_fill(c->p.n, c->p.g, c->field + c->n3 * v);
 }
 
@@ -216,6 +217,18 @@ void orc_transfer(orc_ctx *c) {
  * P can be read from the post-fill field. */
 static inline double Pimg(const double *u, int n, int g, int z, int y, int x) {
   return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+}
+
+/* Neighbour rule of omega's stencils (anisotropic.c:177-216, calc.c, mobility.c
+ * laplace): the low neighbour of cell 0 is 0 when that side has a neighbour
+ * (periodic box) and the mirror cell 1 when it is physical (left < 0, i ==
+ * ix1: n_left = ix1 + 1); the high neighbour of cell n - 1 is 0 in both
+ * cases (with left < 0 the first branch applies and sets n_right = -1). */
+static inline double lo_nb(const double *a, size_t l, size_t step, int i, int phys) {
+  return i > 0 ? a[l - step] : (phys ? a[l + step] : 0.0);
+}
+static inline double hi_nb(const double *a, size_t l, size_t step, int i, int n) {
+  return i < n - 1 ? a[l + step] : 0.0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -233,20 +246,19 @@ static void ac_f2(const orc_ctx *c, int pv, const double *u, double *gx_, double
   const int n = c->p.n;
   const double hx = c->p.hx, hy = c->p.hy, hz = c->p.hz;
   const double *L = c->lam + 9 * pv;
+  const int ph = c->p.physical;
+  const size_t nn = (size_t)n * n;
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
-        double a_left = i > 0 ? u[IDX(n, k, j, i - 1)] : 0.0;
-        double a_right = i < n - 1 ? u[IDX(n, k, j, i + 1)] : 0.0;
-        double a_top = j > 0 ? u[IDX(n, k, j - 1, i)] : 0.0;
-        double a_bottom = j < n - 1 ? u[IDX(n, k, j + 1, i)] : 0.0;
-        double a_front = k > 0 ? u[IDX(n, k - 1, j, i)] : 0.0;
-        double a_back = k < n - 1 ? u[IDX(n, k + 1, j, i)] : 0.0;
+        const size_t l = IDX(n, k, j, i);
+        double a_left = lo_nb(u, l, 1, i, ph), a_right = hi_nb(u, l, 1, i, n);
+        double a_top = lo_nb(u, l, n, j, ph), a_bottom = hi_nb(u, l, n, j, n);
+        double a_front = lo_nb(u, l, nn, k, ph), a_back = hi_nb(u, l, nn, k, n);
         double gradx = (a_right - a_left) / (2.0 * hx);
         double grady = (a_top - a_bottom) / (2.0 * hy);
         double gradz = (a_front - a_back) / (2.0 * hz);
-        size_t l = IDX(n, k, j, i);
         gx_[l] = L[0] * gradx + L[1] * grady + L[2] * gradz;
         gy_[l] = L[3] * gradx + L[4] * grady + L[5] * gradz;
         gz_[l] = L[6] * gradx + L[7] * grady + L[8] * gradz;
@@ -255,17 +267,15 @@ static void ac_f2(const orc_ctx *c, int pv, const double *u, double *gx_, double
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
-        double a_left = i > 0 ? gx_[IDX(n, k, j, i - 1)] : 0.0;
-        double a_right = i < n - 1 ? gx_[IDX(n, k, j, i + 1)] : 0.0;
-        double a_top = j > 0 ? gy_[IDX(n, k, j - 1, i)] : 0.0;
-        double a_bottom = j < n - 1 ? gy_[IDX(n, k, j + 1, i)] : 0.0;
-        double a_front = k > 0 ? gz_[IDX(n, k - 1, j, i)] : 0.0;
-        double a_back = k < n - 1 ? gz_[IDX(n, k + 1, j, i)] : 0.0;
+        const size_t l = IDX(n, k, j, i);
+        double a_left = lo_nb(gx_, l, 1, i, ph), a_right = hi_nb(gx_, l, 1, i, n);
+        double a_top = lo_nb(gy_, l, n, j, ph), a_bottom = hi_nb(gy_, l, n, j, n);
+        double a_front = lo_nb(gz_, l, nn, k, ph), a_back = hi_nb(gz_, l, nn, k, n);
         double tmp = 0.0;
         tmp += (a_right - a_left) / 2.0 / hx;
         tmp += (a_top - a_bottom) / 2.0 / hy;
         tmp += (a_front - a_back) / 2.0 / hz;
-        f2[IDX(n, k, j, i)] = tmp;
+        f2[l] = tmp;
       }
 }
 
@@ -299,7 +309,9 @@ static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
         f -= LE * fp;
         out[l] = f;
       }
-  /* boundary coupling, order left,right,top,bottom,front,back */
+  /* boundary coupling, order left,right,top,bottom,front,back; only on sides
+   * with a neighbour (anisotropic.c:106-158, left >= 0 ...) */
+  if (!p->physical) {
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] += LE * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
@@ -312,6 +324,7 @@ static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
     for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] += LE * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] += LE * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+  }
   if (p->aniso == 1) ac_f2(c, pv, u, c->t0, c->t1, c->t2, c->t3);
   if (p->aniso == 2) ac_f2_full(c, pv, u, c->t3);
   const double *felas = p->elastic == 1 ? c->felas + c->n3 * pv : NULL;
@@ -371,6 +384,8 @@ static void ch_nonlinear(orc_ctx *c, int v, double *out) {
     f += LCI * ch_f(cc);
     t[l] = f;
   }
+  const int ph = p->physical;
+  if (!ph) {
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++) t[IDX(n, k, j, 0)] -= LCI * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
@@ -383,22 +398,23 @@ static void ch_nonlinear(orc_ctx *c, int v, double *out) {
     for (int i = 0; i < n; i++) t[IDX(n, 0, j, i)] -= LCI * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) t[IDX(n, n - 1, j, i)] -= LCI * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+  }
+  const size_t nn = (size_t)n * n;
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
-        double tc = t[IDX(n, k, j, i)];
-        double f_left = i > 0 ? t[IDX(n, k, j, i - 1)] : 0.0;
-        double f_right = i < n - 1 ? t[IDX(n, k, j, i + 1)] : 0.0;
-        double f_top = j > 0 ? t[IDX(n, k, j - 1, i)] : 0.0;
-        double f_bottom = j < n - 1 ? t[IDX(n, k, j + 1, i)] : 0.0;
-        double f_front = k > 0 ? t[IDX(n, k - 1, j, i)] : 0.0;
-        double f_back = k < n - 1 ? t[IDX(n, k + 1, j, i)] : 0.0;
+        const size_t l = IDX(n, k, j, i);
+        double tc = t[l];
+        double f_left = lo_nb(t, l, 1, i, ph), f_right = hi_nb(t, l, 1, i, n);
+        double f_top = lo_nb(t, l, n, j, ph), f_bottom = hi_nb(t, l, n, j, n);
+        double f_front = lo_nb(t, l, nn, k, ph), f_back = hi_nb(t, l, nn, k, n);
         double r = (f_left + f_right - 2.0 * tc) / hx / hx;
         r += (f_top + f_bottom - 2.0 * tc) / hy / hy;
         r += (f_front + f_back - 2.0 * tc) / hz / hz;
-        out[IDX(n, k, j, i)] = r;
+        out[l] = r;
       }
+  if (ph) return;  /* no outside chemical potential on physical sides */
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] -= ch_mu_out(c, u, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
@@ -516,6 +532,7 @@ static double calphad_f(const orc_ctx *c, size_t l, int pv, double gbcc, double 
  * on quadratic fields away from the box faces (any Lambda, cross terms
  * included), and the periodic image against a shifted field. */
 static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int x);
+static inline int mirror_in(int x, int n);
 static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
   const int n = c->p.n, g = c->p.g;
   const double hx = c->p.hx, hy = c->p.hy, hz = c->p.hz;
@@ -524,7 +541,9 @@ static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
-#define U(dz, dy, dx) pimg_or_in(u, n, g, k + (dz), j + (dy), i + (dx))
+#define U(dz, dy, dx)                                                                                          \
+  (c->p.physical ? u[IDX(n, mirror_in(k + (dz), n), mirror_in(j + (dy), n), mirror_in(i + (dx), n))]            \
+                 : pimg_or_in(u, n, g, k + (dz), j + (dy), i + (dx)))
         const double m = U(0, 0, 0);
         const double fxx = (U(0, 0, 1) - 2 * m + U(0, 0, -1)) / hx / hx;
         const double fyy = (U(0, 1, 0) - 2 * m + U(0, -1, 0)) / hy / hy;
@@ -551,6 +570,9 @@ static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int
   if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
   return Pimg(u, n, g, z, y, x);
 }
+/* physical sides (anisotropic_hip.c:546-591): n_left = i + 1 at i == ix1,
+ * n_right = i - 1 at i == ix4 - 1, per axis; the mirror stays in the box */
+static inline int mirror_in(int x, int n) { return x < 0 ? -x : (x >= n ? 2 * (n - 1) - x : x); }
 
 /* ------------------------------------------------------------------ */
 /* Variable-mobility CH: ch_calc_FU_VariableMobility (mobility.c:838-858)
@@ -592,34 +614,35 @@ static void vm_mobility(const orc_ctx *c, int q, double *M, double *Cmax) {
   *Cmax = cm;
 }
 
-static void lap_zero(int n, double hx, double hy, double hz, const double *a, double *out) {
-#pragma omp parallel for schedule(static)
-  for (int k = 0; k < n; k++)
-    for (int j = 0; j < n; j++)
-      for (int i = 0; i < n; i++) {
-        const double ac = a[IDX(n, k, j, i)];
-        const double al = i > 0 ? a[IDX(n, k, j, i - 1)] : 0.0, ar = i < n - 1 ? a[IDX(n, k, j, i + 1)] : 0.0;
-        const double at = j > 0 ? a[IDX(n, k, j - 1, i)] : 0.0, ab = j < n - 1 ? a[IDX(n, k, j + 1, i)] : 0.0;
-        const double af = k > 0 ? a[IDX(n, k - 1, j, i)] : 0.0, ak = k < n - 1 ? a[IDX(n, k + 1, j, i)] : 0.0;
-        out[IDX(n, k, j, i)] = (al + ar - 2 * ac) / hx / hx + (at + ab - 2 * ac) / hy / hy + (af + ak - 2 * ac) / hz / hz;
-      }
-}
-
-void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out) {
+static void lap_zero(int n, double hx, double hy, double hz, const double *a, double *out, int ph) {
+  const size_t nn = (size_t)n * n;
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
         const size_t l = IDX(n, k, j, i);
-        const int hl = i > 0, hr = i < n - 1, ht = j > 0, hb = j < n - 1, hf = k > 0, hk = k < n - 1;
+        const double ac = a[l];
+        const double al = lo_nb(a, l, 1, i, ph), ar = hi_nb(a, l, 1, i, n);
+        const double at = lo_nb(a, l, n, j, ph), ab = hi_nb(a, l, n, j, n);
+        const double af = lo_nb(a, l, nn, k, ph), ak = hi_nb(a, l, nn, k, n);
+        out[l] = (al + ar - 2 * ac) / hx / hx + (at + ab - 2 * ac) / hy / hy + (af + ak - 2 * ac) / hz / hz;
+      }
+}
+
+static void div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out, int ph) {
+  const size_t nn = (size_t)n * n;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        const size_t l = IDX(n, k, j, i);
         const double am = a[l], mm = M[l];
-        const double al = hl ? a[l - 1] : 0.0, ml = hl ? M[l - 1] : 0.0;
-        const double ar = hr ? a[l + 1] : 0.0, mr = hr ? M[l + 1] : 0.0;
-        const double at = ht ? a[l - n] : 0.0, mt = ht ? M[l - n] : 0.0;
-        const double ab = hb ? a[l + n] : 0.0, mb = hb ? M[l + n] : 0.0;
-        const size_t nn = (size_t)n * n;
-        const double af = hf ? a[l - nn] : 0.0, mf = hf ? M[l - nn] : 0.0;
-        const double ak = hk ? a[l + nn] : 0.0, mk = hk ? M[l + nn] : 0.0;
+        const double al = lo_nb(a, l, 1, i, ph), ml = lo_nb(M, l, 1, i, ph);
+        const double ar = hi_nb(a, l, 1, i, n), mr = hi_nb(M, l, 1, i, n);
+        const double at = lo_nb(a, l, n, j, ph), mt = lo_nb(M, l, n, j, ph);
+        const double ab = hi_nb(a, l, n, j, n), mb = hi_nb(M, l, n, j, n);
+        const double af = lo_nb(a, l, nn, k, ph), mf = lo_nb(M, l, nn, k, ph);
+        const double ak = hi_nb(a, l, nn, k, n), mk = hi_nb(M, l, nn, k, n);
         double t = 0.0;
         t += (ml + mm) * (al - am) / 2.0 / hx / hx;
         t += (mr + mm) * (ar - am) / 2.0 / hx / hx;
@@ -629,6 +652,9 @@ void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, con
         t += (mk + mm) * (ak - am) / 2.0 / hz / hz;
         out[l] = t;
       }
+}
+void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out) {
+  div_m_grad(n, hx, hy, hz, M, a, out, 0);
 }
 
 static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
@@ -656,9 +682,11 @@ static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
     for (int i = 0; i < n; i++) arr[IDX(n, 0, j, i)] += coef * Pimg(u, n, g, -1, j, i) / hz / hz;       \
   for (int j = 0; j < n; j++)                                                                            \
     for (int i = 0; i < n; i++) arr[IDX(n, n - 1, j, i)] += coef * Pimg(u, n, g, n, j, i) / hz / hz;
-  FACES(ft, epn2)
-  lap_zero(n, hx, hy, hz, ft, f1);
+  const int ph = p->physical;
+  if (!ph) { FACES(ft, epn2) }
+  lap_zero(n, hx, hy, hz, ft, f1, ph);
   for (size_t l = 0; l < c->n3; l++) f1[l] = -Cq * f1[l];
+  if (!ph) {
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++) f1[IDX(n, k, j, 0)] -= Cq * ch_mu_out(c, u, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
@@ -671,13 +699,14 @@ static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
     for (int i = 0; i < n; i++) f1[IDX(n, 0, j, i)] -= Cq * ch_mu_out(c, u, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) f1[IDX(n, n - 1, j, i)] -= Cq * ch_mu_out(c, u, n, j, i) / hz / hz;
+  }
   /* f2 = -div((M - C) grad(epn2 Lap_T c - f0 + faces)) */
-  lap_zero(n, hx, hy, hz, u, ft);
+  lap_zero(n, hx, hy, hz, u, ft, ph);
   for (size_t l = 0; l < c->n3; l++) ft[l] = epn2 * ft[l] - f0[l];
-  FACES(ft, epn2)
+  if (!ph) { FACES(ft, epn2) }
 #undef FACES
   for (size_t l = 0; l < c->n3; l++) M[l] = M[l] - Cq;
-  orc_div_m_grad(n, hx, hy, hz, M, ft, w);
+  div_m_grad(n, hx, hy, hz, M, ft, w, ph);
   for (size_t l = 0; l < c->n3; l++) out[l] = LCI * (f1[l] + -w[l]);
   free(f1);
 }
@@ -970,7 +999,7 @@ void orc_elastic(orc_ctx *c) {
     for (int z = 0; z < N; z++)
       for (int y = 0; y < N; y++)
         for (int x = 0; x < N; x++) f[IDX(n, z + g, y + g, x + g)] = acc[2 * (((size_t)z * N + y) * N + x)];
-    orc_ghost_fill(n, g, f);
+    if (!p->physical) orc_ghost_fill(n, g, f);  /* physical sides: f keeps 0 outside the interior */
     memcpy(c->felas + c->n3 * pp, f, sizeof(double) * c->n3);
   }
   free(spec);
@@ -1281,27 +1310,31 @@ static void cell_grad(const double *u, int n, int k, int j, int i, double hx, do
 void orc_diag(orc_ctx *c, double *stats) {
   const orc_params *p = &c->p;
   const int n = p->n, g = p->g, nac = p->nac, nch = p->nch;
+  /* the loops run over [ix2, ix3) (field.c:457-527): [g, n - g) with neighbours on
+   * both sides, [0, n - g) when the low side is physical (init_para, vars.c:58-157:
+   * left < 0 gives ix2 = 0, ix3 = nx - nghost) */
+  const int lo = p->physical ? 0 : g;
   const double hx = p->hx, hy = p->hy, hz = p->hz, epn2 = p->epn2;
   const int K = n - 2 * g;
   double *plane = calloc((size_t)n, sizeof(double));
   for (int v = 0; v < c->V; v++) {
     const double *u = c->field + c->n3 * v;
     double mx = -1.0e30, mn = 1.0e30, vol = 0.0;
-    for (int k = g; k < n - g; k++)
-      for (int j = g; j < n - g; j++)
-        for (int i = g; i < n - g; i++) {
+    for (int k = lo; k < n - g; k++)
+      for (int j = lo; j < n - g; j++)
+        for (int i = lo; i < n - g; i++) {
           double t = u[IDX(n, k, j, i)];
           if (t > mx) mx = t;
           if (t < mn) mn = t;
         }
 #pragma omp parallel for schedule(static)
-    for (int k = g; k < n - g; k++) {
+    for (int k = lo; k < n - g; k++) {
       double s = 0.0;
-      for (int j = g; j < n - g; j++)
-        for (int i = g; i < n - g; i++) s += u[IDX(n, k, j, i)] + 1.0;
+      for (int j = lo; j < n - g; j++)
+        for (int i = lo; i < n - g; i++) s += u[IDX(n, k, j, i)] + 1.0;
       plane[k] = s;
     }
-    for (int k = g; k < n - g; k++) vol += plane[k];
+    for (int k = lo; k < n - g; k++) vol += plane[k];
     vol *= hx * hy * hz;
     stats[4 * v + 0] = mx;
     stats[4 * v + 1] = mn;
@@ -1311,11 +1344,11 @@ void orc_diag(orc_ctx *c, double *stats) {
   /* AC energy (common to all AC fields) */
   if (nac > 0) {
 #pragma omp parallel for schedule(static)
-    for (int k = g; k < n - g - 1; k++) {
+    for (int k = lo; k < n - g - 1; k++) {
       double e = 0.0;
       double *um = malloc(sizeof(double) * nac);
-      for (int j = g; j < n - g - 1; j++)
-        for (int i = g; i < n - g - 1; i++) {
+      for (int j = lo; j < n - g - 1; j++)
+        for (int i = lo; i < n - g - 1; i++) {
           for (int m = 0; m < nac; m++) {
             double ux, uy, uz;
             cell_grad(c->field + c->n3 * m, n, k, j, i, hx, hy, hz, &um[m], &ux, &uy, &uz);
@@ -1333,7 +1366,7 @@ void orc_diag(orc_ctx *c, double *stats) {
       plane[k] = e;
     }
     double eng = 0.0;
-    for (int k = g; k < n - g - 1; k++) eng += plane[k];
+    for (int k = lo; k < n - g - 1; k++) eng += plane[k];
     eng = eng * hx * hy * hz;
     for (int v = 0; v < nac; v++) stats[4 * v + 3] = eng;
   }
@@ -1341,10 +1374,10 @@ void orc_diag(orc_ctx *c, double *stats) {
   for (int q = 0; q < nch; q++) {
     const double *u = c->field + c->n3 * (nac + q);
 #pragma omp parallel for schedule(static)
-    for (int k = g; k < n - g - 1; k++) {
+    for (int k = lo; k < n - g - 1; k++) {
       double e = 0.0;
-      for (int j = g; j < n - g - 1; j++)
-        for (int i = g; i < n - g - 1; i++) {
+      for (int j = lo; j < n - g - 1; j++)
+        for (int i = lo; i < n - g - 1; i++) {
           double cm, ux, uy, uz;
           cell_grad(u, n, k, j, i, hx, hy, hz, &cm, &ux, &uy, &uz);
           e += epn2 * (ux * ux + uy * uy + uz * uz) / 2;
@@ -1353,17 +1386,17 @@ void orc_diag(orc_ctx *c, double *stats) {
       plane[k] = e;
     }
     engtmp = 0.0;
-    for (int k = g; k < n - g - 1; k++) engtmp += plane[k];
+    for (int k = lo; k < n - g - 1; k++) engtmp += plane[k];
     engtmp = engtmp * hx * hy * hz;
     stats[4 * (nac + q) + 3] = engtmp;
   }
   double couple = 0.0;
   if (nac > 0 && nch > 0) {
 #pragma omp parallel for schedule(static)
-    for (int k = g; k < n - g - 1; k++) {
+    for (int k = lo; k < n - g - 1; k++) {
       double e = 0.0;
-      for (int j = g; j < n - g - 1; j++)
-        for (int i = g; i < n - g - 1; i++) {
+      for (int j = lo; j < n - g - 1; j++)
+        for (int i = lo; i < n - g - 1; i++) {
           for (int q = 0; q < nch && q < nac; q++) {
             double cm, ux, uy, uz;
             cell_grad(c->field + c->n3 * q, n, k, j, i, hx, hy, hz, &cm, &ux, &uy, &uz);
@@ -1378,7 +1411,7 @@ void orc_diag(orc_ctx *c, double *stats) {
       plane[k] = e;
     }
     couple = engtmp;
-    for (int k = g; k < n - g - 1; k++) couple += plane[k];
+    for (int k = lo; k < n - g - 1; k++) couple += plane[k];
     couple = couple * hx * hy * hz;
   }
   stats[4 * c->V] = couple;

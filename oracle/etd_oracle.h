@@ -61,6 +61,7 @@ typedef struct orc_params {
   int free_energy;    /* 0: Landau polynomial (omega), 1: CALPHAD Ti-Al-V (alpha) */
   const int *mask;    /* NULL, or n^3 grain mask: N -= 0.5 u where mask == 0 (alpha) */
   int varmob;         /* 1: variable-mobility CH (omega mobility.c; nch == 3, nac >= 1) */
+  int physical;       /* 1: physical boundaries on every side (periodic = 0, one box) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

@@ -28,6 +28,7 @@ class OrcParams(C.Structure):
         ("C", C.c_double * 36),
         ("eps0", _dp), ("lam", _dp), ("V", _dp), ("Vi", _dp), ("D", _dp),
         ("free_energy", C.c_int), ("mask", C.POINTER(C.c_int)), ("varmob", C.c_int),
+        ("physical", C.c_int),
     ]
 
 
@@ -174,7 +175,8 @@ class Oracle:
         self._lam = np.ascontiguousarray(np.asarray(lam, dtype=np.float64).reshape(-1))
         Vm = prm.get("Vmat")
         if Vm is None:
-            Vm, Vim, D = eigbasis(n, 3)
+            # boundary id 3 (neighbours on both sides) for a periodic box, 0 (physical both sides) otherwise
+            Vm, Vim, D = eigbasis(n, 3 if prm.get("periodic", 1) else 0)
         else:
             Vim = prm["Vinv"]
             D = prm["D"]
@@ -188,6 +190,7 @@ class Oracle:
         p.D = _ptr(self._D)
         p.free_energy = prm.get("free_energy", 0)
         p.varmob = prm.get("varmob", 0)
+        p.physical = 0 if prm.get("periodic", 1) else 1
         mask = prm.get("mask")
         if mask is not None:
             self._mask = np.ascontiguousarray(np.asarray(mask, dtype=np.int32).reshape(-1))

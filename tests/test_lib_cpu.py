@@ -116,9 +116,9 @@ def test_read_matrices_file_format(tmp_path):
 
 
 def test_unsupported_configurations_are_rejected():
-    """Multi-subdomain grids and the non-cube / non-periodic cases return
-    ETD_ERR_UNSUPPORTED before any device work."""
-    for field, val in (("px", 2), ("periodic", 0), ("variable_mobility", 1), ("ny", 14)):
+    """Multi-subdomain grids, non-cubes and variable mobility without its three
+    CH fields return ETD_ERR_UNSUPPORTED before any device work."""
+    for field, val in (("px", 2), ("variable_mobility", 1), ("ny", 14)):
         p = pyetd.params_from_dict(dict(n=12, nac=1, nch=0, dt=0.01, epn2=1.0))
         setattr(p, field, val)
         ctx = C.c_void_p()
