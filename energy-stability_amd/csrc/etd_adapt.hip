@@ -696,11 +696,11 @@ extern "C" int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_s
     snprintf(c->err, sizeof(c->err), "adaptive stepping needs a periodic box (its energy is the periodic one)");
     return ETD_ERR_UNSUPPORTED;
   }
-  if (c->p.free_energy != 0 || c->mask || c->p.variable_mobility) {
+  if (c->p.free_energy != 0 || c->mask || c->p.variable_mobility || c->p.elastic == 2) {
     // the acceptance test needs the free energy the step descends; it is the Landau / double-well one
     snprintf(c->err, sizeof(c->err),
              "adaptive stepping accepts on the Landau / double-well energy: not with the CALPHAD driving force, "
-             "a grain mask or variable mobility");
+             "a grain mask, variable mobility or elasticity on the CH fields");
     return ETD_ERR_UNSUPPORTED;
   }
   return c->mixed ? run_adaptive<float>(c, o, st, log, log_cap, log_rows)

@@ -98,7 +98,8 @@ static int validate(etd_ctx *c, const etd_params *p) {
     return seterr(c, ETD_ERR_ARG, "ANISOTROPIC = %d is not 0, 1 or 2", p->anisotropic);
   if (p->free_energy != 0 && p->free_energy != 1)
     return seterr(c, ETD_ERR_ARG, "free_energy = %d is not 0 (Landau) or 1 (CALPHAD)", p->free_energy);
-  if (p->elastic != 0 && p->elastic != 1) return seterr(c, ETD_ERR_UNSUPPORTED, "ELASTIC = %d is not supported", p->elastic);
+  if (p->elastic < 0 || p->elastic > 3) return seterr(c, ETD_ERR_ARG, "ELASTIC = %d is not 0..3", p->elastic);
+  if (p->elastic == 2 && p->nch > 16) return seterr(c, ETD_ERR_UNSUPPORTED, "elasticity supports nch <= 16");
   if (p->nac < 0 || p->nch < 0 || p->nac + p->nch < 1) return seterr(c, ETD_ERR_ARG, "nac = 0 and nch = 0");
   if (p->nac + p->nch > ETD_MAX_VAR) return seterr(c, ETD_ERR_UNSUPPORTED, "too many fields");
   if (p->nghost < 1 || p->nx < 2 * p->nghost + 2) return seterr(c, ETD_ERR_ARG, "nghost / nx");
@@ -242,12 +243,15 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   free(dd);
   HCHK(c, hipMalloc(&c->lam, sizeof(double) * 9 * (V > 0 ? V : 1)));
   HCHK(c, hipMemcpy(c->lam, p->lam, sizeof(double) * 9 * (p->nac > 0 ? p->nac : 1), hipMemcpyHostToDevice));
-  // elasticity
-  if (p->elastic == 1 && p->nac > 0) {
+  // elasticity: ELASTIC = 1 on the AC fields, 2 on the CH fields (elastic.c:689-762); 3 (BOTH_FUNCTION)
+  // matches neither branch of elastic_calculate, so the reference computes no force
+  c->nel = p->elastic == 1 ? p->nac : p->elastic == 2 ? p->nch : 0;
+  c->el_v0 = p->elastic == 2 ? p->nac : 0;
+  if (c->nel > 0) {
     c->fft_npass = fft_plan_radices(c->N, c->fft_radix);
     if (c->fft_npass < 0) return seterr(c, ETD_ERR_UNSUPPORTED, "N = %d has prime factors other than 2, 3, 5", c->N);
-    c->npair = (p->nac + 1) / 2;
-    c->ring = p->nac + 2;
+    c->npair = (c->nel + 1) / 2;
+    c->ring = c->nel + 2;
     if (!c->mixed) {
       HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->N3 * c->npair));
       HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->N3 * c->npair));
@@ -270,7 +274,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     free(tw);
     etd_dev_consts *k = (etd_dev_consts *)calloc(1, sizeof(etd_dev_consts));
     c4_from_voigt(p->C, k->C4);
-    for (int q = 0; q < p->nac; q++) {
+    for (int q = 0; q < c->nel; q++) {
       const double *e = p->eps0 + 9 * q;
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
@@ -280,8 +284,8 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
           k->sig[9 * q + i * 3 + j] = s;
         }
     }
-    for (int a = 0; a < p->nac; a++)
-      for (int b = 0; b < p->nac; b++) {
+    for (int a = 0; a < c->nel; a++)
+      for (int b = 0; b < c->nel; b++) {
         double t0 = 0.0;
         const double *ea = p->eps0 + 9 * a, *eb = p->eps0 + 9 * b;
         for (int i = 0; i < 3; i++)
@@ -303,7 +307,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     HCHK(c, hipMalloc(&c->vm_part, sizeof(double) * (3 * 256 + 3)));
   }
   // the mixed path's CH centring relies on the constant-mobility term's conservation
-  if (c->mixed && c->nch > 0 && !p->variable_mobility && p->periodic) {
+  if (c->mixed && c->nch > 0 && !p->variable_mobility && p->periodic && p->elastic != 2) {
     HCHK(c, hipMalloc(&c->ch_sum, sizeof(double) * c->nch * (1 + CH_BLOCKS)));
     HCHK(c, hipMalloc(&c->ch_const, sizeof(float) * c->nch * c->n3));
   }
@@ -524,7 +528,7 @@ void stage_forward(etd_ctx *c) {
   launch_ghost_fill<T>(c, F.u, V, c->n3);
   // an adaptive acceptance test may already have evaluated felas of exactly
   // these fields (same ghost fill, same kernels): reuse it
-  if (c->p.elastic == 1 && c->nac > 0 && !c->elastic_fresh) launch_elastic<T>(c);
+  if (c->nel > 0 && !c->elastic_fresh) launch_elastic<T>(c);
   c->elastic_fresh = 0;
   launch_nonlinear<T>(c, F.N1, nullptr, 1);  // mixed: CH terms centred, N1 - N(c_mean)
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
@@ -547,7 +551,7 @@ int step_once_t(etd_ctx *c) {
   if (c->p.etd2) {
     if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
     launch_ghost_fill<T>(c, F.u, V, n3);
-    if (c->p.elastic == 1 && c->nac > 0) launch_elastic<T>(c);
+    if (c->nel > 0) launch_elastic<T>(c);
     launch_nonlinear<T>(c, F.N2, F.N1);  // N(a) - N(u_n)
     modeprod3<T>(c, 1, F.N2, F.spec, V, 0);
     launch_phiscale<T>(c, F.spec, (const T *)nullptr, F.spec, 0, V, 1);
@@ -622,7 +626,7 @@ template <typename T>
 static int k_elastic_t(etd_ctx *c, double *host_felas) {
   launch_elastic<T>(c);
   if (host_felas) {
-    const size_t cnt = c->n3 * c->nac;
+    const size_t cnt = c->n3 * c->nel;
     T *d;
     HCHK(c, hipMalloc(&d, sizeof(T) * cnt));
     launch_felas_materialize<T>(c, d);
@@ -643,7 +647,7 @@ static int k_elastic_t(etd_ctx *c, double *host_felas) {
 
 extern "C" int etd_k_elastic(etd_ctx *c, double *host_felas) {
   if (!c) return ETD_ERR_ARG;
-  if (c->p.elastic != 1 || c->nac == 0) return seterr(c, ETD_ERR_ARG, "elasticity is off");
+  if (c->nel == 0) return seterr(c, ETD_ERR_ARG, "elasticity is off");
   return c->mixed ? k_elastic_t<float>(c, host_felas) : k_elastic_t<double>(c, host_felas);
 }
 

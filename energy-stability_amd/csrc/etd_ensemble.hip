@@ -68,13 +68,13 @@ extern "C" int etd_ensemble_create(etd_ctx *const *members, int count, int steps
     for (int j = 0; j < i; j++)
       if (e->m[j] == c) return enerr(e, ETD_ERR_ARG, "a context appears twice");
     // the captured felas ring positions must come back to the same phase
-    if (c->p.elastic == 1 && c->nac > 0) {
+    if (c->nel > 0) {
       const int eps = c->p.etd2 ? 2 : 1, R = c->ring;
-      const int need = R / gcd_i((eps * c->nac) % R == 0 ? R : (eps * c->nac) % R, R);
+      const int need = R / gcd_i((eps * c->nel) % R == 0 ? R : (eps * c->nel) % R, R);
       if (steps % need) {
         char msg[256];
-        snprintf(msg, sizeof msg, "member %d (nac %d, %s): steps per launch must be a multiple of %d", i, c->nac,
-                 c->p.etd2 ? "ETD2" : "ETD1", need);
+        snprintf(msg, sizeof msg, "member %d (%d elastic fields, %s): steps per launch must be a multiple of %d", i,
+                 c->nel, c->p.etd2 ? "ETD2" : "ETD1", need);
         return enerr(e, ETD_ERR_ARG, msg);
       }
       e->evals_per_launch[i] = (long long)steps * eps;
@@ -186,7 +186,7 @@ extern "C" int etd_autotune(const etd_params *p, const double *V, const double *
       for (int f = 0; f < 2 && rc == ETD_OK; f++) {
         etd_tuning t = {mp_cands[a], c->mixed ? sw_cands[b] : 0, fft_cands[f], 0.0};
         if (c->mixed && t.split_waves == 8 && (c->n + 15) / 16 > 14) continue;
-        if (t.fft_variant == 1 && (!fft16_supported(c->N) || !(c->p.elastic == 1 && c->nac > 0))) continue;
+        if (t.fft_variant == 1 && (!fft16_supported(c->N) || !(c->nel > 0))) continue;
         if ((rc = etd_set_tuning(c, &t))) break;
         if ((rc = etd_time_steps(c, 1, 0, ms, nullptr))) break;  // warm: first launches
         if ((rc = etd_time_steps(c, steps, 0, ms, nullptr))) break;

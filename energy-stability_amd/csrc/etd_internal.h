@@ -66,6 +66,9 @@ struct etd_ctx {
   double *tw;              /* N complex twiddles exp(-2 pi i e / N) */
   etd_dev_consts *dconst;
   int npair, ring;
+  /* the fields the elastic force acts on: AC fields (ELASTIC = 1) or CH
+   * fields (ELASTIC = 2), nel of them from field el_v0; nel = 0 without */
+  int nel, el_v0;
   long long elastic_evals; /* completed elastic evaluations */
   int elastic_fresh;       /* the latest evaluation is of the current fields (adaptive energy test) */
   int fft_radix[16], fft_npass;

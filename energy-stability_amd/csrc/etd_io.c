@@ -199,7 +199,10 @@ int etd_parse_elastic(const char *path, etd_params *p) {
   if (!f) return ETD_ERR_IO;
   int ok = fscanf(f, "%lf", &p->elastic_scale) == 1;
   for (int i = 0; i < 36 && ok; i++) ok &= fscanf(f, "%lf", &p->C[i]) == 1;
-  for (int v = 0; v < p->nac && ok; v++)
+  /* one eigenstrain per variant the force acts on: the AC fields, or the CH fields for ELASTIC = 2 */
+  const int nvar = p->elastic == 2 ? p->nch : p->nac;
+  if (nvar > ETD_MAX_VAR) ok = 0;
+  for (int v = 0; v < nvar && ok; v++)
     for (int t = 0; t < 9 && ok; t++) ok &= fscanf(f, "%lf", &p->eps0[9 * v + t]) == 1;
   fclose(f);
   return ok ? ETD_OK : ETD_ERR_IO;

@@ -1019,12 +1019,15 @@ __device__ __forceinline__ double ch_f(double cc) { return cc * (cc * cc - 1.0);
 struct ch_h {
   double x, y, z;  // 1 / h^2
 };
+// E, q: the elastic force ring and the CH field's index in it (ELASTIC = 2: t += LCI felas, calc.c:90-101)
 template <typename T>
-__device__ __forceinline__ double ch_t_at(const T *c, const nl_args &a, const ch_h &ih, int k, int j, int i) {
+__device__ __forceinline__ double ch_t_at(const T *c, const T *E, int q, const nl_args &a, const ch_h &ih, int k, int j,
+                                          int i) {
   const int n = a.n, g = a.g;
   const double cc = c[IDX(n, k, j, i)];
   double f = -a.LCI * a.KCI * cc;
   f += a.LCI * ch_f(cc);
+  if (a.elastic == 2) f += a.LCI * felas_at(E, a, q, k, j, i);
   const double ce = a.LCI * a.epn2;
   if (a.phys) return f;  // no outside values on physical sides
   if (i == 0) f -= ce * Pimg(c, n, g, k, j, -1) * ih.x;
@@ -1039,7 +1042,7 @@ __device__ __forceinline__ double ch_t_at(const T *c, const nl_args &a, const ch
 // chemical potential LCI (epn2 Lap c - CH_f(c)) at an outside cell, from the
 // periodic image (ch_mu, trans.c:1433-1937)
 template <typename T>
-__device__ double ch_mu_out(const T *c, const nl_args &a, const ch_h &ih, int z, int y, int x) {
+__device__ double ch_mu_out(const T *c, const T *E, int q, const nl_args &a, const ch_h &ih, int z, int y, int x) {
   const int n = a.n, g = a.g;
   const double v0 = Pimg(c, n, g, z, y, x);
   double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) * ih.x;
@@ -1047,6 +1050,12 @@ __device__ double ch_mu_out(const T *c, const nl_args &a, const ch_h &ih, int z,
   mu += (Pimg(c, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(c, n, g, z + 1, y, x)) * ih.z;
   mu = a.LCI * a.epn2 * mu;
   mu -= a.LCI * ch_f(v0);
+  if (a.elastic == 2) {
+    // ELASTIC = 2 (SWITCH_CH_FIELDMU, ScLETD.h:153-156): the force of the latest evaluation at the interior image
+    const int N = a.N;
+    const size_t off = ((size_t)(wrap_e(z, n, g) - g) * N + (wrap_e(y, n, g) - g)) * N + (wrap_e(x, n, g) - g);
+    mu -= a.LCI * (double)E[(size_t)felas_slot(a, q, 0) * N * N * N + off];
+  }
   return mu;
 }
 
@@ -1091,7 +1100,8 @@ __host__ __device__ __forceinline__ long shell_count(int n) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__restrict__ out, const T *__restrict__ sub,
-                                               const float *__restrict__ nc, nl_args a, unsigned nbi8) {
+                                               const float *__restrict__ nc, const T *__restrict__ E, int q, nl_args a,
+                                               unsigned nbi8) {
   const int n = a.n;
   const ch_h ih = {a.ihx2, a.ihy2, a.ihz2};
   const unsigned b = blockIdx.x;
@@ -1103,11 +1113,20 @@ __global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__res
     int i, j, k;
     idx3(li, NI, i, j, k);
     const size_t dz = (size_t)n * n, l = IDX(n, k + 2, j + 2, i + 2);
-    auto tp = [&](double cc) { return -a.LCI * a.KCI * cc + a.LCI * ch_f(cc); };
-    const double tc = tp(c[l]);
-    double r = (tp(c[l - 1]) + tp(c[l + 1]) - 2.0 * tc) * ih.x;
-    r += (tp(c[l - n]) + tp(c[l + n]) - 2.0 * tc) * ih.y;
-    r += (tp(c[l - dz]) + tp(c[l + dz]) - 2.0 * tc) * ih.z;
+    double tc, r;
+    if (a.elastic == 2) {  // t carries LCI felas (calc.c:90-101)
+      auto te = [&](int dk, int dj, int di) { return ch_t_at(c, E, q, a, ih, k + 2 + dk, j + 2 + dj, i + 2 + di); };
+      tc = te(0, 0, 0);
+      r = (te(0, 0, -1) + te(0, 0, 1) - 2.0 * tc) * ih.x;
+      r += (te(0, -1, 0) + te(0, 1, 0) - 2.0 * tc) * ih.y;
+      r += (te(-1, 0, 0) + te(1, 0, 0) - 2.0 * tc) * ih.z;
+    } else {
+      auto tp = [&](double cc) { return -a.LCI * a.KCI * cc + a.LCI * ch_f(cc); };
+      tc = tp(c[l]);
+      r = (tp(c[l - 1]) + tp(c[l + 1]) - 2.0 * tc) * ih.x;
+      r += (tp(c[l - n]) + tp(c[l + n]) - 2.0 * tc) * ih.y;
+      r += (tp(c[l - dz]) + tp(c[l + dz]) - 2.0 * tc) * ih.z;
+    }
     if (sub) r -= sub[l];
     if (nc) r -= nc[l];
     out[l] = (T)r;
@@ -1118,14 +1137,14 @@ __global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__res
   int i, j, k;
   shell_point(s, n, i, j, k);
   const size_t l = IDX(n, k, j, i);
-  const double tc = ch_t_at(c, a, ih, k, j, i);
-  const double f_right = i < n - 1 ? ch_t_at(c, a, ih, k, j, i + 1) : 0.0;
-  const double f_bottom = j < n - 1 ? ch_t_at(c, a, ih, k, j + 1, i) : 0.0;
-  const double f_back = k < n - 1 ? ch_t_at(c, a, ih, k + 1, j, i) : 0.0;
+  const double tc = ch_t_at(c, E, q, a, ih, k, j, i);
+  const double f_right = i < n - 1 ? ch_t_at(c, E, q, a, ih, k, j, i + 1) : 0.0;
+  const double f_bottom = j < n - 1 ? ch_t_at(c, E, q, a, ih, k, j + 1, i) : 0.0;
+  const double f_back = k < n - 1 ? ch_t_at(c, E, q, a, ih, k + 1, j, i) : 0.0;
   // low neighbour of cell 0: 0 beside a neighbour, the mirror (cell 1) on a physical side
-  const double f_left = i > 0 ? ch_t_at(c, a, ih, k, j, i - 1) : (a.phys ? f_right : 0.0);
-  const double f_top = j > 0 ? ch_t_at(c, a, ih, k, j - 1, i) : (a.phys ? f_bottom : 0.0);
-  const double f_front = k > 0 ? ch_t_at(c, a, ih, k - 1, j, i) : (a.phys ? f_back : 0.0);
+  const double f_left = i > 0 ? ch_t_at(c, E, q, a, ih, k, j, i - 1) : (a.phys ? f_right : 0.0);
+  const double f_top = j > 0 ? ch_t_at(c, E, q, a, ih, k, j - 1, i) : (a.phys ? f_bottom : 0.0);
+  const double f_front = k > 0 ? ch_t_at(c, E, q, a, ih, k - 1, j, i) : (a.phys ? f_back : 0.0);
   double r = (f_left + f_right - 2.0 * tc) * ih.x;
   r += (f_top + f_bottom - 2.0 * tc) * ih.y;
   r += (f_front + f_back - 2.0 * tc) * ih.z;
@@ -1135,12 +1154,12 @@ __global__ __launch_bounds__(256) void k_ch_nl(const T *__restrict__ c, T *__res
     out[l] = (T)r;
     return;
   }
-  if (i == 0) r -= ch_mu_out(c, a, ih, k, j, -1) * ih.x;
-  if (i == n - 1) r -= ch_mu_out(c, a, ih, k, j, n) * ih.x;
-  if (j == 0) r -= ch_mu_out(c, a, ih, k, -1, i) * ih.y;
-  if (j == n - 1) r -= ch_mu_out(c, a, ih, k, n, i) * ih.y;
-  if (k == 0) r -= ch_mu_out(c, a, ih, -1, j, i) * ih.z;
-  if (k == n - 1) r -= ch_mu_out(c, a, ih, n, j, i) * ih.z;
+  if (i == 0) r -= ch_mu_out(c, E, q, a, ih, k, j, -1) * ih.x;
+  if (i == n - 1) r -= ch_mu_out(c, E, q, a, ih, k, j, n) * ih.x;
+  if (j == 0) r -= ch_mu_out(c, E, q, a, ih, k, -1, i) * ih.y;
+  if (j == n - 1) r -= ch_mu_out(c, E, q, a, ih, k, n, i) * ih.y;
+  if (k == 0) r -= ch_mu_out(c, E, q, a, ih, -1, j, i) * ih.z;
+  if (k == n - 1) r -= ch_mu_out(c, E, q, a, ih, n, j, i) * ih.z;
   if (sub) r -= sub[l];
   if (nc) r -= nc[l];
   out[l] = (T)r;
@@ -1166,7 +1185,7 @@ static nl_args make_nl(etd_ctx *c) {
   a.aniso = c->p.anisotropic;
   a.elastic = c->p.elastic;
   a.ring = c->ring > 0 ? c->ring : 1;
-  a.Lbase = (c->elastic_evals - 1) * (long long)c->nac;
+  a.Lbase = (c->elastic_evals - 1) * (long long)c->nel;
   a.slot0 = (int)(((a.Lbase % a.ring) + a.ring) % a.ring);
   a.LE = c->p.LE;
   a.KE = c->p.KE;
@@ -1291,7 +1310,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_vm_out(const T *__restrict__ c, const T *__restrict__ eta,
                                                 const T *__restrict__ ft1, const T *__restrict__ ft2,
                                                 T *__restrict__ out, const T *__restrict__ sub,
-                                                const double *__restrict__ cmax, vm_args m, int q, nl_args a) {
+                                                const double *__restrict__ cmax, vm_args m, int q, nl_args a,
+                                                const T *__restrict__ E) {
   const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = xcd_block() * blockDim.x + threadIdx.x;
@@ -1302,12 +1322,12 @@ __global__ __launch_bounds__(256) void k_vm_out(const T *__restrict__ c, const T
   const ch_h ih = {a.ihx2, a.ihy2, a.ihz2};
   double f1 = -C * lap_zero_at(ft1, n, l, k, j, i, a);
   if (!a.phys) {
-    if (i == 0) f1 -= C * ch_mu_out(c, a, ih, k, j, -1) * ih.x;
-    if (i == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, j, n) * ih.x;
-    if (j == 0) f1 -= C * ch_mu_out(c, a, ih, k, -1, i) * ih.y;
-    if (j == n - 1) f1 -= C * ch_mu_out(c, a, ih, k, n, i) * ih.y;
-    if (k == 0) f1 -= C * ch_mu_out(c, a, ih, -1, j, i) * ih.z;
-    if (k == n - 1) f1 -= C * ch_mu_out(c, a, ih, n, j, i) * ih.z;
+    if (i == 0) f1 -= C * ch_mu_out(c, E, q, a, ih, k, j, -1) * ih.x;
+    if (i == n - 1) f1 -= C * ch_mu_out(c, E, q, a, ih, k, j, n) * ih.x;
+    if (j == 0) f1 -= C * ch_mu_out(c, E, q, a, ih, k, -1, i) * ih.y;
+    if (j == n - 1) f1 -= C * ch_mu_out(c, E, q, a, ih, k, n, i) * ih.y;
+    if (k == 0) f1 -= C * ch_mu_out(c, E, q, a, ih, -1, j, i) * ih.z;
+    if (k == n - 1) f1 -= C * ch_mu_out(c, E, q, a, ih, n, j, i) * ih.z;
   }
   // div((M - C) grad ft2), mobility.c:435-722: (m_nb + m_mid)(a_nb - a_mid) / 2h^2, 0 outside the box
   const size_t dz = (size_t)n * n;
@@ -1342,16 +1362,16 @@ static void launch_vm(etd_ctx *c, const T *u, T *out, const T *sub, T *scratch, 
     k_vm_ft<T><<<nb, 256, 0, c->stream>>>(cf, scratch, scratch + n3, a);
     k_vm_out<T><<<nb, 256, 0, c->stream>>>(cf, u, scratch, scratch + n3, out + n3 * (c->nac + q),
                                            sub ? sub + n3 * (c->nac + q) : nullptr, c->vm_part + 3 * VM_BLOCKS, m,
-                                           q, a);
+                                           q, a, fields<T>(c).E);
   }
 }
 
 template <typename T>
-static void launch_ch_nl(etd_ctx *c, const T *cf, T *out, const T *sub, const float *nc, const nl_args &a) {
+static void launch_ch_nl(etd_ctx *c, const T *cf, T *out, const T *sub, const float *nc, int q, const nl_args &a) {
   const long NI = c->n - 4;
   const unsigned nbi8 = round8((size_t)((NI * NI * NI + 255) / 256));
   const unsigned nbs = (unsigned)((shell_count(c->n) + 255) / 256);
-  k_ch_nl<T><<<nbi8 + nbs, 256, 0, c->stream>>>(cf, out, sub, nc, a, nbi8);
+  k_ch_nl<T><<<nbi8 + nbs, 256, 0, c->stream>>>(cf, out, sub, nc, fields<T>(c).E, q, a, nbi8);
 }
 
 // N for every variable into out (V x n3); when sub != NULL the result is
@@ -1400,7 +1420,7 @@ void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre) {
     for (int q = 0; q < c->nch; q++) {
       const int v = c->nac + q;
       const float *nc = ch_centre && sizeof(T) == 4 && c->ch_const ? c->ch_const + n3 * q : nullptr;
-      launch_ch_nl<T>(c, F.u + n3 * v, out + n3 * v, sub ? sub + n3 * v : nullptr, nc, a);
+      launch_ch_nl<T>(c, F.u + n3 * v, out + n3 * v, sub ? sub + n3 * v : nullptr, nc, q, a);
     }
   etd_prof_end(c, PC_NONLIN);
 }
@@ -2079,14 +2099,15 @@ void launch_elastic(etd_ctx *c) {
   const size_t N3 = c->N3;
   const unsigned nb = (unsigned)((N3 + 255) / 256);
   etd_prof_begin(c, PC_ELASTIC);
-  const long long Lbase = c->elastic_evals * (long long)c->nac;
+  const long long Lbase = c->elastic_evals * (long long)c->nel;
+  const T *uel = F.u + c->n3 * c->el_v0;  // the fields the force acts on (AC or CH)
   const int slot0 = (int)(Lbase % c->ring);
   // two-stage FFT: the pack is the forward x pass's input and the unpack the
   // inverse z pass's output
   // (the fused passes exist only where the two-stage kernel does: N = 16 R2
   // with an instantiated R2; elsewhere pack, radix passes and unpack)
   const bool fused = c->fft_variant == 1 && fft16_supported(N);
-  fft_io io{F.u, F.E, c->n, c->g, c->nac, c->ring, slot0};
+  fft_io io{uel, F.E, c->n, c->g, c->nel, c->ring, slot0};
   if (fused) {
     etd_prof_begin(c, PC_FFT);
     launch_fft16<1, T2, 1>(c, (T2 *)F.Z, c->npair, 0, io);
@@ -2094,7 +2115,7 @@ void launch_elastic(etd_ctx *c) {
     launch_fft16<1, T2>(c, (T2 *)F.Z, c->npair, 2);
     etd_prof_end(c, PC_FFT);
   } else {
-    k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(F.u, (T2 *)F.Z, c->n, c->g, N, c->nac);
+    k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(uel, (T2 *)F.Z, c->n, c->g, N, c->nel);
     fft3d<T>(c, F.Z, c->npair, +1);
   }
   {
@@ -2103,7 +2124,7 @@ void launch_elastic(etd_ctx *c) {
     const unsigned nbh = (unsigned)(((size_t)N * N * (N / 2 + 1) + 255) / 256);
     const T2 *Zc = (const T2 *)F.Z;
     T2 *Yc = (T2 *)F.Y;
-    switch (c->nac) {
+    switch (c->nel) {
 #define NACC(X)                                                                                    case X:                                                                                            k_el_contract_t<X, T2><<<nbh, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);                  k_el_contract_mirror<X, T2><<<nbm, 256, 0, c->stream>>>(Zc, Yc, N, c->npair, c->dconst);            break;
       NACC(1) NACC(2) NACC(3) NACC(4) NACC(5) NACC(6) NACC(7) NACC(8) NACC(9) NACC(10) NACC(11) NACC(12) NACC(13)
       NACC(14) NACC(15) NACC(16)
@@ -2119,7 +2140,7 @@ void launch_elastic(etd_ctx *c) {
     etd_prof_end(c, PC_FFT);
   } else {
     fft3d<T>(c, F.Y, c->npair, -1);
-    k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nac, c->ring, slot0);
+    k_el_unpack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>((const T2 *)F.Y, F.E, N, c->nel, c->ring, slot0);
   }
   c->elastic_evals++;
   etd_prof_end(c, PC_ELASTIC);
@@ -2140,7 +2161,7 @@ __global__ void k_felas_mat(T *felas, const T *E, nl_args a) {
 template <typename T>
 void launch_felas_materialize(etd_ctx *c, T *felas) {
   nl_args a = make_nl(c);
-  k_felas_mat<T><<<dim3((unsigned)((c->n3 + 255) / 256), c->nac), 256, 0, c->stream>>>(felas, fields<T>(c).E, a);
+  k_felas_mat<T><<<dim3((unsigned)((c->n3 + 255) / 256), c->nel), 256, 0, c->stream>>>(felas, fields<T>(c).E, a);
 }
 
 // ------------------------------------------------------------------ energy
@@ -2712,7 +2733,7 @@ void ch_mean_mark(etd_ctx *c, const float *u) {
   float *uni = c->f32.w1;
   for (int q = 0; q < c->nch; q++) {
     k_ch_fill<<<nb, 256, 0, c->stream>>>(uni, c->n3, c->ch_sum, q, invN3);
-    launch_ch_nl<float>(c, uni, c->ch_const + c->n3 * q, nullptr, nullptr, a);
+    launch_ch_nl<float>(c, uni, c->ch_const + c->n3 * q, nullptr, nullptr, q, a);
   }
   c->ch_stale = 0;
 }

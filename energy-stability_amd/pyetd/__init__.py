@@ -373,10 +373,12 @@ def params_from_dict(prm):
     for i in range(36):
         p.C[i] = Cv[i]
     nac = p.nac
-    if nac:
-        eps = np.asarray(prm.get("eps0", np.zeros((nac, 3, 3))), dtype=np.float64).reshape(-1)[: 9 * nac]
+    nel = p.nch if p.elastic == 2 else nac  # the variants the elastic force acts on
+    if nel:
+        eps = np.asarray(prm.get("eps0", np.zeros((nel, 3, 3))), dtype=np.float64).reshape(-1)[: 9 * nel]
         for i, x in enumerate(eps):
             p.eps0[i] = x
+    if nac:
         lam = prm.get("lam")
         if lam is None:
             lam = np.array([np.diag(kk)] * nac)
@@ -482,10 +484,12 @@ class Etd:
         _check(lib().etd_k_transfer(self.ctx), self.ctx, "transfer")
 
     def elastic(self, want=True):
+        """etd_k_elastic; the force on the AC fields (ELASTIC = 1) or the CH fields (2)."""
         if want:
-            out = np.empty(self.p.nac * self.n ** 3)
+            nel = self.p.nac if self.p.elastic == 1 else self.p.nch if self.p.elastic == 2 else 0
+            out = np.empty(max(nel, 1) * self.n ** 3)
             _check(lib().etd_k_elastic(self.ctx, _ptr(out)), self.ctx, "elastic")
-            return out.reshape(self.p.nac, self.n, self.n, self.n)
+            return out[: nel * self.n ** 3].reshape(nel, self.n, self.n, self.n)
         _check(lib().etd_k_elastic(self.ctx, None), self.ctx, "elastic")
         return None
 

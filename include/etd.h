@@ -57,13 +57,16 @@ typedef struct etd_params {
   char data_dir[1024];
   double dt, t_total;
   double epn2;
-  int periodic;            /* must be 1 */
+  int periodic;            /* 1; or 0: physical boundaries on every side of the one box (boundary
+                              id 0 basis, h = L / nx, no ghost fill or outside values) */
   int nac, nch;
   int restart, restart_iter;
   char restart_dir[1024];
   double xmin, ymin, zmin, xmax, ymax, zmax;
   int etd2;
-  int elastic;             /* 0 none, 1 AC (2 CH / 3 both: unsupported) */
+  int elastic;             /* 0 none; 1 on the AC fields; 2 on the CH fields (t += LCI felas, and
+                              -LCI felas in the outside chemical potential); 3 (BOTH_FUNCTION)
+                              matches no branch of elastic_calculate: no force, as in the reference */
   int anisotropic;         /* 0 none; 1 F2 = div(Lambda grad u), omega (anisotropic.c:164-627);
                               2 the compact Lambda_ij d_i d_j stencil of the beta->alpha driver
                               (anisotropic_hip.c:528-647) */
@@ -195,10 +198,12 @@ int etd_write_field(etd_ctx *c, int var, int ioutput);
 
 /* ---- per-kernel entry points (unit parity; host buffers in/out) ---- */
 
-/* transfer() (trans.c:1939-1996): periodic ghost fill of every field. */
+/* transfer() (trans.c:1939-1996): periodic ghost fill of every field (nothing
+ * on a box with physical sides). */
 int etd_k_transfer(etd_ctx *c);
-/* elastic_calculate (elastic.c:689-762); felas_p copied to host_felas
- * (nac * n^3 doubles) when non-NULL. */
+/* elastic_calculate (elastic.c:689-762); the force of every field it acts on
+ * copied to host_felas when non-NULL: nac * n^3 doubles for ELASTIC = 1 (AC
+ * fields), nch * n^3 for ELASTIC = 2 (CH fields). */
 int etd_k_elastic(etd_ctx *c, double *host_felas);
 /* The 3D FFT of the elastic path (replaces fft_forward / fft_backward,
  * fft3d.c:98-138) on N^3 interleaved complex values, N = n - 2 nghost, in

@@ -36,6 +36,7 @@
 #include "etd_oracle.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -58,6 +59,12 @@ struct orc_ctx {
   double *t0, *t1, *t2, *t3;  /* n3 scratch */
   int *mask;             /* copy of p.mask (or NULL) */
 };
+
+/* the fields the elastic force acts on (elastic.c:689-762): the AC fields for
+ * ELASTIC = 1 (AC_FUNCTION), the CH fields for 2 (CH_FUNCTION); 3
+ * (BOTH_FUNCTION) matches neither branch, so no force is computed */
+static int el_count(const orc_params *p) { return p->elastic == 1 ? p->nac : p->elastic == 2 ? p->nch : 0; }
+static int el_first(const orc_params *p) { return p->elastic == 2 ? p->nac : 0; }
 
 int orc_threads(void) {
 #ifdef _OPENMP
@@ -361,12 +368,16 @@ static inline double ch_f(double cc) { return orc_mutation == 1 ? -cc * (cc * cc
 static double ch_mu_out(const orc_ctx *c, const double *u, int z, int y, int x) {
   const orc_params *p = &c->p;
   const int n = p->n, g = p->g;
+  const int q = (int)((u - c->field) / (ptrdiff_t)c->n3) - p->nac;  /* the CH field u is */
   double v0 = Pimg(u, n, g, z, y, x);
   double mu = (Pimg(u, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(u, n, g, z, y, x + 1)) / p->hx / p->hx;
   mu += (Pimg(u, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(u, n, g, z, y + 1, x)) / p->hy / p->hy;
   mu += (Pimg(u, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(u, n, g, z + 1, y, x)) / p->hz / p->hz;
   mu = p->LCI * p->epn2 * mu;
   mu -= p->LCI * ch_f(v0);
+  /* ELASTIC = 2: minus LCI felas at the outside cell (SWITCH_CH_FIELDMU, ScLETD.h:153-156), from
+   * the enlarged slabs of the force's own exchange (elastic.c:757-759): its interior image */
+  if (p->elastic == 2) mu -= p->LCI * Pimg(c->felas + c->n3 * q, n, g, z, y, x);
   return mu;
 }
 
@@ -377,11 +388,13 @@ static void ch_nonlinear(orc_ctx *c, int v, double *out) {
   const double hx = p->hx, hy = p->hy, hz = p->hz;
   const double *u = c->field + c->n3 * v;
   double *t = c->t0;
+  const double *fe = p->elastic == 2 ? c->felas + c->n3 * (v - p->nac) : NULL;
 #pragma omp parallel for schedule(static)
   for (size_t l = 0; l < c->n3; l++) {
     double cc = u[l];
     double f = -LCI * KCI * cc;
     f += LCI * ch_f(cc);
+    if (fe) f += LCI * fe[l];  /* ELASTIC == CH_FUNCTION (calc.c:90-101) */
     t[l] = f;
   }
   const int ph = p->physical;
@@ -902,7 +915,7 @@ typedef struct {
 
 static void bfast_init(const orc_params *p, bfast *b) {
   c4_from_voigt(p->C, b->C4);
-  for (int q = 0; q < p->nac; q++) {
+  for (int q = 0; q < el_count(p); q++) {
     const double *e = p->eps0 + 9 * q;
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) {
@@ -912,8 +925,8 @@ static void bfast_init(const orc_params *p, bfast *b) {
         b->sig[9 * q + i * 3 + j] = s;
       }
   }
-  for (int pp = 0; pp < p->nac; pp++)
-    for (int qq = 0; qq < p->nac; qq++) {
+  for (int pp = 0; pp < el_count(p); pp++)
+    for (int qq = 0; qq < el_count(p); qq++) {
       double t0 = 0.0;
       const double *ep = p->eps0 + 9 * pp, *eq = p->eps0 + 9 * qq;
       for (int i = 0; i < 3; i++)
@@ -948,7 +961,7 @@ static void bfast_row(const orc_params *p, const bfast *b, int N, int gx, int gy
   double a[3], w[3];
   for (int j = 0; j < 3; j++) a[j] = nv[0] * sp[0 * 3 + j] + nv[1] * sp[1 * 3 + j] + nv[2] * sp[2 * 3 + j];
   for (int k = 0; k < 3; k++) w[k] = a[0] * om[0 * 3 + k] + a[1] * om[1 * 3 + k] + a[2] * om[2 * 3 + k];
-  for (int qq = 0; qq < p->nac; qq++) {
+  for (int qq = 0; qq < el_count(p); qq++) {
     const double *sq = b->sig + 9 * qq;
     double t1 = 0.0;
     for (int k = 0; k < 3; k++) t1 += w[k] * (sq[k * 3 + 0] * nv[0] + sq[k * 3 + 1] * nv[1] + sq[k * 3 + 2] * nv[2]);
@@ -958,8 +971,8 @@ static void bfast_row(const orc_params *p, const bfast *b, int N, int gx, int gy
 
 void orc_elastic(orc_ctx *c) {
   const orc_params *p = &c->p;
-  const int n = p->n, g = p->g, N = n - 2 * g, nac = p->nac;
-  if (p->elastic != 1 || nac == 0 || nac > 64) return;
+  const int n = p->n, g = p->g, N = n - 2 * g, nac = el_count(p), v0 = el_first(p);
+  if (nac == 0 || nac > 64) return;
   size_t N3 = (size_t)N * N * N;
   double *spec = malloc(sizeof(double) * 2 * N3 * nac);
   double *acc = malloc(sizeof(double) * 2 * N3);
@@ -967,7 +980,7 @@ void orc_elastic(orc_ctx *c) {
   bfast_init(p, bf);
   for (int q = 0; q < nac; q++) {
     double *s = spec + 2 * N3 * q;
-    const double *u = c->field + c->n3 * q;
+    const double *u = c->field + c->n3 * (v0 + q);
     for (int z = 0; z < N; z++)
       for (int y = 0; y < N; y++)
         for (int x = 0; x < N; x++) {
@@ -1138,9 +1151,10 @@ orc_ctx *orc_create(const orc_params *p) {
   memcpy(c->Vm, p->V, sizeof(double) * nn);
   memcpy(c->Vim, p->Vi, sizeof(double) * nn);
   memcpy(c->D, p->D, sizeof(double) * n);
-  c->eps0 = calloc((size_t)9 * (p->nac > 0 ? p->nac : 1), sizeof(double));
+  const int neps = el_count(p) > p->nac ? el_count(p) : p->nac;
+  c->eps0 = calloc((size_t)9 * (neps > 0 ? neps : 1), sizeof(double));
   c->lam = calloc((size_t)9 * (p->nac > 0 ? p->nac : 1), sizeof(double));
-  if (p->eps0) memcpy(c->eps0, p->eps0, sizeof(double) * 9 * p->nac);
+  if (p->eps0) memcpy(c->eps0, p->eps0, sizeof(double) * 9 * neps);
   if (p->lam) memcpy(c->lam, p->lam, sizeof(double) * 9 * p->nac);
   c->p.eps0 = c->eps0;
   c->p.lam = c->lam;
@@ -1167,7 +1181,7 @@ orc_ctx *orc_create(const orc_params *p) {
   c->field = calloc(c->n3 * c->V, sizeof(double));
   c->N1 = calloc(c->n3 * c->V, sizeof(double));
   c->N2 = calloc(c->n3 * c->V, sizeof(double));
-  c->felas = calloc(c->n3 * (p->nac > 0 ? p->nac : 1), sizeof(double));
+  c->felas = calloc(c->n3 * (el_count(p) > 0 ? el_count(p) : 1), sizeof(double));
   c->fbuf = calloc(c->n3, sizeof(double));
   c->phi1 = calloc(c->n3 * 2, sizeof(double));
   c->phi2 = calloc(c->n3 * 2, sizeof(double));
@@ -1215,7 +1229,7 @@ void orc_step(orc_ctx *c) {
   double *uh = malloc(sizeof(double) * n3), *nh = malloc(sizeof(double) * n3);
   double *w1 = malloc(sizeof(double) * n3), *w2 = malloc(sizeof(double) * n3);
   orc_transfer(c);
-  if (p->elastic == 1) orc_elastic(c);
+  if (el_count(p) > 0) orc_elastic(c);
   for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N1 + n3 * v);
   for (int v = 0; v < c->V; v++) {
     double *u = c->field + n3 * v;
@@ -1226,7 +1240,7 @@ void orc_step(orc_ctx *c) {
   }
   if (p->etd2) {
     orc_transfer(c);
-    if (p->elastic == 1) orc_elastic(c);
+    if (el_count(p) > 0) orc_elastic(c);
     for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N2 + n3 * v);
     for (int v = 0; v < c->V; v++) {
       double *dn = c->N2 + n3 * v;
@@ -1261,7 +1275,7 @@ void orc_step_pair(orc_ctx *c, double dt, double *ufull) {
   double *u0h = malloc(sizeof(double) * n3 * c->V), *n0h = malloc(sizeof(double) * n3 * c->V);
   double *w1 = malloc(sizeof(double) * n3), *w2 = malloc(sizeof(double) * n3);
   orc_transfer(c);
-  if (c->p.elastic == 1) orc_elastic(c);
+  if (el_count(&c->p) > 0) orc_elastic(c);
   for (int v = 0; v < c->V; v++) orc_nonlinear(c, v, c->N1 + n3 * v);
   for (int v = 0; v < c->V; v++) {
     fwd3(c, c->field + n3 * v, u0h + n3 * v, w1, w2);
