@@ -158,6 +158,12 @@ int etd_get_field(etd_ctx *c, int var, double *host);
  * nonlinear term gets -0.5 u where mask == 0.  NULL removes it. */
 int etd_set_mask(etd_ctx *c, const int *mask);
 
+/* The Gibbs energies that free_energy = 1 drives with: bcc and hcp Ti-Al-V at
+ * the beta->alpha driver's fixed compositions and T = 1093 K (Free_energy,
+ * HCP_energy, BCC_energy: source_beta_to_alpha/src/vars.c:144-299; used at
+ * anisotropic_hip.c:2029-2048).  Host only, no context needed. */
+void etd_calphad(double *gbcc, double *ghcp);
+
 /* Seeded initial state: interior U[mean-amp, mean+amp) from splitmix64
  * ((x >> 11) * 2^-53), ghosts as periodic images.  Replaces init_field
  * (field.c:742-777) for the synthetic workloads. */
