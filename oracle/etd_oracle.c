@@ -22,6 +22,13 @@
  *   orc_step (ETD1)  pinned by dense expm(dt*L) at n = 6 with N == 0.
  *   orc_step (ETD2)  pinned by order-2 convergence.
  *   orc_diag         pinned by hand-computed 2^3 / 3^3 grids.
+ *   orc_group_*      decomposed runs (procs > 1): a group of one box is the
+ *                    single-box oracle bit for bit; check_soln_new of the
+ *                    reference's own input64.ini start (2x2x2 boxes) equals
+ *                    the first line of its shipped energy trace; face ghosts
+ *                    and outside values are global periodic images; the
+ *                    force is that of one box holding the global interior
+ *                    (tests/test_oracle_decomposed.py).
  *   CH (ch_nonlinear, h_ch)
  *                    the reference's CH_f is broken (formula.c:40-56); the
  *                    double well W = (c^2-1)^2/4 of field.c:575 / trans.c:993
@@ -38,6 +45,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -58,6 +66,20 @@ struct orc_ctx {
   double *phi1, *phi2;   /* (nac? 1:0 + nch? 1:0) tables, zxy layout */
   double *t0, *t1, *t2, *t3;  /* n3 scratch */
   int *mask;             /* copy of p.mask (or NULL) */
+  int nfel;              /* felas arrays allocated */
+  /* decomposed run (orc_group): the boxes this one exchanges with, and its
+   * coordinates in the process grid (NULL / 0 for a single box) */
+  const struct orc_group *grp;
+  int bx, by, bz;
+};
+
+/* A decomposed run on one periodic domain: px x py x pz boxes of the same
+ * extended edge n, box (bx, by, bz) at index (bz py + by) px + bx owning the
+ * global interior cells [bx N, (bx + 1) N) x ... (ω's Cartesian ranks,
+ * mpi.c / ScLETD.c:20-47, one box per rank). */
+struct orc_group {
+  int px, py, pz;
+  orc_ctx **box;
 };
 
 /* the fields the elastic force acts on (elastic.c:689-762): the AC fields for
@@ -222,8 +244,33 @@ void orc_transfer(orc_ctx *c) {
  * CH chemical potential at the outside cells (ch_mu, trans.c:1433-1937) are
  * read from these slabs.  Interior cells are untouched by the ghost fill, so
  * P can be read from the post-fill field. */
-static inline double Pimg(const double *u, int n, int g, int z, int y, int x) {
-  return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+static inline int box_shift(int e, int n, int g) { return e < g ? -1 : (e >= n - g ? 1 : 0); }
+static inline const orc_ctx *nb_box(const orc_ctx *c, int sx, int sy, int sz) {
+  const struct orc_group *G = c->grp;
+  if (!G) return c;
+  int x = (c->bx + sx + G->px) % G->px, y = (c->by + sy + G->py) % G->py, z = (c->bz + sz + G->pz) % G->pz;
+  return G->box[((size_t)z * G->py + y) * G->px + x];
+}
+/* the same array of box b as u is of box c (fields, felas, the elastic
+ * transfer buffer) */
+static const double *same_array(const orc_ctx *c, const orc_ctx *b, const double *u) {
+  if (b == c) return u;
+  uintptr_t a = (uintptr_t)u;
+  size_t nf = (size_t)c->n3 * c->V, ne = (size_t)c->n3 * c->nfel;
+  if (a >= (uintptr_t)c->field && a < (uintptr_t)(c->field + nf)) return b->field + (u - c->field);
+  if (a >= (uintptr_t)c->felas && a < (uintptr_t)(c->felas + ne)) return b->felas + (u - c->felas);
+  if (a >= (uintptr_t)c->fbuf && a < (uintptr_t)(c->fbuf + c->n3)) return b->fbuf + (u - c->fbuf);
+  return u;
+}
+/* P(z, y, x) for -2 <= z, y, x < n + 2: in a decomposed run the interior
+ * image lies in the neighbouring box the position points into (a corner
+ * image in the diagonal box), as the relayed slabs of left_right_pack /
+ * top_bottom_pack carry it (trans.c:7-152, 253-925). */
+static inline double PimgC(const orc_ctx *c, const double *u, int z, int y, int x) {
+  const int n = c->p.n, g = c->p.g;
+  const double *src = u;
+  if (c->grp) src = same_array(c, nb_box(c, box_shift(x, n, g), box_shift(y, n, g), box_shift(z, n, g)), u);
+  return src[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
 }
 
 /* Neighbour rule of omega's stencils (anisotropic.c:177-216, calc.c, mobility.c
@@ -290,7 +337,7 @@ static double calphad_f(const orc_ctx *c, size_t l, int pv, double gbcc, double 
 static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2);
 static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
   const orc_params *p = &c->p;
-  const int n = p->n, g = p->g, nac = p->nac;
+  const int n = p->n, nac = p->nac;
   const double LE = p->LE, KE = p->KE, epn2 = p->epn2;
   const double hx = p->hx, hy = p->hy, hz = p->hz;
   const double *u = c->field + c->n3 * pv;
@@ -320,17 +367,17 @@ static void ac_nonlinear(orc_ctx *c, int pv, double *out) {
    * with a neighbour (anisotropic.c:106-158, left >= 0 ...) */
   if (!p->physical) {
   for (int k = 0; k < n; k++)
-    for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] += LE * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, 0)] += LE * epn2 * PimgC(c, u, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
-    for (int j = 0; j < n; j++) out[IDX(n, k, j, n - 1)] += LE * epn2 * Pimg(u, n, g, k, j, n) / hx / hx;
+    for (int j = 0; j < n; j++) out[IDX(n, k, j, n - 1)] += LE * epn2 * PimgC(c, u, k, j, n) / hx / hx;
   for (int k = 0; k < n; k++)
-    for (int i = 0; i < n; i++) out[IDX(n, k, 0, i)] += LE * epn2 * Pimg(u, n, g, k, -1, i) / hy / hy;
+    for (int i = 0; i < n; i++) out[IDX(n, k, 0, i)] += LE * epn2 * PimgC(c, u, k, -1, i) / hy / hy;
   for (int k = 0; k < n; k++)
-    for (int i = 0; i < n; i++) out[IDX(n, k, n - 1, i)] += LE * epn2 * Pimg(u, n, g, k, n, i) / hy / hy;
+    for (int i = 0; i < n; i++) out[IDX(n, k, n - 1, i)] += LE * epn2 * PimgC(c, u, k, n, i) / hy / hy;
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] += LE * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
+    for (int i = 0; i < n; i++) out[IDX(n, 0, j, i)] += LE * epn2 * PimgC(c, u, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] += LE * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+    for (int i = 0; i < n; i++) out[IDX(n, n - 1, j, i)] += LE * epn2 * PimgC(c, u, n, j, i) / hz / hz;
   }
   if (p->aniso == 1) ac_f2(c, pv, u, c->t0, c->t1, c->t2, c->t3);
   if (p->aniso == 2) ac_f2_full(c, pv, u, c->t3);
@@ -367,23 +414,22 @@ static inline double ch_f(double cc) { return orc_mutation == 1 ? -cc * (cc * cc
 
 static double ch_mu_out(const orc_ctx *c, const double *u, int z, int y, int x) {
   const orc_params *p = &c->p;
-  const int n = p->n, g = p->g;
   const int q = (int)((u - c->field) / (ptrdiff_t)c->n3) - p->nac;  /* the CH field u is */
-  double v0 = Pimg(u, n, g, z, y, x);
-  double mu = (Pimg(u, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(u, n, g, z, y, x + 1)) / p->hx / p->hx;
-  mu += (Pimg(u, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(u, n, g, z, y + 1, x)) / p->hy / p->hy;
-  mu += (Pimg(u, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(u, n, g, z + 1, y, x)) / p->hz / p->hz;
+  double v0 = PimgC(c, u, z, y, x);
+  double mu = (PimgC(c, u, z, y, x - 1) + (-2.0) * v0 + PimgC(c, u, z, y, x + 1)) / p->hx / p->hx;
+  mu += (PimgC(c, u, z, y - 1, x) + (-2.0) * v0 + PimgC(c, u, z, y + 1, x)) / p->hy / p->hy;
+  mu += (PimgC(c, u, z - 1, y, x) + (-2.0) * v0 + PimgC(c, u, z + 1, y, x)) / p->hz / p->hz;
   mu = p->LCI * p->epn2 * mu;
   mu -= p->LCI * ch_f(v0);
   /* ELASTIC = 2: minus LCI felas at the outside cell (SWITCH_CH_FIELDMU, ScLETD.h:153-156), from
    * the enlarged slabs of the force's own exchange (elastic.c:757-759): its interior image */
-  if (p->elastic == 2) mu -= p->LCI * Pimg(c->felas + c->n3 * q, n, g, z, y, x);
+  if (p->elastic == 2) mu -= p->LCI * PimgC(c, c->felas + c->n3 * q, z, y, x);
   return mu;
 }
 
 static void ch_nonlinear(orc_ctx *c, int v, double *out) {
   const orc_params *p = &c->p;
-  const int n = p->n, g = p->g;
+  const int n = p->n;
   const double LCI = p->LCI, KCI = p->KCI, epn2 = p->epn2;
   const double hx = p->hx, hy = p->hy, hz = p->hz;
   const double *u = c->field + c->n3 * v;
@@ -400,17 +446,17 @@ static void ch_nonlinear(orc_ctx *c, int v, double *out) {
   const int ph = p->physical;
   if (!ph) {
   for (int k = 0; k < n; k++)
-    for (int j = 0; j < n; j++) t[IDX(n, k, j, 0)] -= LCI * epn2 * Pimg(u, n, g, k, j, -1) / hx / hx;
+    for (int j = 0; j < n; j++) t[IDX(n, k, j, 0)] -= LCI * epn2 * PimgC(c, u, k, j, -1) / hx / hx;
   for (int k = 0; k < n; k++)
-    for (int j = 0; j < n; j++) t[IDX(n, k, j, n - 1)] -= LCI * epn2 * Pimg(u, n, g, k, j, n) / hx / hx;
+    for (int j = 0; j < n; j++) t[IDX(n, k, j, n - 1)] -= LCI * epn2 * PimgC(c, u, k, j, n) / hx / hx;
   for (int k = 0; k < n; k++)
-    for (int i = 0; i < n; i++) t[IDX(n, k, 0, i)] -= LCI * epn2 * Pimg(u, n, g, k, -1, i) / hy / hy;
+    for (int i = 0; i < n; i++) t[IDX(n, k, 0, i)] -= LCI * epn2 * PimgC(c, u, k, -1, i) / hy / hy;
   for (int k = 0; k < n; k++)
-    for (int i = 0; i < n; i++) t[IDX(n, k, n - 1, i)] -= LCI * epn2 * Pimg(u, n, g, k, n, i) / hy / hy;
+    for (int i = 0; i < n; i++) t[IDX(n, k, n - 1, i)] -= LCI * epn2 * PimgC(c, u, k, n, i) / hy / hy;
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) t[IDX(n, 0, j, i)] -= LCI * epn2 * Pimg(u, n, g, -1, j, i) / hz / hz;
+    for (int i = 0; i < n; i++) t[IDX(n, 0, j, i)] -= LCI * epn2 * PimgC(c, u, -1, j, i) / hz / hz;
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) t[IDX(n, n - 1, j, i)] -= LCI * epn2 * Pimg(u, n, g, n, j, i) / hz / hz;
+    for (int i = 0; i < n; i++) t[IDX(n, n - 1, j, i)] -= LCI * epn2 * PimgC(c, u, n, j, i) / hz / hz;
   }
   const size_t nn = (size_t)n * n;
 #pragma omp parallel for schedule(static)
@@ -544,10 +590,10 @@ static double calphad_f(const orc_ctx *c, size_t l, int pv, double gbcc, double 
  * enlarged slabs, i.e. the periodic image (one box, periodic).  Pinned: exact
  * on quadratic fields away from the box faces (any Lambda, cross terms
  * included), and the periodic image against a shifted field. */
-static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int x);
+static inline double pimg_or_in(const orc_ctx *c, const double *u, int n, int z, int y, int x);
 static inline int mirror_in(int x, int n);
 static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
-  const int n = c->p.n, g = c->p.g;
+  const int n = c->p.n;
   const double hx = c->p.hx, hy = c->p.hy, hz = c->p.hz;
   const double *L = c->lam + 9 * pv;
 #pragma omp parallel for schedule(static)
@@ -556,7 +602,7 @@ static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
       for (int i = 0; i < n; i++) {
 #define U(dz, dy, dx)                                                                                          \
   (c->p.physical ? u[IDX(n, mirror_in(k + (dz), n), mirror_in(j + (dy), n), mirror_in(i + (dx), n))]            \
-                 : pimg_or_in(u, n, g, k + (dz), j + (dy), i + (dx)))
+                 : pimg_or_in(c, u, n, k + (dz), j + (dy), i + (dx)))
         const double m = U(0, 0, 0);
         const double fxx = (U(0, 0, 1) - 2 * m + U(0, 0, -1)) / hx / hx;
         const double fyy = (U(0, 1, 0) - 2 * m + U(0, -1, 0)) / hy / hy;
@@ -579,9 +625,9 @@ static void ac_f2_full(const orc_ctx *c, int pv, const double *u, double *f2) {
       }
 }
 /* a cell inside the box reads the field, one outside its periodic image */
-static inline double pimg_or_in(const double *u, int n, int g, int z, int y, int x) {
+static inline double pimg_or_in(const orc_ctx *c, const double *u, int n, int z, int y, int x) {
   if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
-  return Pimg(u, n, g, z, y, x);
+  return PimgC(c, u, z, y, x);
 }
 /* physical sides (anisotropic_hip.c:546-591): n_left = i + 1 at i == ix1,
  * n_right = i - 1 at i == ix4 - 1, per axis; the mirror stays in the box */
@@ -672,7 +718,7 @@ void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, con
 
 static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
   const orc_params *p = &c->p;
-  const int n = p->n, g = p->g, q = v - p->nac;
+  const int n = p->n, q = v - p->nac;
   const double LCI = p->LCI, KCI = p->KCI, epn2 = p->epn2, hx = p->hx, hy = p->hy, hz = p->hz;
   const double *u = c->field + c->n3 * v;
   double *M = c->t0, *f0 = c->t1, *ft = c->t2, *w = c->t3;
@@ -684,17 +730,17 @@ static void ch_nonlinear_vm(orc_ctx *c, int v, double *out) {
   for (size_t l = 0; l < c->n3; l++) ft[l] = -f0[l] + KCI * u[l];
 #define FACES(arr, coef)                                                                                 \
   for (int k = 0; k < n; k++)                                                                            \
-    for (int j = 0; j < n; j++) arr[IDX(n, k, j, 0)] += coef * Pimg(u, n, g, k, j, -1) / hx / hx;       \
+    for (int j = 0; j < n; j++) arr[IDX(n, k, j, 0)] += coef * PimgC(c, u, k, j, -1) / hx / hx;       \
   for (int k = 0; k < n; k++)                                                                            \
-    for (int j = 0; j < n; j++) arr[IDX(n, k, j, n - 1)] += coef * Pimg(u, n, g, k, j, n) / hx / hx;    \
+    for (int j = 0; j < n; j++) arr[IDX(n, k, j, n - 1)] += coef * PimgC(c, u, k, j, n) / hx / hx;    \
   for (int k = 0; k < n; k++)                                                                            \
-    for (int i = 0; i < n; i++) arr[IDX(n, k, 0, i)] += coef * Pimg(u, n, g, k, -1, i) / hy / hy;       \
+    for (int i = 0; i < n; i++) arr[IDX(n, k, 0, i)] += coef * PimgC(c, u, k, -1, i) / hy / hy;       \
   for (int k = 0; k < n; k++)                                                                            \
-    for (int i = 0; i < n; i++) arr[IDX(n, k, n - 1, i)] += coef * Pimg(u, n, g, k, n, i) / hy / hy;    \
+    for (int i = 0; i < n; i++) arr[IDX(n, k, n - 1, i)] += coef * PimgC(c, u, k, n, i) / hy / hy;    \
   for (int j = 0; j < n; j++)                                                                            \
-    for (int i = 0; i < n; i++) arr[IDX(n, 0, j, i)] += coef * Pimg(u, n, g, -1, j, i) / hz / hz;       \
+    for (int i = 0; i < n; i++) arr[IDX(n, 0, j, i)] += coef * PimgC(c, u, -1, j, i) / hz / hz;       \
   for (int j = 0; j < n; j++)                                                                            \
-    for (int i = 0; i < n; i++) arr[IDX(n, n - 1, j, i)] += coef * Pimg(u, n, g, n, j, i) / hz / hz;
+    for (int i = 0; i < n; i++) arr[IDX(n, n - 1, j, i)] += coef * PimgC(c, u, n, j, i) / hz / hz;
   const int ph = p->physical;
   if (!ph) { FACES(ft, epn2) }
   lap_zero(n, hx, hy, hz, ft, f1, ph);
@@ -1181,7 +1227,8 @@ orc_ctx *orc_create(const orc_params *p) {
   c->field = calloc(c->n3 * c->V, sizeof(double));
   c->N1 = calloc(c->n3 * c->V, sizeof(double));
   c->N2 = calloc(c->n3 * c->V, sizeof(double));
-  c->felas = calloc(c->n3 * (el_count(p) > 0 ? el_count(p) : 1), sizeof(double));
+  c->nfel = el_count(p) > 0 ? el_count(p) : 1;
+  c->felas = calloc(c->n3 * c->nfel, sizeof(double));
   c->fbuf = calloc(c->n3, sizeof(double));
   c->phi1 = calloc(c->n3 * 2, sizeof(double));
   c->phi2 = calloc(c->n3 * 2, sizeof(double));
@@ -1430,4 +1477,232 @@ void orc_diag(orc_ctx *c, double *stats) {
   }
   stats[4 * c->V] = couple;
   free(plane);
+}
+
+/* ------------------------------------------------------------------ */
+/* Decomposed runs: px x py x pz periodic boxes (omega's procs line with
+ * periodic = 1), every box with the same extended edge n and the same
+ * boundary id 3 eigenbasis (each side has a neighbour, vars.c:488-582).
+ * h = L / (p N) comes from init_para (vars.c:58-157) through the caller's
+ * hx, hy, hz. */
+
+orc_group *orc_group_create(orc_ctx **boxes, int px, int py, int pz) {
+  if (!boxes || px < 1 || py < 1 || pz < 1) return NULL;
+  const int P = px * py * pz;
+  const orc_params *p0 = &boxes[0]->p;
+  for (int b = 0; b < P; b++) {
+    const orc_params *p = &boxes[b]->p;
+    if (p->n != p0->n || p->g != p0->g || p->nac != p0->nac || p->nch != p0->nch || p->physical || p->varmob)
+      return NULL;
+  }
+  orc_group *G = calloc(1, sizeof(orc_group));
+  G->px = px;
+  G->py = py;
+  G->pz = pz;
+  G->box = malloc(sizeof(orc_ctx *) * P);
+  for (int b = 0; b < P; b++) {
+    G->box[b] = boxes[b];
+    boxes[b]->grp = G;
+    boxes[b]->bx = b % px;
+    boxes[b]->by = (b / px) % py;
+    boxes[b]->bz = b / (px * py);
+  }
+  return G;
+}
+
+void orc_group_destroy(orc_group *G) {
+  if (!G) return;
+  for (int b = 0; b < G->px * G->py * G->pz; b++) G->box[b]->grp = NULL;
+  free(G->box);
+  free(G);
+}
+
+/* transfer() across the boxes (trans.c:1939-1996 with the slab packing of
+ * 7-251): every send reads the sender's state before any unpack, and unpack
+ * writes x ghosts, then y ghosts over every plane, then z ghosts, so the last
+ * writer of an edge or corner cell is the z (then y) exchange, which carries
+ * whole planes (rows) of the neighbour including its own ghost cells.  That
+ * is the single-box sweep of orc_ghost_fill with the source box being the
+ * neighbour in the sweep's direction; running each sweep over all boxes
+ * before the next keeps every read on pre-transfer data (a z sweep reads
+ * interior planes, a y sweep interior rows whose x ghosts only the x sweep
+ * writes). */
+static void group_fill(orc_group *G, double *(*arr)(orc_ctx *, int), int v) {
+  const int P = G->px * G->py * G->pz;
+  const int n = G->box[0]->p.n, g = G->box[0]->p.g;
+  for (int b = 0; b < P; b++) {
+    orc_ctx *c = G->box[b];
+    double *u = arr(c, v);
+    for (int k = 0; k < n; k++) {
+      if (!(k < g || k >= n - g)) continue;
+      const double *src = arr((orc_ctx *)nb_box(c, 0, 0, box_shift(k, n, g)), v);
+      memcpy(u + IDX(n, k, 0, 0), src + IDX(n, wrap_e(k, n, g), 0, 0), sizeof(double) * (size_t)n * n);
+    }
+  }
+  for (int b = 0; b < P; b++) {
+    orc_ctx *c = G->box[b];
+    double *u = arr(c, v);
+    for (int k = g; k < n - g; k++)
+      for (int j = 0; j < n; j++) {
+        if (!(j < g || j >= n - g)) continue;
+        const double *src = arr((orc_ctx *)nb_box(c, 0, box_shift(j, n, g), 0), v);
+        memcpy(u + IDX(n, k, j, 0), src + IDX(n, k, wrap_e(j, n, g), 0), sizeof(double) * (size_t)n);
+      }
+  }
+  for (int b = 0; b < P; b++) {
+    orc_ctx *c = G->box[b];
+    double *u = arr(c, v);
+    const double *lo = arr((orc_ctx *)nb_box(c, -1, 0, 0), v), *hi = arr((orc_ctx *)nb_box(c, 1, 0, 0), v);
+    for (int k = g; k < n - g; k++)
+      for (int j = g; j < n - g; j++)
+        for (int i = 0; i < n; i++) {
+          if (i < g) u[IDX(n, k, j, i)] = lo[IDX(n, k, j, wrap_e(i, n, g))];
+          else if (i >= n - g) u[IDX(n, k, j, i)] = hi[IDX(n, k, j, wrap_e(i, n, g))];
+        }
+  }
+}
+static double *arr_field(orc_ctx *c, int v) { return c->field + c->n3 * v; }
+static double *arr_fbuf(orc_ctx *c, int v) { (void)v; return c->fbuf; }
+
+void orc_group_transfer(orc_group *G) {
+  const int V = G->box[0]->V;
+  for (int v = 0; v < V; v++) group_fill(G, arr_field, v);
+}
+
+/* elastic_calculate (elastic.c:689-762) on the global interior of
+ * (px N)^3 cells, the distributed FFT of fftMPI (fft3d.c:98-138) taken as
+ * one transform: B_pq at the global wavenumbers, felas_p written into each
+ * box's interior of its persistent transfer buffer f, which the exchange of
+ * elastic_calculate_ElasDri (its own persistent requests on f, 669-686)
+ * then fills as transfer() fills a field; felas_p = f. */
+void orc_group_elastic(orc_group *G) {
+  orc_ctx *c0 = G->box[0];
+  const orc_params *p = &c0->p;
+  const int n = p->n, g = p->g, N = n - 2 * g, nel = el_count(p), v0 = el_first(p);
+  if (nel == 0 || nel > 64 || G->px != G->py || G->px != G->pz) return;
+  const int P = G->px * G->py * G->pz, Ng = G->px * N;
+  const size_t N3 = (size_t)Ng * Ng * Ng;
+  double *spec = malloc(sizeof(double) * 2 * N3 * nel);
+  double *acc = malloc(sizeof(double) * 2 * N3);
+  bfast *bf = malloc(sizeof(bfast));
+  bfast_init(p, bf);
+  for (int q = 0; q < nel; q++) {
+    double *s = spec + 2 * N3 * q;
+    for (int b = 0; b < P; b++) {
+      const orc_ctx *c = G->box[b];
+      const double *u = c->field + c->n3 * (v0 + q);
+      for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+          for (int x = 0; x < N; x++) {
+            size_t m = (((size_t)(c->bz * N + z)) * Ng + (c->by * N + y)) * Ng + (c->bx * N + x);
+            s[2 * m] = u[IDX(n, z + g, y + g, x + g)];
+            s[2 * m + 1] = 0.0;
+          }
+    }
+    orc_fft3d(Ng, s, +1);
+  }
+  for (int pp = 0; pp < nel; pp++) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < Ng; z++) {
+      double row[64];
+      for (int y = 0; y < Ng; y++)
+        for (int x = 0; x < Ng; x++) {
+          size_t m = ((size_t)z * Ng + y) * Ng + x;
+          bfast_row(p, bf, Ng, x, y, z, pp, row);
+          double re = 0.0, im = 0.0;
+          for (int qq = 0; qq < nel; qq++) {
+            re += spec[2 * N3 * qq + 2 * m] * row[qq];
+            im += spec[2 * N3 * qq + 2 * m + 1] * row[qq];
+          }
+          acc[2 * m] = re;
+          acc[2 * m + 1] = im;
+        }
+    }
+    orc_fft3d(Ng, acc, -1);
+    for (int b = 0; b < P; b++) {
+      orc_ctx *c = G->box[b];
+      for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+          for (int x = 0; x < N; x++) {
+            size_t m = (((size_t)(c->bz * N + z)) * Ng + (c->by * N + y)) * Ng + (c->bx * N + x);
+            c->fbuf[IDX(n, z + g, y + g, x + g)] = acc[2 * m];
+          }
+    }
+    group_fill(G, arr_fbuf, 0);
+    for (int b = 0; b < P; b++) memcpy(G->box[b]->felas + G->box[b]->n3 * pp, G->box[b]->fbuf, sizeof(double) * c0->n3);
+  }
+  free(spec);
+  free(acc);
+  free(bf);
+}
+
+/* One time step of every box: ScLETD.c:86-189 with transfer / elastic across
+ * the boxes; each box's nonlinear terms read its neighbours (outside values)
+ * before any box is advanced. */
+void orc_group_step(orc_group *G) {
+  const int P = G->px * G->py * G->pz;
+  orc_ctx *c0 = G->box[0];
+  const orc_params *p = &c0->p;
+  const size_t n3 = c0->n3;
+  const int V = c0->V;
+  double *uh = malloc(sizeof(double) * n3), *nh = malloc(sizeof(double) * n3);
+  double *w1 = malloc(sizeof(double) * n3), *w2 = malloc(sizeof(double) * n3);
+  orc_group_transfer(G);
+  if (el_count(p) > 0) orc_group_elastic(G);
+  for (int b = 0; b < P; b++)
+    for (int v = 0; v < V; v++) orc_nonlinear(G->box[b], v, G->box[b]->N1 + n3 * v);
+  for (int b = 0; b < P; b++) {
+    orc_ctx *c = G->box[b];
+    for (int v = 0; v < V; v++) {
+      double *u = c->field + n3 * v;
+      fwd3(c, u, uh, w1, w2);
+      fwd3(c, c->N1 + n3 * v, nh, w1, w2);
+      orc_phi_update(c, v, uh, nh);
+      bwd3(c, uh, u, w1, w2);
+    }
+  }
+  if (p->etd2) {
+    orc_group_transfer(G);
+    if (el_count(p) > 0) orc_group_elastic(G);
+    for (int b = 0; b < P; b++)
+      for (int v = 0; v < V; v++) orc_nonlinear(G->box[b], v, G->box[b]->N2 + n3 * v);
+    for (int b = 0; b < P; b++) {
+      orc_ctx *c = G->box[b];
+      for (int v = 0; v < V; v++) {
+        double *dn = c->N2 + n3 * v;
+        const double *n1 = c->N1 + n3 * v;
+        for (size_t l = 0; l < n3; l++) dn[l] = dn[l] - n1[l];
+        fwd3(c, dn, nh, w1, w2);
+        const double *ph2 = c->phi2 + n3 * (v >= p->nac);
+        for (size_t l = 0; l < n3; l++) nh[l] = ph2[l] * nh[l];
+        bwd3(c, nh, uh, w1, w2);
+        double *u = c->field + n3 * v;
+        for (size_t l = 0; l < n3; l++) u[l] += uh[l];
+      }
+    }
+  }
+  free(uh); free(nh); free(w1); free(w2);
+}
+
+/* check_soln_new over the boxes: each box's sums over its own ranges, then
+ * MPI_Reduce (field.c:529-532): max, min, and the sums of vol and eng in box
+ * order. */
+void orc_group_diag(orc_group *G, double *stats) {
+  const int P = G->px * G->py * G->pz, V = G->box[0]->V;
+  double *s = malloc(sizeof(double) * (4 * V + 1));
+  for (int b = 0; b < P; b++) {
+    orc_diag(G->box[b], s);
+    if (b == 0) {
+      memcpy(stats, s, sizeof(double) * (4 * V + 1));
+      continue;
+    }
+    for (int v = 0; v < V; v++) {
+      if (s[4 * v] > stats[4 * v]) stats[4 * v] = s[4 * v];
+      if (s[4 * v + 1] < stats[4 * v + 1]) stats[4 * v + 1] = s[4 * v + 1];
+      stats[4 * v + 2] += s[4 * v + 2];
+      stats[4 * v + 3] += s[4 * v + 3];
+    }
+    stats[4 * V] += s[4 * V];
+  }
+  free(s);
 }

@@ -6,8 +6,9 @@
  * and the cpu_baseline leg of bench.py may load it.  The product library
  * (libetd, HIP) never links or calls it.
  *
- * It restates, for ONE periodic subdomain (procs = 1,1,1, periodic = 1), the
- * algorithm of the reference's beta->omega CPU path:
+ * It restates, for one subdomain (procs = 1,1,1) or a periodic decomposition
+ * into boxes (orc_group), the algorithm of the reference's beta->omega CPU
+ * path:
  *   source_beta_to_omega/src/ScLETD.c:84-191   stage order (ETD1 + optional ETD2)
  *   source_beta_to_omega/src/calc.c:8-68,70-473,489-684
  *   source_beta_to_omega/src/anisotropic.c:9-627
@@ -100,6 +101,21 @@ void orc_calphad_terms(double *t);
 void orc_div_m_grad(int n, double hx, double hy, double hz, const double *M, const double *a, double *out);
 /* stats: per field v (nac+nch) 4 doubles max,min,vol,eng; then couple eng */
 void orc_diag(orc_ctx *c, double *stats);
+
+/* Decomposed runs (omega's procs px,py,pz with periodic = 1): boxes[b] is
+ * the box at (b % px, (b / px) % py, b / (px py)), all created with the same
+ * parameters (n, g, field counts, h = L / (p N)); the group borrows them
+ * (destroy the group before the boxes).  Each box is then stepped by
+ * orc_group_step only; orc_nonlinear on a box reads its neighbours' fields
+ * for the outside values. */
+typedef struct orc_group orc_group;
+orc_group *orc_group_create(orc_ctx **boxes, int px, int py, int pz);
+void orc_group_destroy(orc_group *G);
+void orc_group_transfer(orc_group *G);
+void orc_group_elastic(orc_group *G);
+void orc_group_step(orc_group *G);
+/* stats as orc_diag, reduced over the boxes */
+void orc_group_diag(orc_group *G, double *stats);
 
 /* closed-form eigenbasis of the truncated 1D second-difference operator */
 int orc_eigbasis(int n, int id, double *V, double *Vi, double *D);

@@ -68,6 +68,13 @@ def lib():
         L.orc_calphad.argtypes = [_dp, _dp]
         L.orc_calphad_terms.argtypes = [_dp]
         L.orc_div_m_grad.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_group_create.restype = C.c_void_p
+        L.orc_group_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
+        L.orc_group_destroy.argtypes = [C.c_void_p]
+        L.orc_group_transfer.argtypes = [C.c_void_p]
+        L.orc_group_elastic.argtypes = [C.c_void_p]
+        L.orc_group_step.argtypes = [C.c_void_p]
+        L.orc_group_diag.argtypes = [C.c_void_p, _dp]
         _LIB = L
     return _LIB
 
@@ -252,6 +259,49 @@ class Oracle:
 
     def bpq(self, N, gx, gy, gz, p, q):
         return lib().orc_bpq(C.byref(self._p), N, gx, gy, gz, p, q)
+
+
+class OracleGroup:
+    """A decomposed run: px x py x pz boxes (omega's procs line, periodic),
+    box b at (b % px, (b // px) % py, b // (px py)), each an Oracle with the
+    same parameters (h = L / (p N) in prm["h"])."""
+
+    def __init__(self, prm, procs):
+        self.px, self.py, self.pz = procs
+        self.P = self.px * self.py * self.pz
+        self.boxes = [Oracle(prm) for _ in range(self.P)]
+        self.n = prm["n"]
+        self.V = self.boxes[0].V
+        arr = (C.c_void_p * self.P)(*[b.ctx for b in self.boxes])
+        self.grp = lib().orc_group_create(arr, self.px, self.py, self.pz)
+        if not self.grp:
+            raise ValueError("orc_group_create rejected the boxes")
+
+    def __del__(self):
+        if getattr(self, "grp", None):
+            lib().orc_group_destroy(self.grp)
+            self.grp = None
+
+    def box(self, b):
+        return self.boxes[b]
+
+    def coords(self, b):
+        return b % self.px, (b // self.px) % self.py, b // (self.px * self.py)
+
+    def transfer(self):
+        lib().orc_group_transfer(self.grp)
+
+    def elastic(self):
+        lib().orc_group_elastic(self.grp)
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            lib().orc_group_step(self.grp)
+
+    def diag(self):
+        st = np.zeros(4 * self.V + 1)
+        lib().orc_group_diag(self.grp, _ptr(st))
+        return st[:-1].reshape(self.V, 4), st[-1]
 
 
 def ghost_fill(u, g=2):
