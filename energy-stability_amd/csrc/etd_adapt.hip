@@ -692,6 +692,10 @@ extern "C" int etd_run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_s
     snprintf(c->err, sizeof(c->err), "learned controller requested without etd_set_controller");
     return ETD_ERR_ARG;
   }
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   if (!c->p.periodic) {
     snprintf(c->err, sizeof(c->err), "adaptive stepping needs a periodic box (its energy is the periodic one)");
     return ETD_ERR_UNSUPPORTED;

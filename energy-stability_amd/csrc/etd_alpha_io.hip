@@ -81,6 +81,10 @@ static int put_file(etd_ctx *c, const char *fn, const void *a, size_t asz, const
 // has their code, disabled: vars.c:990-1010).
 extern "C" int etd_write_checkpoint(etd_ctx *c, int irun, int chk) {
   if (!c || (chk != 0 && chk != 1)) return ETD_ERR_ARG;
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   double *h = (double *)malloc(sizeof(double) * c->n3);
   if (!h) return ETD_ERR_NOMEM;
   int rc = ETD_OK;
@@ -101,6 +105,10 @@ extern "C" int etd_write_checkpoint(etd_ctx *c, int irun, int chk) {
 // when present (else the CH fields are left as they are).
 extern "C" int etd_read_checkpoint(etd_ctx *c, int chk, int *irun) {
   if (!c || !irun || (chk != 0 && chk != 1)) return ETD_ERR_ARG;
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   double *h = (double *)malloc(sizeof(double) * c->n3);
   if (!h) return ETD_ERR_NOMEM;
   int rc = ETD_OK;
@@ -164,6 +172,10 @@ static int field2b_t(etd_ctx *c, int irun) {
 // n^3 uint16 (ghosts included, x fastest).
 extern "C" int etd_write_field2b(etd_ctx *c, int irun) {
   if (!c) return ETD_ERR_ARG;
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   return c->mixed ? field2b_t<float>(c, irun) : field2b_t<double>(c, irun);
 }
 
@@ -184,6 +196,10 @@ static int detector_t(etd_ctx *c, int var, unsigned char *bits, float *binary) {
 
 extern "C" int etd_detector_feed(etd_ctx *c, int var, unsigned char *bits, float *binary) {
   if (!c || !bits || var < 0 || var >= c->nac) return ETD_ERR_ARG;
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   const int d = c->N / 2;
   if (c->N % 2 || ((long)d * d * d) % 8) {
     snprintf(c->err, sizeof(c->err), "detector feed: N = %d must be even with (N/2)^3 a multiple of 8", c->N);

@@ -84,8 +84,21 @@ static void c4_from_voigt(const double *C2, double *C4) {
 static int validate(etd_ctx *c, const etd_params *p) {
   if (p->nx != p->ny || p->nx != p->nz)
     return seterr(c, ETD_ERR_UNSUPPORTED, "nx, ny, nz must be equal (the axis rotation of calc.c:585-629 needs a cube)");
-  if (p->px != 1 || p->py != 1 || p->pz != 1)
-    return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz = %d,%d,%d: one subdomain per GPU (use 1,1,1)", p->px, p->py, p->pz);
+  if (p->px < 1 || p->py < 1 || p->pz < 1 || (long)p->px * p->py * p->pz > 512)
+    return seterr(c, ETD_ERR_ARG, "px,py,pz = %d,%d,%d", p->px, p->py, p->pz);
+  if (p->px * p->py * p->pz > 1) {
+    // a decomposed run: every box on this GPU, the reference's localized-ETD numerics
+    if (p->periodic != 1)
+      return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz = %d,%d,%d: boxes with physical outer sides are not built "
+                    "(periodic = 1 only)", p->px, p->py, p->pz);
+    if (p->precision != ETD_PREC_FP64)
+      return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz > 1 runs in FP64 (precision 0)");
+    if (p->variable_mobility || p->free_energy)
+      return seterr(c, ETD_ERR_UNSUPPORTED, "px,py,pz > 1 with VariableMobility or the CALPHAD force is not built");
+    if (p->elastic && (p->px != p->py || p->px != p->pz))
+      return seterr(c, ETD_ERR_UNSUPPORTED, "elasticity over px,py,pz = %d,%d,%d boxes needs px = py = pz (a cubic "
+                    "global grid)", p->px, p->py, p->pz);
+  }
   if (p->periodic != 0 && p->periodic != 1) return seterr(c, ETD_ERR_ARG, "periodic = %d is not 0 or 1", p->periodic);
   if (p->variable_mobility != 0 && p->variable_mobility != 1)
     return seterr(c, ETD_ERR_ARG, "VariableMobility = %d is not 0 or 1", p->variable_mobility);
@@ -133,6 +146,14 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   const int n = c->n, V = c->V;
   c->n3 = (size_t)n * n * n;
   c->N3 = (size_t)c->N * c->N * c->N;
+  c->px = p->px;
+  c->py = p->py;
+  c->pz = p->pz;
+  c->P = p->px * p->py * p->pz;
+  c->box = 0;
+  c->bs = (size_t)V * c->n3;
+  c->Ne = c->px * c->N;  // px = py = pz whenever elastic; the grid edge only matters then
+  c->Ne3 = (size_t)c->Ne * c->Ne * c->Ne;
   *out = c;  // from here on etd_close releases partial state
   c->hV = (double *)malloc(sizeof(double) * n * n);
   c->hVi = (double *)malloc(sizeof(double) * n * n);
@@ -182,13 +203,18 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   c->split_zs = c->split_f16 && c->split_kernel == 10 && f16_zs_supported(p->nx);
   if (c->split_kernel == 10 && !c->split_zs) c->split_kernel = 0;  // BF16, or n > 256: the default of those
   if (!c->mixed) {
-    HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * V));  // [u | N1]
-    c->N1 = c->u + n3 * V;
-    HCHK(c, hipMemsetAsync(c->u, 0, sizeof(double) * n3 * 2 * V, c->stream));
-    if (p->etd2) HCHK(c, hipMalloc(&c->N2, sizeof(double) * n3 * V));
-    HCHK(c, hipMalloc(&c->w1, sizeof(double) * n3 * 2 * V));
-    HCHK(c, hipMalloc(&c->w2, sizeof(double) * n3 * 2 * V));
-    HCHK(c, hipMalloc(&c->spec, sizeof(double) * n3 * 2 * V));
+    const size_t PV = (size_t)c->P * V;  // every box's fields
+    HCHK(c, hipMalloc(&c->u, sizeof(double) * n3 * 2 * PV));  // [u | N1]
+    c->N1 = c->u + n3 * PV;
+    HCHK(c, hipMemsetAsync(c->u, 0, sizeof(double) * n3 * 2 * PV, c->stream));
+    if (p->etd2) HCHK(c, hipMalloc(&c->N2, sizeof(double) * n3 * PV));
+    HCHK(c, hipMalloc(&c->w1, sizeof(double) * n3 * 2 * PV));
+    HCHK(c, hipMalloc(&c->w2, sizeof(double) * n3 * 2 * PV));
+    HCHK(c, hipMalloc(&c->spec, sizeof(double) * n3 * 2 * PV));
+    c->bu = c->u;
+    c->bN1 = c->N1;
+    c->bN2 = c->N2;
+    c->bspec = c->spec;
     HCHK(c, hipMalloc(&c->gtmp, sizeof(double) * n3 * 3));
     // eigenbasis as MFMA A fragments
     const int MT = (n + 15) / 16, nkc = (n + 15) / 16;
@@ -248,29 +274,29 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   c->nel = p->elastic == 1 ? p->nac : p->elastic == 2 ? p->nch : 0;
   c->el_v0 = p->elastic == 2 ? p->nac : 0;
   if (c->nel > 0) {
-    c->fft_npass = fft_plan_radices(c->N, c->fft_radix);
-    if (c->fft_npass < 0) return seterr(c, ETD_ERR_UNSUPPORTED, "N = %d has prime factors other than 2, 3, 5", c->N);
+    c->fft_npass = fft_plan_radices(c->Ne, c->fft_radix);
+    if (c->fft_npass < 0) return seterr(c, ETD_ERR_UNSUPPORTED, "N = %d has prime factors other than 2, 3, 5", c->Ne);
     c->npair = (c->nel + 1) / 2;
     c->ring = c->nel + 2;
     if (!c->mixed) {
-      HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->N3 * c->npair));
-      HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->N3 * c->npair));
-      HCHK(c, hipMalloc(&c->E, sizeof(double) * c->N3 * c->ring));
-      HCHK(c, hipMemsetAsync(c->E, 0, sizeof(double) * c->N3 * c->ring, c->stream));
+      HCHK(c, hipMalloc(&c->Z, sizeof(double) * 2 * c->Ne3 * c->npair));
+      HCHK(c, hipMalloc(&c->Y, sizeof(double) * 2 * c->Ne3 * c->npair));
+      HCHK(c, hipMalloc(&c->E, sizeof(double) * c->Ne3 * c->ring));
+      HCHK(c, hipMemsetAsync(c->E, 0, sizeof(double) * c->Ne3 * c->ring, c->stream));
     } else {
       HCHK(c, hipMalloc(&c->f32.Z, sizeof(float) * 2 * c->N3 * c->npair));
       HCHK(c, hipMalloc(&c->f32.Y, sizeof(float) * 2 * c->N3 * c->npair));
       HCHK(c, hipMalloc(&c->f32.E, sizeof(float) * c->N3 * c->ring));
       HCHK(c, hipMemsetAsync(c->f32.E, 0, sizeof(float) * c->N3 * c->ring, c->stream));
     }
-    double *tw = (double *)malloc(sizeof(double) * 2 * c->N);
+    double *tw = (double *)malloc(sizeof(double) * 2 * c->Ne);
     const double pi = 3.14159265358979323846;
-    for (int e = 0; e < c->N; e++) {
-      tw[2 * e] = cos(2.0 * pi * e / c->N);
-      tw[2 * e + 1] = -sin(2.0 * pi * e / c->N);
+    for (int e = 0; e < c->Ne; e++) {
+      tw[2 * e] = cos(2.0 * pi * e / c->Ne);
+      tw[2 * e + 1] = -sin(2.0 * pi * e / c->Ne);
     }
-    HCHK(c, hipMalloc(&c->tw, sizeof(double) * 2 * c->N));
-    HCHK(c, hipMemcpy(c->tw, tw, sizeof(double) * 2 * c->N, hipMemcpyHostToDevice));
+    HCHK(c, hipMalloc(&c->tw, sizeof(double) * 2 * c->Ne));
+    HCHK(c, hipMemcpy(c->tw, tw, sizeof(double) * 2 * c->Ne, hipMemcpyHostToDevice));
     free(tw);
     etd_dev_consts *k = (etd_dev_consts *)calloc(1, sizeof(etd_dev_consts));
     c4_from_voigt(p->C, k->C4);
@@ -332,6 +358,11 @@ extern "C" void etd_close(etd_ctx *c) {
   (void)hipFree(c->ch_const);
   if (c->ad_hpart) (void)hipHostFree(c->ad_hpart);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->bu) {  // the buffers' starts, whatever box a failed call left in view
+    c->u = c->bu;
+    c->N2 = c->bN2;
+    c->spec = c->bspec;
+  }
   (void)hipFree(c->u);
   (void)hipFree(c->N2);
   (void)hipFree(c->w1);
@@ -449,13 +480,16 @@ extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   c->elastic_fresh = 0;
   c->ch_stale = 1;
   if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
-  HCHK(c, hipMemcpyAsync(c->u + c->n3 * var, host, sizeof(double) * c->n3, hipMemcpyHostToDevice, c->stream));
+  for (int b = 0; b < c->P; b++)  // a decomposed run: the boxes one after another
+    HCHK(c, hipMemcpyAsync(c->bu + b * c->bs + c->n3 * var, host + b * c->n3, sizeof(double) * c->n3,
+                           hipMemcpyHostToDevice, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
 }
 
 extern "C" int etd_set_mask(etd_ctx *c, const int *mask) {
   if (!c) return ETD_ERR_ARG;
+  if (mask && c->P > 1) return seterr(c, ETD_ERR_UNSUPPORTED, "set_mask: one box only (px,py,pz = 1,1,1)");
   if (!mask) {
     if (c->mask) HCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->mask);
@@ -478,19 +512,62 @@ extern "C" int etd_set_mask(etd_ctx *c, const int *mask) {
 extern "C" int etd_get_field(etd_ctx *c, int var, double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "get_field: bad variable %d", var);
   if (c->mixed) return get_f32(c, host, c->f32.u + c->n3 * var, c->n3);
-  HCHK(c, hipMemcpyAsync(host, c->u + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
+  for (int b = 0; b < c->P; b++)
+    HCHK(c, hipMemcpyAsync(host + b * c->n3, c->bu + b * c->bs + c->n3 * var, sizeof(double) * c->n3,
+                           hipMemcpyDeviceToHost, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
 }
 
 extern "C" int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, double mean, double amp) {
   if (!c || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "init_noise: bad variable %d", var);
-  double *h = (double *)malloc(sizeof(double) * c->n3);
+  double *h = (double *)malloc(sizeof(double) * c->n3 * c->P);
   if (!h) return ETD_ERR_NOMEM;
-  etd_host_noise(c->n, c->g, seed, mean, amp, h);
+  // box b of a decomposed run draws from seed + 1000003 b
+  for (int b = 0; b < c->P; b++) etd_host_noise(c->n, c->g, seed + 1000003ULL * b, mean, amp, h + b * c->n3);
   int rc = etd_set_field(c, var, h);
   free(h);
   return rc;
+}
+
+// ------------------------------------------------------------------ boxes
+// The per-box launches of a decomposed run see box b's fields through u, N1,
+// N2 and spec (and make_nl's box coordinates); box 0 is the buffers' start,
+// which every other launch uses.
+static void box_view(etd_ctx *c, int b) {
+  const size_t o = (size_t)b * c->bs;
+  c->u = c->bu + o;
+  c->N1 = c->bN1 + o;
+  c->N2 = c->bN2 ? c->bN2 + o : nullptr;
+  c->spec = c->bspec + o;
+  c->box = b;
+}
+
+// the nonlinear terms of every box (out, sub: box 0's first field, the
+// others at stride bs); each box reads its neighbours' fields for the
+// outside values, so all boxes are evaluated before any is advanced
+template <typename T>
+static void nonlinear_boxes(etd_ctx *c, T *out, const T *sub, int ch_centre) {
+  if (c->P == 1) {
+    launch_nonlinear<T>(c, out, sub, ch_centre);
+    return;
+  }
+  if constexpr (sizeof(T) == 8) {
+    for (int b = 0; b < c->P; b++) {
+      box_view(c, b);
+      launch_nonlinear<T>(c, out + b * c->bs, sub ? sub + b * c->bs : nullptr, ch_centre);
+    }
+    box_view(c, 0);
+  }
+}
+
+// phi scaling of every box: u-hat of box b at spec + b bs, N-hat after all
+// boxes' u-hat (the forward transform's input was [u of all boxes | N1 of all])
+template <typename T>
+static void phiscale_boxes(etd_ctx *c, T *spec, const T *nhat0, int order2) {
+  const size_t bs = c->bs;
+  for (int b = 0; b < c->P; b++)
+    launch_phiscale<T>(c, spec + b * bs, nhat0 ? nhat0 + b * bs : nullptr, spec + b * bs, 0, c->V, order2);
 }
 
 // ------------------------------------------------------------------ step
@@ -530,9 +607,9 @@ void stage_forward(etd_ctx *c) {
   // these fields (same ghost fill, same kernels): reuse it
   if (c->nel > 0 && !c->elastic_fresh) launch_elastic<T>(c);
   c->elastic_fresh = 0;
-  launch_nonlinear<T>(c, F.N1, nullptr, 1);  // mixed: CH terms centred, N1 - N(c_mean)
+  nonlinear_boxes<T>(c, F.N1, (const T *)nullptr, 1);  // mixed: CH terms centred, N1 - N(c_mean)
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
-  modeprod3<T>(c, 1, F.u, F.spec, 2 * V, 0);
+  modeprod3<T>(c, 1, F.u, F.spec, 2 * V * c->P, 0);
   // u stays centred: every caller overwrites it with the backward transform;
   // the ETD2 stage needs N1 itself (N(a) - N1)
   if constexpr (sizeof(T) == 4)
@@ -546,16 +623,17 @@ int step_once_t(etd_ctx *c) {
   const etd_fields<T> F = fields<T>(c);
   if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
   stage_forward<T>(c);
-  launch_phiscale<T>(c, F.spec, F.spec + n3 * V, F.spec, 0, V, 0);
-  modeprod3<T>(c, 0, F.spec, F.u, V, 0);
+  const int PV = V * c->P;
+  phiscale_boxes<T>(c, F.spec, F.spec + n3 * PV, 0);
+  modeprod3<T>(c, 0, F.spec, F.u, PV, 0);
   if (c->p.etd2) {
     if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
     launch_ghost_fill<T>(c, F.u, V, n3);
     if (c->nel > 0) launch_elastic<T>(c);
-    launch_nonlinear<T>(c, F.N2, F.N1);  // N(a) - N(u_n)
-    modeprod3<T>(c, 1, F.N2, F.spec, V, 0);
-    launch_phiscale<T>(c, F.spec, (const T *)nullptr, F.spec, 0, V, 1);
-    modeprod3<T>(c, 0, F.spec, F.u, V, 1);  // u = a + S phi2 S^-1 dN
+    nonlinear_boxes<T>(c, F.N2, (const T *)F.N1, 0);  // N(a) - N(u_n)
+    modeprod3<T>(c, 1, F.N2, F.spec, PV, 0);
+    phiscale_boxes<T>(c, F.spec, (const T *)nullptr, 1);
+    modeprod3<T>(c, 0, F.spec, F.u, PV, 1);  // u = a + S phi2 S^-1 dN
     if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
   } else {
     if constexpr (sizeof(T) == 4) ch_mean_restore_centered(c, F.u);  // adds the mean back
@@ -572,7 +650,33 @@ template int step_once_t<double>(etd_ctx *);
 template int step_once_t<float>(etd_ctx *);
 
 static int step_once(etd_ctx *c) { return c->mixed ? step_once_t<float>(c) : step_once_t<double>(c); }
-static int diag_any(etd_ctx *c, double *st) { return c->mixed ? launch_diag<float>(c, st) : launch_diag<double>(c, st); }
+// check_soln_new of a decomposed run: each box over its own ranges, then the
+// reduction of field.c:513-516 (max, min, and the sums in box order)
+static int diag_boxes(etd_ctx *c, double *st) {
+  const int nv = 4 * c->V + 1;
+  double *sb = (double *)malloc(sizeof(double) * nv);
+  if (!sb) return ETD_ERR_NOMEM;
+  int rc = ETD_OK;
+  for (int b = 0; b < c->P && !rc; b++) {
+    box_view(c, b);
+    rc = launch_diag<double>(c, b == 0 ? st : sb);
+    if (rc || b == 0) continue;
+    for (int v = 0; v < c->V; v++) {
+      if (sb[4 * v] > st[4 * v]) st[4 * v] = sb[4 * v];
+      if (sb[4 * v + 1] < st[4 * v + 1]) st[4 * v + 1] = sb[4 * v + 1];
+      st[4 * v + 2] += sb[4 * v + 2];
+      st[4 * v + 3] += sb[4 * v + 3];
+    }
+    st[4 * c->V] += sb[4 * c->V];
+  }
+  box_view(c, 0);
+  free(sb);
+  return rc;
+}
+static int diag_any(etd_ctx *c, double *st) {
+  if (c->P > 1) return diag_boxes(c, st);
+  return c->mixed ? launch_diag<float>(c, st) : launch_diag<double>(c, st);
+}
 
 extern "C" int etd_step(etd_ctx *c, int nsteps) {
   if (!c || nsteps < 0) return ETD_ERR_ARG;
@@ -601,9 +705,12 @@ extern "C" int etd_write_csv(etd_ctx *c, int iter, const double *stats) {
 
 extern "C" int etd_write_field(etd_ctx *c, int var, int ioutput) {
   if (!c || var < 0 || var >= c->V) return ETD_ERR_ARG;
-  double *h = (double *)malloc(sizeof(double) * c->n3);
+  double *h = (double *)malloc(sizeof(double) * c->n3 * c->P);
+  if (!h) return ETD_ERR_NOMEM;
   int rc = etd_get_field(c, var, h);
-  if (!rc) rc = etd_field_file(c->p.data_dir, var, ioutput, c->n, h);
+  for (int b = 0; b < c->P && !rc; b++)  // one file per box, named by its rank coordinates
+    rc = etd_field_file(c->p.data_dir, var, ioutput, c->n, h + b * c->n3, b % c->px, (b / c->px) % c->py,
+                        b / (c->px * c->py));
   free(h);
   return rc;
 }
@@ -626,10 +733,26 @@ template <typename T>
 static int k_elastic_t(etd_ctx *c, double *host_felas) {
   launch_elastic<T>(c);
   if (host_felas) {
-    const size_t cnt = c->n3 * c->nel;
+    // felas p of box b at (p P + b) n3
+    const size_t cnt = c->n3 * c->nel * c->P;
     T *d;
     HCHK(c, hipMalloc(&d, sizeof(T) * cnt));
-    launch_felas_materialize<T>(c, d);
+    if (c->P == 1) {
+      launch_felas_materialize<T>(c, d);
+    } else if constexpr (sizeof(T) == 8) {
+      T *t;
+      HCHK(c, hipMalloc(&t, sizeof(T) * c->n3 * c->nel));
+      for (int b = 0; b < c->P; b++) {
+        box_view(c, b);
+        launch_felas_materialize<T>(c, t);
+        for (int q = 0; q < c->nel; q++)
+          HCHK(c, hipMemcpyAsync(d + ((size_t)q * c->P + b) * c->n3, t + (size_t)q * c->n3, sizeof(T) * c->n3,
+                                 hipMemcpyDeviceToDevice, c->stream));
+      }
+      box_view(c, 0);
+      HCHK(c, hipStreamSynchronize(c->stream));
+      HCHK(c, hipFree(t));
+    }
     int rc = ETD_OK;
     if constexpr (sizeof(T) == 8) {
       HCHK(c, hipMemcpyAsync(host_felas, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
@@ -654,7 +777,7 @@ extern "C" int etd_k_elastic(etd_ctx *c, double *host_felas) {
 extern "C" int etd_k_fft3d(etd_ctx *c, double *data, int dir) {
   if (!c || !data || (dir != 1 && dir != -1)) return ETD_ERR_ARG;
   if (!c->tw) return seterr(c, ETD_ERR_ARG, "fft3d: the context has no elastic FFT plan");
-  const size_t cnt = 2 * c->N3;
+  const size_t cnt = 2 * c->Ne3;
   if (!c->mixed) {
     double *d;
     HCHK(c, hipMalloc(&d, sizeof(double) * cnt));
@@ -684,9 +807,11 @@ extern "C" int etd_k_nonlinear(etd_ctx *c, int var, double *host_out) {
     HCHK(c, hipGetLastError());
     return get_f32(c, host_out, c->f32.N1 + c->n3 * var, c->n3);
   }
-  launch_nonlinear<double>(c, c->N1, nullptr);
+  nonlinear_boxes<double>(c, c->N1, (const double *)nullptr, 0);
   HCHK(c, hipGetLastError());
-  HCHK(c, hipMemcpyAsync(host_out, c->N1 + c->n3 * var, sizeof(double) * c->n3, hipMemcpyDeviceToHost, c->stream));
+  for (int b = 0; b < c->P; b++)  // a decomposed run: the boxes one after another
+    HCHK(c, hipMemcpyAsync(host_out + b * c->n3, c->N1 + b * c->bs + c->n3 * var, sizeof(double) * c->n3,
+                           hipMemcpyDeviceToHost, c->stream));
   HCHK(c, hipStreamSynchronize(c->stream));
   return ETD_OK;
 }

@@ -58,6 +58,7 @@ extern "C" int etd_ensemble_create(etd_ctx *const *members, int count, int steps
   int dev = -1;
   for (int i = 0; i < count; i++) {
     etd_ctx *c = members[i];
+    if (c && c->P > 1) return ETD_ERR_UNSUPPORTED;  // members are single boxes
     if (!c) return enerr(e, ETD_ERR_ARG, "null member");
     e->m[i] = c;
     int d;

@@ -1,7 +1,9 @@
 /*
  * etd_internal.h -- device state and kernel launchers of libetd (gfx950).
  *
- * HBM layout (plain FP64 path), n = extended edge, n3 = n^3, V = nac + nch:
+ * HBM layout (plain FP64 path), n = extended edge, n3 = n^3, V = nac + nch
+ * (a decomposed run repeats every field buffer per box, box-major; the
+ * elastic buffers then cover the global grid of edge Ne = px N):
  *   u     V x n3   fields, xyz (x fastest), ghosts included
  *   N1    V x n3   N(u_n); N2 V x n3: N(a) - N(u_n) for the ETD2 corrector
  *   w1,w2 2V x n3  mode-product intermediates (u and N batched together)
@@ -50,6 +52,16 @@ struct etd_ctx {
   int device;
   int n, g, N, V, nac, nch;
   size_t n3, N3;
+  /* decomposed run (procs px, py, pz, periodic): P boxes of edge n, box b at
+   * (b % px, (b / px) % py, b / (px py)); every field buffer holds the boxes
+   * one after another, box b's V fields at b bs (bs = V n3).  box: the box
+   * the per-box launches see (u, N1, N2, spec point at its fields); Ne: edge
+   * of the elastic grid (px N; N for one box).  bu, bN1, bN2, bspec: box 0,
+   * the buffers' starts. */
+  int px, py, pz, P, box;
+  int Ne;
+  size_t Ne3, bs;
+  double *bu, *bN1, *bN2, *bspec;
   hipStream_t stream;
   char err[512];
   /* host copies */
@@ -136,7 +148,8 @@ extern "C" {
 /* host-side helpers in etd_io.c */
 void etd_host_noise(int n, int g, unsigned long long seed, double mean, double amp, double *u);
 int etd_csv_rows(const char *work_dir, int nac, int nch, int iter, const double *stats);
-int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u);
+/* outputfield (field.c:17-56): one box, (cx, cy, cz) its Cartesian rank coordinates */
+int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u, int cx, int cy, int cz);
 /* CALPHAD Gibbs energies (free_energy 1); variable-mobility x(1-x), D_A, D_G of Cu, Mn, Ni */
 void etd_calphad(double *gbcc, double *ghcp);
 void etd_vm_constants(double X[3], double DA[3], double DG[3]);

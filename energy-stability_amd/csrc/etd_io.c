@@ -404,9 +404,10 @@ __attribute__((visibility("hidden"))) int etd_csv_rows(const char *work_dir, int
 }
 
 /* outputfield (field.c:17-37): text %+1.15lf, one value per line, full n^3 */
-__attribute__((visibility("hidden"))) int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u) {
+__attribute__((visibility("hidden"))) int etd_field_file(const char *data_dir, int var, int ioutput, int n, const double *u,
+                                                       int cx, int cy, int cz) {
   char fn[2048];
-  snprintf(fn, sizeof fn, "%seta%d_%06d_%02d%02d%02d.dat", data_dir, var, ioutput, 0, 0, 0);
+  snprintf(fn, sizeof fn, "%seta%d_%06d_%02d%02d%02d.dat", data_dir, var, ioutput, cx, cy, cz);
   FILE *f = fopen(fn, "w");
   if (!f) return ETD_ERR_IO;
   for (size_t l = 0; l < (size_t)n * n * n; l++) fprintf(f, "%+1.15lf\n", u[l]);

@@ -82,8 +82,15 @@ __device__ __forceinline__ void idx3(size_t l, int n, int &i, int &j, int &k) {
 // comes from the pre-transfer state, z faces override y faces override x
 // faces.  In place: z phase, then y phase (reads x-ghost cells of interior
 // planes, not yet rewritten), then x phase (reads interior cells only).
+// In a decomposed run (px x py x pz boxes) the source of the low (high)
+// ghosts is the neighbouring box before (after) this one along the sweep's
+// axis, lo / hi elements away: the z and y exchanges carry whole planes and
+// rows of the neighbour, its own ghost cells included, and every sweep reads
+// only cells the sweeps before it do not write, so running each sweep over
+// all boxes before the next reads pre-transfer data, as the reference's
+// send-everything-then-unpack order does.
 template <typename T>
-__global__ void k_ghost_z(T *base, size_t fs, int n, int g) {
+__global__ void k_ghost_z(T *base, size_t fs, int n, int g, long long lo, long long hi) {
   T *u = base + fs * blockIdx.y;
   const size_t n2 = (size_t)n * n;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,10 +98,10 @@ __global__ void k_ghost_z(T *base, size_t fs, int n, int g) {
   int pl = (int)(t / n2);
   size_t r = t % n2;
   int k = pl < g ? pl : n - 2 * g + pl;
-  u[(size_t)k * n2 + r] = u[(size_t)wrap_e(k, n, g) * n2 + r];
+  u[(size_t)k * n2 + r] = u[(pl < g ? lo : hi) + (long long)((size_t)wrap_e(k, n, g) * n2 + r)];
 }
 template <typename T>
-__global__ void k_ghost_y(T *base, size_t fs, int n, int g) {
+__global__ void k_ghost_y(T *base, size_t fs, int n, int g, long long lo, long long hi) {
   T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = 2 * g;
@@ -103,10 +110,10 @@ __global__ void k_ghost_y(T *base, size_t fs, int n, int g) {
   int rr = (int)((t / n) % rows);
   int k = g + (int)(t / ((size_t)n * rows));
   int j = rr < g ? rr : n - 2 * g + rr;
-  u[IDX(n, k, j, i)] = u[IDX(n, k, wrap_e(j, n, g), i)];
+  u[IDX(n, k, j, i)] = u[(rr < g ? lo : hi) + (long long)IDX(n, k, wrap_e(j, n, g), i)];
 }
 template <typename T>
-__global__ void k_ghost_x(T *base, size_t fs, int n, int g) {
+__global__ void k_ghost_x(T *base, size_t fs, int n, int g, long long lo, long long hi) {
   T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int cols = 2 * g, m = n - 2 * g;
@@ -115,9 +122,17 @@ __global__ void k_ghost_x(T *base, size_t fs, int n, int g) {
   int j = g + (int)((t / cols) % m);
   int k = g + (int)(t / ((size_t)cols * m));
   int i = cc < g ? cc : n - 2 * g + cc;
-  u[IDX(n, k, j, i)] = u[IDX(n, k, j, wrap_e(i, n, g))];
+  u[IDX(n, k, j, i)] = u[(cc < g ? lo : hi) + (long long)IDX(n, k, j, wrap_e(i, n, g))];
 }
 
+// element offset from box b's fields to those of the box (sx, sy, sz) away
+static long long host_box_off(const etd_ctx *c, int b, int sx, int sy, int sz) {
+  const int bx = b % c->px, by = (b / c->px) % c->py, bz = b / (c->px * c->py);
+  const int x = (bx + sx + c->px) % c->px, y = (by + sy + c->py) % c->py, z = (bz + sz + c->pz) % c->pz;
+  return (long long)((z * c->py + y) * c->px + x - b) * (long long)c->bs;
+}
+
+// base: box 0's first field; the boxes' fields follow at stride bs
 template <typename T>
 void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
   if (!c->p.periodic) return;  // physical sides: no neighbour, unpack writes nothing (trans.c:154-251)
@@ -125,9 +140,15 @@ void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
   const size_t n2 = (size_t)n * n;
   size_t tz = 2 * g * n2, ty = (size_t)(n - 2 * g) * 2 * g * n, tx = (size_t)(n - 2 * g) * (n - 2 * g) * 2 * g;
   etd_prof_begin(c, PC_HALO);
-  k_ghost_z<T><<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
-  k_ghost_y<T><<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
-  k_ghost_x<T><<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(base, fstride, n, g);
+  for (int b = 0; b < c->P; b++)
+    k_ghost_z<T><<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, 0, -1), host_box_off(c, b, 0, 0, 1));
+  for (int b = 0; b < c->P; b++)
+    k_ghost_y<T><<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, -1, 0), host_box_off(c, b, 0, 1, 0));
+  for (int b = 0; b < c->P; b++)
+    k_ghost_x<T><<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, -1, 0, 0), host_box_off(c, b, 1, 0, 0));
   etd_prof_end(c, PC_HALO);
 }
 
@@ -760,11 +781,41 @@ struct nl_args {
   int fe;                   // free energy: 0 Landau polynomial, 1 CALPHAD (alpha)
   double gbcc, ghcp;        // CALPHAD Gibbs energies (fe == 1)
   const unsigned char *mask;  // grain mask or NULL
+  // decomposed run: px x py x pz boxes (P of them), this box at (bx, by, bz),
+  // bs elements between the same field of consecutive boxes, and this box's
+  // first interior cell in the global grid of edge N (the elastic grid)
+  int P, px, py, pz, bx, by, bz;
+  long long bs;
+  int bxo, byo, bzo;
 };
 
+// -1 / 0 / +1: the extended coordinate e lies before, in or after the interior
+__device__ __forceinline__ int box_shift(int e, int n, int g) { return e < g ? -1 : (e >= n - g ? 1 : 0); }
+// element offset from this box's array to the same array of the box (sx, sy,
+// sz) away, periodically; 0 for one box
+__device__ __forceinline__ long long box_off(const nl_args &a, int sx, int sy, int sz) {
+  if (a.P == 1) return 0;
+  int x = a.bx + sx, y = a.by + sy, z = a.bz + sz;
+  x += x < 0 ? a.px : (x >= a.px ? -a.px : 0);
+  y += y < 0 ? a.py : (y >= a.py ? -a.py : 0);
+  z += z < 0 ? a.pz : (z >= a.pz ? -a.pz : 0);
+  return (long long)((z * a.py + y) * a.px + x - ((a.bz * a.py + a.by) * a.px + a.bx)) * a.bs;
+}
+// The periodic interior image of extended position (z, y, x), -2 <= z, y, x
+// < n + 2, which the exchanged and relayed slabs hold (trans.c:7-925): in a
+// decomposed run it lies in the box the position points into (the diagonal
+// box for an edge or corner position).
 template <typename T>
-__device__ __forceinline__ double Pimg(const T *u, int n, int g, int z, int y, int x) {
-  return u[IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+__device__ __forceinline__ double Pimg(const T *u, const nl_args &a, int z, int y, int x) {
+  const int n = a.n, g = a.g;
+  return u[box_off(a, box_shift(x, n, g), box_shift(y, n, g), box_shift(z, n, g)) +
+           (long long)IDX(n, wrap_e(z, n, g), wrap_e(y, n, g), wrap_e(x, n, g))];
+}
+// the global interior coordinate of extended coordinate e of a box whose
+// interior starts at o, periodic in a grid of edge Ng
+__device__ __forceinline__ int gwrap(int e, int o, int g, int Ng) {
+  const int X = o + e - g;
+  return X < 0 ? X + Ng : (X >= Ng ? X - Ng : X);
 }
 
 // Elastic driving force of AC field p at extended cell (k,j,i).  The
@@ -786,7 +837,7 @@ __device__ __forceinline__ double felas_at(const T *E, const nl_args &a, int p, 
   int slot = a.slot0 + p - lag;
   if (slot >= a.ring) slot -= a.ring;
   if (slot < 0) slot += a.ring;
-  const int wk = wrap_e(k, n, g) - g, wj = wrap_e(j, n, g) - g, wi = wrap_e(i, n, g) - g;
+  const int wk = gwrap(k, a.bzo, g, N), wj = gwrap(j, a.byo, g, N), wi = gwrap(i, a.bxo, g, N);
   return E[(size_t)slot * N * N * N + ((size_t)wk * N + wj) * N + wi];
 }
 
@@ -803,7 +854,7 @@ __device__ __forceinline__ felas_cell felas_locate(const nl_args &a, int k, int 
   int lag = gk + gj + gi - 1;
   felas_cell c;
   c.lag = lag < 0 ? 0 : lag;
-  const int wk = wrap_e(k, n, g) - g, wj = wrap_e(j, n, g) - g, wi = wrap_e(i, n, g) - g;
+  const int wk = gwrap(k, a.bzo, g, N), wj = gwrap(j, a.byo, g, N), wi = gwrap(i, a.bxo, g, N);
   c.off = ((size_t)wk * N + wj) * N + wi;
   return c;
 }
@@ -836,13 +887,19 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
   idx3(l, n, i, j, k);
   const unsigned lo = (unsigned)l;
   // explicit outside coupling (the reference's calc_mu values): a cell lies on
-  // at most one box face per axis, and its image offset is the same for every
-  // field; -1 when the cell is on no face of that axis
+  // at most one box face per axis (fx, fy, fz), and its image offset is the
+  // same for every field; in a decomposed run the image lies in the
+  // neighbouring, possibly diagonal, box
   const int wk = wrap_e(k, n, g), wj = wrap_e(j, n, g), wi = wrap_e(i, n, g);
-  int ox = i == 0 ? (int)IDX(n, wk, wj, wrap_e(-1, n, g)) : i == n - 1 ? (int)IDX(n, wk, wj, wrap_e(n, n, g)) : -1;
-  int oy = j == 0 ? (int)IDX(n, wk, wrap_e(-1, n, g), wi) : j == n - 1 ? (int)IDX(n, wk, wrap_e(n, n, g), wi) : -1;
-  int oz = k == 0 ? (int)IDX(n, wrap_e(-1, n, g), wj, wi) : k == n - 1 ? (int)IDX(n, wrap_e(n, n, g), wj, wi) : -1;
-  if (a.phys) ox = oy = oz = -1;  // no outside values on physical sides
+  const int sk = box_shift(k, n, g), sj = box_shift(j, n, g), si = box_shift(i, n, g);
+  const bool fx = !a.phys && (i == 0 || i == n - 1), fy = !a.phys && (j == 0 || j == n - 1),
+             fz = !a.phys && (k == 0 || k == n - 1);  // no outside values on physical sides
+  const long long ox = !fx ? 0 : i == 0 ? (long long)IDX(n, wk, wj, wrap_e(-1, n, g)) + box_off(a, -1, sj, sk)
+                                       : (long long)IDX(n, wk, wj, wrap_e(n, n, g)) + box_off(a, 1, sj, sk);
+  const long long oy = !fy ? 0 : j == 0 ? (long long)IDX(n, wk, wrap_e(-1, n, g), wi) + box_off(a, si, -1, sk)
+                                       : (long long)IDX(n, wk, wrap_e(n, n, g), wi) + box_off(a, si, 1, sk);
+  const long long oz = !fz ? 0 : k == 0 ? (long long)IDX(n, wrap_e(-1, n, g), wj, wi) + box_off(a, si, sj, -1)
+                                       : (long long)IDX(n, wrap_e(n, n, g), wj, wi) + box_off(a, si, sj, 1);
   // physical sides: felas is 0 outside the interior
   const bool fel_zero = a.phys && (k < g || k >= n - g || j < g || j >= n - g || i < g || i >= n - g);
   double s = 0.0;
@@ -883,9 +940,9 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
         const double fp = a.a2 * up - a.a3 * up * up + a.a4 * s * up;
         f -= a.LE * fp;
       }
-      if (ox >= 0) f += ce * (double)up_f[ox] / a.hx / a.hx;
-      if (oy >= 0) f += ce * (double)up_f[oy] / a.hy / a.hy;
-      if (oz >= 0) f += ce * (double)up_f[oz] / a.hz / a.hz;
+      if (fx) f += ce * (double)up_f[ox] / a.hx / a.hx;
+      if (fy) f += ce * (double)up_f[oy] / a.hy / a.hy;
+      if (fz) f += ce * (double)up_f[oz] / a.hz / a.hz;
       if (with_felas) f -= a.LE * a.es * (double)fel[e];
       if (mask_here && a.mask && a.mask[lo] == 0) f -= 0.5 * up;
       if (sub) f -= sbv[e];
@@ -960,7 +1017,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_f2_full(const T *__restrict__ u, T *__restrict__ out,
                                                  const T *__restrict__ sub, const T *__restrict__ E,
                                                  const double *__restrict__ lam, nl_args a, int p, int with_felas) {
-  const int n = a.n, g = a.g;
+  const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= n3) return;
@@ -978,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_f2_full(const T *__restrict__ u, T *__r
       x = x < 0 ? -x : (x >= n ? 2 * (n - 1) - x : x);
     }
     if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) return u[IDX(n, z, y, x)];
-    return Pimg(u, n, g, z, y, x);
+    return Pimg(u, a, z, y, x);
   };
   const double m = U(0, 0, 0);
   const double fxx = (U(0, 0, 1) - 2 * m + U(0, 0, -1)) * a.ihx2;
@@ -1023,19 +1080,19 @@ struct ch_h {
 template <typename T>
 __device__ __forceinline__ double ch_t_at(const T *c, const T *E, int q, const nl_args &a, const ch_h &ih, int k, int j,
                                           int i) {
-  const int n = a.n, g = a.g;
+  const int n = a.n;
   const double cc = c[IDX(n, k, j, i)];
   double f = -a.LCI * a.KCI * cc;
   f += a.LCI * ch_f(cc);
   if (a.elastic == 2) f += a.LCI * felas_at(E, a, q, k, j, i);
   const double ce = a.LCI * a.epn2;
   if (a.phys) return f;  // no outside values on physical sides
-  if (i == 0) f -= ce * Pimg(c, n, g, k, j, -1) * ih.x;
-  if (i == n - 1) f -= ce * Pimg(c, n, g, k, j, n) * ih.x;
-  if (j == 0) f -= ce * Pimg(c, n, g, k, -1, i) * ih.y;
-  if (j == n - 1) f -= ce * Pimg(c, n, g, k, n, i) * ih.y;
-  if (k == 0) f -= ce * Pimg(c, n, g, -1, j, i) * ih.z;
-  if (k == n - 1) f -= ce * Pimg(c, n, g, n, j, i) * ih.z;
+  if (i == 0) f -= ce * Pimg(c, a, k, j, -1) * ih.x;
+  if (i == n - 1) f -= ce * Pimg(c, a, k, j, n) * ih.x;
+  if (j == 0) f -= ce * Pimg(c, a, k, -1, i) * ih.y;
+  if (j == n - 1) f -= ce * Pimg(c, a, k, n, i) * ih.y;
+  if (k == 0) f -= ce * Pimg(c, a, -1, j, i) * ih.z;
+  if (k == n - 1) f -= ce * Pimg(c, a, n, j, i) * ih.z;
   return f;
 }
 
@@ -1043,17 +1100,17 @@ __device__ __forceinline__ double ch_t_at(const T *c, const T *E, int q, const n
 // periodic image (ch_mu, trans.c:1433-1937)
 template <typename T>
 __device__ double ch_mu_out(const T *c, const T *E, int q, const nl_args &a, const ch_h &ih, int z, int y, int x) {
-  const int n = a.n, g = a.g;
-  const double v0 = Pimg(c, n, g, z, y, x);
-  double mu = (Pimg(c, n, g, z, y, x - 1) + (-2.0) * v0 + Pimg(c, n, g, z, y, x + 1)) * ih.x;
-  mu += (Pimg(c, n, g, z, y - 1, x) + (-2.0) * v0 + Pimg(c, n, g, z, y + 1, x)) * ih.y;
-  mu += (Pimg(c, n, g, z - 1, y, x) + (-2.0) * v0 + Pimg(c, n, g, z + 1, y, x)) * ih.z;
+  const int g = a.g;
+  const double v0 = Pimg(c, a, z, y, x);
+  double mu = (Pimg(c, a, z, y, x - 1) + (-2.0) * v0 + Pimg(c, a, z, y, x + 1)) * ih.x;
+  mu += (Pimg(c, a, z, y - 1, x) + (-2.0) * v0 + Pimg(c, a, z, y + 1, x)) * ih.y;
+  mu += (Pimg(c, a, z - 1, y, x) + (-2.0) * v0 + Pimg(c, a, z + 1, y, x)) * ih.z;
   mu = a.LCI * a.epn2 * mu;
   mu -= a.LCI * ch_f(v0);
   if (a.elastic == 2) {
     // ELASTIC = 2 (SWITCH_CH_FIELDMU, ScLETD.h:153-156): the force of the latest evaluation at the interior image
     const int N = a.N;
-    const size_t off = ((size_t)(wrap_e(z, n, g) - g) * N + (wrap_e(y, n, g) - g)) * N + (wrap_e(x, n, g) - g);
+    const size_t off = ((size_t)gwrap(z, a.bzo, g, N) * N + gwrap(y, a.byo, g, N)) * N + gwrap(x, a.bxo, g, N);
     mu -= a.LCI * (double)E[(size_t)felas_slot(a, q, 0) * N * N * N + off];
   }
   return mu;
@@ -1179,7 +1236,7 @@ static nl_args make_nl(etd_ctx *c) {
   nl_args a;
   a.n = c->n;
   a.g = c->g;
-  a.N = c->N;
+  a.N = c->Ne;  // the elastic grid: the felas ring holds its interior
   a.nac = c->nac;
   a.nch = c->nch;
   a.aniso = c->p.anisotropic;
@@ -1207,6 +1264,17 @@ static nl_args make_nl(etd_ctx *c) {
   a.gbcc = c->gbcc;
   a.ghcp = c->ghcp;
   a.mask = c->mask;
+  a.P = c->P;
+  a.px = c->px;
+  a.py = c->py;
+  a.pz = c->pz;
+  a.bx = c->box % c->px;
+  a.by = (c->box / c->px) % c->py;
+  a.bz = c->box / (c->px * c->py);
+  a.bs = (long long)c->bs;
+  a.bxo = a.bx * c->N;
+  a.byo = a.by * c->N;
+  a.bzo = a.bz * c->N;
   return a;
 }
 
@@ -1280,7 +1348,7 @@ __device__ __forceinline__ double lap_zero_at(const T *a, int n, size_t l, int k
 template <typename T>
 __global__ __launch_bounds__(256) void k_vm_ft(const T *__restrict__ c, T *__restrict__ ft1, T *__restrict__ ft2,
                                                nl_args a) {
-  const int n = a.n, g = a.g;
+  const int n = a.n;
   const size_t n3 = (size_t)n * n * n;
   const size_t l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= n3) return;
@@ -1296,12 +1364,12 @@ __global__ __launch_bounds__(256) void k_vm_ft(const T *__restrict__ c, T *__res
     ft2[l] = (T)t2;
     return;
   }
-  if (i == 0) fc += a.epn2 * Pimg(c, n, g, k, j, -1) * a.ihx2;
-  if (i == n - 1) fc += a.epn2 * Pimg(c, n, g, k, j, n) * a.ihx2;
-  if (j == 0) fc += a.epn2 * Pimg(c, n, g, k, -1, i) * a.ihy2;
-  if (j == n - 1) fc += a.epn2 * Pimg(c, n, g, k, n, i) * a.ihy2;
-  if (k == 0) fc += a.epn2 * Pimg(c, n, g, -1, j, i) * a.ihz2;
-  if (k == n - 1) fc += a.epn2 * Pimg(c, n, g, n, j, i) * a.ihz2;
+  if (i == 0) fc += a.epn2 * Pimg(c, a, k, j, -1) * a.ihx2;
+  if (i == n - 1) fc += a.epn2 * Pimg(c, a, k, j, n) * a.ihx2;
+  if (j == 0) fc += a.epn2 * Pimg(c, a, k, -1, i) * a.ihy2;
+  if (j == n - 1) fc += a.epn2 * Pimg(c, a, k, n, i) * a.ihy2;
+  if (k == 0) fc += a.epn2 * Pimg(c, a, -1, j, i) * a.ihz2;
+  if (k == n - 1) fc += a.epn2 * Pimg(c, a, n, j, i) * a.ihz2;
   ft1[l] = (T)(t1 + fc);
   ft2[l] = (T)(t2 + fc);
 }
@@ -1799,14 +1867,14 @@ bool fft16_supported(int N) {
 
 template <int SGN, typename T2, int IO = 0>
 static bool launch_fft16(etd_ctx *c, T2 *data, int npair, int axis, fft_io io = fft_io{}) {
-  const int N = c->N;
+  const int N = c->Ne;
   if (!fft16_supported(N)) return false;
   dim3 grid((unsigned)((size_t)N * N / 16), npair);
   const double2 *tw = (const double2 *)c->tw;
   switch (N / 16) {
 #define F16(R)                                                                                   \
   case R:                                                                                        \
-    k_fft16<SGN, R, T2, IO><<<grid, 256, 0, c->stream>>>(data, c->N3, axis, tw, io);             \
+    k_fft16<SGN, R, T2, IO><<<grid, 256, 0, c->stream>>>(data, c->Ne3, axis, tw, io);            \
     return true;
     F16(2) F16(3) F16(4) F16(5) F16(6) F16(8) F16(9) F16(10) F16(12) F16(15) F16(16)
 #undef F16
@@ -1818,21 +1886,21 @@ template <typename T>
 void fft3d(etd_ctx *c, T *data, int npair, int sgn) {
   using T2 = typename cplx<T>::t;
   fft_plan pl;
-  pl.N = c->N;
+  pl.N = c->Ne;
   pl.npass = c->fft_npass;
   for (int i = 0; i < 16; i++) pl.radix[i] = c->fft_radix[i];
-  const size_t N2 = (size_t)c->N * c->N;
+  const size_t N2 = (size_t)c->Ne * c->Ne;
   etd_prof_begin(c, PC_FFT);
   dim3 grid((unsigned)((N2 + FFT_LINES - 1) / FFT_LINES), npair);
-  size_t lds = 2 * FFT_LINES * (size_t)c->N * sizeof(double2);
+  size_t lds = 2 * FFT_LINES * (size_t)c->Ne * sizeof(double2);
   for (int ax = 0; ax < 3; ax++) {
     if (c->fft_variant == 1 && (sgn > 0 ? launch_fft16<1, T2>(c, (T2 *)data, npair, ax)
                                         : launch_fft16<-1, T2>(c, (T2 *)data, npair, ax)))
       continue;
     if (sgn > 0)
-      k_fft_axis<1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+      k_fft_axis<1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->Ne3, ax, pl, (const double2 *)c->tw);
     else
-      k_fft_axis<-1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->N3, ax, pl, (const double2 *)c->tw);
+      k_fft_axis<-1, T2><<<grid, 256, lds, c->stream>>>((T2 *)data, c->Ne3, ax, pl, (const double2 *)c->tw);
   }
   etd_prof_end(c, PC_FFT);
 }
@@ -1848,6 +1916,29 @@ __global__ void k_el_pack(const T *__restrict__ u, typename cplx<T>::t *__restri
   idx3(m, N, x, y, z);
   const size_t n3 = (size_t)n * n * n;
   const size_t src = IDX(n, z + g, y + g, x + g);
+  const int pa = 2 * r, pb = 2 * r + 1;
+  typename cplx<T>::t zz;
+  zz.x = u[n3 * pa + src];
+  zz.y = pb < nac ? u[n3 * pb + src] : (T)0;
+  Z[N3 * r + m] = zz;
+}
+
+// the same over a decomposed run's boxes (px per axis, fields of box b at
+// b bs): global cell (x, y, z) of the (px N)^3 grid is interior cell
+// (x % N, y % N, z % N) of box (x / N, y / N, z / N)
+template <typename T>
+__global__ void k_el_pack_boxes(const T *__restrict__ u, typename cplx<T>::t *__restrict__ Z, int n, int g, int N,
+                                int nac, int px, long long bs) {
+  const int Ng = px * N;
+  const size_t N3 = (size_t)Ng * Ng * Ng;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N3) return;
+  const int r = blockIdx.y;
+  int x, y, z;
+  idx3(m, Ng, x, y, z);
+  const size_t n3 = (size_t)n * n * n;
+  const int b = ((z / N) * px + y / N) * px + x / N;
+  const long long src = b * bs + (long long)IDX(n, z % N + g, y % N + g, x % N + g);
   const int pa = 2 * r, pb = 2 * r + 1;
   typename cplx<T>::t zz;
   zz.x = u[n3 * pa + src];
@@ -2095,8 +2186,8 @@ template <typename T>
 void launch_elastic(etd_ctx *c) {
   using T2 = typename cplx<T>::t;
   const etd_fields<T> F = fields<T>(c);
-  const int N = c->N;
-  const size_t N3 = c->N3;
+  const int N = c->Ne;  // the global interior of a decomposed run (fftMPI's distributed transform)
+  const size_t N3 = c->Ne3;
   const unsigned nb = (unsigned)((N3 + 255) / 256);
   etd_prof_begin(c, PC_ELASTIC);
   const long long Lbase = c->elastic_evals * (long long)c->nel;
@@ -2106,7 +2197,7 @@ void launch_elastic(etd_ctx *c) {
   // inverse z pass's output
   // (the fused passes exist only where the two-stage kernel does: N = 16 R2
   // with an instantiated R2; elsewhere pack, radix passes and unpack)
-  const bool fused = c->fft_variant == 1 && fft16_supported(N);
+  const bool fused = c->fft_variant == 1 && fft16_supported(N) && c->P == 1;
   fft_io io{uel, F.E, c->n, c->g, c->nel, c->ring, slot0};
   if (fused) {
     etd_prof_begin(c, PC_FFT);
@@ -2115,7 +2206,11 @@ void launch_elastic(etd_ctx *c) {
     launch_fft16<1, T2>(c, (T2 *)F.Z, c->npair, 2);
     etd_prof_end(c, PC_FFT);
   } else {
-    k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(uel, (T2 *)F.Z, c->n, c->g, N, c->nel);
+    if (c->P == 1)
+      k_el_pack<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(uel, (T2 *)F.Z, c->n, c->g, N, c->nel);
+    else
+      k_el_pack_boxes<T><<<dim3(nb, c->npair), 256, 0, c->stream>>>(uel, (T2 *)F.Z, c->n, c->g, c->N, c->nel, c->px,
+                                                                     (long long)c->bs);
     fft3d<T>(c, F.Z, c->npair, +1);
   }
   {
@@ -2552,6 +2647,10 @@ __global__ __launch_bounds__(256) void k_diag_finish(const double *__restrict__ 
 
 extern "C" int etd_diag_noalloc(etd_ctx *c, double *stats_dev) {
   if (!c || !stats_dev) return ETD_ERR_ARG;
+  if (c->P > 1) {
+    snprintf(c->err, sizeof(c->err), "%s: one box only (px,py,pz = 1,1,1)", __func__);
+    return ETD_ERR_UNSUPPORTED;
+  }
   const size_t need = diag_partials(c);
   if (need > DIAG_SCRATCH_DOUBLES || c->V + c->nch + 2 > 256) {
     snprintf(c->err, sizeof(c->err),

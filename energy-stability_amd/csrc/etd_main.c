@@ -10,14 +10,20 @@
  * writing phi_ac_<n>.csv / phi_ch_<n>.csv / phi_couple.csv under work_dir.
  * Errors print the reference's message and exit(1).
  *
- * Initial state: restart = 1 reads <dir>/input/field_iter<it>_ac<n>_000.txt
- * (interior, one value per line, field.c:137-154,746-754); otherwise the
- * seeded noise of the synthetic workloads (ETD_SEED, default 1): AC fields
- * U[-0.01, 0.01], CH fields 0.1 + U[-0.005, 0.005].
+ * A procs line other than 1,1,1 runs the reference's decomposition on this
+ * GPU: px x py x pz boxes of nx^3, exchanged as its ranks would.
  *
- * Field dumps: every 6500 iterations (ETD_OUTPUT_EVERY overrides), inside the
+ * Initial state: restart = 1 reads <dir>/input/field_iter<it>_ac<n>_<cx><cy><cz>.txt
+ * per box (interior, one value per line, field.c:137-154,746-754);
+ * ETD_INIT=sphere the reference's own start for nch = 0 (init_field,
+ * field.c:757-766: the sphere of init_field_sphere_center in AC field 3, the
+ * other fields 0); otherwise the seeded noise of the synthetic workloads
+ * (ETD_SEED, default 1): AC fields U[-0.01, 0.01], CH fields
+ * 0.1 + U[-0.005, 0.005].
+ *
+ * Field dumps: every 6500 iterations (ETD_OUTPUT_EVERY overrides; 0 none), inside the
  * diagnostics as in check_soln_new (field.c:667-668), outputfield writes
- * <data_dir>eta<v>_<ioutput:06d>_000000.dat with ioutput starting at iter/100
+ * <data_dir>eta<v>_<ioutput:06d>_<cx:02d><cy:02d><cz:02d>.dat per box with ioutput starting at iter/100
  * (ScLETD.c:83) and counting up per dump (field.c:55).  The reference's loop
  * starts at variant 3 (field.c:23); every AC variant is written here, a
  * superset of its files.
@@ -51,25 +57,62 @@
 
 #include "etd.h"
 
-static int read_restart(etd_ctx *c, const char *dir, const char *kind, int var, int idx, int iter, int n, int g) {
-  char fn[2048];
-  snprintf(fn, sizeof fn, "%s/input/field_iter%d_%s%d_%d%d%d.txt", dir, iter, kind, idx, 0, 0, 0);
-  FILE *f = fopen(fn, "r");
-  if (!f) {
-    printf("fopen error %s!\n", fn);
-    return -1;
+/* restart files (field.c:137-154, 746-754): one per box, named by its rank
+ * coordinates, interior values */
+static int read_restart(etd_ctx *c, const etd_params *p, const char *dir, const char *kind, int var, int idx,
+                        int iter) {
+  const int n = p->nx, g = p->nghost, P = p->px * p->py * p->pz;
+  const size_t n3 = (size_t)n * n * n;
+  double *u = calloc(n3 * P, sizeof(double));
+  if (!u) return -1;
+  for (int b = 0; b < P; b++) {
+    char fn[2048];
+    snprintf(fn, sizeof fn, "%s/input/field_iter%d_%s%d_%d%d%d.txt", dir, iter, kind, idx, b % p->px,
+             (b / p->px) % p->py, b / (p->px * p->py));
+    FILE *f = fopen(fn, "r");
+    if (!f) {
+      printf("fopen error %s!\n", fn);
+      free(u);
+      return -1;
+    }
+    double *ub = u + b * n3;
+    for (int k = g; k < n - g; k++)
+      for (int j = g; j < n - g; j++)
+        for (int i = g; i < n - g; i++)
+          if (fscanf(f, "%lf", &ub[((size_t)k * n + j) * n + i]) != 1) {
+            fclose(f);
+            free(u);
+            return -1;
+          }
+    fclose(f);
   }
-  double *u = calloc((size_t)n * n * n, sizeof(double));
-  for (int k = g; k < n - g; k++)
-    for (int j = g; j < n - g; j++)
-      for (int i = g; i < n - g; i++)
-        if (fscanf(f, "%lf", &u[((size_t)k * n + j) * n + i]) != 1) {
-          fclose(f);
-          free(u);
-          return -1;
-        }
-  fclose(f);
   int rc = etd_set_field(c, var, u);
+  free(u);
+  return rc;
+}
+
+/* init_field_sphere_center (field.c:379-405), the reference's start for
+ * nch = 0 (init_field, field.c:757-766): AC field 3 is 1 where a box's own
+ * coordinate (cx nx + i, cy ny + j, cz nz + k), ghost cells included, lies
+ * within 8 of nx px / 2 on every axis, and 0 elsewhere; the other fields 0. */
+static int init_sphere(etd_ctx *c, const etd_params *p) {
+  const int n = p->nx, P = p->px * p->py * p->pz;
+  const size_t n3 = (size_t)n * n * n;
+  double *u = calloc(n3 * P, sizeof(double));
+  if (!u) return -1;
+  int rc = 0;
+  for (int v = 0; v < p->nac + p->nch && !rc; v++) rc = etd_set_field(c, v, u);
+  const double cnt = n * p->px / 2;
+  for (int b = 0; b < P; b++) {
+    const int cx = b % p->px, cy = (b / p->px) % p->py, cz = b / (p->px * p->py);
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          const double x = cx * n + i - cnt, y = cy * n + j - cnt, z = cz * n + k - cnt;
+          u[b * n3 + ((size_t)k * n + j) * n + i] = x * x + y * y + z * z < 8.0 * 8.0 ? 1.0 : 0.0;
+        }
+  }
+  if (!rc) rc = etd_set_field(c, 3, u);
   free(u);
   return rc;
 }
@@ -244,11 +287,18 @@ int main(int argc, char **argv) {
   printf("ELASTIC\t\t%d\n", p.elastic);
   printf("ANISOTROPIC\t\t%d\n", p.anisotropic);
   const int V = p.nac + p.nch;
+  const char *init = getenv("ETD_INIT");
   if (p.restart == 1) {
     for (int v = 0; v < p.nac; v++)
-      if (read_restart(c, dir, "ac", v, v, p.restart_iter, p.nx, p.nghost)) exit(1);
+      if (read_restart(c, &p, dir, "ac", v, v, p.restart_iter)) exit(1);
     for (int v = 0; v < p.nch; v++)
-      if (read_restart(c, dir, "ch", p.nac + v, v, p.restart_iter, p.nx, p.nghost)) exit(1);
+      if (read_restart(c, &p, dir, "ch", p.nac + v, v, p.restart_iter)) exit(1);
+  } else if (init && !strcmp(init, "sphere")) {
+    if (p.nch != 0 || p.nac < 4) {
+      printf("ETD_INIT=sphere is the reference's nch = 0 start in AC field 3 (nac >= 4)\n");
+      exit(1);
+    }
+    if (init_sphere(c, &p)) exit(1);
   } else {
     const char *s = getenv("ETD_SEED");
     unsigned long long seed = s ? strtoull(s, NULL, 10) : 1ULL;
@@ -261,7 +311,7 @@ int main(int argc, char **argv) {
   double tt = iter * p.dt;
   int ioutput = iter / 100;
   const char *oe = getenv("ETD_OUTPUT_EVERY");
-  const int out_every = oe && atoi(oe) > 0 ? atoi(oe) : 6500;
+  const int out_every = oe ? atoi(oe) : 6500;  // 0 or less: no field dumps
   double *stats = malloc(sizeof(double) * (4 * V + 1));
   struct timespec a, b;
   while (tt < p.t_total) {
@@ -269,7 +319,7 @@ int main(int argc, char **argv) {
       fprintf(stderr, "check_soln_new: %s\n", etd_last_error(c));
       exit(1);
     }
-    if (iter % out_every == 0) {
+    if (out_every > 0 && iter % out_every == 0) {
       for (int v = 0; v < p.nac; v++)
         if (etd_write_field(c, v, ioutput)) {
           printf("fopen error writing eta%d under %s\n", v, p.data_dir);

@@ -359,8 +359,10 @@ def params_from_dict(prm):
     p.kkx, p.kky, p.kkz = kk
     h = prm.get("h", (1.0, 1.0, 1.0))
     N = n - 2 * p.nghost
+    procs = prm.get("procs", (1, 1, 1))  # a decomposed run: boxes per axis
+    p.px, p.py, p.pz = procs
     p.xmin = p.ymin = p.zmin = 0.0
-    p.xmax, p.ymax, p.zmax = h[0] * N, h[1] * N, h[2] * N
+    p.xmax, p.ymax, p.zmax = h[0] * N * procs[0], h[1] * N * procs[1], h[2] * N * procs[2]
     p.hx, p.hy, p.hz = h
     p.LE = prm.get("LE", 1.0)
     p.KE = prm.get("KE", 42.0)
@@ -393,13 +395,16 @@ def params_from_dict(prm):
 
 
 class Etd:
-    """One periodic subdomain on the GPU."""
+    """One subdomain on the GPU, or a decomposed run (px x py x pz boxes):
+    then fields, nonlinear terms and forces come per box, box b at index b
+    of the leading axis (b = (bz py + by) px + bx)."""
 
     def __init__(self, params, V=None, Vi=None, D=None):
         self.ctx = C.c_void_p()
         self.p = params
         self.n = params.nx
         self.V = params.nac + params.nch
+        self.P = params.px * params.py * params.pz
         if V is not None:
             Vc = np.ascontiguousarray(np.asarray(V).T).reshape(-1)
             Vic = np.ascontiguousarray(np.asarray(Vi).T).reshape(-1)
@@ -430,6 +435,7 @@ class Etd:
         lib().etd_get_params(obj.ctx, C.byref(obj.p))
         obj.n = obj.p.nx
         obj.V = obj.p.nac + obj.p.nch
+        obj.P = obj.p.px * obj.p.py * obj.p.pz
         return obj
 
     def close(self):
@@ -457,16 +463,20 @@ class Etd:
         assert m.size == self.n ** 3
         _check(L.etd_set_mask(self.ctx, m.ctypes.data_as(C.POINTER(C.c_int))), self.ctx, "set_mask")
 
+    def _boxes(self, a):
+        n = self.n
+        return a.reshape(n, n, n) if self.P == 1 else a.reshape(self.P, n, n, n)
+
     def set_field(self, var, data):
         a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
-        if a.size != self.n ** 3:
+        if a.size != self.P * self.n ** 3:
             raise EtdError("set_field: wrong size")
         _check(lib().etd_set_field(self.ctx, var, _ptr(a)), self.ctx, "set_field")
 
     def get_field(self, var):
-        a = np.empty(self.n ** 3)
+        a = np.empty(self.P * self.n ** 3)
         _check(lib().etd_get_field(self.ctx, var, _ptr(a)), self.ctx, "get_field")
-        return a.reshape(self.n, self.n, self.n)
+        return self._boxes(a)
 
     def init_noise(self, var, seed, mean, amp):
         _check(lib().etd_init_noise(self.ctx, var, seed, mean, amp), self.ctx, "init_noise")
@@ -487,9 +497,11 @@ class Etd:
         """etd_k_elastic; the force on the AC fields (ELASTIC = 1) or the CH fields (2)."""
         if want:
             nel = self.p.nac if self.p.elastic == 1 else self.p.nch if self.p.elastic == 2 else 0
-            out = np.empty(max(nel, 1) * self.n ** 3)
+            out = np.empty(max(nel, 1) * self.P * self.n ** 3)
             _check(lib().etd_k_elastic(self.ctx, _ptr(out)), self.ctx, "elastic")
-            return out[: nel * self.n ** 3].reshape(nel, self.n, self.n, self.n)
+            n = self.n
+            out = out[: nel * self.P * n ** 3]
+            return out.reshape(nel, n, n, n) if self.P == 1 else out.reshape(nel, self.P, n, n, n)
         _check(lib().etd_k_elastic(self.ctx, None), self.ctx, "elastic")
         return None
 
@@ -501,9 +513,9 @@ class Etd:
         return a
 
     def nonlinear(self, var):
-        out = np.empty(self.n ** 3)
+        out = np.empty(self.P * self.n ** 3)
         _check(lib().etd_k_nonlinear(self.ctx, var, _ptr(out)), self.ctx, "nonlinear")
-        return out.reshape(self.n, self.n, self.n)
+        return self._boxes(out)
 
     def modeprod(self, A, u, rotate=True, slowest=False):
         """One mode product by A (n x n): over the fastest axis (PUX,

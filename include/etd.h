@@ -51,7 +51,12 @@ enum { ETD_PREC_FP64 = 0, ETD_PREC_MIXED = 1, ETD_PREC_MIXED_F16 = 2 };
  * (anisotropic.c:645-673) and the constants hard-coded in formula.c:9-29. */
 typedef struct etd_params {
   int nx, ny, nz;          /* extended edge (must be equal) */
-  int px, py, pz;          /* process grid (must be 1,1,1: one subdomain) */
+  int px, py, pz;          /* process grid: 1,1,1 one subdomain; otherwise the
+                              reference's decomposition into px x py x pz boxes
+                              of nx^3, all on this GPU, exchanged as its ranks
+                              exchange them (periodic = 1, FP64, Landau force,
+                              constant mobility; px = py = pz when elastic).
+                              Box b sits at (b % px, (b / px) % py, b / (px py)). */
   int nghost;
   char work_dir[1024];
   char data_dir[1024];
@@ -149,7 +154,8 @@ const char *etd_last_error(const etd_ctx *c);
 const char *etd_error_string(int code);
 int etd_get_params(const etd_ctx *c, etd_params *p);
 
-/* Host <-> device copies of one field (n^3 doubles, ghosts included). */
+/* Host <-> device copies of one field (n^3 doubles, ghosts included; a
+ * decomposed context takes and gives its boxes one after another, P n^3). */
 int etd_set_field(etd_ctx *c, int var, const double *host);
 int etd_get_field(etd_ctx *c, int var, double *host);
 
@@ -165,7 +171,8 @@ int etd_set_mask(etd_ctx *c, const int *mask);
 void etd_calphad(double *gbcc, double *ghcp);
 
 /* Seeded initial state: interior U[mean-amp, mean+amp) from splitmix64
- * ((x >> 11) * 2^-53), ghosts as periodic images.  Replaces init_field
+ * ((x >> 11) * 2^-53), ghosts as periodic images (box b of a decomposed
+ * context: its own images, seed + 1000003 b).  Replaces init_field
  * (field.c:742-777) for the synthetic workloads. */
 int etd_init_noise(etd_ctx *c, int var, unsigned long long seed, double mean, double amp);
 
@@ -176,7 +183,9 @@ int etd_step(etd_ctx *c, int nsteps);
 
 /* check_soln_new (field.c:407-678) without file output:
  * stats[4*v + {0,1,2,3}] = max, min, vol, eng for each variable v, then
- * stats[4*(nac+nch)] = couple energy (0 unless nac > 0 and nch > 0). */
+ * stats[4*(nac+nch)] = couple energy (0 unless nac > 0 and nch > 0); over
+ * the boxes of a decomposed context reduced as its MPI_Reduce calls do
+ * (max, min, sums; field.c:513-516). */
 int etd_diag(etd_ctx *c, double *stats);
 
 /* The allocation-free variant of etd_diag; it has no counterpart in the
@@ -198,8 +207,8 @@ int etd_diag_noalloc(etd_ctx *c, double *stats_dev);
  * writes the headers. */
 int etd_write_csv(etd_ctx *c, int iter, const double *stats);
 
-/* Writes <data_dir>eta<v>_<ioutput:06d>_000000.dat, text %+1.15lf, full n^3
- * (outputfield, field.c:17-56). */
+/* Writes <data_dir>eta<v>_<ioutput:06d>_<cx:02d><cy:02d><cz:02d>.dat per box,
+ * text %+1.15lf, full n^3 (outputfield, field.c:17-56). */
 int etd_write_field(etd_ctx *c, int var, int ioutput);
 
 /* ---- per-kernel entry points (unit parity; host buffers in/out) ---- */
@@ -209,15 +218,18 @@ int etd_write_field(etd_ctx *c, int var, int ioutput);
 int etd_k_transfer(etd_ctx *c);
 /* elastic_calculate (elastic.c:689-762); the force of every field it acts on
  * copied to host_felas when non-NULL: nac * n^3 doubles for ELASTIC = 1 (AC
- * fields), nch * n^3 for ELASTIC = 2 (CH fields). */
+ * fields), nch * n^3 for ELASTIC = 2 (CH fields); a decomposed context gives
+ * P n^3 per field (field p of box b at (p P + b) n^3) and transforms its
+ * global (px N)^3 interior, as fftMPI's distributed transform does. */
 int etd_k_elastic(etd_ctx *c, double *host_felas);
 /* The 3D FFT of the elastic path (replaces fft_forward / fft_backward,
- * fft3d.c:98-138) on N^3 interleaved complex values, N = n - 2 nghost, in
- * place; dir = +1 forward exp(-i..), -1 backward exp(+i..), both unscaled.
- * Needs a context created with ELASTIC = 1. */
+ * fft3d.c:98-138) on N^3 interleaved complex values, N = n - 2 nghost (px N
+ * for a decomposed context), in place; dir = +1 forward exp(-i..), -1
+ * backward exp(+i..), both unscaled.  Needs a context with elasticity. */
 int etd_k_fft3d(etd_ctx *c, double *data, int dir);
 /* ac_calc_FU / ch_calc_FU (calc.c:8-68,475-487) for variable var on the
- * current device state (after etd_k_transfer / etd_k_elastic). */
+ * current device state (after etd_k_transfer / etd_k_elastic); P n^3 for a
+ * decomposed context. */
 int etd_k_nonlinear(etd_ctx *c, int var, double *host_out);
 /* PUX/PUY/PUZ (calc.c:631-684): out = A x_1 in, optionally rotated
  * (flat (a,b,c) -> (b,c,a)); A column-major n x n; n^3 in/out.

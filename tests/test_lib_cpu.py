@@ -116,14 +116,19 @@ def test_read_matrices_file_format(tmp_path):
 
 
 def test_unsupported_configurations_are_rejected():
-    """Multi-subdomain grids, non-cubes and variable mobility without its three
-    CH fields return ETD_ERR_UNSUPPORTED before any device work."""
-    for field, val in (("px", 2), ("variable_mobility", 1), ("ny", 14)):
+    """Decomposed runs outside what is built (mixed precision, physical outer
+    sides, a non-cubic grid of boxes under elasticity), non-cubes and variable
+    mobility without its three CH fields return ETD_ERR_UNSUPPORTED before
+    any device work."""
+    cases = (dict(px=2, precision=2), dict(px=2, periodic=0), dict(px=2, elastic=1), dict(variable_mobility=1),
+             dict(ny=14))
+    for over in cases:
         p = pyetd.params_from_dict(dict(n=12, nac=1, nch=0, dt=0.01, epn2=1.0))
-        setattr(p, field, val)
+        for field, val in over.items():
+            setattr(p, field, val)
         ctx = C.c_void_p()
         rc = pyetd.lib().etd_create(C.byref(p), None, None, None, C.byref(ctx))
-        assert rc == -3, field
+        assert rc == -3, over
 
 
 def test_precision_flag_is_checked_before_device_work():
