@@ -374,6 +374,8 @@ def cpu_baseline(W, prm, init, full):
             o.step()
         s = time.perf_counter() - t0
         del o
+        print(f"cpu baseline: {steps} steps of n = {p['n']} on {threads} thread(s): {s:.2f} s", file=sys.stderr,
+              flush=True)
         return s
 
     big_steps = 20 if full else 1
