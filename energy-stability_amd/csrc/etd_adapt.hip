@@ -493,6 +493,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     ACHK(hipHostMalloc(&c->ad_hpart, sizeof(double) * RBLOCKS * RCOLS, hipHostMallocDefault));
   }
   T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec, *SP2 = SP + cnt;
+  c->carry_ok = 0;  // the attempts transform u themselves and move it afterwards
   const double dt_fixed = c->p.dt;
   // energy-increase slack: the rounding level of the storage type times the
   // energy's scale, for the scheme energy the larger absolute term sum of

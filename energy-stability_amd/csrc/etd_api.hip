@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <rocblas/rocblas.h>
+
 #include "etd_internal.h"
 
 #define HCHK(c, call)                                                                  \
@@ -172,6 +174,8 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
     if (v == 0 || v == 1 || v == 2 || v == 4 || v == 16) c->mp_variant = v;
   }
   c->mp_nb = 8;
+  const char *cv = getenv("ETD_CARRY");
+  c->no_carry = cv && atoi(cv) == 0;  // every step transforms u itself
   c->fft_variant = 1;
   const char *fv = getenv("ETD_FFT");
   if (fv && atoi(fv) == 0) c->fft_variant = 0;
@@ -363,6 +367,8 @@ extern "C" void etd_close(etd_ctx *c) {
     c->N2 = c->bN2;
     c->spec = c->bspec;
   }
+  (void)hipFree(c->cr_buf);
+  if (c->cr_blas) rocblas_destroy_handle((rocblas_handle)c->cr_blas);
   (void)hipFree(c->u);
   (void)hipFree(c->N2);
   (void)hipFree(c->w1);
@@ -478,6 +484,7 @@ static int get_f32(etd_ctx *c, double *host, const float *dev, size_t cnt) {
 extern "C" int etd_set_field(etd_ctx *c, int var, const double *host) {
   if (!c || !host || var < 0 || var >= c->V) return seterr(c, ETD_ERR_ARG, "set_field: bad variable %d", var);
   c->elastic_fresh = 0;
+  c->carry_ok = 0;
   c->ch_stale = 1;
   if (c->mixed) return put_f32(c, c->f32.u + c->n3 * var, host, c->n3);
   for (int b = 0; b < c->P; b++)  // a decomposed run: the boxes one after another
@@ -598,18 +605,37 @@ void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumul
 // place, three inverse products return u in xyz.
 // ghost fill, elasticity, N1 = N(u), spec = V^-1 x3 [u | N1]: the part of a
 // step that does not depend on dt
+//
+// With the AC spectrum carried (carry_ok: spec[0 .. nac) is the spectrum the
+// AC fields of u were transformed back from), only [u_CH | N1] is
+// transformed; the AC spectrum of the ghost-filled fields is the carried one
+// plus the transform of the fill's changes (carry_correct).
 template <typename T>
 void stage_forward(etd_ctx *c) {
   const int V = c->V;
   const etd_fields<T> F = fields<T>(c);
-  launch_ghost_fill<T>(c, F.u, V, c->n3);
+  const bool carry = c->carry_ok && carry_possible(c);
+  c->carry_ok = 0;
+  if (carry) {
+    T *dz, *dy, *dx;
+    carry_deltas<T>(c, &dz, &dy, &dx);
+    launch_ghost_fill<T>(c, F.u, c->nac, c->n3, dz, dy, dx);
+    if (c->nch) launch_ghost_fill<T>(c, F.u + c->n3 * c->nac, c->nch, c->n3);
+  } else {
+    launch_ghost_fill<T>(c, F.u, V, c->n3);
+  }
   // an adaptive acceptance test may already have evaluated felas of exactly
   // these fields (same ghost fill, same kernels): reuse it
   if (c->nel > 0 && !c->elastic_fresh) launch_elastic<T>(c);
   c->elastic_fresh = 0;
   nonlinear_boxes<T>(c, F.N1, (const T *)nullptr, 1);  // mixed: CH terms centred, N1 - N(c_mean)
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
-  modeprod3<T>(c, 1, F.u, F.spec, 2 * V * c->P, 0);
+  if (carry) {
+    modeprod3<T>(c, 1, F.u + c->n3 * c->nac, F.spec + c->n3 * c->nac, c->nch + V, 0);
+    carry_correct<T>(c, F.spec, 1.0, (const T *)nullptr, 0.0, F.spec);
+  } else {
+    modeprod3<T>(c, 1, F.u, F.spec, 2 * V * c->P, 0);
+  }
   // u stays centred: every caller overwrites it with the backward transform;
   // the ETD2 stage needs N1 itself (N(a) - N1)
   if constexpr (sizeof(T) == 4)
@@ -628,12 +654,25 @@ int step_once_t(etd_ctx *c) {
   modeprod3<T>(c, 0, F.spec, F.u, PV, 0);
   if (c->p.etd2) {
     if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
-    launch_ghost_fill<T>(c, F.u, V, n3);
+    if (c->carry_next) {  // the fill's changes of a enter the final spectrum too
+      T *dz, *dy, *dx;
+      carry_deltas<T>(c, &dz, &dy, &dx);
+      launch_ghost_fill<T>(c, F.u, c->nac, n3, dz, dy, dx);
+      if (c->nch) launch_ghost_fill<T>(c, F.u + n3 * c->nac, c->nch, n3);
+    } else {
+      launch_ghost_fill<T>(c, F.u, V, n3);
+    }
     if (c->nel > 0) launch_elastic<T>(c);
     nonlinear_boxes<T>(c, F.N2, (const T *)F.N1, 0);  // N(a) - N(u_n)
-    modeprod3<T>(c, 1, F.N2, F.spec, PV, 0);
-    phiscale_boxes<T>(c, F.spec, (const T *)nullptr, 1);
-    modeprod3<T>(c, 0, F.spec, F.u, PV, 1);  // u = a + S phi2 S^-1 dN
+    // the corrector's spectra in the N-hat half (free after the phi update),
+    // so that spec[0 .. V) keeps a-hat
+    T *dsp = F.spec + n3 * PV;
+    modeprod3<T>(c, 1, F.N2, dsp, PV, 0);
+    phiscale_boxes<T>(c, dsp, (const T *)nullptr, 1);
+    modeprod3<T>(c, 0, dsp, F.u, PV, 1);  // u = a + S phi2 S^-1 dN
+    // the spectrum of u for the next step: a-hat + S^-1 (the fill's changes
+    // of a) + phi2 S^-1 dN
+    if (c->carry_next) carry_correct<T>(c, F.spec, 1.0, dsp, 1.0, F.spec);
     if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
   } else {
     if constexpr (sizeof(T) == 4) ch_mean_restore_centered(c, F.u);  // adds the mean back
@@ -649,7 +688,20 @@ template void stage_forward<float>(etd_ctx *);
 template int step_once_t<double>(etd_ctx *);
 template int step_once_t<float>(etd_ctx *);
 
-static int step_once(etd_ctx *c) { return c->mixed ? step_once_t<float>(c) : step_once_t<double>(c); }
+// one fixed step; afterwards spec[0 .. nac) is the AC spectrum of u, which
+// the next step carries (the adaptive driver and the ensemble call
+// step_once_t directly and never set carry_ok)
+static int step_once(etd_ctx *c) {
+  if (carry_possible(c)) {
+    const int rc = c->mixed ? carry_prepare<float>(c) : carry_prepare<double>(c);
+    if (rc) return seterr(c, rc, "carry: cannot allocate the correction buffers");
+  }
+  c->carry_next = carry_possible(c);
+  const int rc = c->mixed ? step_once_t<float>(c) : step_once_t<double>(c);
+  c->carry_ok = rc == ETD_OK && c->carry_next;
+  c->carry_next = 0;
+  return rc;
+}
 // check_soln_new of a decomposed run: each box over its own ranges, then the
 // reduction of field.c:513-516 (max, min, and the sums in box order)
 static int diag_boxes(etd_ctx *c, double *st) {
@@ -720,6 +772,7 @@ extern "C" int etd_write_field(etd_ctx *c, int var, int ioutput) {
 // products); host buffers stay FP64 and are converted at the copy.
 extern "C" int etd_k_transfer(etd_ctx *c) {
   if (!c) return ETD_ERR_ARG;
+  c->carry_ok = 0;
   if (c->mixed)
     launch_ghost_fill<float>(c, c->f32.u, c->V, c->n3);
   else

@@ -1,0 +1,198 @@
+// etd_carry.hip -- the AC spectrum carried from one step to the next (gfx950).
+//
+// After a step, u = V (x) V (x) V u-hat exactly as the backward transform
+// computed it from u-hat.  The next step transforms the ghost-filled field
+// u_f = u + d, where the ghost fill (transfer, trans.c:1939-1996) changed
+// only the shell of 2g cells, so
+//   S^-1 u_f = u-hat + S^-1 d,  S^-1 = Vi (x) Vi (x) Vi,
+// and S^-1 d costs a few small products instead of three full mode products
+// over every AC field.  d splits into the cells each sweep of the fill wrote:
+//   z sweep: planes k in G (all j, i),  y sweep: k interior, rows j in G,
+//   x sweep: k, j interior, columns i in G    (G = {0 .. g-1, n-g .. n-1}),
+// and for u-hat[k^][j^][i^] (x fastest, the layout every forward transform
+// ends in)
+//   S^-1 d = sum_{s} Vi[k^][G_s] P_s[j^][i^] + Vi[j^][G_s] R_s[k^][i^]
+//                                           + Vi[i^][G_s] X_s[k^][j^]
+//   P_s = Vi d_z,s Vi^T,  R_s = Vi_I d_y,s Vi^T,  X_s = Vi_I d_x,s Vi_I^T
+// with Vi_I the columns of Vi at the interior cells.  The two products of
+// each P, R, X are plain batched GEMMs (rocBLAS, strided-batched over the
+// AC fields and the 2g sweep slabs), and k_carry_expand adds the three sums
+// to the carried spectrum in one pass.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <rocblas/rocblas.h>
+
+#include "etd_internal.h"
+
+// out = alpha s1 + beta s2 (s2 may be null) + S^-1 d over nac fields.  A
+// block owns one j^ of one field and KB values of k^, a thread one i^ (and
+// i^ + 256 ... where n exceeds the block): the
+// four P values of (j^, i^) and the thread's Vi[i^][G] stay in registers
+// across the KB rows; R comes from L2 (coalesced along i^), X and Vi[k^][G],
+// Vi[j^][G] are uniform.  All loads of the KB rows are issued before any is
+// used.
+template <typename T, int KB>
+__global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, T alpha, const T *__restrict__ s2,
+                                                      T beta, T *__restrict__ out, const T *__restrict__ P,
+                                                      const T *__restrict__ R, const T *__restrict__ X,
+                                                      const T *__restrict__ G4, int n, int with_corr) {
+  constexpr int G = 4;  // 2 g ghost layers, g = 2 (carry_possible)
+  const int jh = blockIdx.x, v = blockIdx.z, k0 = blockIdx.y * KB;
+  const unsigned n2 = (unsigned)n * n;
+  const T *Pv = P + (size_t)v * G * n2, *Rv = R + (size_t)v * G * n2, *Xv = X + (size_t)v * G * n2;
+  const T *s1v = s1 + (size_t)v * n2 * n, *s2v_ = s2 ? s2 + (size_t)v * n2 * n : nullptr;
+  T *ov = out + (size_t)v * n2 * n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {  // n may exceed the block
+    const unsigned l0 = (unsigned)jh * n + i;
+    T sv[KB], s2v[KB], rv[KB][G];
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+      const int kh = k0 + kb < n ? k0 + kb : n - 1;  // past the edge: a valid row, not stored
+      const unsigned l = l0 + (unsigned)kh * n2;
+      sv[kb] = s1v[l];
+      s2v[kb] = s2v_ ? s2v_[l] : (T)0;
+#pragma unroll
+      for (int s = 0; s < G; s++) rv[kb][s] = Rv[s * n2 + (unsigned)kh * n + i];
+    }
+    T pj[G], gi[G], gj[G];
+#pragma unroll
+    for (int s = 0; s < G; s++) {
+      pj[s] = Pv[s * n2 + l0];
+      gi[s] = G4[i * G + s];
+      gj[s] = G4[jh * G + s];
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+      const int kh = k0 + kb;
+      if (kh >= n) break;
+      T corr = (T)0;
+      if (with_corr) {
+#pragma unroll
+        for (int s = 0; s < G; s++) {
+          corr += G4[kh * G + s] * pj[s];
+          corr += gj[s] * rv[kb][s];
+          corr += gi[s] * Xv[s * n2 + (unsigned)kh * n + jh];
+        }
+      }
+      T val = alpha * sv[kb] + corr;
+      if (s2v_) val += beta * s2v[kb];
+      ov[l0 + (unsigned)kh * n2] = val;
+    }
+  }
+}
+
+// buffer layout (elements of T): dz, dy, dx (the recorded changes), t1, P, R,
+// X (nac 2g n^2 each), Vi and Vi^T (row-major n^2 each), G4 (n x 2g)
+template <typename T>
+static size_t carry_elems(const etd_ctx *c, size_t *off) {
+  const size_t n = c->n, g = c->g, G = 2 * g, m = n - 2 * g, nac = c->nac, n2 = n * n;
+  size_t o = 0;
+  off[0] = o, o += nac * G * n2;     // dz
+  off[1] = o, o += nac * m * G * n;  // dy
+  off[2] = o, o += nac * m * m * G;  // dx
+  off[3] = o, o += nac * G * n2;     // t1
+  off[4] = o, o += nac * G * n2;     // P
+  off[5] = o, o += nac * G * n2;     // R
+  off[6] = o, o += nac * G * n2;     // X
+  off[7] = o, o += n2;               // Vi
+  off[8] = o, o += n2;               // Vi^T
+  off[9] = o, o += n * G;            // G4
+  return o;
+}
+
+bool carry_possible(const etd_ctx *c) { return c->P == 1 && c->nac > 0 && c->g == 2 && !c->no_carry; }
+
+template <typename T>
+int carry_prepare(etd_ctx *c) {
+  if (c->cr_buf) return ETD_OK;
+  rocblas_handle hb;
+  if (rocblas_create_handle(&hb) != rocblas_status_success) return ETD_ERR_DEVICE;
+  rocblas_set_stream(hb, c->stream);
+  c->cr_blas = hb;
+  size_t off[10];
+  const size_t tot = carry_elems<T>(c, off);
+  if (hipMalloc(&c->cr_buf, sizeof(T) * tot) != hipSuccess) return ETD_ERR_NOMEM;
+  const int n = c->n, g = c->g, G = 2 * g;
+  T *h = (T *)malloc(sizeof(T) * (2 * (size_t)n * n + (size_t)n * G));
+  if (!h) return ETD_ERR_NOMEM;
+  // hVi is column-major: Vi(r, c) = hVi[r + n c]
+  for (int r = 0; r < n; r++)
+    for (int q = 0; q < n; q++) {
+      h[(size_t)r * n + q] = (T)c->hVi[r + (size_t)n * q];                  // Vi[r][q]
+      h[(size_t)n * n + (size_t)q * n + r] = (T)c->hVi[r + (size_t)n * q];  // Vi^T[q][r]
+    }
+  for (int a = 0; a < n; a++)
+    for (int s = 0; s < G; s++) {
+      const int gi = s < g ? s : n - 2 * g + s;
+      h[2 * (size_t)n * n + (size_t)a * G + s] = (T)c->hVi[a + (size_t)n * gi];
+    }
+  hipError_t e = hipMemcpy((T *)c->cr_buf + off[7], h, sizeof(T) * (2 * (size_t)n * n + (size_t)n * G),
+                           hipMemcpyHostToDevice);
+  free(h);
+  return e == hipSuccess ? ETD_OK : ETD_ERR_DEVICE;
+}
+
+template <typename T>
+void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx) {
+  size_t off[10];
+  carry_elems<T>(c, off);
+  T *b = (T *)c->cr_buf;
+  *dz = b + off[0];
+  *dy = b + off[1];
+  *dx = b + off[2];
+}
+
+// row-major C (M x N, row stride ldc) = A (M x K, lda) B (K x N, ldb) for
+// `batch` matrices (strides sa, sb, sc; 0 shares an operand), through the
+// column-major BLAS as C^T = B^T A^T
+static void gemm_rm(etd_ctx *c, const double *A, long lda, long sa, const double *B, long ldb, long sb, double *C,
+                    long ldc, long sc, int M, int N, int K, int batch) {
+  const double one = 1.0, zero = 0.0;
+  rocblas_dgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M, K,
+                                &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+}
+static void gemm_rm(etd_ctx *c, const float *A, long lda, long sa, const float *B, long ldb, long sb, float *C,
+                    long ldc, long sc, int M, int N, int K, int batch) {
+  const float one = 1.0f, zero = 0.0f;
+  rocblas_sgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M, K,
+                                &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+}
+
+template <typename T>
+void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out) {
+  const int n = c->n, g = c->g, G = 2 * g, m = n - 2 * g, nac = c->nac;
+  const long n2 = (long)n * n;
+  size_t off[10];
+  carry_elems<T>(c, off);
+  T *b = (T *)c->cr_buf;
+  T *dz = b + off[0], *dy = b + off[1], *dx = b + off[2], *t1 = b + off[3];
+  T *P = b + off[4], *R = b + off[5], *X = b + off[6];
+  const T *Vi = b + off[7], *ViT = b + off[8], *G4 = b + off[9];
+  const int nb = nac * G;
+  etd_prof_begin(c, PC_MODEPROD);
+  if (c->p.periodic) {
+    // z sweep: P = Vi (d Vi^T), d = dz[v][s] (n x n)
+    gemm_rm(c, dz, n, n2, ViT, n, 0, t1, n, n2, n, n, n, nb);
+    gemm_rm(c, Vi, n, 0, t1, n, n2, P, n, n2, n, n, n, nb);
+    // y sweep: R = Vi_I (d Vi^T), d = dy[v][s] (m x n)
+    gemm_rm(c, dy, n, (long)m * n, ViT, n, 0, t1, n, n2, m, n, n, nb);
+    gemm_rm(c, Vi + g, n, 0, t1, n, n2, R, n, n2, n, n, m, nb);
+    // x sweep: X = Vi_I (d Vi_I^T), d = dx[v][s] (m x m), Vi_I^T = rows g .. g+m-1 of Vi^T
+    gemm_rm(c, dx, m, (long)m * m, ViT + (long)g * n, n, 0, t1, n, n2, m, n, m, nb);
+    gemm_rm(c, Vi + g, n, 0, t1, n, n2, X, n, n2, n, n, m, nb);
+  }  // physical sides: no ghost fill, nothing to add
+  constexpr int KB = 7;
+  const dim3 grid((unsigned)n, (unsigned)((n + KB - 1) / KB), (unsigned)nac);
+  k_carry_expand<T, KB><<<grid, 256, 0, c->stream>>>(s1, (T)alpha, s2, (T)beta, out, P, R, X, G4, n,
+                                                     c->p.periodic);
+  etd_prof_end(c, PC_MODEPROD);
+}
+
+template int carry_prepare<double>(etd_ctx *);
+template int carry_prepare<float>(etd_ctx *);
+template void carry_deltas<double>(etd_ctx *, double **, double **, double **);
+template void carry_deltas<float>(etd_ctx *, float **, float **, float **);
+template void carry_correct<double>(etd_ctx *, const double *, double, const double *, double, double *);
+template void carry_correct<float>(etd_ctx *, const float *, double, const float *, double, float *);

@@ -59,6 +59,10 @@ extern "C" int etd_ensemble_create(etd_ctx *const *members, int count, int steps
   for (int i = 0; i < count; i++) {
     etd_ctx *c = members[i];
     if (c && c->P > 1) return ETD_ERR_UNSUPPORTED;  // members are single boxes
+    if (c) {  // a replayed graph transforms u itself every step
+      c->no_carry = 1;
+      c->carry_ok = 0;
+    }
     if (!c) return enerr(e, ETD_ERR_ARG, "null member");
     e->m[i] = c;
     int d;

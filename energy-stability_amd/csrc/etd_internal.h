@@ -126,6 +126,14 @@ struct etd_ctx {
   double *vm_part;
   double vm_X[3], vm_DA[3], vm_DG[3];  /* x (1 - x), D_A, D_G of Cu, Mn, Ni */
   double gbcc, ghcp;       /* CALPHAD Gibbs energies (free_energy 1) */
+  /* the AC spectrum carried between fixed steps (etd_carry.hip): carry_ok
+   * when spec[0 .. nac) holds the spectrum the AC fields of u were
+   * transformed back from and nothing has changed u since; no_carry turns it
+   * off (ETD_CARRY=0, ensemble members); cr_buf the correction's buffers */
+  int carry_ok, no_carry;
+  void *cr_blas;           /* rocBLAS handle of the correction's batched GEMMs */
+  int carry_next;          /* inside step_once: leave spec[0 .. nac) as the AC spectrum of u */
+  void *cr_buf;
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
@@ -175,7 +183,8 @@ inline etd_fields<float> fields<float>(const etd_ctx *c) {
 /* T = double (plain FP64 path) or float (FP32 storage of the mixed path;
  * arithmetic in registers stays FP64 in these kernels) */
 template <typename T>
-void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride);
+void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride, T *dz = nullptr, T *dy = nullptr,
+                       T *dx = nullptr);
 template <typename T>
 void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2);
 template <typename T>
@@ -216,6 +225,19 @@ void ch_mean_restore_centered(etd_ctx *c, float *u);
  * The step leaves u centred after the forward transform (the backward
  * transform overwrites it) and undoes it in ch_mean_restore_centered. */
 void ch_center(etd_ctx *c, float *u, float *N, int sign);
+
+/* the carried AC spectrum (etd_carry.hip): carry_possible says whether this
+ * context may carry; carry_prepare allocates the buffers; carry_deltas gives
+ * the buffers the ghost fill records its changes of the AC fields into;
+ * carry_correct writes out = alpha s1 + beta s2 + S^-1 (recorded changes)
+ * for the nac AC fields (s2 may be null) */
+bool carry_possible(const etd_ctx *c);
+template <typename T>
+int carry_prepare(etd_ctx *c);
+template <typename T>
+void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx);
+template <typename T>
+void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out);
 
 /* step pieces (etd_api.hip) */
 template <typename T>

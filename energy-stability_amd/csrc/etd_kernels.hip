@@ -89,8 +89,11 @@ __device__ __forceinline__ void idx3(size_t l, int n, int &i, int &j, int &k) {
 // only cells the sweeps before it do not write, so running each sweep over
 // all boxes before the next reads pre-transfer data, as the reference's
 // send-everything-then-unpack order does.
+// dlt (or null): the change new - old of every cell a sweep writes, per
+// field as 2g contiguous matrices: z [plane][j][i], y [row][k - g][i], x
+// [column][k - g][j - g].
 template <typename T>
-__global__ void k_ghost_z(T *base, size_t fs, int n, int g, long long lo, long long hi) {
+__global__ void k_ghost_z(T *base, size_t fs, int n, int g, long long lo, long long hi, T *dlt) {
   T *u = base + fs * blockIdx.y;
   const size_t n2 = (size_t)n * n;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,10 +101,12 @@ __global__ void k_ghost_z(T *base, size_t fs, int n, int g, long long lo, long l
   int pl = (int)(t / n2);
   size_t r = t % n2;
   int k = pl < g ? pl : n - 2 * g + pl;
-  u[(size_t)k * n2 + r] = u[(pl < g ? lo : hi) + (long long)((size_t)wrap_e(k, n, g) * n2 + r)];
+  const T v = u[(pl < g ? lo : hi) + (long long)((size_t)wrap_e(k, n, g) * n2 + r)];
+  if (dlt) dlt[(size_t)2 * g * n2 * blockIdx.y + t] = (T)((double)v - (double)u[(size_t)k * n2 + r]);
+  u[(size_t)k * n2 + r] = v;
 }
 template <typename T>
-__global__ void k_ghost_y(T *base, size_t fs, int n, int g, long long lo, long long hi) {
+__global__ void k_ghost_y(T *base, size_t fs, int n, int g, long long lo, long long hi, T *dlt) {
   T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = 2 * g;
@@ -110,10 +115,14 @@ __global__ void k_ghost_y(T *base, size_t fs, int n, int g, long long lo, long l
   int rr = (int)((t / n) % rows);
   int k = g + (int)(t / ((size_t)n * rows));
   int j = rr < g ? rr : n - 2 * g + rr;
-  u[IDX(n, k, j, i)] = u[(rr < g ? lo : hi) + (long long)IDX(n, k, wrap_e(j, n, g), i)];
+  const T v = u[(rr < g ? lo : hi) + (long long)IDX(n, k, wrap_e(j, n, g), i)];
+  if (dlt)
+    dlt[(size_t)(n - 2 * g) * rows * n * blockIdx.y + ((size_t)rr * (n - 2 * g) + (k - g)) * n + i] =
+        (T)((double)v - (double)u[IDX(n, k, j, i)]);
+  u[IDX(n, k, j, i)] = v;
 }
 template <typename T>
-__global__ void k_ghost_x(T *base, size_t fs, int n, int g, long long lo, long long hi) {
+__global__ void k_ghost_x(T *base, size_t fs, int n, int g, long long lo, long long hi, T *dlt) {
   T *u = base + fs * blockIdx.y;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int cols = 2 * g, m = n - 2 * g;
@@ -122,7 +131,11 @@ __global__ void k_ghost_x(T *base, size_t fs, int n, int g, long long lo, long l
   int j = g + (int)((t / cols) % m);
   int k = g + (int)(t / ((size_t)cols * m));
   int i = cc < g ? cc : n - 2 * g + cc;
-  u[IDX(n, k, j, i)] = u[(cc < g ? lo : hi) + (long long)IDX(n, k, j, wrap_e(i, n, g))];
+  const T v = u[(cc < g ? lo : hi) + (long long)IDX(n, k, j, wrap_e(i, n, g))];
+  if (dlt)
+    dlt[(size_t)m * m * cols * blockIdx.y + ((size_t)cc * m + (k - g)) * m + (j - g)] =
+        (T)((double)v - (double)u[IDX(n, k, j, i)]);
+  u[IDX(n, k, j, i)] = v;
 }
 
 // element offset from box b's fields to those of the box (sx, sy, sz) away
@@ -132,9 +145,10 @@ static long long host_box_off(const etd_ctx *c, int b, int sx, int sy, int sz) {
   return (long long)((z * c->py + y) * c->px + x - b) * (long long)c->bs;
 }
 
-// base: box 0's first field; the boxes' fields follow at stride bs
+// base: box 0's first field; the boxes' fields follow at stride bs.  dz, dy,
+// dx (one box only): record each sweep's changes (k_ghost_z/y/x)
 template <typename T>
-void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
+void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride, T *dz, T *dy, T *dx) {
   if (!c->p.periodic) return;  // physical sides: no neighbour, unpack writes nothing (trans.c:154-251)
   const int n = c->n, g = c->g;
   const size_t n2 = (size_t)n * n;
@@ -142,13 +156,13 @@ void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride) {
   etd_prof_begin(c, PC_HALO);
   for (int b = 0; b < c->P; b++)
     k_ghost_z<T><<<dim3((unsigned)((tz + 255) / 256), nfields), 256, 0, c->stream>>>(
-        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, 0, -1), host_box_off(c, b, 0, 0, 1));
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, 0, -1), host_box_off(c, b, 0, 0, 1), dz);
   for (int b = 0; b < c->P; b++)
     k_ghost_y<T><<<dim3((unsigned)((ty + 255) / 256), nfields), 256, 0, c->stream>>>(
-        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, -1, 0), host_box_off(c, b, 0, 1, 0));
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, 0, -1, 0), host_box_off(c, b, 0, 1, 0), dy);
   for (int b = 0; b < c->P; b++)
     k_ghost_x<T><<<dim3((unsigned)((tx + 255) / 256), nfields), 256, 0, c->stream>>>(
-        base + b * c->bs, fstride, n, g, host_box_off(c, b, -1, 0, 0), host_box_off(c, b, 1, 0, 0));
+        base + b * c->bs, fstride, n, g, host_box_off(c, b, -1, 0, 0), host_box_off(c, b, 1, 0, 0), dx);
   etd_prof_end(c, PC_HALO);
 }
 
@@ -2863,7 +2877,7 @@ void ch_mean_restore(etd_ctx *c, float *u) {
 }
 
 #define ETD_INST(T)                                                                   \
-  template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t);                   \
+  template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t, T *, T *, T *);    \
   template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \
   template void launch_phiscale_pair<T>(etd_ctx *, const T *, const T *, T *, double, T *, double); \
   template void launch_nonlinear<T>(etd_ctx *, T *, const T *, int);                 \

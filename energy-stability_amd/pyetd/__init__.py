@@ -399,7 +399,18 @@ class Etd:
     then fields, nonlinear terms and forces come per box, box b at index b
     of the leading axis (b = (bz py + by) px + bx)."""
 
-    def __init__(self, params, V=None, Vi=None, D=None):
+    def __init__(self, params, V=None, Vi=None, D=None, carry=True):
+        """carry=False: every step transforms u itself (ETD_CARRY=0) instead of
+        carrying the AC spectrum from the step before."""
+        if not carry:
+            os.environ["ETD_CARRY"] = "0"
+        try:
+            self._create(params, V, Vi, D)
+        finally:
+            if not carry:
+                os.environ.pop("ETD_CARRY", None)
+
+    def _create(self, params, V, Vi, D):
         self.ctx = C.c_void_p()
         self.p = params
         self.n = params.nx
@@ -421,8 +432,8 @@ class Etd:
             raise EtdError(f"etd_create: {lib().etd_error_string(rc).decode()} {err}")
 
     @classmethod
-    def from_dict(cls, prm):
-        return cls(params_from_dict(prm))
+    def from_dict(cls, prm, carry=True):
+        return cls(params_from_dict(prm), carry=carry)
 
     @classmethod
     def open(cls, directory, suffix):

@@ -37,8 +37,8 @@ def instances(count, seed):
     return out
 
 
-def make(prm, tuning):
-    g = pyetd.Etd.from_dict(prm)
+def make(prm, tuning, carry=True):
+    g = pyetd.Etd.from_dict(prm, carry=carry)
     g.set_tuning(tuning)
     for v, f in enumerate(W.initial_fields(prm)):
         g.set_field(v, f)
@@ -84,7 +84,8 @@ def main():
     for i in range(min(args.check, len(prms))):
         # the same history as the ensemble member (warm step, reset, one warm
         # graph launch, reset): the felas ring then holds the same values
-        s = make(prms[i], tunings[prms[i]["n"]])
+        # (and, as the graph does, transforming u every step)
+        s = make(prms[i], tunings[prms[i]["n"]], carry=False)
         s.step(1)
         for v, f in enumerate(W.initial_fields(prms[i])):
             s.set_field(v, f)
