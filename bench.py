@@ -88,7 +88,7 @@ def fresh(pyetd, prm, fields):
 
 def fixed_dt(pyetd, prm, fields, steps, warmup, calib=0):
     """etd_time_steps: warmup steps, then `steps` steps between HIP events on
-    the library's stream (diagnostics included); returns (ms[8], launches[7])."""
+    the library's stream (diagnostics included); returns (ms[10], launches[8])."""
     g = fresh(pyetd, prm, fields)
     if calib:
         g.calib_copy(calib)  # plain copy kernel, exactly this many bytes each way
@@ -100,8 +100,9 @@ def fixed_dt(pyetd, prm, fields, steps, warmup, calib=0):
 
 
 def breakdown(ms, K):
-    return {"mode_products": ms[1] / K, "elastic_fft": ms[2] / K, "fft_passes": ms[7] / K, "nonlinear": ms[3] / K,
-            "phi_scaling": ms[4] / K, "energy": ms[5] / K, "halo": ms[6] / K}
+    return {"mode_products": ms[1] / K, "carry_correction": ms[8] / K, "elastic_fft": ms[2] / K,
+            "fft_passes": ms[7] / K, "nonlinear": ms[3] / K, "phi_scaling": ms[4] / K, "energy": ms[5] / K,
+            "halo": ms[6] / K}
 
 
 def horizon_runs(pyetd, prm, fields, kw, count, mlp=None, log_cap=0):
@@ -196,13 +197,17 @@ def main():
     fp = None
     if args.only in ("all", "fp64"):
         ms, launches = fixed_dt(pyetd, prm, init, K, Wm, args.calib_copy_mib << 20)
-        mp_flops = 9 * V * 2.0 * n ** 4  # 3 forward (2V fields) + 3 backward (V) mode products per step
+        # field passes the mode products made (3 axes x the fields of each
+        # transform: 2V forward and V backward, or nch + V forward when the
+        # AC spectrum is carried), counted by the library
+        mp_flops = ms[9] * 2.0 * n ** 4
         mp_launch_ms = ms[1] / max(1, launches[0])
         fp = {"sim_time_per_s": dt * K / (ms[0] / 1e3), "steps": K, "warmup": Wm, "ms_per_step": ms[0] / K,
               "grid_point_variant_steps_per_s": N ** 3 * V * K / (ms[0] / 1e3), "breakdown_ms_per_step": breakdown(ms, K),
               "roofline": {"kernel": "k_modeprod_dl<13,4,2,2> (FP64 MFMA 4x4x4 mode products)", "bound": "mfma",
-                           "achieved": mp_flops * K / (ms[1] / 1e3) / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
-                           "unit": "TFLOP/s", "frac": mp_flops * K / (ms[1] / 1e3) / PEAK_FP64_MFMA,
+                           "achieved": mp_flops / (ms[1] / 1e3) / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
+                           "unit": "TFLOP/s", "frac": mp_flops / (ms[1] / 1e3) / PEAK_FP64_MFMA,
+                           "field_passes_per_step": ms[9] / K,
                            "launch_ms_avg": mp_launch_ms, "launches": launches[0],
                            "traffic": measured_traffic("k_modeprod_dl_192cube_13fields_hbm_counters.md")}}
 
@@ -217,7 +222,7 @@ def main():
         npair = (nac + 1) // 2
         fft_bytes = 2 * npair * N ** 3 * 8.0  # read + write one float2 per point and packed field
         mp_launch_ms = ms[1] / max(1, launches[0])
-        mp_fields_avg = (3 * 2 * V + 3 * V) / 6.0
+        mp_fields_avg = ms[9] / max(1, launches[0])  # fields per launch (a launch is one axis pass)
         mp_bytes = 2 * 4.0 * n ** 3 * mp_fields_avg  # read + write FP32 per field
         nterm = 6 if args.precision == 1 else 3  # BF16 split-3: six products; FP16 split-2: three
         mp_mfma = nterm * 2.0 * n ** 4 * mp_fields_avg
@@ -233,7 +238,7 @@ def main():
                 "unit": "GB/s", "frac": mp_bytes / (mp_launch_ms / 1e3) / PEAK_HBM,
                 "traffic": measured_traffic("k_modeprod_split_192cube_hbm_counters.md" if args.precision == 1 else
                                             "k_modeprod_f16_192cube_hbm_counters.md"), "launch_ms": mp_launch_ms,
-                "launches": launches[0], "ms_per_step": ms[1] / K,
+                "launches": launches[0], "fields_per_launch": mp_fields_avg, "ms_per_step": ms[1] / K,
                 "mfma_tflops": mp_mfma / (mp_launch_ms / 1e3) / 1e12,
                 "mfma_frac_16bit": mp_mfma / (mp_launch_ms / 1e3) / PEAK_BF16_MFMA}
         mx["rooflines"] = [r_fft, r_mp]

@@ -493,7 +493,22 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     ACHK(hipHostMalloc(&c->ad_hpart, sizeof(double) * RBLOCKS * RCOLS, hipHostMallocDefault));
   }
   T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec, *SP2 = SP + cnt;
-  c->carry_ok = 0;  // the attempts transform u themselves and move it afterwards
+  // ETD1 attempts carry the AC spectrum: the second half step starts from the
+  // first's phi-scaled spectrum SP2, and after an accepted attempt u is the
+  // backward transform of spec (u_h2) or, extrapolated, of 2 spec - SP (SP
+  // is not written again before the next attempt's forward stage).  A
+  // rejection moves u back to S0, whose spectrum is not kept: the next
+  // attempt transforms it.
+  const bool carry_run = !c->p.etd2 && carry_possible(c);
+  if (carry_run) {
+    const int prc = carry_prepare<T>(c);
+    if (prc) {
+      snprintf(c->err, sizeof(c->err), "carry: cannot allocate the correction buffers");
+      return prc;
+    }
+  }
+  c->carry_ok = 0;
+  carry_base(c, nullptr, 1.0, nullptr, 0.0);
   const double dt_fixed = c->p.dt;
   // energy-increase slack: the rounding level of the storage type times the
   // energy's scale, for the scheme energy the larger absolute term sum of
@@ -588,6 +603,10 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
         ch_mean_restore_centered(c, SB);
         ch_mean_restore_centered(c, F.u);
       }
+      if (carry_run) {
+        c->carry_ok = 1;
+        carry_base(c, SP2, 1.0, nullptr, 0.0);
+      }
       if ((rc = step_once_t<T>(c))) break;
     } else {
       c->p.dt = dt;
@@ -657,9 +676,17 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       plain_next = dt * clampd(fac, o->shrink, o->grow);
       rej_since = 0;
       fallback = 0;
+      if (carry_run) {
+        c->carry_ok = 1;
+        if (o->extrapolate)
+          carry_base(c, nullptr, 2.0, SP, -1.0);
+        else
+          carry_base(c, nullptr, 1.0, nullptr, 0.0);
+      }
     } else {
       ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
       c->elastic_fresh = 0;  // the latest evaluation is of the rejected candidate
+      c->carry_ok = 0;
       if (!err_ok) {
         st->rejected_err++;
         dt = dt * clampd(fac, o->shrink, 1.0);
@@ -675,6 +702,8 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     }
   }
   c->p.dt = dt_fixed;
+  c->carry_ok = 0;  // the base may be 2 spec - SP: fixed steps start afresh
+  carry_base(c, nullptr, 1.0, nullptr, 0.0);
   st->t = c->t;
   st->dt_last = hist[0];
   st->energy = E;

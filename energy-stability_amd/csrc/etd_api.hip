@@ -176,6 +176,7 @@ extern "C" int etd_create(const etd_params *p, const double *Vh, const double *V
   c->mp_nb = 8;
   const char *cv = getenv("ETD_CARRY");
   c->no_carry = cv && atoi(cv) == 0;  // every step transforms u itself
+  carry_base(c, nullptr, 1.0, nullptr, 0.0);
   c->fft_variant = 1;
   const char *fv = getenv("ETD_FFT");
   if (fv && atoi(fv) == 0) c->fft_variant = 0;
@@ -584,6 +585,7 @@ template <typename T>
 void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumulate) {
   const size_t n3 = c->n3;
   const etd_fields<T> F = fields<T>(c);
+  if (c->prof) c->mp_fields += 3LL * nf;
   if constexpr (sizeof(T) == 8) {
     const double *A = forward ? c->Vi4 : c->V4;
     launch_modeprod(c, A, in, n3, F.w1, n3, nf, 1, 0);
@@ -606,16 +608,19 @@ void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumul
 // ghost fill, elasticity, N1 = N(u), spec = V^-1 x3 [u | N1]: the part of a
 // step that does not depend on dt
 //
-// With the AC spectrum carried (carry_ok: spec[0 .. nac) is the spectrum the
-// AC fields of u were transformed back from), only [u_CH | N1] is
-// transformed; the AC spectrum of the ghost-filled fields is the carried one
-// plus the transform of the fill's changes (carry_correct).
+// With the AC spectrum carried (carry_ok: the AC fields of u were transformed
+// back from carry_alpha s1 + carry_beta s2, by default spec[0 .. nac)), only
+// [u_CH | N1] is transformed; the AC spectrum of the ghost-filled fields is
+// the carried one plus the transform of the fill's changes (carry_correct).
 template <typename T>
 void stage_forward(etd_ctx *c) {
   const int V = c->V;
   const etd_fields<T> F = fields<T>(c);
   const bool carry = c->carry_ok && carry_possible(c);
+  const T *b1 = c->carry_s1 ? (const T *)c->carry_s1 : F.spec, *b2 = (const T *)c->carry_s2;
+  const double ba = c->carry_alpha, bb = c->carry_beta;
   c->carry_ok = 0;
+  carry_base(c, nullptr, 1.0, nullptr, 0.0);
   if (carry) {
     T *dz, *dy, *dx;
     carry_deltas<T>(c, &dz, &dy, &dx);
@@ -632,7 +637,7 @@ void stage_forward(etd_ctx *c) {
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
   if (carry) {
     modeprod3<T>(c, 1, F.u + c->n3 * c->nac, F.spec + c->n3 * c->nac, c->nch + V, 0);
-    carry_correct<T>(c, F.spec, 1.0, (const T *)nullptr, 0.0, F.spec);
+    carry_correct<T>(c, b1, ba, b2, bb, F.spec);
   } else {
     modeprod3<T>(c, 1, F.u, F.spec, 2 * V * c->P, 0);
   }
@@ -699,6 +704,7 @@ static int step_once(etd_ctx *c) {
   c->carry_next = carry_possible(c);
   const int rc = c->mixed ? step_once_t<float>(c) : step_once_t<double>(c);
   c->carry_ok = rc == ETD_OK && c->carry_next;
+  carry_base(c, nullptr, 1.0, nullptr, 0.0);
   c->carry_next = 0;
   return rc;
 }
@@ -971,7 +977,7 @@ extern "C" int etd_calib_copy(etd_ctx *c, size_t bytes) {
 extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches) {
   if (!c || nsteps < 1 || !ms) return ETD_ERR_ARG;
   // enough event slots for every bracketed launch group of nsteps steps
-  const int per_step = 2 * (2 * 12 + 2 * 4 + 8 + 4);
+  const int per_step = 2 * (2 * 12 + 2 * 4 + 8 + 6);
   const int need = per_step * nsteps + 16;
   if (c->cap_ev < need) {
     for (int i = 0; i < c->cap_ev; i++) (void)hipEventDestroy(c->ev[i]);
@@ -991,6 +997,7 @@ extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms,
   HCHK(c, hipStreamSynchronize(c->stream));
   c->prof = 1;
   c->nev = 0;
+  c->mp_fields = 0;
   double *st = (double *)malloc(sizeof(double) * (4 * c->V + 1));
   HCHK(c, hipEventRecord(t0, c->stream));
   int rc = ETD_OK;
@@ -1007,6 +1014,7 @@ extern "C" int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms,
   HCHK(c, hipEventElapsedTime(&tot, t0, t1));
   for (int k = 0; k < 1 + PC_COUNT; k++) ms[k] = 0.0;
   ms[0] = tot;
+  ms[1 + PC_COUNT] = (double)c->mp_fields;
   int cnt[PC_COUNT] = {0};
   for (int i = 0; i < c->nev; i++) {
     if (c->ev_class[i] < 0) continue;

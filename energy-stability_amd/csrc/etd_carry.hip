@@ -27,58 +27,78 @@
 #include "etd_internal.h"
 
 // out = alpha s1 + beta s2 (s2 may be null) + S^-1 d over nac fields.  A
-// block owns one j^ of one field and KB values of k^, a thread one i^ (and
-// i^ + 256 ... where n exceeds the block): the
-// four P values of (j^, i^) and the thread's Vi[i^][G] stay in registers
-// across the KB rows; R comes from L2 (coalesced along i^), X and Vi[k^][G],
-// Vi[j^][G] are uniform.  All loads of the KB rows are issued before any is
-// used.
-template <typename T, int KB>
+// block owns KB values of k^ and JB values of j^ of one field, a thread one
+// i^ (and i^ + 256 ... where n exceeds the block).  The thread's R values of
+// its KB rows k^ stay in registers across the JB rows j^, its P values of one
+// j^ across the KB rows k^; X, Vi[k^][G] and Vi[j^][G] are uniform.  All
+// loads of JU rows j^ (KB values of s1 and s2 each, their P values) are
+// issued before any is used.  The rows a block writes are JB consecutive rows of KB planes.
+// out may be s1: every element is read by the thread that writes it, before
+// it writes it, and by no other thread.
+template <typename T, int KB, int JB, int JU>
 __global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, T alpha, const T *__restrict__ s2,
                                                       T beta, T *__restrict__ out, const T *__restrict__ P,
                                                       const T *__restrict__ R, const T *__restrict__ X,
                                                       const T *__restrict__ G4, int n, int with_corr) {
   constexpr int G = 4;  // 2 g ghost layers, g = 2 (carry_possible)
-  const int jh = blockIdx.x, v = blockIdx.z, k0 = blockIdx.y * KB;
+  const int j0 = blockIdx.x * JB, k0 = blockIdx.y * KB, v = blockIdx.z;
+  const int nk = n - k0 < KB ? n - k0 : KB, nj = n - j0 < JB ? n - j0 : JB;
   const unsigned n2 = (unsigned)n * n;
   const T *Pv = P + (size_t)v * G * n2, *Rv = R + (size_t)v * G * n2, *Xv = X + (size_t)v * G * n2;
   const T *s1v = s1 + (size_t)v * n2 * n, *s2v_ = s2 ? s2 + (size_t)v * n2 * n : nullptr;
   T *ov = out + (size_t)v * n2 * n;
+  T gk[KB][G];  // Vi[k^][G] of the block's rows (uniform)
+#pragma unroll
+  for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+    for (int s = 0; s < G; s++) gk[kb][s] = with_corr ? G4[(k0 + (kb < nk ? kb : 0)) * G + s] : (T)0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {  // n may exceed the block
-    const unsigned l0 = (unsigned)jh * n + i;
-    T sv[KB], s2v[KB], rv[KB][G];
+    T rv[KB][G], gi[G];
 #pragma unroll
     for (int kb = 0; kb < KB; kb++) {
-      const int kh = k0 + kb < n ? k0 + kb : n - 1;  // past the edge: a valid row, not stored
-      const unsigned l = l0 + (unsigned)kh * n2;
-      sv[kb] = s1v[l];
-      s2v[kb] = s2v_ ? s2v_[l] : (T)0;
+      const unsigned kh = (unsigned)(k0 + (kb < nk ? kb : 0));
 #pragma unroll
-      for (int s = 0; s < G; s++) rv[kb][s] = Rv[s * n2 + (unsigned)kh * n + i];
-    }
-    T pj[G], gi[G], gj[G];
-#pragma unroll
-    for (int s = 0; s < G; s++) {
-      pj[s] = Pv[s * n2 + l0];
-      gi[s] = G4[i * G + s];
-      gj[s] = G4[jh * G + s];
+      for (int s = 0; s < G; s++) rv[kb][s] = with_corr ? Rv[s * n2 + kh * n + i] : (T)0;
     }
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
-      const int kh = k0 + kb;
-      if (kh >= n) break;
-      T corr = (T)0;
-      if (with_corr) {
+    for (int s = 0; s < G; s++) gi[s] = with_corr ? G4[i * G + s] : (T)0;
+    for (int jb = 0; jb < nj; jb += JU) {  // JU rows j^ per pass, all their loads in flight together
+      T sv[JU][KB], s2v[JU][KB], pj[JU][G];
 #pragma unroll
-        for (int s = 0; s < G; s++) {
-          corr += G4[kh * G + s] * pj[s];
-          corr += gj[s] * rv[kb][s];
-          corr += gi[s] * Xv[s * n2 + (unsigned)kh * n + jh];
+      for (int u = 0; u < JU; u++) {
+        const unsigned l0 = (unsigned)(j0 + (jb + u < nj ? jb + u : jb)) * n + i;  // past the edge: valid, not stored
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+          const unsigned l = l0 + (unsigned)(k0 + (kb < nk ? kb : 0)) * n2;
+          sv[u][kb] = s1v[l];
+          s2v[u][kb] = s2v_ ? s2v_[l] : (T)0;
+        }
+#pragma unroll
+        for (int s = 0; s < G; s++) pj[u][s] = with_corr ? Pv[s * n2 + l0] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < JU; u++) {
+        if (jb + u >= nj) break;
+        const int jh = j0 + jb + u;
+        const unsigned l0 = (unsigned)jh * n + i;
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+          if (kb >= nk) break;
+          const unsigned kh = (unsigned)(k0 + kb);
+          T corr = (T)0;
+          if (with_corr) {
+#pragma unroll
+            for (int s = 0; s < G; s++) {
+              corr += gk[kb][s] * pj[u][s];
+              corr += G4[jh * G + s] * rv[kb][s];
+              corr += gi[s] * Xv[s * n2 + kh * n + jh];
+            }
+          }
+          T val = alpha * sv[u][kb] + corr;
+          if (s2v_) val += beta * s2v[u][kb];
+          ov[l0 + kh * n2] = val;
         }
       }
-      T val = alpha * sv[kb] + corr;
-      if (s2v_) val += beta * s2v[kb];
-      ov[l0 + (unsigned)kh * n2] = val;
     }
   }
 }
@@ -171,7 +191,7 @@ void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double be
   T *P = b + off[4], *R = b + off[5], *X = b + off[6];
   const T *Vi = b + off[7], *ViT = b + off[8], *G4 = b + off[9];
   const int nb = nac * G;
-  etd_prof_begin(c, PC_MODEPROD);
+  etd_prof_begin(c, PC_CARRY);
   if (c->p.periodic) {
     // z sweep: P = Vi (d Vi^T), d = dz[v][s] (n x n)
     gemm_rm(c, dz, n, n2, ViT, n, 0, t1, n, n2, n, n, n, nb);
@@ -183,11 +203,11 @@ void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double be
     gemm_rm(c, dx, m, (long)m * m, ViT + (long)g * n, n, 0, t1, n, n2, m, n, m, nb);
     gemm_rm(c, Vi + g, n, 0, t1, n, n2, X, n, n2, n, n, m, nb);
   }  // physical sides: no ghost fill, nothing to add
-  constexpr int KB = 7;
-  const dim3 grid((unsigned)n, (unsigned)((n + KB - 1) / KB), (unsigned)nac);
-  k_carry_expand<T, KB><<<grid, 256, 0, c->stream>>>(s1, (T)alpha, s2, (T)beta, out, P, R, X, G4, n,
-                                                     c->p.periodic);
-  etd_prof_end(c, PC_MODEPROD);
+  constexpr int KB = 4, JB = 8, JU = 2;
+  const dim3 grid((unsigned)((n + JB - 1) / JB), (unsigned)((n + KB - 1) / KB), (unsigned)nac);
+  k_carry_expand<T, KB, JB, JU><<<grid, 256, 0, c->stream>>>(s1, (T)alpha, s2, (T)beta, out, P, R, X, G4, n,
+                                                             c->p.periodic);
+  etd_prof_end(c, PC_CARRY);
 }
 
 template int carry_prepare<double>(etd_ctx *);

@@ -185,7 +185,7 @@ extern "C" int etd_autotune(const etd_params *p, const double *V, const double *
   static const int sw_cands[] = {8, 4};
   static const int fft_cands[] = {1, 0};
   best->ms_per_step = 1e300;
-  double ms[1 + PC_COUNT];
+  double ms[2 + PC_COUNT];
   for (int a = 0; a < (c->mixed ? 1 : 5) && rc == ETD_OK; a++)
     for (int b = 0; b < (c->mixed ? 2 : 1) && rc == ETD_OK; b++)
       for (int f = 0; f < 2 && rc == ETD_OK; f++) {

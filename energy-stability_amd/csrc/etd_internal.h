@@ -126,14 +126,18 @@ struct etd_ctx {
   double *vm_part;
   double vm_X[3], vm_DA[3], vm_DG[3];  /* x (1 - x), D_A, D_G of Cu, Mn, Ni */
   double gbcc, ghcp;       /* CALPHAD Gibbs energies (free_energy 1) */
-  /* the AC spectrum carried between fixed steps (etd_carry.hip): carry_ok
-   * when spec[0 .. nac) holds the spectrum the AC fields of u were
-   * transformed back from and nothing has changed u since; no_carry turns it
-   * off (ETD_CARRY=0, ensemble members); cr_buf the correction's buffers */
+  /* the AC spectrum carried between steps (etd_carry.hip): carry_ok when
+   * the AC fields of u were transformed back from the spectrum
+   * carry_alpha s1 + carry_beta s2 (s1 null: spec[0 .. nac); s2 null: none)
+   * and nothing has changed u since; no_carry turns it off (ETD_CARRY=0,
+   * ensemble members); cr_buf the correction's buffers */
   int carry_ok, no_carry;
+  const void *carry_s1, *carry_s2;
+  double carry_alpha, carry_beta;
   void *cr_blas;           /* rocBLAS handle of the correction's batched GEMMs */
   int carry_next;          /* inside step_once: leave spec[0 .. nac) as the AC spectrum of u */
   void *cr_buf;
+  long long mp_fields;     /* field passes of the mode-product launches while timing */
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
@@ -148,7 +152,7 @@ struct etd_ctx {
 };
 
 /* profile classes; PC_FFT (the 3D FFT passes) is nested inside PC_ELASTIC */
-enum { PC_MODEPROD = 0, PC_ELASTIC, PC_NONLIN, PC_PHI, PC_ENERGY, PC_HALO, PC_FFT, PC_COUNT };
+enum { PC_MODEPROD = 0, PC_ELASTIC, PC_NONLIN, PC_PHI, PC_ENERGY, PC_HALO, PC_FFT, PC_CARRY, PC_COUNT };
 
 #ifdef __cplusplus
 extern "C" {
@@ -232,6 +236,14 @@ void ch_center(etd_ctx *c, float *u, float *N, int sign);
  * carry_correct writes out = alpha s1 + beta s2 + S^-1 (recorded changes)
  * for the nac AC fields (s2 may be null) */
 bool carry_possible(const etd_ctx *c);
+/* what the next carried step starts from: alpha s1 + beta s2 (s1 null:
+ * spec[0 .. nac); s2 null: nothing) */
+static inline void carry_base(etd_ctx *c, const void *s1, double alpha, const void *s2, double beta) {
+  c->carry_s1 = s1;
+  c->carry_alpha = alpha;
+  c->carry_s2 = s2;
+  c->carry_beta = beta;
+}
 template <typename T>
 int carry_prepare(etd_ctx *c);
 template <typename T>

@@ -608,8 +608,10 @@ class Etd:
         return st.as_dict(), log[: rows.value].copy()
 
     def time_steps(self, nsteps, with_diag=True):
-        ms = np.zeros(8)
-        launches = (C.c_int * 7)()
+        """etd_time_steps: (ms[10], launches[8]); ms[9] is the count of
+        field passes the mode-product launches made."""
+        ms = np.zeros(10)
+        launches = (C.c_int * 8)()
         _check(lib().etd_time_steps(self.ctx, nsteps, 1 if with_diag else 0, _ptr(ms), launches), self.ctx,
                "time_steps")
         return ms, list(launches)

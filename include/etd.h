@@ -246,8 +246,10 @@ int etd_k_phiscale(etd_ctx *c, int var, double *uhat, const double *nhat);
 /* ---- timing: runs nsteps on the context stream between HIP events ---- */
 /* ms[0] = total, ms[1] = mode-product kernels, ms[2] = elastic (FFT +
  * contraction), ms[3] = nonlinear, ms[4] = phi scaling, ms[5] = energy,
- * ms[6] = halo, ms[7] = the 3D FFT passes alone (part of ms[2]); launches[k]
- * counts the launch groups of class k (7 ints). */
+ * ms[6] = halo, ms[7] = the 3D FFT passes alone (part of ms[2]), ms[8] = the
+ * carried spectrum's correction, ms[9] = the number of field passes the
+ * mode-product launches made (a count: fields x 3 axes); launches[k] counts
+ * the launch groups of class k (8 ints). */
 int etd_time_steps(etd_ctx *c, int nsteps, int with_diag, double *ms, int *launches);
 
 /* ---- adaptive ETD and the learned dt controller ----

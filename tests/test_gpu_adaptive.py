@@ -54,8 +54,8 @@ def test_all_accepted_without_extrapolation_is_half_steps(precision, etd2, elast
     prm = small_params(n=16, nac=3, nch=1, elastic=elastic, aniso=1, dt=dt, etd2=etd2)
     prm["precision"] = precision
     f = noise_fields(prm, amp=0.3)
-    a = gpu(prm, f)
-    b = gpu(dict(prm, dt=dt / 2), f, carry=False)  # the attempts transform u each step, as this does
+    a = gpu(prm, f, carry=False)  # every attempt transforms u, as plain stepping without the carry does
+    b = gpu(dict(prm, dt=dt / 2), f, carry=False)
     st, log = a.run_adaptive(log_cap=16, t_end=3 * dt, dt0=dt, tol=1e9, grow=1.0, shrink=1.0, extrapolate=0,
                              energy_test=0)
     assert st["accepted"] == 3 and st["rejected_err"] == 0 and st["step_evals"] == 9
@@ -94,6 +94,41 @@ def test_plain_controller_matches_oracle_restatement(elastic):
     assert np.abs(log[:, 3] - ro[:, 3]).max() < 1e-10 * np.abs(ro[:, 3]).max()
     for v in range(4):
         assert np.abs(g.get_field(v) - o.get_field(v)).max() < 1e-10
+
+
+@pytest.mark.parametrize("precision,extrapolate", [(0, 1), (2, 1), (0, 0)])
+def test_carried_attempts_match_transformed_attempts(precision, extrapolate):
+    """ETD1 attempts that carry the AC spectrum (the second half step from the
+    first's phi-scaled spectrum, the next attempt from spec or 2 spec - SP)
+    make the same decisions as attempts that transform u, with fields equal
+    to roundoff; rejections (which drop the carry) included."""
+    say(f"adaptive carried vs transformed precision={precision} extrapolate={extrapolate}")
+    prm = dict(small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, dt=0.01, KE=3.2, landau=(0.2, 3.0, 3.0)),
+               precision=precision)
+    f = noise_fields(prm, amp=0.05, ch_amp=0.05)
+    if precision:
+        f = [x.astype(np.float32).astype(np.float64) for x in f]
+    kw = dict(t_end=0.6, dt0=0.2, dt_min=1e-5, dt_max=0.3, tol=2e-4, extrapolate=extrapolate)
+    a, b = gpu(prm, f, carry=True), gpu(prm, f, carry=False)
+    sa, la = a.run_adaptive(log_cap=1000, **kw)
+    sb, lb = b.run_adaptive(log_cap=1000, **kw)
+    say(f"  carried {sa}  transformed {sb}")
+    assert sa["rejected_err"] + sa["rejected_energy"] > 0 and sa["accepted"] > 3
+    for k in ("accepted", "rejected_err", "rejected_energy", "step_evals"):
+        assert sa[k] == sb[k], k
+    # dt follows sqrt(tol / err), and err is a difference of two states of
+    # size tol: a roundoff-level change of the fields moves dt at 1e-11
+    assert np.abs(la[:, 1] - lb[:, 1]).max() <= (1e-9 if precision == 0 else 1e-3) * lb[:, 1].max()
+    d = max(np.abs(a.get_field(v) - b.get_field(v)).max() for v in range(4))
+    say(f"  max |d| = {d:.2e}")
+    assert d < (1e-11 if precision == 0 else 1e-5)
+    # a fixed step after the run starts afresh (the run may end on 2 spec - SP)
+    a.step(1)
+    b.step(1)
+    d = max(np.abs(a.get_field(v) - b.get_field(v)).max() for v in range(4))
+    assert d < (1e-11 if precision == 0 else 1e-5)
+    a.close()
+    b.close()
 
 
 def scheme_energy(prm, fields):
