@@ -58,15 +58,24 @@ CONTROLLER = os.path.join(ROOT, "energy-stability_amd", "controller", "mlp.npz")
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def measured_traffic(name):
-    """HBM bytes per launch from a committed counter summary, or None."""
+def measured_traffic(name, key="traffic_bytes_per_launch_avg"):
+    """A figure (HBM bytes per launch, or per field pass) from a committed
+    counter summary, or None."""
     path = os.path.join(PROFILES, name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if "traffic_bytes_per_launch_avg:" in line:
-            return float(line.split("traffic_bytes_per_launch_avg:")[1].strip(" *\n"))
+        if key + ":" in line:
+            return float(line.split(key + ":")[1].strip(" *\n"))
     return None
+
+
+def traffic_per_launch(name, fields_per_launch):
+    """HBM bytes of a mode-product launch of this many fields, from the
+    summary's bytes per field pass (a launch of f fields moves f field
+    passes)."""
+    per_field = measured_traffic(name, "traffic_bytes_per_field_pass")
+    return None if per_field is None else per_field * fields_per_launch
 
 
 def cpu_model():
@@ -209,7 +218,8 @@ def main():
                            "unit": "TFLOP/s", "frac": mp_flops / (ms[1] / 1e3) / PEAK_FP64_MFMA,
                            "field_passes_per_step": ms[9] / K,
                            "launch_ms_avg": mp_launch_ms, "launches": launches[0],
-                           "traffic": measured_traffic("k_modeprod_dl_192cube_13fields_hbm_counters.md")}}
+                           "traffic": traffic_per_launch("k_modeprod_dl_192cube_13fields_hbm_counters.md",
+                                                         ms[9] / max(1, launches[0]))}}
 
     # ---- mixed precision, fixed dt: per-kernel timing of the headline path's kernels
     mx = None
@@ -236,8 +246,9 @@ def main():
                            "FP32 accumulate)"),
                 "bound": "hbm", "achieved": mp_bytes / (mp_launch_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": mp_bytes / (mp_launch_ms / 1e3) / PEAK_HBM,
-                "traffic": measured_traffic("k_modeprod_split_192cube_hbm_counters.md" if args.precision == 1 else
-                                            "k_modeprod_f16_192cube_hbm_counters.md"), "launch_ms": mp_launch_ms,
+                "traffic": traffic_per_launch("k_modeprod_split_192cube_hbm_counters.md" if args.precision == 1 else
+                                              "k_modeprod_f16_192cube_hbm_counters.md", mp_fields_avg),
+                "launch_ms": mp_launch_ms,
                 "launches": launches[0], "fields_per_launch": mp_fields_avg, "ms_per_step": ms[1] / K,
                 "mfma_tflops": mp_mfma / (mp_launch_ms / 1e3) / 1e12,
                 "mfma_frac_16bit": mp_mfma / (mp_launch_ms / 1e3) / PEAK_BF16_MFMA}

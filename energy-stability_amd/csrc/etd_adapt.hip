@@ -595,7 +595,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
       stage_forward<T>(c);  // N(u_n) and the forward transforms, shared
       // the full step and the first half step from one phi pass
-      launch_phiscale_pair<T>(c, F.spec, F.spec + cnt, SP, dt, SP2, 0.5 * dt);
+      phi_stage<T>(c, SP, dt, SP2, 0.5 * dt);
       modeprod3<T>(c, 0, SP, SB, V, 0);
       modeprod3<T>(c, 0, SP2, F.u, V, 0);
       c->p.dt = 0.5 * dt;

@@ -369,6 +369,7 @@ extern "C" void etd_close(etd_ctx *c) {
     c->spec = c->bspec;
   }
   (void)hipFree(c->cr_buf);
+  (void)hipFree(c->cr_phi);
   if (c->cr_blas) rocblas_destroy_handle((rocblas_handle)c->cr_blas);
   (void)hipFree(c->u);
   (void)hipFree(c->N2);
@@ -611,7 +612,8 @@ void modeprod3(etd_ctx *c, int forward, const T *in, T *out, int nf, int accumul
 // With the AC spectrum carried (carry_ok: the AC fields of u were transformed
 // back from carry_alpha s1 + carry_beta s2, by default spec[0 .. nac)), only
 // [u_CH | N1] is transformed; the AC spectrum of the ghost-filled fields is
-// the carried one plus the transform of the fill's changes (carry_correct).
+// the carried one plus the transform of the fill's changes, which phi_stage
+// forms in its pass (carry_pend).
 template <typename T>
 void stage_forward(etd_ctx *c) {
   const int V = c->V;
@@ -620,6 +622,7 @@ void stage_forward(etd_ctx *c) {
   const T *b1 = c->carry_s1 ? (const T *)c->carry_s1 : F.spec, *b2 = (const T *)c->carry_s2;
   const double ba = c->carry_alpha, bb = c->carry_beta;
   c->carry_ok = 0;
+  c->carry_pend = 0;
   carry_base(c, nullptr, 1.0, nullptr, 0.0);
   if (carry) {
     T *dz, *dy, *dx;
@@ -637,7 +640,12 @@ void stage_forward(etd_ctx *c) {
   if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, -1);
   if (carry) {
     modeprod3<T>(c, 1, F.u + c->n3 * c->nac, F.spec + c->n3 * c->nac, c->nch + V, 0);
-    carry_correct<T>(c, b1, ba, b2, bb, F.spec);
+    // the correction joins the phi update (phi_stage)
+    c->carry_pend = 1;
+    c->pend_s1 = b1;
+    c->pend_alpha = ba;
+    c->pend_s2 = b2;
+    c->pend_beta = bb;
   } else {
     modeprod3<T>(c, 1, F.u, F.spec, 2 * V * c->P, 0);
   }
@@ -648,6 +656,31 @@ void stage_forward(etd_ctx *c) {
 }
 
 template <typename T>
+void phi_stage(etd_ctx *c, T *out, double dt, T *out2, double dt2) {
+  const etd_fields<T> F = fields<T>(c);
+  const size_t n3 = c->n3;
+  const int V = c->V, nac = c->nac;
+  const T *nhat = F.spec + n3 * V * c->P;
+  if (!c->carry_pend) {
+    if (out2)
+      launch_phiscale_pair<T>(c, F.spec, nhat, out, dt, out2, dt2, 0, V);
+    else
+      phiscale_boxes<T>(c, F.spec, nhat, 0);
+    return;
+  }
+  c->carry_pend = 0;  // only single-box contexts carry: N-hat is at spec + n3 V
+  carry_correct_phi<T>(c, (const T *)c->pend_s1, c->pend_alpha, (const T *)c->pend_s2, c->pend_beta, nhat, out, dt,
+                       out2, dt2);
+  if (c->nch) {  // the CH fields were transformed: their plain phi update
+    const size_t o = n3 * nac;
+    if (out2)
+      launch_phiscale_pair<T>(c, F.spec + o, nhat + o, out + o, dt, out2 + o, dt2, nac, c->nch);
+    else
+      launch_phiscale<T>(c, F.spec + o, nhat + o, out + o, nac, c->nch, 0);
+  }
+}
+
+template <typename T>
 int step_once_t(etd_ctx *c) {
   const int V = c->V;
   const size_t n3 = c->n3;
@@ -655,7 +688,7 @@ int step_once_t(etd_ctx *c) {
   if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
   stage_forward<T>(c);
   const int PV = V * c->P;
-  phiscale_boxes<T>(c, F.spec, F.spec + n3 * PV, 0);
+  phi_stage<T>(c, F.spec, c->p.dt, (T *)nullptr, 0.0);  // in place, at the context's dt
   modeprod3<T>(c, 0, F.spec, F.u, PV, 0);
   if (c->p.etd2) {
     if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
@@ -688,6 +721,8 @@ int step_once_t(etd_ctx *c) {
 
 template void modeprod3<double>(etd_ctx *, int, const double *, double *, int, int);
 template void modeprod3<float>(etd_ctx *, int, const float *, float *, int, int);
+template void phi_stage<double>(etd_ctx *, double *, double, double *, double);
+template void phi_stage<float>(etd_ctx *, float *, double, float *, double);
 template void stage_forward<double>(etd_ctx *);
 template void stage_forward<float>(etd_ctx *);
 template int step_once_t<double>(etd_ctx *);

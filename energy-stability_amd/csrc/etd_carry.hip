@@ -26,32 +26,79 @@
 
 #include "etd_internal.h"
 
-// out = alpha s1 + beta s2 (s2 may be null) + S^-1 d over nac fields.  A
-// block owns KB values of k^ and JB values of j^ of one field, a thread one
-// i^ (and i^ + 256 ... where n exceeds the block).  The thread's R values of
-// its KB rows k^ stay in registers across the JB rows j^, its P values of one
-// j^ across the KB rows k^; X, Vi[k^][G] and Vi[j^][G] are uniform.  All
-// loads of JU rows j^ (KB values of s1 and s2 each, their P values) are
-// issued before any is used.  The rows a block writes are JB consecutive rows of KB planes.
-// out may be s1: every element is read by the thread that writes it, before
-// it writes it, and by no other thread.
-template <typename T, int KB, int JB, int JU>
-__global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, T alpha, const T *__restrict__ s2,
-                                                      T beta, T *__restrict__ out, const T *__restrict__ P,
-                                                      const T *__restrict__ R, const T *__restrict__ X,
-                                                      const T *__restrict__ G4, int n, int with_corr) {
+// phi1 and 1 - phi1 H' of one AC point from H and e = exp(-dt H), with
+// phi_at's formula and |H| > 1e-8 switch
+__device__ __forceinline__ phi_pt phi_sep(double H, double e, double dt) {
+  phi_pt r;
+  const bool big = fabs(H) > 1.0e-8;
+  r.phi1 = big ? (1.0 - e) / H : dt;
+  r.phi2 = 0.0;
+  r.lin = big ? 1.0 - r.phi1 * H : 1.0;
+  return r;
+}
+
+// exp(-dt H) of an AC point factors over the axes: -dt H = dt LE epn2
+// (kkz Dz + kky Dy + kkx Dx) - dt LE KE, so with ax[a n + i] = exp(dt LE epn2
+// kk_a D_a[i]) (a = x, y, z; and at dt2 from 3 n) and e0 = exp(-dt LE KE),
+// exp(-dt H) = e0 ax_z[k^] ax_y[j^] ax_x[i^].  Every factor is at most 1
+// (D <= 0), so the product overflows only where e0 = exp(-dt H) of the
+// zero mode does, and then the update is not finite either way.
+__global__ void k_phi_axes(const double *DDX, const double *DDY, const double *DDZ, int n, phi_args a,
+                           int two, double *ax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double c = a.LE * a.epn2, c2 = c * a.dt2;
+  ax[i] = exp(c * a.dt * a.kkx * DDX[i]);
+  ax[n + i] = exp(c * a.dt * a.kky * DDY[i]);
+  ax[2 * n + i] = exp(c * a.dt * a.kkz * DDZ[i]);
+  if (two) {
+    ax[3 * n + i] = exp(c2 * a.kkx * DDX[i]);
+    ax[4 * n + i] = exp(c2 * a.kky * DDY[i]);
+    ax[5 * n + i] = exp(c2 * a.kkz * DDZ[i]);
+  }
+}
+
+// u-hat = alpha s1 + beta s2 (s2 may be null) + S^-1 d over nac fields, and
+// out = u-hat (PH = 0), or the first-order phi update of u-hat and N-hat at
+// step a.dt (PH = 1; and into out2 at step a.dt2, PH = 2) as k_phiscale
+// forms it from u-hat stored in T, with exp(-dt H) from the axis factors
+// ax (k_phi_axes) instead of one exp per point.  A block owns KB values of k^
+// and JB values of j^ of one field, a thread one i^ (and i^ + 256 ... where n
+// exceeds the block).  The thread's R values of its KB rows k^ stay in
+// registers across the JB rows j^, its P values of one j^ across the KB rows
+// k^; X, Vi[k^][G] and Vi[j^][G] are uniform.  All loads of JU rows j^ (KB
+// values of s1, s2 and N-hat each, their P values) are issued before any is
+// used.  The rows a block writes are JB consecutive rows of KB planes.  out
+// may be s1 or s2: every element is read by the thread that writes it,
+// before it writes it, and by no other thread (so no __restrict__ on them).
+template <typename T, int KB, int JB, int JU, int PH>
+__global__ __launch_bounds__(256) void k_carry_expand(const T *s1, T alpha, const T *s2, T beta, T *out,
+                                                      const T *__restrict__ P, const T *__restrict__ R,
+                                                      const T *__restrict__ X, const T *__restrict__ G4, int n,
+                                                      int with_corr, const T *nhat, T *out2,
+                                                      const double *__restrict__ DDX,
+                                                      const double *__restrict__ DDY,
+                                                      const double *__restrict__ DDZ, phi_args a,
+                                                      const double *__restrict__ ax, double2 e0) {
   constexpr int G = 4;  // 2 g ghost layers, g = 2 (carry_possible)
   const int j0 = blockIdx.x * JB, k0 = blockIdx.y * KB, v = blockIdx.z;
   const int nk = n - k0 < KB ? n - k0 : KB, nj = n - j0 < JB ? n - j0 : JB;
   const unsigned n2 = (unsigned)n * n;
+  const size_t fo = (size_t)v * n2 * n;
   const T *Pv = P + (size_t)v * G * n2, *Rv = R + (size_t)v * G * n2, *Xv = X + (size_t)v * G * n2;
-  const T *s1v = s1 + (size_t)v * n2 * n, *s2v_ = s2 ? s2 + (size_t)v * n2 * n : nullptr;
-  T *ov = out + (size_t)v * n2 * n;
+  const T *s1v = s1 + fo, *s2v_ = s2 ? s2 + fo : nullptr, *nv_ = PH ? nhat + fo : nullptr;
+  T *ov = out + fo, *ov2 = PH == 2 ? out2 + fo : nullptr;
   T gk[KB][G];  // Vi[k^][G] of the block's rows (uniform)
+  double ez[KB], ez2[KB];  // the z factors of exp(-dt H) of the block's rows
+  const double E[2] = {e0.x, e0.y};
 #pragma unroll
-  for (int kb = 0; kb < KB; kb++)
+  for (int kb = 0; kb < KB; kb++) {
+    const int kh = k0 + (kb < nk ? kb : 0);
 #pragma unroll
-    for (int s = 0; s < G; s++) gk[kb][s] = with_corr ? G4[(k0 + (kb < nk ? kb : 0)) * G + s] : (T)0;
+    for (int s = 0; s < G; s++) gk[kb][s] = with_corr ? G4[kh * G + s] : (T)0;
+    ez[kb] = PH ? ax[2 * n + kh] : 0.0;
+    ez2[kb] = PH == 2 ? ax[5 * n + kh] : 0.0;
+  }
   for (int i = threadIdx.x; i < n; i += blockDim.x) {  // n may exceed the block
     T rv[KB][G], gi[G];
 #pragma unroll
@@ -62,8 +109,16 @@ __global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, 
     }
 #pragma unroll
     for (int s = 0; s < G; s++) gi[s] = with_corr ? G4[i * G + s] : (T)0;
+    const double dxi = PH ? a.kkx * DDX[i] : 0.0, exi = PH ? ax[i] : 0.0, exi2 = PH == 2 ? ax[3 * n + i] : 0.0;
     for (int jb = 0; jb < nj; jb += JU) {  // JU rows j^ per pass, all their loads in flight together
-      T sv[JU][KB], s2v[JU][KB], pj[JU][G];
+      T sv[JU][KB], s2v[JU][KB], nv[JU][KB], pj[JU][G];
+      double ey[JU], ey2[JU];
+#pragma unroll
+      for (int u = 0; u < JU; u++) {
+        const int jh = j0 + (jb + u < nj ? jb + u : jb);
+        ey[u] = PH ? ax[n + jh] : 0.0;
+        ey2[u] = PH == 2 ? ax[4 * n + jh] : 0.0;
+      }
 #pragma unroll
       for (int u = 0; u < JU; u++) {
         const unsigned l0 = (unsigned)(j0 + (jb + u < nj ? jb + u : jb)) * n + i;  // past the edge: valid, not stored
@@ -72,6 +127,7 @@ __global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, 
           const unsigned l = l0 + (unsigned)(k0 + (kb < nk ? kb : 0)) * n2;
           sv[u][kb] = s1v[l];
           s2v[u][kb] = s2v_ ? s2v_[l] : (T)0;
+          nv[u][kb] = PH ? nv_[l] : (T)0;
         }
 #pragma unroll
         for (int s = 0; s < G; s++) pj[u][s] = with_corr ? Pv[s * n2 + l0] : (T)0;
@@ -96,7 +152,20 @@ __global__ __launch_bounds__(256) void k_carry_expand(const T *__restrict__ s1, 
           }
           T val = alpha * sv[u][kb] + corr;
           if (s2v_) val += beta * s2v[u][kb];
-          ov[l0 + kh * n2] = val;
+          if constexpr (PH == 0) {
+            ov[l0 + kh * n2] = val;
+          } else {
+            // H of this point as k_phiscale forms it (AC kind); exp(-dt H)
+            // from the per-axis factors
+            const double tmp = a.kkz * DDZ[kh] + a.kky * DDY[jh] + dxi;
+            const double H = -a.LE * (tmp * a.epn2 - a.KE);
+            const phi_pt ph = phi_sep(H, E[0] * ez[kb] * ey[u] * exi, a.dt);
+            ov[l0 + kh * n2] = (T)(ph.lin * (double)val + ph.phi1 * (double)nv[u][kb]);
+            if constexpr (PH == 2) {
+              const phi_pt p2 = phi_sep(H, E[1] * ez2[kb] * ey2[u] * exi2, a.dt2);
+              ov2[l0 + kh * n2] = (T)(p2.lin * (double)val + p2.phi1 * (double)nv[u][kb]);
+            }
+          }
         }
       }
     }
@@ -134,6 +203,7 @@ int carry_prepare(etd_ctx *c) {
   size_t off[10];
   const size_t tot = carry_elems<T>(c, off);
   if (hipMalloc(&c->cr_buf, sizeof(T) * tot) != hipSuccess) return ETD_ERR_NOMEM;
+  if (hipMalloc(&c->cr_phi, sizeof(double) * 6 * (size_t)c->n) != hipSuccess) return ETD_ERR_NOMEM;
   const int n = c->n, g = c->g, G = 2 * g;
   T *h = (T *)malloc(sizeof(T) * (2 * (size_t)n * n + (size_t)n * G));
   if (!h) return ETD_ERR_NOMEM;
@@ -180,8 +250,11 @@ static void gemm_rm(etd_ctx *c, const float *A, long lda, long sa, const float *
                                 &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
 }
 
+// P, R, X from the recorded changes (six batched GEMMs; nothing to do on
+// physical sides, which have no ghost fill)
 template <typename T>
-void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out) {
+static void carry_gemms(etd_ctx *c) {
+  if (!c->p.periodic) return;
   const int n = c->n, g = c->g, G = 2 * g, m = n - 2 * g, nac = c->nac;
   const long n2 = (long)n * n;
   size_t off[10];
@@ -189,24 +262,57 @@ void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double be
   T *b = (T *)c->cr_buf;
   T *dz = b + off[0], *dy = b + off[1], *dx = b + off[2], *t1 = b + off[3];
   T *P = b + off[4], *R = b + off[5], *X = b + off[6];
-  const T *Vi = b + off[7], *ViT = b + off[8], *G4 = b + off[9];
+  const T *Vi = b + off[7], *ViT = b + off[8];
   const int nb = nac * G;
-  etd_prof_begin(c, PC_CARRY);
-  if (c->p.periodic) {
-    // z sweep: P = Vi (d Vi^T), d = dz[v][s] (n x n)
-    gemm_rm(c, dz, n, n2, ViT, n, 0, t1, n, n2, n, n, n, nb);
-    gemm_rm(c, Vi, n, 0, t1, n, n2, P, n, n2, n, n, n, nb);
-    // y sweep: R = Vi_I (d Vi^T), d = dy[v][s] (m x n)
-    gemm_rm(c, dy, n, (long)m * n, ViT, n, 0, t1, n, n2, m, n, n, nb);
-    gemm_rm(c, Vi + g, n, 0, t1, n, n2, R, n, n2, n, n, m, nb);
-    // x sweep: X = Vi_I (d Vi_I^T), d = dx[v][s] (m x m), Vi_I^T = rows g .. g+m-1 of Vi^T
-    gemm_rm(c, dx, m, (long)m * m, ViT + (long)g * n, n, 0, t1, n, n2, m, n, m, nb);
-    gemm_rm(c, Vi + g, n, 0, t1, n, n2, X, n, n2, n, n, m, nb);
-  }  // physical sides: no ghost fill, nothing to add
+  // z sweep: P = Vi (d Vi^T), d = dz[v][s] (n x n)
+  gemm_rm(c, dz, n, n2, ViT, n, 0, t1, n, n2, n, n, n, nb);
+  gemm_rm(c, Vi, n, 0, t1, n, n2, P, n, n2, n, n, n, nb);
+  // y sweep: R = Vi_I (d Vi^T), d = dy[v][s] (m x n)
+  gemm_rm(c, dy, n, (long)m * n, ViT, n, 0, t1, n, n2, m, n, n, nb);
+  gemm_rm(c, Vi + g, n, 0, t1, n, n2, R, n, n2, n, n, m, nb);
+  // x sweep: X = Vi_I (d Vi_I^T), d = dx[v][s] (m x m), Vi_I^T = rows g .. g+m-1 of Vi^T
+  gemm_rm(c, dx, m, (long)m * m, ViT + (long)g * n, n, 0, t1, n, n2, m, n, m, nb);
+  gemm_rm(c, Vi + g, n, 0, t1, n, n2, X, n, n2, n, n, m, nb);
+}
+
+template <typename T, int PH>
+static void carry_expand(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out, const T *nhat,
+                         T *out2, const phi_args &a) {
+  const int n = c->n, nac = c->nac;
+  size_t off[10];
+  carry_elems<T>(c, off);
+  const T *b = (const T *)c->cr_buf;
+  double2 e0 = make_double2(0.0, 0.0);
+  if (PH) {
+    k_phi_axes<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->DDX, c->DDY, c->DDZ, n, a, PH == 2,
+                                                                   c->cr_phi);
+    e0 = make_double2(exp(-a.dt * a.LE * a.KE), exp(-a.dt2 * a.LE * a.KE));
+  }
   constexpr int KB = 4, JB = 8, JU = 2;
   const dim3 grid((unsigned)((n + JB - 1) / JB), (unsigned)((n + KB - 1) / KB), (unsigned)nac);
-  k_carry_expand<T, KB, JB, JU><<<grid, 256, 0, c->stream>>>(s1, (T)alpha, s2, (T)beta, out, P, R, X, G4, n,
-                                                             c->p.periodic);
+  k_carry_expand<T, KB, JB, JU, PH><<<grid, 256, 0, c->stream>>>(s1, (T)alpha, s2, (T)beta, out, b + off[4],
+                                                                 b + off[5], b + off[6], b + off[9], n,
+                                                                 c->p.periodic, nhat, out2, c->DDX, c->DDY,
+                                                                 c->DDZ, a, c->cr_phi, e0);
+}
+
+template <typename T>
+void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out) {
+  etd_prof_begin(c, PC_CARRY);
+  carry_gemms<T>(c);
+  carry_expand<T, 0>(c, s1, alpha, s2, beta, out, nullptr, nullptr, phi_args_of(c, 0.0, 0.0));
+  etd_prof_end(c, PC_CARRY);
+}
+
+template <typename T>
+void carry_correct_phi(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, const T *nhat, T *out,
+                       double dt, T *out2, double dt2) {
+  etd_prof_begin(c, PC_CARRY);
+  carry_gemms<T>(c);
+  if (out2)
+    carry_expand<T, 2>(c, s1, alpha, s2, beta, out, nhat, out2, phi_args_of(c, dt, dt2));
+  else
+    carry_expand<T, 1>(c, s1, alpha, s2, beta, out, nhat, nullptr, phi_args_of(c, dt, 0.0));
   etd_prof_end(c, PC_CARRY);
 }
 
@@ -216,3 +322,7 @@ template void carry_deltas<double>(etd_ctx *, double **, double **, double **);
 template void carry_deltas<float>(etd_ctx *, float **, float **, float **);
 template void carry_correct<double>(etd_ctx *, const double *, double, const double *, double, double *);
 template void carry_correct<float>(etd_ctx *, const float *, double, const float *, double, float *);
+template void carry_correct_phi<double>(etd_ctx *, const double *, double, const double *, double, const double *,
+                                        double *, double, double *, double);
+template void carry_correct_phi<float>(etd_ctx *, const float *, double, const float *, double, const float *,
+                                       float *, double, float *, double);

@@ -134,9 +134,15 @@ struct etd_ctx {
   int carry_ok, no_carry;
   const void *carry_s1, *carry_s2;
   double carry_alpha, carry_beta;
+  /* set by stage_forward: the AC spectrum still lacks the fill's changes,
+   * which phi_stage adds from this base on its way */
+  int carry_pend;
+  const void *pend_s1, *pend_s2;
+  double pend_alpha, pend_beta;
   void *cr_blas;           /* rocBLAS handle of the correction's batched GEMMs */
   int carry_next;          /* inside step_once: leave spec[0 .. nac) as the AC spectrum of u */
   void *cr_buf;
+  double *cr_phi;          /* the per-axis factors of exp(-dt H) of the phi-fused correction */
   long long mp_fields;     /* field passes of the mode-product launches while timing */
   /* energy partials */
   double *part;            /* device partial sums */
@@ -189,10 +195,64 @@ inline etd_fields<float> fields<float>(const etd_ctx *c) {
 template <typename T>
 void launch_ghost_fill(etd_ctx *c, T *base, int nfields, size_t fstride, T *dz = nullptr, T *dy = nullptr,
                        T *dx = nullptr);
+/* phi scaling: ac_updateU_new / ch_updateU_new (calc.c:489-535) and
+ * prepare_U2_new (553-567) with phi1, phi2 (vars.c:302-359) recomputed
+ * per point from D */
+struct phi_args {
+  int n, nac;
+  double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
+  double dt2;  // with a second output: its step
+};
+
+// phi1, phi2 and 1 - phi1 H' of one point for one kind of field (AC or CH);
+// the formula and the |H| > 1e-8 switch are the reference's (vars.c:315-325)
+struct phi_pt {
+  double phi1, phi2, lin;
+};
+__device__ __forceinline__ phi_pt phi_at(double H, double dt) {
+  phi_pt r;
+  if (fabs(H) > 1.0e-8) {
+    const double e = exp(-dt * H);
+    r.phi1 = (1.0 - e) / H;
+    r.phi2 = (1.0 - r.phi1 / dt) / H;
+  } else {
+    r.phi1 = dt;
+    r.phi2 = dt / 2;
+  }
+  const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
+  r.lin = 1.0 - r.phi1 * Hp;
+  return r;
+}
+
+static inline phi_args phi_args_of(const etd_ctx *c, double dt, double dt2) {
+  phi_args a;
+  a.n = c->n;
+  a.nac = c->nac;
+  a.dt = dt;
+  a.epn2 = c->p.epn2;
+  a.LE = c->p.LE;
+  a.KE = c->p.KE;
+  a.LCI = c->p.LCI;
+  a.KCI = c->p.KCI;
+  a.kkx = c->p.kkx;
+  a.kky = c->p.kky;
+  a.kkz = c->p.kkz;
+  a.dt2 = dt2;
+  return a;
+}
 template <typename T>
 void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2);
+/* first order over fields first_var .. first_var + nvars (pointers at
+ * field first_var): out at step dt, out2 at step dt2 */
 template <typename T>
-void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2);
+void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2,
+                          int first_var, int nvars);
+/* the first-order phi update of the forward stage's spectra (spec, N-hat
+ * after all boxes' u-hat) into out (and out2 at step dt2; single: out =
+ * spec, dt = the context's dt); a carried AC spectrum that stage_forward
+ * left pending is corrected in the same pass */
+template <typename T>
+void phi_stage(etd_ctx *c, T *out, double dt, T *out2, double dt2);
 template <typename T>
 void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre = 0);
 template <typename T>
@@ -250,6 +310,11 @@ template <typename T>
 void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx);
 template <typename T>
 void carry_correct(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, T *out);
+/* the same spectrum, phi-updated with N-hat (nac fields from nhat) at step
+ * dt into out, and at step dt2 into out2 unless out2 is null */
+template <typename T>
+void carry_correct_phi(etd_ctx *c, const T *s1, double alpha, const T *s2, double beta, const T *nhat, T *out,
+                       double dt, T *out2, double dt2);
 
 /* step pieces (etd_api.hip) */
 template <typename T>

@@ -656,32 +656,6 @@ void launch_modeprod(etd_ctx *c, const double *A4, const double *in, size_t in_f
 // (553-567) on the rotated spectral layout l = i' + n j' + n^2 k'.  phi1 and
 // phi2 (vars.c:302-359) are recomputed in registers from D instead of read
 // from n^3 tables.
-struct phi_args {
-  int n, nac;
-  double dt, epn2, LE, KE, LCI, KCI, kkx, kky, kkz;
-  double dt2;  // with a second output: its step
-};
-
-// phi1, phi2 and 1 - phi1 H' of one point for one kind of field (AC or CH);
-// the formula and the |H| > 1e-8 switch are the reference's (vars.c:315-325)
-struct phi_pt {
-  double phi1, phi2, lin;
-};
-__device__ __forceinline__ phi_pt phi_at(double H, double dt) {
-  phi_pt r;
-  if (fabs(H) > 1.0e-8) {
-    const double e = exp(-dt * H);
-    r.phi1 = (1.0 - e) / H;
-    r.phi2 = (1.0 - r.phi1 / dt) / H;
-  } else {
-    r.phi1 = dt;
-    r.phi2 = dt / 2;
-  }
-  const double Hp = fabs(H) < 1.0e-8 ? 0.0 : H;
-  r.lin = 1.0 - r.phi1 * Hp;
-  return r;
-}
-
 // One thread per spectral point, looping over the fields: every AC field
 // shares H (and so phi), as do the CH fields, so exp and the divisions run
 // once per point and kind instead of once per field.
@@ -742,45 +716,23 @@ __global__ void k_phiscale(const T *uhat, const T *nhat, T *outp, size_t fs, int
 // current dt; out may be uhat
 template <typename T>
 void launch_phiscale(etd_ctx *c, const T *uhat, const T *nhat, T *out, int first_var, int nvars, int order2) {
-  phi_args a;
-  a.n = c->n;
-  a.nac = c->nac;
-  a.dt = c->p.dt;
-  a.epn2 = c->p.epn2;
-  a.LE = c->p.LE;
-  a.KE = c->p.KE;
-  a.LCI = c->p.LCI;
-  a.KCI = c->p.KCI;
-  a.kkx = c->p.kkx;
-  a.kky = c->p.kky;
-  a.kkz = c->p.kkz;
-  a.dt2 = 0.0;
+  const phi_args a = phi_args_of(c, c->p.dt, 0.0);
   etd_prof_begin(c, PC_PHI);
   k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, first_var, nvars,
                                                                       c->DDX, c->DDY, c->DDZ, a, order2, nullptr);
   etd_prof_end(c, PC_PHI);
 }
 
-// first order, all variables: out = phi-update at step dt, out2 at step dt2,
-// from one read of uhat and nhat (the step-doubling pair)
+// first order over fields first_var .. first_var + nvars: out = phi-update
+// at step dt, out2 at step dt2, from one read of uhat and nhat (the
+// step-doubling pair)
 template <typename T>
-void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2) {
-  phi_args a;
-  a.n = c->n;
-  a.nac = c->nac;
-  a.dt = dt;
-  a.epn2 = c->p.epn2;
-  a.LE = c->p.LE;
-  a.KE = c->p.KE;
-  a.LCI = c->p.LCI;
-  a.KCI = c->p.KCI;
-  a.kkx = c->p.kkx;
-  a.kky = c->p.kky;
-  a.kkz = c->p.kkz;
-  a.dt2 = dt2;
+void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, double dt, T *out2, double dt2,
+                          int first_var, int nvars) {
+  const phi_args a = phi_args_of(c, dt, dt2);
   etd_prof_begin(c, PC_PHI);
-  k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, 0, c->V, c->DDX,
-                                                                      c->DDY, c->DDZ, a, 0, out2);
+  k_phiscale<T><<<(unsigned)((c->n3 + 255) / 256), 256, 0, c->stream>>>(uhat, nhat, out, c->n3, first_var, nvars,
+                                                                      c->DDX, c->DDY, c->DDZ, a, 0, out2);
   etd_prof_end(c, PC_PHI);
 }
 
@@ -2879,7 +2831,7 @@ void ch_mean_restore(etd_ctx *c, float *u) {
 #define ETD_INST(T)                                                                   \
   template void launch_ghost_fill<T>(etd_ctx *, T *, int, size_t, T *, T *, T *);    \
   template void launch_phiscale<T>(etd_ctx *, const T *, const T *, T *, int, int, int); \
-  template void launch_phiscale_pair<T>(etd_ctx *, const T *, const T *, T *, double, T *, double); \
+  template void launch_phiscale_pair<T>(etd_ctx *, const T *, const T *, T *, double, T *, double, int, int); \
   template void launch_nonlinear<T>(etd_ctx *, T *, const T *, int);                 \
   template void launch_elastic<T>(etd_ctx *);                                        \
   template void fft3d<T>(etd_ctx *, T *, int, int);                                  \
