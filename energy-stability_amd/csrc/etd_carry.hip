@@ -193,19 +193,16 @@ static size_t carry_elems(const etd_ctx *c, size_t *off) {
 
 bool carry_possible(const etd_ctx *c) { return c->P == 1 && c->nac > 0 && c->g == 2 && !c->no_carry; }
 
+// all or nothing: on any failure nothing stays allocated, and the next step
+// tries again
 template <typename T>
 int carry_prepare(etd_ctx *c) {
   if (c->cr_buf) return ETD_OK;
-  rocblas_handle hb;
-  if (rocblas_create_handle(&hb) != rocblas_status_success) return ETD_ERR_DEVICE;
-  rocblas_set_stream(hb, c->stream);
-  c->cr_blas = hb;
+  const int n = c->n, g = c->g, G = 2 * g;
   size_t off[10];
   const size_t tot = carry_elems<T>(c, off);
-  if (hipMalloc(&c->cr_buf, sizeof(T) * tot) != hipSuccess) return ETD_ERR_NOMEM;
-  if (hipMalloc(&c->cr_phi, sizeof(double) * 6 * (size_t)c->n) != hipSuccess) return ETD_ERR_NOMEM;
-  const int n = c->n, g = c->g, G = 2 * g;
-  T *h = (T *)malloc(sizeof(T) * (2 * (size_t)n * n + (size_t)n * G));
+  const size_t nh = 2 * (size_t)n * n + (size_t)n * G;
+  T *h = (T *)malloc(sizeof(T) * nh);
   if (!h) return ETD_ERR_NOMEM;
   // hVi is column-major: Vi(r, c) = hVi[r + n c]
   for (int r = 0; r < n; r++)
@@ -218,10 +215,27 @@ int carry_prepare(etd_ctx *c) {
       const int gi = s < g ? s : n - 2 * g + s;
       h[2 * (size_t)n * n + (size_t)a * G + s] = (T)c->hVi[a + (size_t)n * gi];
     }
-  hipError_t e = hipMemcpy((T *)c->cr_buf + off[7], h, sizeof(T) * (2 * (size_t)n * n + (size_t)n * G),
-                           hipMemcpyHostToDevice);
+  void *buf = nullptr;
+  double *phi = nullptr;
+  rocblas_handle hb = nullptr;
+  int rc = ETD_OK;
+  if (hipMalloc(&buf, sizeof(T) * tot) != hipSuccess || hipMalloc(&phi, sizeof(double) * 6 * (size_t)n) != hipSuccess)
+    rc = ETD_ERR_NOMEM;
+  else if (hipMemcpy((T *)buf + off[7], h, sizeof(T) * nh, hipMemcpyHostToDevice) != hipSuccess ||
+           rocblas_create_handle(&hb) != rocblas_status_success ||
+           rocblas_set_stream(hb, c->stream) != rocblas_status_success)
+    rc = ETD_ERR_DEVICE;
   free(h);
-  return e == hipSuccess ? ETD_OK : ETD_ERR_DEVICE;
+  if (rc) {
+    if (hb) rocblas_destroy_handle(hb);
+    (void)hipFree(buf);
+    (void)hipFree(phi);
+    return rc;
+  }
+  c->cr_blas = hb;
+  c->cr_phi = phi;
+  c->cr_buf = buf;
+  return ETD_OK;
 }
 
 template <typename T>

@@ -1723,7 +1723,14 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
   const int tid = threadIdx.x;
   const int ll = axis == 0 ? tid >> 4 : tid & 15;  // line within the group
   const int j = axis == 0 ? tid & 15 : tid >> 4;   // thread within the line
-  const unsigned L = blockIdx.x * 16 + ll;          // N^2 % 16 == 0: every line exists
+  // The FP32 unpack writes each real field's row segment of 16 x as 64 bytes,
+  // half a 128-byte line whose other half the next group of 16 lines writes.
+  // Workgroups go to the 8 XCDs in turn and each XCD has its own L2, so there
+  // each XCD takes a contiguous range of line groups (gridDim.x = N^2 / 16 is
+  // a multiple of 8): both halves of a line meet in one L2.
+  unsigned bx = blockIdx.x;
+  if constexpr (IO == 2 && sizeof(T2) == 8) bx = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const unsigned L = bx * 16 + ll;                  // N^2 % 16 == 0: every line exists
   const unsigned x = L % N, o = L / N;
   const size_t base = axis == 0 ? (size_t)L * N : (axis == 1 ? x + (size_t)N * N * o : x + (size_t)N * o);
   const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)N : (size_t)N * N);
@@ -1748,7 +1755,7 @@ __global__ __launch_bounds__(256) void k_fft16(T2 *data, size_t fs, int axis, co
 #pragma unroll
       for (int k = 0; k < R2; k++) {
         const int e = tid + 256 * k, l2 = e / N, xx = e - l2 * N;
-        const unsigned Lg = blockIdx.x * 16 + l2;
+        const unsigned Lg = bx * 16 + l2;
         const size_t src = IDX(io.n, (int)(Lg / N) + io.g, (int)(Lg % N) + io.g, xx + io.g);
         va[k] = u[n3 * pa + src];
         vb[k] = pb < io.nac ? u[n3 * pb + src] : (T)0;
