@@ -868,20 +868,13 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
                                        : (long long)IDX(n, wrap_e(n, n, g), wj, wi) + box_off(a, si, sj, 1);
   // physical sides: felas is 0 outside the interior
   const bool fel_zero = a.phys && (k < g || k >= n - g || j < g || j >= n - g || i < g || i >= n - g);
-  double s = 0.0;
-  for (int m0 = 0; m0 < a.nac; m0 += FB) {
-    T um[FB];
-#pragma unroll
-    for (int e = 0; e < FB; e++) um[e] = u[n3 * (unsigned)(m0 + e < a.nac ? m0 + e : a.nac - 1) + lo];
-#pragma unroll
-    for (int e = 0; e < FB; e++)
-      if (m0 + e < a.nac) s = s + (double)um[e] * (double)um[e];
-  }
   const felas_cell fc = felas_locate(a, k, j, i);
   const unsigned N3 = (unsigned)a.N * a.N * a.N, foff = (unsigned)fc.off;
   const double ce = a.LE * a.epn2;
-  for (int p0 = 0; p0 < a.nac; p0 += FB) {
-    T upv[FB], fel[FB], sbv[FB];
+  // batch p0 of the fields: u, felas and the ETD2 difference, all loads issued
+  // before any is used
+  T upv[FB], fel[FB], sbv[FB];
+  auto load_batch = [&](int p0) {
 #pragma unroll
     for (int e = 0; e < FB; e++) {
       const unsigned p = (unsigned)(p0 + e < a.nac ? p0 + e : a.nac - 1);
@@ -889,6 +882,21 @@ __global__ __launch_bounds__(256) void k_nonlin_ac(const T *__restrict__ u, T *_
       if (with_felas) fel[e] = fel_zero ? (T)0 : E[(unsigned)felas_slot(a, (int)p, fc.lag) * N3 + foff];
       if (sub) sbv[e] = sub[n3 * p + lo];
     }
+  };
+  // one batch holds every field (nac <= FB): its loads serve both passes
+  const bool one = a.nac <= FB;
+  if (one) load_batch(0);
+  double s = 0.0;
+  for (int m0 = 0; m0 < a.nac; m0 += FB) {
+    T um[FB];
+#pragma unroll
+    for (int e = 0; e < FB; e++) um[e] = one ? upv[e] : u[n3 * (unsigned)(m0 + e < a.nac ? m0 + e : a.nac - 1) + lo];
+#pragma unroll
+    for (int e = 0; e < FB; e++)
+      if (m0 + e < a.nac) s = s + (double)um[e] * (double)um[e];
+  }
+  for (int p0 = 0; p0 < a.nac; p0 += FB) {
+    if (!one) load_batch(p0);
 #pragma unroll
     for (int e = 0; e < FB; e++) {
       const int p = p0 + e;
