@@ -37,6 +37,10 @@ resident in HBM before every timed region.
   configuration (10 steps).  --cpu-full times the 64^3 x 4 configuration in
   full (100 steps, median of 3, 1 core and all cores) and 20 steps of the
   192^3 workload instead.
+--equal-error prints instead the nucleated state compared at equal error: the
+plain FP64 path at dt / 2^k against a Richardson-extrapolated FP64 reference,
+the headline path's error and time on the same state, and the FP64 step (and
+speed) that matches the headline's error.
 Prints one JSON line.
 """
 import argparse
@@ -135,6 +139,76 @@ def horizon_runs(pyetd, prm, fields, kw, count, mlp=None, log_cap=0):
     return walls, st, log, out
 
 
+def equal_error(args, pyetd, W, CT, prm, nsteps=100, levels=7):
+    """The nucleated state (one seeded nucleus per 16^3 cells, where the
+    growth of the nuclei limits dt) over t = 0 -> nsteps dt, compared at equal
+    error.  The reference is the plain FP64 path at dt / 2^(levels - 1) and
+    dt / 2^(levels - 2), Richardson-extrapolated (ETD1 is first order:
+    2 u(h) - u(2h) cancels the O(h) error).  Errors are max |d eta| over the
+    interior of the AC fields.  The plain FP64 path runs at dt / 2^k for
+    k < levels (timed with HIP events, no diagnostics), the headline path
+    (mixed precision + learned controller, tol as given) once; the FP64 step
+    that matches the headline's error is found from the finest runs' first-order
+    error constant, and the speedup is the headline's sim-time/s over the
+    FP64 path's at that step."""
+    import numpy as np
+    g0, nac, nch = prm["g"], prm["nac"], prm["nch"]
+    dt = prm["dt"]
+    t_end = dt * nsteps
+    nq = dict(prm, init="nuclei", nuclei_per_cell=16.0 ** -3, nuclei_radius=(3.0, 6.0))
+    ninit = W.initial_fields(nq)
+    inner = (slice(g0, -g0),) * 3
+
+    def fixed(k):
+        q = dict(nq, dt=dt / 2 ** k)
+        g = fresh(pyetd, q, ninit)
+        steps = nsteps * 2 ** k
+        g.time_steps(2, with_diag=False)  # warm: first launches and the carried spectrum's buffers
+        for v, f in enumerate(ninit):  # restart from the initial state
+            g.set_field(v, f)
+        ms, _ = g.time_steps(steps, with_diag=False)
+        eta = [g.get_field(v)[inner].copy() for v in range(nac)]
+        g.close()
+        print(f"equal-error: FP64 dt = {q['dt']:.4g}, {steps} steps, {ms[0] / 1e3:.2f} s", file=sys.stderr,
+              flush=True)
+        return eta, ms[0] / steps
+
+    runs = {k: fixed(k) for k in range(levels)}
+    ref = [2.0 * a - b for a, b in zip(runs[levels - 1][0], runs[levels - 2][0])]
+    err = lambda eta: max(float(np.abs(a - r).max()) for a, r in zip(eta, ref))
+    fixed_rows = [{"dt": dt / 2 ** k, "steps": nsteps * 2 ** k, "ms_per_step": runs[k][1],
+                   "sim_time_per_s": (dt / 2 ** k) / (runs[k][1] / 1e3), "max_deta": err(runs[k][0])}
+                  for k in range(levels)]
+    # the headline path on the same state
+    mlp = CT.load(args.weights) if args.controller == "learned" else None
+    kw = dict(t_end=t_end, tol=args.tol, dt0=dt, dt_min=1e-5, dt_max=1.0, controller=args.controller)
+    horizon_runs(pyetd, dict(nq, precision=args.precision), ninit, kw, 1, mlp)  # warm
+    walls, st, log, hf = horizon_runs(pyetd, dict(nq, precision=args.precision), ninit, kw, 1, mlp, log_cap=100000)
+    if mlp is not None:
+        mlp.close()
+    e_head = err([hf[v][inner] for v in range(nac)])
+    head = {"precision": args.precision, "controller": args.controller, "tol": args.tol, "wall_s": walls[0],
+            "sim_time_per_s": t_end / walls[0], "accepted": st["accepted"],
+            "rejected": st["rejected_err"] + st["rejected_energy"], "max_deta": e_head,
+            "energy_monotone": monotone(log[:, 3])}
+    # first-order error constant from the finest run that is not part of the reference
+    kc = levels - 3
+    c1 = fixed_rows[kc]["max_deta"] / fixed_rows[kc]["dt"]
+    dt_eq = e_head / c1
+    ms_fp64 = statistics.median(r["ms_per_step"] for r in fixed_rows)
+    fp_eq = dt_eq / (ms_fp64 / 1e3)
+    return {"metric": "sim-time/s at equal error, nucleated 192^3 x 13", "unit": "sim-time/s",
+            "workload": f"{prm['name']} from {round((prm['n'] - 2 * g0) ** 3 / 16 ** 3)} seeded nuclei "
+                        f"(radius 3-6 cells), t = 0 -> {t_end:g}",
+            "reference": f"plain FP64 path at dt / {2 ** (levels - 1)} and dt / {2 ** (levels - 2)}, "
+                         "Richardson-extrapolated; errors max |d eta| over the interior",
+            "reference_self_check_max_deta": err(runs[levels - 1][0]),
+            "fixed_dt_fp64": fixed_rows, "headline": head,
+            "fp64_first_order_constant": c1, "fp64_dt_at_headline_error": dt_eq,
+            "fp64_sim_time_per_s_at_headline_error": fp_eq,
+            "speedup_at_equal_error": head["sim_time_per_s"] / fp_eq}
+
+
 def monotone(E):
     import numpy as np
     E = np.asarray(E)
@@ -179,6 +253,8 @@ def main():
                     help="run one plain device copy of this size first (HBM counter calibration)")
     ap.add_argument("--only", choices=["all", "fp64", "mixed-fixed", "headline"], default="all",
                     help="run one part (for profiling)")
+    ap.add_argument("--equal-error", action="store_true",
+                    help="instead of the bench line: the nucleated state at equal error (about 3 minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: libetd runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
@@ -197,6 +273,9 @@ def main():
     dt = prm["dt"]
     t_end = dt * args.horizon_steps
     K, Wm = args.steps, args.warmup
+    if args.equal_error:
+        print(json.dumps(equal_error(args, pyetd, W, CT, prm)))
+        return
     init = W.initial_fields(prm)
     out = {"metric": METRIC, "unit": "sim-time/s", "n_gpus": 1, "steps": K, "warmup": Wm, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
