@@ -10,7 +10,8 @@
 #include "etd_internal.h"
 
 #define RBLOCKS 2048
-#define RCOLS 8
+#define RCOLS 9
+#define ERR_COL 8  // k_maxdiff_extrap's column, beside the acceptance statistics' eight
 
 // ------------------------------------------------------------------ reductions
 // Interior cells visited one (z, y) row per wave, lanes along x: one 32-bit
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(256) void k_accept_stats(const T *__restrict__ u, c
 // would hide a NaN cell from the error estimate)
 __host__ __device__ __forceinline__ double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
-// max |u - b| per block into part[block * RCOLS] (NaN if any cell is NaN) and,
+// max |u - b| per block into part[block * RCOLS + ERR_COL] (NaN if any cell is NaN) and,
 // with extrapolate, u = 2 u - b (local extrapolation of the step-doubling
 // pair) in the same pass
 template <typename T>
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void k_maxdiff_extrap(T *__restrict__ u, const
   __shared__ double sh[4];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x * RCOLS] = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
+  if (threadIdx.x == 0) part[blockIdx.x * RCOLS + ERR_COL] = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
 }
 
 
@@ -325,6 +326,7 @@ static int fetch_part(etd_ctx *c, int cols) {
   ACHK(hipMemcpyAsync(c->ad_hpart, c->ad_part, sizeof(double) * RBLOCKS * RCOLS, hipMemcpyDeviceToHost, c->stream));
   ACHK(hipStreamSynchronize(c->stream));
   (void)cols;
+  c->ad_fetches++;
   return ETD_OK;
 }
 
@@ -620,13 +622,16 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     st->step_evals += 3;
     ACHK(hipGetLastError());
     double err = 0.0;
+    // the error's partial maxima ride along with the acceptance statistics'
+    // fetch (one host synchronisation per attempt instead of two)
     k_maxdiff_extrap<T><<<RBLOCKS, 256, 0, c->stream>>>(F.u, SB, cnt, o->extrapolate, c->ad_part);
-    if ((rc = fetch_part(c, 1))) break;
-    for (int i = 0; i < RBLOCKS; i++) err = nanmax(err, c->ad_hpart[i * RCOLS]);
     double Enew, Eabs_new;
     run_feat nf;
     int nhf = 0;
+    const long long f0 = c->ad_fetches;
     if ((rc = energy_of<T>(c, ekind, with_el, &Enew, &Eabs_new, S0, &nf, &nhf))) break;
+    if (c->ad_fetches == f0 && (rc = fetch_part(c, RCOLS))) break;  // that energy fetched nothing
+    for (int i = 0; i < RBLOCKS; i++) err = nanmax(err, c->ad_hpart[i * RCOLS + ERR_COL]);
     const double scale = ekind == 0 ? fmax(Escale, fabs(E)) : fmax(Eabs, Eabs_new);
     // a non-finite candidate (error estimate or energy) is never accepted:
     // it is rejected with the largest shrink, and at dt_min the run stops

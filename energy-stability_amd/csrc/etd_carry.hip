@@ -17,7 +17,8 @@
 // with Vi_I the columns of Vi at the interior cells.  The two products of
 // each P, R, X are plain batched GEMMs (rocBLAS, strided-batched over the
 // AC fields and the 2g sweep slabs), and k_carry_expand adds the three sums
-// to the carried spectrum in one pass.
+// to the carried spectrum in one pass, which in a step also applies the AC
+// phi update (carry_correct_phi).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -41,8 +42,9 @@ __device__ __forceinline__ phi_pt phi_sep(double H, double e, double dt) {
 // (kkz Dz + kky Dy + kkx Dx) - dt LE KE, so with ax[a n + i] = exp(dt LE epn2
 // kk_a D_a[i]) (a = x, y, z; and at dt2 from 3 n) and e0 = exp(-dt LE KE),
 // exp(-dt H) = e0 ax_z[k^] ax_y[j^] ax_x[i^].  Every factor is at most 1
-// (D <= 0), so the product overflows only where e0 = exp(-dt H) of the
-// zero mode does, and then the update is not finite either way.
+// (D <= 0; LE, epn2, kk >= 0), so the product overflows only where e0 =
+// exp(-dt H) of the zero mode does, and then the update is not finite either
+// way.
 __global__ void k_phi_axes(const double *DDX, const double *DDY, const double *DDZ, int n, phi_args a,
                            int two, double *ax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -191,7 +193,13 @@ static size_t carry_elems(const etd_ctx *c, size_t *off) {
   return o;
 }
 
-bool carry_possible(const etd_ctx *c) { return c->P == 1 && c->nac > 0 && c->g == 2 && !c->no_carry; }
+// one box, two ghost layers, and coefficients whose exp(-dt H) factors stay
+// at most 1 (k_phi_axes)
+bool carry_possible(const etd_ctx *c) {
+  const etd_params &p = c->p;
+  return c->P == 1 && c->nac > 0 && c->g == 2 && !c->no_carry && p.LE >= 0 && p.epn2 >= 0 && p.kkx >= 0 &&
+         p.kky >= 0 && p.kkz >= 0;
+}
 
 // all or nothing: on any failure nothing stays allocated, and the next step
 // tries again

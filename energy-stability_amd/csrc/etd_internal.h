@@ -144,6 +144,7 @@ struct etd_ctx {
   void *cr_buf;
   double *cr_phi;          /* the per-axis factors of exp(-dt H) of the phi-fused correction */
   long long mp_fields;     /* field passes of the mode-product launches while timing */
+  long long ad_fetches;    /* host fetches of the adaptive partial sums */
   /* energy partials */
   double *part;            /* device partial sums */
   double *hpart;           /* pinned host copy */
