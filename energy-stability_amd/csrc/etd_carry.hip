@@ -14,9 +14,11 @@
 //   S^-1 d = sum_{s} Vi[k^][G_s] P_s[j^][i^] + Vi[j^][G_s] R_s[k^][i^]
 //                                           + Vi[i^][G_s] X_s[k^][j^]
 //   P_s = Vi d_z,s Vi^T,  R_s = Vi_I d_y,s Vi^T,  X_s = Vi_I d_x,s Vi_I^T
-// with Vi_I the columns of Vi at the interior cells.  The two products of
-// each P, R, X are plain batched GEMMs (rocBLAS, strided-batched over the
-// AC fields and the 2g sweep slabs), and k_carry_expand adds the three sums
+// with Vi_I the columns of Vi at the interior cells.  Recorded as n x n
+// matrices with zeros outside the interior rows (y) or rows and columns (x),
+// every one of them is Vi D Vi^T, so all three families are two plain
+// strided-batched GEMMs (rocBLAS) over the nac 2g 3 matrices, and
+// k_carry_expand adds the three sums
 // to the carried spectrum in one pass, which in a step also applies the AC
 // phi update (carry_correct_phi).
 #include <hip/hip_runtime.h>
@@ -174,22 +176,23 @@ __global__ __launch_bounds__(256) void k_carry_expand(const T *s1, T alpha, cons
   }
 }
 
-// buffer layout (elements of T): dz, dy, dx (the recorded changes), t1, P, R,
-// X (nac 2g n^2 each), Vi and Vi^T (row-major n^2 each), G4 (n x 2g)
+// buffer layout (elements of T): dz, dy, dx (the recorded changes, nac 2g
+// n x n matrices each, in that order), t1 (3 nac 2g n^2), P, R, X (nac 2g
+// n^2 each, in that order), Vi and Vi^T (row-major n^2 each), G4 (n x 2g)
 template <typename T>
 static size_t carry_elems(const etd_ctx *c, size_t *off) {
-  const size_t n = c->n, g = c->g, G = 2 * g, m = n - 2 * g, nac = c->nac, n2 = n * n;
+  const size_t n = c->n, g = c->g, G = 2 * g, nac = c->nac, n2 = n * n;
   size_t o = 0;
-  off[0] = o, o += nac * G * n2;     // dz
-  off[1] = o, o += nac * m * G * n;  // dy
-  off[2] = o, o += nac * m * m * G;  // dx
-  off[3] = o, o += nac * G * n2;     // t1
-  off[4] = o, o += nac * G * n2;     // P
-  off[5] = o, o += nac * G * n2;     // R
-  off[6] = o, o += nac * G * n2;     // X
-  off[7] = o, o += n2;               // Vi
-  off[8] = o, o += n2;               // Vi^T
-  off[9] = o, o += n * G;            // G4
+  off[0] = o, o += nac * G * n2;      // dz
+  off[1] = o, o += nac * G * n2;      // dy
+  off[2] = o, o += nac * G * n2;      // dx
+  off[3] = o, o += 3 * nac * G * n2;  // t1
+  off[4] = o, o += nac * G * n2;      // P
+  off[5] = o, o += nac * G * n2;      // R
+  off[6] = o, o += nac * G * n2;      // X
+  off[7] = o, o += n2;                // Vi
+  off[8] = o, o += n2;                // Vi^T
+  off[9] = o, o += n * G;             // G4
   return o;
 }
 
@@ -230,6 +233,8 @@ int carry_prepare(etd_ctx *c) {
   if (hipMalloc(&buf, sizeof(T) * tot) != hipSuccess || hipMalloc(&phi, sizeof(double) * 6 * (size_t)n) != hipSuccess)
     rc = ETD_ERR_NOMEM;
   else if (hipMemcpy((T *)buf + off[7], h, sizeof(T) * nh, hipMemcpyHostToDevice) != hipSuccess ||
+           // the zeros of dy and dx outside what the fill records
+           hipMemset((T *)buf + off[1], 0, sizeof(T) * (off[3] - off[1])) != hipSuccess ||
            rocblas_create_handle(&hb) != rocblas_status_success ||
            rocblas_set_stream(hb, c->stream) != rocblas_status_success)
     rc = ETD_ERR_DEVICE;
@@ -272,29 +277,19 @@ static void gemm_rm(etd_ctx *c, const float *A, long lda, long sa, const float *
                                 &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
 }
 
-// P, R, X from the recorded changes (six batched GEMMs; nothing to do on
-// physical sides, which have no ghost fill)
+// P, R, X from the recorded changes: T = D Vi^T, then Vi T, over the
+// 3 nac 2g matrices D of dz, dy, dx (nothing to do on physical sides, which
+// have no ghost fill)
 template <typename T>
 static void carry_gemms(etd_ctx *c) {
   if (!c->p.periodic) return;
-  const int n = c->n, g = c->g, G = 2 * g, m = n - 2 * g, nac = c->nac;
+  const int n = c->n, nb = 3 * c->nac * 2 * c->g;
   const long n2 = (long)n * n;
   size_t off[10];
   carry_elems<T>(c, off);
   T *b = (T *)c->cr_buf;
-  T *dz = b + off[0], *dy = b + off[1], *dx = b + off[2], *t1 = b + off[3];
-  T *P = b + off[4], *R = b + off[5], *X = b + off[6];
-  const T *Vi = b + off[7], *ViT = b + off[8];
-  const int nb = nac * G;
-  // z sweep: P = Vi (d Vi^T), d = dz[v][s] (n x n)
-  gemm_rm(c, dz, n, n2, ViT, n, 0, t1, n, n2, n, n, n, nb);
-  gemm_rm(c, Vi, n, 0, t1, n, n2, P, n, n2, n, n, n, nb);
-  // y sweep: R = Vi_I (d Vi^T), d = dy[v][s] (m x n)
-  gemm_rm(c, dy, n, (long)m * n, ViT, n, 0, t1, n, n2, m, n, n, nb);
-  gemm_rm(c, Vi + g, n, 0, t1, n, n2, R, n, n2, n, n, m, nb);
-  // x sweep: X = Vi_I (d Vi_I^T), d = dx[v][s] (m x m), Vi_I^T = rows g .. g+m-1 of Vi^T
-  gemm_rm(c, dx, m, (long)m * m, ViT + (long)g * n, n, 0, t1, n, n2, m, n, m, nb);
-  gemm_rm(c, Vi + g, n, 0, t1, n, n2, X, n, n2, n, n, m, nb);
+  gemm_rm(c, b + off[0], n, n2, b + off[8], n, 0, b + off[3], n, n2, n, n, n, nb);
+  gemm_rm(c, b + off[7], n, 0, b + off[3], n, n2, b + off[4], n, n2, n, n, n, nb);
 }
 
 template <typename T, int PH>

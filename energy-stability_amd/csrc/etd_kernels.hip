@@ -90,8 +90,9 @@ __device__ __forceinline__ void idx3(size_t l, int n, int &i, int &j, int &k) {
 // all boxes before the next reads pre-transfer data, as the reference's
 // send-everything-then-unpack order does.
 // dlt (or null): the change new - old of every cell a sweep writes, per
-// field as 2g contiguous matrices: z [plane][j][i], y [row][k - g][i], x
-// [column][k - g][j - g].
+// field as 2g contiguous n x n matrices: z [plane][j][i], y [row][k][i], x
+// [column][k][j] (y and x write their interior rows / columns only; the
+// rest of their matrices stays zero).
 template <typename T>
 __global__ void k_ghost_z(T *base, size_t fs, int n, int g, long long lo, long long hi, T *dlt) {
   T *u = base + fs * blockIdx.y;
@@ -117,8 +118,7 @@ __global__ void k_ghost_y(T *base, size_t fs, int n, int g, long long lo, long l
   int j = rr < g ? rr : n - 2 * g + rr;
   const T v = u[(rr < g ? lo : hi) + (long long)IDX(n, k, wrap_e(j, n, g), i)];
   if (dlt)
-    dlt[(size_t)(n - 2 * g) * rows * n * blockIdx.y + ((size_t)rr * (n - 2 * g) + (k - g)) * n + i] =
-        (T)((double)v - (double)u[IDX(n, k, j, i)]);
+    dlt[(size_t)rows * n * n * blockIdx.y + ((size_t)rr * n + k) * n + i] = (T)((double)v - (double)u[IDX(n, k, j, i)]);
   u[IDX(n, k, j, i)] = v;
 }
 template <typename T>
@@ -133,8 +133,7 @@ __global__ void k_ghost_x(T *base, size_t fs, int n, int g, long long lo, long l
   int i = cc < g ? cc : n - 2 * g + cc;
   const T v = u[(cc < g ? lo : hi) + (long long)IDX(n, k, j, wrap_e(i, n, g))];
   if (dlt)
-    dlt[(size_t)m * m * cols * blockIdx.y + ((size_t)cc * m + (k - g)) * m + (j - g)] =
-        (T)((double)v - (double)u[IDX(n, k, j, i)]);
+    dlt[(size_t)cols * n * n * blockIdx.y + ((size_t)cc * n + k) * n + j] = (T)((double)v - (double)u[IDX(n, k, j, i)]);
   u[IDX(n, k, j, i)] = v;
 }
 
