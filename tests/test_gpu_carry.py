@@ -102,6 +102,29 @@ def test_carried_steps_on_an_edge_wider_than_a_block(precision):
     b.close()
 
 
+def test_carried_steps_do_not_drift_over_a_long_run():
+    """300 FP64 steps with elasticity and a CH field: each carried step
+    starts from the spectrum the last backward transform consumed, so the
+    carried and the transformed runs stay together at roundoff instead of
+    drifting apart."""
+    say("carry over 300 steps")
+    prm = small_params(n=20, nac=3, nch=1, etd2=0, elastic=1, aniso=1, dt=0.02, landau=OMEGA_LANDAU, KE=42.0)
+    f = noise_fields(prm, amp=0.3, seed=11)
+    a, b = gpu(prm, True), gpu(prm, False)
+    for v, x in enumerate(f):
+        a.set_field(v, x)
+        b.set_field(v, x)
+    worst = []
+    for rep in range(3):
+        a.step(100)
+        b.step(100)
+        worst.append(max(float(np.abs(a.get_field(v) - b.get_field(v)).max()) for v in range(4)))
+    say(f"  carried vs transformed after 100, 200, 300 steps: {worst}")
+    assert max(worst) < 1e-11
+    a.close()
+    b.close()
+
+
 def test_setting_a_field_drops_the_carried_spectrum():
     """Step, replace the fields, step again: the same as a fresh context that
     starts from the replaced fields."""
