@@ -621,6 +621,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     }
     st->step_evals += 3;
     ACHK(hipGetLastError());
+    if ((rc = carry_status(c))) break;
     double err = 0.0;
     // the error's partial maxima ride along with the acceptance statistics'
     // fetch (one host synchronisation per attempt instead of two)

@@ -716,7 +716,7 @@ int step_once_t(etd_ctx *c) {
     if constexpr (sizeof(T) == 4) ch_mean_restore_centered(c, F.u);  // adds the mean back
   }
   HCHK(c, hipGetLastError());
-  return ETD_OK;
+  return carry_status(c);
 }
 
 template void modeprod3<double>(etd_ctx *, int, const double *, double *, int, int);

@@ -23,6 +23,7 @@
 // phi update (carry_correct_phi).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <rocblas/rocblas.h>
@@ -263,18 +264,31 @@ void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx) {
 
 // row-major C (M x N, row stride ldc) = A (M x K, lda) B (K x N, ldb) for
 // `batch` matrices (strides sa, sb, sc; 0 shares an operand), through the
-// column-major BLAS as C^T = B^T A^T
+// column-major BLAS as C^T = B^T A^T; a failure is kept in c->blas_status
+// for the step to report
 static void gemm_rm(etd_ctx *c, const double *A, long lda, long sa, const double *B, long ldb, long sb, double *C,
                     long ldc, long sc, int M, int N, int K, int batch) {
   const double one = 1.0, zero = 0.0;
-  rocblas_dgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M, K,
-                                &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+  const rocblas_status st =
+      rocblas_dgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M,
+                                    K, &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+  if (st != rocblas_status_success && !c->blas_status) c->blas_status = (int)st;
 }
 static void gemm_rm(etd_ctx *c, const float *A, long lda, long sa, const float *B, long ldb, long sb, float *C,
                     long ldc, long sc, int M, int N, int K, int batch) {
   const float one = 1.0f, zero = 0.0f;
-  rocblas_sgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M, K,
-                                &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+  const rocblas_status st =
+      rocblas_sgemm_strided_batched((rocblas_handle)c->cr_blas, rocblas_operation_none, rocblas_operation_none, N, M,
+                                    K, &one, B, ldb, sb, A, lda, sa, &zero, C, ldc, sc, batch);
+  if (st != rocblas_status_success && !c->blas_status) c->blas_status = (int)st;
+}
+
+int carry_status(etd_ctx *c) {
+  const int st = c->blas_status;
+  c->blas_status = 0;
+  if (!st) return ETD_OK;
+  snprintf(c->err, sizeof(c->err), "carry: rocBLAS strided-batched GEMM failed (status %d)", st);
+  return ETD_ERR_DEVICE;
 }
 
 // P, R, X from the recorded changes: T = D Vi^T, then Vi T, over the

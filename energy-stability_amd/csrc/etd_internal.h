@@ -143,6 +143,7 @@ struct etd_ctx {
   int carry_next;          /* inside step_once: leave spec[0 .. nac) as the AC spectrum of u */
   void *cr_buf;
   double *cr_phi;          /* the per-axis factors of exp(-dt H) of the phi-fused correction */
+  int blas_status;         /* first failing rocBLAS status of the correction since the last check */
   long long mp_fields;     /* field passes of the mode-product launches while timing */
   long long ad_fetches;    /* host fetches of the adaptive partial sums */
   /* energy partials */
@@ -307,6 +308,9 @@ static inline void carry_base(etd_ctx *c, const void *s1, double alpha, const vo
 }
 template <typename T>
 int carry_prepare(etd_ctx *c);
+/* ETD_OK, or the error of a failed correction GEMM since the last call
+ * (message in c->err) */
+int carry_status(etd_ctx *c);
 template <typename T>
 void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx);
 template <typename T>

@@ -44,6 +44,8 @@ CASES = [
     dict(n=16, nac=3, nch=1, etd2=0, elastic=1, aniso=0, precision=2, steps=6),
     dict(n=20, nac=4, nch=1, etd2=1, elastic=1, aniso=1, precision=2, steps=5),
     dict(n=16, nac=2, nch=1, etd2=1, elastic=0, aniso=0, precision=0, steps=5, periodic=0),
+    dict(n=20, nac=3, nch=1, etd2=0, elastic=1, aniso=0, precision=1, steps=5),  # BF16 split-3 products
+    dict(n=16, nac=1, nch=0, etd2=0, elastic=0, aniso=0, precision=0, steps=6),  # one AC field
 ]
 
 
