@@ -495,13 +495,14 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
     ACHK(hipHostMalloc(&c->ad_hpart, sizeof(double) * RBLOCKS * RCOLS, hipHostMallocDefault));
   }
   T *S0 = (T *)c->ad_S0, *SB = (T *)c->ad_SB, *SP = (T *)c->ad_spec, *SP2 = SP + cnt;
-  // ETD1 attempts carry the AC spectrum: the second half step starts from the
-  // first's phi-scaled spectrum SP2, and after an accepted attempt u is the
-  // backward transform of spec (u_h2) or, extrapolated, of 2 spec - SP (SP
-  // is not written again before the next attempt's forward stage).  A
-  // rejection moves u back to S0, whose spectrum is not kept: the next
-  // attempt transforms it.
-  const bool carry_run = !c->p.etd2 && carry_possible(c);
+  // The attempts carry the AC spectrum: the second half step starts from the
+  // first's spectrum (ETD1: phi-scaled, SP2; ETD2: its corrector's result),
+  // and after an accepted attempt u is the backward transform of spec (u_h2)
+  // or, extrapolated, of 2 spec - SP, SP the full step's spectrum (not
+  // written again before the next attempt's forward stage).  A rejection
+  // moves u back to S0, whose spectrum is not kept: the next attempt
+  // transforms it.
+  const bool carry_run = carry_possible(c);
   if (carry_run) {
     const int prc = carry_prepare<T>(c);
     if (prc) {
@@ -611,13 +612,28 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
       }
       if ((rc = step_once_t<T>(c))) break;
     } else {
+      // ETD2: the full step and the first half step share the predictor's
+      // forward stage and one phi pass; each corrector runs on its own
+      // predictor (and, carrying, leaves its step's spectrum: the full
+      // step's in SP, the half step's in spec)
+      if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
+      stage_forward<T>(c);
+      phi_stage<T>(c, SP, dt, SP2, 0.5 * dt);
       c->p.dt = dt;
-      if ((rc = step_once_t<T>(c))) break;
+      modeprod3<T>(c, 0, SP, F.u, V, 0);
+      etd2_stage<T>(c, SP, carry_run ? SP : (T *)nullptr);
       ACHK(hipMemcpyAsync(SB, F.u, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
-      ACHK(hipMemcpyAsync(F.u, S0, sizeof(T) * cnt, hipMemcpyDeviceToDevice, c->stream));
       c->p.dt = 0.5 * dt;
-      if ((rc = step_once_t<T>(c))) break;
-      if ((rc = step_once_t<T>(c))) break;
+      modeprod3<T>(c, 0, SP2, F.u, V, 0);
+      etd2_stage<T>(c, SP2, carry_run ? F.spec : (T *)nullptr);
+      if (carry_run) {
+        c->carry_ok = 1;
+        carry_base(c, nullptr, 1.0, nullptr, 0.0);
+      }
+      c->carry_next = carry_run;
+      rc = step_once_t<T>(c);
+      c->carry_next = 0;
+      if (rc) break;
     }
     st->step_evals += 3;
     ACHK(hipGetLastError());

@@ -680,38 +680,49 @@ void phi_stage(etd_ctx *c, T *out, double dt, T *out2, double dt2) {
   }
 }
 
+// The ETD2 corrector (calc.c:537-583) on the predictor a in u (its spectrum
+// a-hat was transformed back into u; N1 = N(u_n) is in place): u = a +
+// S phi2 S^-1 (N(a) - N1) at the context's dt.  With spec_out, the fill's
+// changes of a are recorded and spec_out = a-hat + S^-1 (those changes) +
+// phi2 S^-1 dN, the spectrum of the new u, which a following step carries
+// (spec_out may be a-hat).
 template <typename T>
-int step_once_t(etd_ctx *c) {
+void etd2_stage(etd_ctx *c, const T *ahat, T *spec_out) {
   const int V = c->V;
   const size_t n3 = c->n3;
   const etd_fields<T> F = fields<T>(c);
+  const int PV = V * c->P;
+  if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
+  if (spec_out) {  // the fill's changes of a enter the final spectrum too
+    T *dz, *dy, *dx;
+    carry_deltas<T>(c, &dz, &dy, &dx);
+    launch_ghost_fill<T>(c, F.u, c->nac, n3, dz, dy, dx);
+    if (c->nch) launch_ghost_fill<T>(c, F.u + n3 * c->nac, c->nch, n3);
+  } else {
+    launch_ghost_fill<T>(c, F.u, V, n3);
+  }
+  if (c->nel > 0) launch_elastic<T>(c);
+  nonlinear_boxes<T>(c, F.N2, (const T *)F.N1, 0);  // N(a) - N(u_n)
+  // the corrector's spectra in the N-hat half (free after the phi update),
+  // so that a-hat stays where it is
+  T *dsp = F.spec + n3 * PV;
+  modeprod3<T>(c, 1, F.N2, dsp, PV, 0);
+  phiscale_boxes<T>(c, dsp, (const T *)nullptr, 1);
+  modeprod3<T>(c, 0, dsp, F.u, PV, 1);  // u = a + S phi2 S^-1 dN
+  if (spec_out) carry_correct<T>(c, ahat, 1.0, dsp, 1.0, spec_out);
+  if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
+}
+
+template <typename T>
+int step_once_t(etd_ctx *c) {
+  const etd_fields<T> F = fields<T>(c);
   if constexpr (sizeof(T) == 4) ch_mean_mark(c, F.u);
   stage_forward<T>(c);
-  const int PV = V * c->P;
+  const int PV = c->V * c->P;
   phi_stage<T>(c, F.spec, c->p.dt, (T *)nullptr, 0.0);  // in place, at the context's dt
   modeprod3<T>(c, 0, F.spec, F.u, PV, 0);
   if (c->p.etd2) {
-    if constexpr (sizeof(T) == 4) ch_center(c, F.u, nullptr, +1);
-    if (c->carry_next) {  // the fill's changes of a enter the final spectrum too
-      T *dz, *dy, *dx;
-      carry_deltas<T>(c, &dz, &dy, &dx);
-      launch_ghost_fill<T>(c, F.u, c->nac, n3, dz, dy, dx);
-      if (c->nch) launch_ghost_fill<T>(c, F.u + n3 * c->nac, c->nch, n3);
-    } else {
-      launch_ghost_fill<T>(c, F.u, V, n3);
-    }
-    if (c->nel > 0) launch_elastic<T>(c);
-    nonlinear_boxes<T>(c, F.N2, (const T *)F.N1, 0);  // N(a) - N(u_n)
-    // the corrector's spectra in the N-hat half (free after the phi update),
-    // so that spec[0 .. V) keeps a-hat
-    T *dsp = F.spec + n3 * PV;
-    modeprod3<T>(c, 1, F.N2, dsp, PV, 0);
-    phiscale_boxes<T>(c, dsp, (const T *)nullptr, 1);
-    modeprod3<T>(c, 0, dsp, F.u, PV, 1);  // u = a + S phi2 S^-1 dN
-    // the spectrum of u for the next step: a-hat + S^-1 (the fill's changes
-    // of a) + phi2 S^-1 dN
-    if (c->carry_next) carry_correct<T>(c, F.spec, 1.0, dsp, 1.0, F.spec);
-    if constexpr (sizeof(T) == 4) ch_mean_restore(c, F.u);
+    etd2_stage<T>(c, F.spec, c->carry_next ? F.spec : (T *)nullptr);
   } else {
     if constexpr (sizeof(T) == 4) ch_mean_restore_centered(c, F.u);  // adds the mean back
   }
@@ -725,6 +736,8 @@ template void phi_stage<double>(etd_ctx *, double *, double, double *, double);
 template void phi_stage<float>(etd_ctx *, float *, double, float *, double);
 template void stage_forward<double>(etd_ctx *);
 template void stage_forward<float>(etd_ctx *);
+template void etd2_stage<double>(etd_ctx *, const double *, double *);
+template void etd2_stage<float>(etd_ctx *, const float *, float *);
 template int step_once_t<double>(etd_ctx *);
 template int step_once_t<float>(etd_ctx *);
 

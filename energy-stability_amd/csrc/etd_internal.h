@@ -255,6 +255,10 @@ void launch_phiscale_pair(etd_ctx *c, const T *uhat, const T *nhat, T *out, doub
  * left pending is corrected in the same pass */
 template <typename T>
 void phi_stage(etd_ctx *c, T *out, double dt, T *out2, double dt2);
+/* the ETD2 corrector on the predictor in u (its spectrum a-hat); with
+ * spec_out, also the spectrum of the result for a carrying step */
+template <typename T>
+void etd2_stage(etd_ctx *c, const T *ahat, T *spec_out);
 template <typename T>
 void launch_nonlinear(etd_ctx *c, T *out, const T *sub, int ch_centre = 0);
 template <typename T>

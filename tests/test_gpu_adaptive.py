@@ -70,6 +70,32 @@ def test_all_accepted_without_extrapolation_is_half_steps(precision, etd2, elast
             assert np.array_equal(a.get_field(v), b.get_field(v))
 
 
+@pytest.mark.parametrize("etd2", [0, 1])
+def test_one_extrapolated_attempt_is_two_half_steps_against_one_full_step(etd2):
+    """One accepted attempt with local extrapolation is 2 u(dt/2, twice) -
+    u(dt, once), bit for bit against plain stepping of both (FP64, no
+    elasticity, every step transforming u): the full step and the first
+    half step share their forward stage (and for ETD2 the predictor's), and
+    each comes out as a step of its own would."""
+    say(f"one extrapolated attempt etd2={etd2}")
+    dt = 2.0 ** -6
+    prm = small_params(n=16, nac=3, nch=1, elastic=0, aniso=1, dt=dt, etd2=etd2)
+    f = noise_fields(prm, amp=0.3)
+    a = gpu(prm, f, carry=False)
+    half = gpu(dict(prm, dt=dt / 2), f, carry=False)
+    full = gpu(prm, f, carry=False)
+    st, _ = a.run_adaptive(log_cap=4, t_end=dt, dt0=dt, tol=1e9, grow=1.0, shrink=1.0, extrapolate=1,
+                           energy_test=0)
+    assert st["accepted"] == 1 and st["step_evals"] == 3
+    half.step(2)
+    full.step(1)
+    for v in range(4):
+        want = 2.0 * half.get_field(v) - full.get_field(v)
+        assert np.array_equal(a.get_field(v), want), v
+    for g in (a, half, full):
+        g.close()
+
+
 @pytest.mark.parametrize("elastic", [0, 1])
 def test_plain_controller_matches_oracle_restatement(elastic):
     say(f"adaptive vs oracle restatement elastic={elastic}")
@@ -96,15 +122,15 @@ def test_plain_controller_matches_oracle_restatement(elastic):
         assert np.abs(g.get_field(v) - o.get_field(v)).max() < 1e-10
 
 
-@pytest.mark.parametrize("precision,extrapolate", [(0, 1), (2, 1), (0, 0)])
-def test_carried_attempts_match_transformed_attempts(precision, extrapolate):
-    """ETD1 attempts that carry the AC spectrum (the second half step from the
-    first's phi-scaled spectrum, the next attempt from spec or 2 spec - SP)
-    make the same decisions as attempts that transform u, with fields equal
-    to roundoff; rejections (which drop the carry) included."""
-    say(f"adaptive carried vs transformed precision={precision} extrapolate={extrapolate}")
-    prm = dict(small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, dt=0.01, KE=3.2, landau=(0.2, 3.0, 3.0)),
-               precision=precision)
+@pytest.mark.parametrize("precision,extrapolate,etd2", [(0, 1, 0), (2, 1, 0), (0, 0, 0), (0, 1, 1), (2, 1, 1)])
+def test_carried_attempts_match_transformed_attempts(precision, extrapolate, etd2):
+    """Attempts that carry the AC spectrum (the second half step from the
+    first's spectrum, the next attempt from spec or 2 spec - SP) make the same
+    decisions as attempts that transform u, with fields equal to roundoff;
+    rejections (which drop the carry) included; ETD1 and ETD2."""
+    say(f"adaptive carried vs transformed precision={precision} extrapolate={extrapolate} etd2={etd2}")
+    prm = dict(small_params(n=16, nac=3, nch=1, elastic=1, aniso=1, dt=0.01, KE=3.2, landau=(0.2, 3.0, 3.0),
+                            etd2=etd2), precision=precision)
     f = noise_fields(prm, amp=0.05, ch_amp=0.05)
     if precision:
         f = [x.astype(np.float32).astype(np.float64) for x in f]
