@@ -118,6 +118,12 @@ def breakdown(ms, K):
             "halo": ms[6] / K}
 
 
+def progress(msg):
+    """A line on stderr as each part of the run ends (the JSON line goes to
+    stdout at the very end)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def horizon_runs(pyetd, prm, fields, kw, count, mlp=None, log_cap=0):
     """`count` adaptive integrations of the horizon from `fields`, each timed
     on its own (set_field and run_adaptive synchronise the stream on both
@@ -134,6 +140,7 @@ def horizon_runs(pyetd, prm, fields, kw, count, mlp=None, log_cap=0):
         t0 = time.perf_counter()
         st, log = g.run_adaptive(log_cap=log_cap, **kw)
         walls.append(time.perf_counter() - t0)
+        progress(f"adaptive integration to t = {kw['t_end']:g}: {st['accepted']} accepted, {walls[-1]:.3f} s")
     out = [g.get_field(v) for v in range(len(fields))]
     g.close()
     return walls, st, log, out
@@ -286,6 +293,7 @@ def main():
     fp = None
     if args.only in ("all", "fp64"):
         ms, launches = fixed_dt(pyetd, prm, init, K, Wm, args.calib_copy_mib << 20)
+        progress(f"plain FP64 fixed dt: {ms[0] / K:.3f} ms per step")
         # field passes the mode products made (3 axes x the fields of each
         # transform: 2V forward and V backward, or nch + V forward when the
         # AC spectrum is carried), counted by the library
@@ -306,6 +314,7 @@ def main():
     if args.only in ("all", "mixed-fixed"):
         q = dict(prm, precision=args.precision)
         ms, launches = fixed_dt(pyetd, q, init, K, Wm, (args.calib_copy_mib << 20) if args.only == "mixed-fixed" else 0)
+        progress(f"mixed fixed dt: {ms[0] / K:.3f} ms per step")
         mx = {"ms_per_step": ms[0] / K, "sim_time_per_s_fixed_dt": dt * K / (ms[0] / 1e3),
               "grid_point_variant_steps_per_s": N ** 3 * V * K / (ms[0] / 1e3), "breakdown_ms_per_step": breakdown(ms, K)}
         fft_launch_ms = ms[7] / max(1, launches[6]) / 3.0  # 3 axis passes per fft3d group
@@ -381,6 +390,7 @@ def main():
             out["speedup_vs_plain_fp64"] = out["value"] / fp["sim_time_per_s"]
         # ---- fixed-error check against the plain FP64 path, and the mixed fixed-dt report
         if not args.quick:
+            progress(f"accuracy: {args.horizon_steps} fixed steps, plain FP64 and mixed")
             rows, ref, sa = AC.compare(prm, dict(prm, precision=args.precision), args.horizon_steps, every=100,
                                        fields=init, log=None, keep_ref=True)
             g2 = fresh(pyetd, dict(prm, precision=args.precision), headline_fields)
@@ -407,6 +417,7 @@ def main():
             nkw = dict(kw, t_end=nt)
             nw, nst, nlog, nfields = horizon_runs(pyetd, dict(nq, precision=args.precision), ninit, nkw,
                                                   args.nuc_repeats, mlp, log_cap=100000)
+            progress(f"nucleated: {nsteps} fixed steps, plain FP64 and mixed")
             nrows, nref, nsa = AC.compare(nq, dict(nq, precision=args.precision), nsteps, every=nsteps,
                                           fields=ninit, log=None, keep_ref=True)
             nmed = statistics.median(nw)
