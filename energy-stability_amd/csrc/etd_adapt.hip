@@ -502,14 +502,7 @@ static int run_adaptive(etd_ctx *c, const etd_adapt_opts *o, etd_adapt_stats *st
   // written again before the next attempt's forward stage).  A rejection
   // moves u back to S0, whose spectrum is not kept: the next attempt
   // transforms it.
-  const bool carry_run = carry_possible(c);
-  if (carry_run) {
-    const int prc = carry_prepare<T>(c);
-    if (prc) {
-      snprintf(c->err, sizeof(c->err), "carry: cannot allocate the correction buffers");
-      return prc;
-    }
-  }
+  const bool carry_run = carry_ready(c);
   c->carry_ok = 0;
   carry_base(c, nullptr, 1.0, nullptr, 0.0);
   const double dt_fixed = c->p.dt;

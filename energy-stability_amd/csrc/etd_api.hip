@@ -745,11 +745,7 @@ template int step_once_t<float>(etd_ctx *);
 // the next step carries (the adaptive driver and the ensemble call
 // step_once_t directly and never set carry_ok)
 static int step_once(etd_ctx *c) {
-  if (carry_possible(c)) {
-    const int rc = c->mixed ? carry_prepare<float>(c) : carry_prepare<double>(c);
-    if (rc) return seterr(c, rc, "carry: cannot allocate the correction buffers");
-  }
-  c->carry_next = carry_possible(c);
+  c->carry_next = carry_ready(c);
   const int rc = c->mixed ? step_once_t<float>(c) : step_once_t<double>(c);
   c->carry_ok = rc == ETD_OK && c->carry_next;
   carry_base(c, nullptr, 1.0, nullptr, 0.0);

@@ -205,8 +205,7 @@ bool carry_possible(const etd_ctx *c) {
          p.kky >= 0 && p.kkz >= 0;
 }
 
-// all or nothing: on any failure nothing stays allocated, and the next step
-// tries again
+// all or nothing: on any failure nothing stays allocated (see carry_ready)
 template <typename T>
 int carry_prepare(etd_ctx *c) {
   if (c->cr_buf) return ETD_OK;
@@ -250,6 +249,19 @@ int carry_prepare(etd_ctx *c) {
   c->cr_phi = phi;
   c->cr_buf = buf;
   return ETD_OK;
+}
+
+// whether this context carries from here on: the buffers exist or were just
+// made.  If they cannot be made (device memory short), the context stops
+// carrying for good and steps transform u, as with ETD_CARRY=0; the failed
+// allocation's error is cleared so that it does not fail the step.
+bool carry_ready(etd_ctx *c) {
+  if (!carry_possible(c)) return false;
+  if ((c->mixed ? carry_prepare<float>(c) : carry_prepare<double>(c)) == ETD_OK) return true;
+  (void)hipGetLastError();
+  c->no_carry = 1;
+  c->carry_ok = 0;
+  return false;
 }
 
 template <typename T>

@@ -315,6 +315,9 @@ int carry_prepare(etd_ctx *c);
 /* ETD_OK, or the error of a failed correction GEMM since the last call
  * (message in c->err) */
 int carry_status(etd_ctx *c);
+/* carry_possible, with the buffers prepared; where they cannot be
+ * allocated the context stops carrying (no_carry) and this is false */
+bool carry_ready(etd_ctx *c);
 template <typename T>
 void carry_deltas(etd_ctx *c, T **dz, T **dy, T **dx);
 template <typename T>
